@@ -1,0 +1,161 @@
+"""Band-detection throughput on one MI355X (patterns/s), with roofline and CPU baseline.
+
+Workload (default): 16,384 synthetic 120x120 uint16 Kikuchi-like FCC patterns, background
+subtraction, Radon 180x90, 9x5 Radon-space kernel (tSigma=1, rSigma=2), 9 bands with
+sub-pixel fit -- BandDetect.find_bands on patterns already resident in HBM.  One step =
+one find_bands pass over the whole batch.
+
+Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "patterns/s (Radon+band detect), batched vs plain, at ≤1e-3-bin peak error"
+LDS_PEAK_GBS = 256 * 128 * 2.4          # 256 CU x 128 B/clk x 2.4 GHz (nominal LDS read bandwidth)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--npats", type=int, default=16384)
+    ap.add_argument("--size", type=int, default=120)
+    ap.add_argument("--dtype", default="uint16")
+    ap.add_argument("--no-background", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-reps", type=int, default=5)
+    return ap.parse_args()
+
+
+def cpu_baseline(pats_host, bg, params, target_s):
+    """Reference-faithful CPU restatement (oracle.ref_cpu) on a bounded sample."""
+    from oracle import ref_cpu
+    H, W = pats_host.shape[1:]
+    ref = ref_cpu.RefBandDetect((H, W), **params)
+    ref.backgroundsub = bg
+    ref.find_bands(pats_host[:64])  # warm-up (also builds the C library if needed)
+    done, t0 = 0, time.perf_counter()
+    while done < pats_host.shape[0]:
+        n = min(512, pats_host.shape[0] - done)
+        ref.find_bands(pats_host[done:done + n], chunksize=512)
+        done += n
+        if time.perf_counter() - t0 >= target_s:
+            break
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
+              f"(patterns shard independently across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    H = W = args.size
+    params = dict(nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9)
+    bd = BandDetect(patDim=(H, W), **params)
+    print(f"[bench] generating {args.npats} {H}x{W} {args.dtype} patterns", file=sys.stderr, flush=True)
+    pats = kikuchi_patterns(args.npats, (H, W), args.dtype, seed=2024, device=dev, chunk=1024)
+    torch.cuda.synchronize()
+    sample = pats[:1024].cpu().numpy()
+    bg = None
+    if not args.no_background:
+        bd.collect_background(patsIn=sample.astype(np.float64), nsample=1024)
+        bg = bd.backgroundsub
+    plan = bd.device_plan()
+    info = plan.info()
+    N = pats.shape[0]
+    out = plan.alloc_out(N)
+    work = plan.workspace(N, N)
+    print(f"[bench] plan {info}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        bd.find_bands_device(pats, out=out, work=work, chunk=N)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bd.find_bands_device(pats, out=out, work=work, chunk=N)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = N * args.steps / (t1 - t0)
+    print(f"[bench] find_bands {ms:.3f} ms/step, {value:.1f} patterns/s", file=sys.stderr, flush=True)
+
+    # ---- per-kernel durations with HIP events on the launch stream ----
+    s = torch.cuda.current_stream(dev)
+    rad = work[:N * bd.nRho * bd.nTheta].view(N, bd.nRho, bd.nTheta)
+    x, flags = bd._prepare(plan, pats)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    plan.radon(x, flags=flags, out=rad)
+    torch.cuda.synchronize()
+    ev[0].record(s)
+    for _ in range(args.roofline_reps):
+        plan.radon(x, flags=flags, out=rad)
+    ev[1].record(s)
+    for _ in range(args.roofline_reps):
+        plan.peaks(rad, out=out)
+    ev[2].record(s)
+    torch.cuda.synchronize()
+    gather_ms = ev[0].elapsed_time(ev[1]) / args.roofline_reps
+    peaks_ms = ev[1].elapsed_time(ev[2]) / args.roofline_reps
+    pairs = int(info["pairs"])
+    lds_bytes = 4.0 * pairs * N                       # algorithmic LDS bytes per gather launch
+    lds_gbs = lds_bytes / (gather_ms * 1e-3) / 1e9
+    esz = {"uint8": 1, "uint16": 2, "float32": 4}[args.dtype]
+    hbm_bytes = N * (H * W * esz + 9 * 28 + 12)       # input + band output per launch
+    hbm_gbs = hbm_bytes / (ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_radon_gather_{H}x{W}_{args.dtype}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline (oracle restatement of the reference CPU path)", file=sys.stderr, flush=True)
+        pps, done, dt = cpu_baseline(sample, bg, params, args.cpu_seconds)
+        cpu = {"value": round(pps, 2), "unit": "patterns/s", "cores": os.cpu_count(),
+               "kind": "port",
+               "sample": f"{done} of the same {H}x{W} {args.dtype} patterns in {dt:.1f} s "
+                         f"(chunks of 512, FFT workers = os.cpu_count(), OMP_NUM_THREADS="
+                         f"{os.environ.get('OMP_NUM_THREADS', 'unset')})"}
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "patterns/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16" if esz == 2 else
+        ("u8" if esz == 1 else "f32"), "data": "synthetic",
+        "config": {"workload": f"plain find_bands, {N} x {H}x{W} {args.dtype} FCC patterns, "
+                               f"{'background subtraction, ' if bg is not None else ''}Radon 180x90, "
+                               f"kernel 9x5, 9 bands", "global_batch": N, "parallelism": "none"},
+        "roofline": {"bound": "lds", "kernel": "k_radon_gather", "achieved": round(lds_gbs, 1),
+                     "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes_per_launch": lds_bytes,
+                     "avg_launch_ms": round(gather_ms, 4), "pairs_per_pattern": pairs},
+        "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_step": hbm_bytes},
+        "kernels_ms": {"k_radon_gather": round(gather_ms, 4), "k_peaks": round(peaks_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

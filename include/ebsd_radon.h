@@ -1,0 +1,110 @@
+/*
+ * ebsd_radon.h -- C ABI of the MI355X (gfx950) band-detection library.
+ *
+ * The library replaces the compute inside PyEBSDIndex's BandDetect.find_bands
+ * (pyebsdindex/band_detect.py:411-526): the fast Radon transform
+ * (pyebsdindex/radon_fast.py:242-310), the Radon-space padding and convolution
+ * (band_detect.py:562-637), the local-max search (band_detect.py:639-664) and the
+ * peak labelling with sub-pixel fit (band_detect.py:667-809), plus the normmax and
+ * theta/rho conversion of band_detect.py:457-460,521-524.
+ *
+ * Conventions: every pointer named d_* is device memory; every other pointer is host
+ * memory.  Functions return 0 on success and a negative code on failure; the message
+ * is available from ebsd_last_error() (per thread).  A plan is immutable after set-up
+ * except through ebsd_plan_set_background/ebsd_plan_set_tuning; calls that use one
+ * plan from several host threads at once must be serialised by the caller.
+ */
+#ifndef EBSD_RADON_H
+#define EBSD_RADON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ebsd_plan ebsd_plan;
+
+/* Pattern element types accepted by ebsd_find_bands (numpy uint8 / uint16 / float32).
+ * OR-ing EBSD_FLAG_NOBG into the dtype argument ignores the plan's background for that
+ * call (used when the caller has already subtracted it, e.g. for float64 input). */
+enum { EBSD_U8 = 0, EBSD_U16 = 1, EBSD_F32 = 2, EBSD_FLAG_NOBG = 0x100 };
+
+/* Output fields of one find_bands call, all device pointers, row-major (N, nBands[,2]).
+ * Mirrors the numeric fields of BandDetect.dataType (band_detect.py:110-115). */
+typedef struct ebsd_bands {
+    float  *d_max;      /* (N, nB)     peak value of the convolved Radon            */
+    float  *d_normmax;  /* (N, nB)     (max - min) / clip(mean - min, 1e-7)          */
+    float  *d_avemax;   /* (N, nB)     mean of the 3x3 neighbourhood                 */
+    float  *d_maxloc;   /* (N, nB, 2)  integer (rho, theta) bin, padding removed     */
+    float  *d_aveloc;   /* (N, nB, 2)  sub-pixel (rho, theta) bin                    */
+    float  *d_width;    /* (N, nB)     FWHM-style width in rho bins                  */
+    float  *d_theta;    /* (N, nB)     radians, detector convention                  */
+    float  *d_rho;      /* (N, nB)     pixels, detector convention                   */
+    int8_t *d_valid;    /* (N, nB)                                                   */
+    float  *d_stats;    /* (N, 3)      min, mean, max of the convolved Radon region  */
+} ebsd_bands;
+
+/* Build a device plan from the reference's own set-up products.
+ *   index_plan: Radon.indexPlan, int64 (nRho, nTheta, plan_len), each line sorted,
+ *               entries >= H*W ending the line      (radon_fast.py:104-165)
+ *   rdnmask:    BandDetect.rdnmask as uint8 (nRho, nTheta)        (band_detect.py:203-251)
+ *   kernel:     BandDetect.kernel[0], float32 (kr, kt), odd sizes  (band_detect.py:277-285)
+ *   pad_r/t:    BandDetect.padding; peak_r/t: BandDetect.peakPad   (band_detect.py:287-290)
+ *   theta_deg:  Radon.theta (nTheta,) float32; rho: Radon.rho (nRho,) float32
+ * Replaces: the plan upload of the OpenCL backend (opencl/band_detect_cl.py:284-291),
+ * which re-sends indexPlan every chunk; here it is compiled once per pattern size. */
+int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_t *index_plan, int plan_len,
+                     const uint8_t *rdnmask, const float *kernel, int kr, int kt, int pad_r,
+                     int pad_t, int peak_r, int peak_t, int nBands, const float *theta_deg,
+                     const float *rho, ebsd_plan **out);
+
+int ebsd_plan_destroy(ebsd_plan *plan);
+
+/* Background subtracted before the Radon (BandDetect.backgroundsub, radon_fast.py:255-259).
+ * bg: host float64 (H*W) or NULL to clear.  exact=1 stages float32(x - bg) per pixel and
+ * accumulates in float64 exactly as the reference; exact=0 subtracts the per-line float64
+ * sum of bg from the exact integer line sum (a float64-exact form of the same quantity). */
+int ebsd_plan_set_background(ebsd_plan *plan, const double *bg, int exact);
+
+/* Kernel launch shape.  Any argument <= 0 keeps the current value.
+ *   gather_pack:    patterns interleaved per Radon work-group (u8: 4/8/16, u16: 2/4/8, f32: 1/2/4)
+ *   gather_threads: threads per Radon work-group (multiple of 64, <= 1024)
+ *   peak_tile:      theta columns per tile of the convolution/peak kernel
+ *   peak_threads:   threads per convolution/peak work-group (multiple of 64, <= 1024) */
+int ebsd_plan_set_tuning(ebsd_plan *plan, int gather_pack, int gather_threads, int peak_tile,
+                         int peak_threads);
+
+/* Workspace (device bytes) that ebsd_find_bands needs to process `chunk` patterns per pass. */
+size_t ebsd_workspace_bytes(const ebsd_plan *plan, int64_t chunk);
+
+/* BandDetect.find_bands on N patterns resident on the device (band_detect.py:411-526).
+ * d_pats: (N, H, W) of `dtype`; d_work: workspace of ws_bytes (>= ebsd_workspace_bytes(plan,1)),
+ * patterns are processed in chunks that fit it.  stream: a hipStream_t (NULL = default).
+ * Replaces: band_detect_cl.BandDetect.find_bands (opencl/band_detect_cl.py:51-199). */
+int ebsd_find_bands(ebsd_plan *plan, const void *d_pats, int dtype, int64_t N,
+                    const ebsd_bands *out, void *d_work, size_t ws_bytes, void *stream);
+
+/* Stage-level entry points (the reference's per-stage methods, for tests and adjoints). */
+/* Radon of N patterns into d_radon (N, nRho, nTheta) float32, -1 where a line is empty.
+ * Replaces: Radon.radon_faster (radon_fast.py:242-279) without padding. */
+int ebsd_radon(ebsd_plan *plan, const void *d_pats, int dtype, int64_t N, float *d_radon,
+               void *stream);
+/* Padding + convolution + local max + labelling of N unpadded Radons (N, nRho, nTheta).
+ * Replaces: rdn_conv + rdn_local_max + band_label (band_detect.py:562-809). */
+int ebsd_peaks(ebsd_plan *plan, const float *d_radon, int64_t N, const ebsd_bands *out,
+               void *stream);
+/* Padded, convolved Radon (N, nRho+2*pad_r, nTheta+2*pad_t) float32 for parity checks.
+ * Replaces: the rdnConv array of rdn_conv (band_detect.py:562-637). */
+int ebsd_conv(ebsd_plan *plan, const float *d_radon, int64_t N, float *d_conv, void *stream);
+
+/* Plan geometry queries. */
+int ebsd_plan_info(const ebsd_plan *plan, int64_t *out8);
+
+const char *ebsd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

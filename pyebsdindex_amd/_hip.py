@@ -1,0 +1,77 @@
+"""ctypes binding of the C-ABI library ``libebsd_radon.so`` (declared in include/ebsd_radon.h).
+
+There is deliberately no fallback: if the library is missing or fails to load, every
+entry point raises ``HipLibraryError``.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libebsd_radon.so")
+
+EBSD_U8, EBSD_U16, EBSD_F32 = 0, 1, 2
+
+#: exported symbols, in declaration order of include/ebsd_radon.h
+SYMBOLS = (
+    "ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background", "ebsd_plan_set_tuning",
+    "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
+    "ebsd_plan_info", "ebsd_last_error",
+)
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+class EbsdBands(ctypes.Structure):
+    """Mirror of ``ebsd_bands`` (device pointers of the output fields)."""
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "d_max", "d_normmax", "d_avemax", "d_maxloc", "d_aveloc", "d_width", "d_theta", "d_rho",
+        "d_valid", "d_stats")]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path: str = LIB_PATH):
+    """Load the library once; raise HipLibraryError if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise HipLibraryError(
+                f"HIP library not found at {path}; build it with `python -m pyebsdindex_amd.build`")
+        try:
+            L = ctypes.CDLL(path)
+        except OSError as e:
+            raise HipLibraryError(f"failed to load {path}: {e}") from e
+        P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+        L.ebsd_plan_create.argtypes = [I, I, I, I, P, I, P, P, I, I, I, I, I, I, I, P, P,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.ebsd_plan_destroy.argtypes = [P]
+        L.ebsd_plan_set_background.argtypes = [P, P, I]
+        L.ebsd_plan_set_tuning.argtypes = [P, I, I, I, I]
+        L.ebsd_workspace_bytes.argtypes = [P, I64]
+        L.ebsd_workspace_bytes.restype = ctypes.c_size_t
+        L.ebsd_find_bands.argtypes = [P, P, I, I64, ctypes.POINTER(EbsdBands), P, ctypes.c_size_t, P]
+        L.ebsd_radon.argtypes = [P, P, I, I64, P, P]
+        L.ebsd_peaks.argtypes = [P, P, I64, ctypes.POINTER(EbsdBands), P]
+        L.ebsd_conv.argtypes = [P, P, I64, P, P]
+        L.ebsd_plan_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
+        L.ebsd_last_error.argtypes = []
+        L.ebsd_last_error.restype = ctypes.c_char_p
+        for name in ("ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background",
+                     "ebsd_plan_set_tuning", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks",
+                     "ebsd_conv", "ebsd_plan_info"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+        return L
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().ebsd_last_error().decode(errors="replace")
+        raise HipLibraryError(f"ebsd library error {rc}: {msg}")

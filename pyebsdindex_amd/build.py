@@ -1,0 +1,40 @@
+"""Build the gfx950 HIP library ``libebsd_radon.so`` in-tree.
+
+``python -m pyebsdindex_amd.build`` (or ``__graft_entry__.build()``) runs hipcc on
+``csrc/ebsd_radon.hip``.  The library is a plain C-ABI shared object (no torch
+symbols); ``pyebsdindex_amd._hip`` loads it with ctypes.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "ebsd_radon.hip")
+LIB = os.path.join(HERE, "libebsd_radon.so")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    # float contraction only where the source asks for fmaf(): keeps the f64 parity
+    # arithmetic (sub-pixel fit, np.interp restatement) free of fused operations
+    "-ffp-contract=off",
+    # no packed-FP32 VALU instructions anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", tmp, SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))
+    sys.exit(0)

@@ -1,0 +1,1056 @@
+// ebsd_radon.hip -- MI355X (gfx950) band detection: fast Radon gather-sum, Radon-space
+// padding + convolution, masked local-max peak search with sub-pixel fit.
+//
+// Reference path: PyEBSDIndex BandDetect.find_bands (pyebsdindex/band_detect.py:411-526)
+// on its CPU/Numba backend: radon_fast.py:242-310, band_detect.py:562-809.
+//
+// Two kernels per chunk of patterns:
+//   k_radon_gather  one work-group holds P patterns interleaved pixel-major in LDS (one
+//                   LDS word = the same pixel of P patterns) and walks the compiled index
+//                   plan: every lane owns one Radon line, reads its 8-entry plan group with
+//                   one 16-byte global load and gathers P patterns per LDS read.
+//   k_peaks         one work-group per pattern, theta-tiled: stage the padded Radon tile,
+//                   separable convolution, theta/rho dilation, equality local max under the
+//                   rho mask, candidate merge into a running top-nBands list, then the
+//                   reference's width / 3x3 sub-pixel fit / normmax / theta-rho conversion.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ebsd_radon.h"
+
+// ---------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(-3, std::string(#x) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+extern "C" const char *ebsd_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------------------
+static constexpr int KMAX = 16;      // max separable taps per axis
+static constexpr int CAND_CAP = 1024; // candidates per tile pass held in LDS
+static constexpr int NB_MAX = 64;     // max bands per pattern
+
+struct PeakGeo {
+    int nR, nT, p0, p1, nRp, nTp;
+    int kr, kt, hr, ht;   // conv sizes, dilation half-windows
+    int cr0, cr1;         // convolved rows kept (padded coordinates)
+    int mr0, mr1;         // rows that can hold a candidate
+    int sr0, sr1, sc0, sc1; // stats region
+    int tile, ntiles;
+    int nB;
+    float u[KMAX], v[KMAX]; // C = sum_a u[a] sum_b v[b] Rpad[r + kr/2 - a][c + kt/2 - b]
+};
+
+struct ebsd_plan {
+    int device = 0, ncu = 0, lds_max = 0;
+    int H = 0, W = 0, nPx = 0, nT = 0, nR = 0, nCells = 0;
+    // compiled gather plan
+    int nbatch = 0;
+    uint4 *d_gidx = nullptr;     // [block][lane] groups of 8 u16 pixel ids
+    uint2 *d_bhdr = nullptr;     // per batch: first block, number of blocks
+    int2 *d_lane = nullptr;      // per batch lane: cell id (-1 = none), entry count
+    std::vector<int> h_cnt;      // per cell entry count
+    std::vector<int64_t> h_off;  // per cell first entry in h_ent
+    std::vector<uint16_t> h_ent; // per cell valid entries, plan order
+    // background
+    int bg_mode = 0;             // 0 none, 1 per-line sums, 2 exact per-pixel
+    double *d_linebg = nullptr;  // nCells
+    double *d_bg = nullptr;      // nPx
+    // peaks
+    PeakGeo g{};
+    uint8_t *d_mask = nullptr;   // nR*nT
+    double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
+    // tuning
+    int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 512;
+    int ptile = 60, pthreads = 256;
+};
+
+// ---------------------------------------------------------------------------------------
+// k_radon_gather
+// ---------------------------------------------------------------------------------------
+struct GatherArgs {
+    const void *pats;
+    int64_t N;
+    int nPx;
+    int nbatch;
+    const uint4 *gidx;
+    const uint2 *bhdr;
+    const int2 *lane;
+    const double *linebg; // fast background mode
+    const double *bg;     // exact background mode
+    float *out;           // (N, nCells)
+    int nCells;
+    int64_t npacks;
+};
+
+template <int WB> struct WordT;
+template <> struct WordT<4> { typedef uint32_t T; };
+template <> struct WordT<8> { typedef uint2 T; };
+template <> struct WordT<16> { typedef uint4 T; };
+
+template <int IN> struct InT;
+template <> struct InT<EBSD_U8> { typedef uint8_t T; };
+template <> struct InT<EBSD_U16> { typedef uint16_t T; };
+template <> struct InT<EBSD_F32> { typedef float T; };
+
+// Word layout: FLT=0: P integers of 8 (u8) or 16 (u16) bits packed little-endian;
+//              FLT=1: P float32 values of float32(x - bg) (reference staging).
+template <int IN, bool FLT, int P> struct GatherCfg {
+    static constexpr int EB = FLT ? 4 : (IN == EBSD_U8 ? 1 : 2); // bytes per element
+    static constexpr int WB = EB * P;                             // bytes per LDS word
+    static constexpr int ND = WB / 4;                             // dwords per word
+};
+
+template <int IN, bool FLT, int P>
+__device__ __forceinline__ void accumulate_word(const uint32_t *d, uint32_t *acc, double *accd)
+{
+    if constexpr (FLT) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) accd[p] += (double)__uint_as_float(d[p]);
+    } else if constexpr (IN == EBSD_U8) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] += (d[p >> 2] >> (8 * (p & 3))) & 0xffu;
+    } else {
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] += (d[p >> 1] >> (16 * (p & 1))) & 0xffffu;
+    }
+}
+
+template <int IN, bool FLT, int P>
+__global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
+{
+    typedef GatherCfg<IN, FLT, P> C;
+    typedef typename WordT<C::WB>::T Word;
+    typedef typename InT<IN>::T Tin;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    Word *lds = reinterpret_cast<Word *>(lds_raw);
+
+    const Tin *pats = reinterpret_cast<const Tin *>(a.pats);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int nPx = a.nPx;
+
+    for (int64_t pack = blockIdx.x; pack < a.npacks; pack += gridDim.x) {
+        const int64_t q0 = pack * P;
+        // ---- stage P patterns, pixel-major, plus one zero word at index nPx ----
+        for (int px = tid; px <= nPx; px += blockDim.x) {
+            uint32_t d[C::ND];
+#pragma unroll
+            for (int k = 0; k < C::ND; ++k) d[k] = 0u;
+            if (px < nPx) {
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const int64_t q = q0 + p;
+                    if (q < a.N) {
+                        Tin x = pats[q * (int64_t)nPx + px];
+                        if constexpr (FLT) {
+                            float f;
+                            if (a.bg) f = (float)((double)x - a.bg[px]);
+                            else f = (float)x;
+                            d[p] = __float_as_uint(f);
+                        } else if constexpr (IN == EBSD_U8) {
+                            d[p >> 2] |= (uint32_t)x << (8 * (p & 3));
+                        } else {
+                            d[p >> 1] |= (uint32_t)x << (16 * (p & 1));
+                        }
+                    }
+                }
+            }
+            Word w;
+            __builtin_memcpy(&w, d, sizeof(Word));
+            lds[px] = w;
+        }
+        __syncthreads();
+
+        // ---- gather: each lane owns one line of the current 64-line batch ----
+        for (int b = wave; b < a.nbatch; b += nwaves) {
+            const uint2 h = a.bhdr[b];
+            const uint4 *g = a.gidx + (size_t)h.x * 64 + lane;
+            const int ngrp = (int)h.y;
+            uint32_t acc[P];
+            double accd[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+            uint4 e = ngrp > 0 ? g[0] : make_uint4(0, 0, 0, 0);
+            for (int gi = 0; gi < ngrp; ++gi) {
+                uint4 en = (gi + 1 < ngrp) ? g[(size_t)(gi + 1) * 64] : e;
+                const uint32_t ew[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t idx = (ew[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+                    Word w = lds[idx];
+                    accumulate_word<IN, FLT, P>(reinterpret_cast<const uint32_t *>(&w), acc, accd);
+                }
+                e = en;
+            }
+            const int2 li = a.lane[(size_t)b * 64 + lane];
+            if (li.x >= 0) {
+                const double cnt = (double)li.y;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const int64_t q = q0 + p;
+                    if (q < a.N) {
+                        float r;
+                        if (li.y == 0) {
+                            r = -1.0f;
+                        } else if constexpr (FLT) {
+                            r = (float)(accd[p] / cnt);
+                        } else {
+                            double s = (double)acc[p];
+                            if (a.linebg) s -= a.linebg[li.x];
+                            r = (float)(s / cnt);
+                        }
+                        a.out[q * (int64_t)a.nCells + li.x] = r;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_peaks
+// ---------------------------------------------------------------------------------------
+struct PeakArgs {
+    const float *radon; // (N, nR, nT)
+    int64_t N;
+    const uint8_t *mask; // (nR, nT)
+    const double *grid;  // theta_deg[nT], rho[nR]
+    ebsd_bands out;
+    PeakGeo g;
+};
+
+// Source cell of padded Radon cell (r, c) in the unpadded (nR, nT) Radon: theta padding
+// mirrors the whole padded rho axis (band_detect.py:573-575), then rho padding repeats
+// the first/last interior rows (band_detect.py:578-580).
+__device__ __forceinline__ int pad_src(const PeakGeo &g, int r, int c)
+{
+    int i = r - g.p0;
+    i = i < 0 ? 0 : (i > g.nR - 1 ? g.nR - 1 : i);
+    int j;
+    if (c < g.p1) {
+        j = g.nT - g.p1 + c;
+        i = g.nR - 1 - i;
+    } else if (c >= g.p1 + g.nT) {
+        j = c - g.p1 - g.nT;
+        i = g.nR - 1 - i;
+    } else {
+        j = c - g.p1;
+    }
+    return i * g.nT + j;
+}
+
+struct Key {
+    float v;
+    int pos;
+};
+
+__device__ __forceinline__ bool key_gt(float av, int ap, float bv, int bp)
+{
+    return av > bv || (av == bv && ap > bp);
+}
+
+// LDS layout of k_peaks (all offsets in floats)
+struct PeakLds {
+    int a_off, b_off, cand_v, cand_p, run_v, run_p, run_nn, red, total;
+};
+
+__host__ __device__ inline int peak_rows(const PeakGeo &g) { return g.cr1 - g.cr0 + 2 * (g.kr / 2); }
+__host__ __device__ inline int peak_cols_max(const PeakGeo &g)
+{
+    return g.tile + 2 * g.ht + 2 * (g.kt / 2);
+}
+
+__host__ __device__ inline PeakLds peak_lds_layout(const PeakGeo &g)
+{
+    PeakLds L;
+    const int rh = peak_rows(g), rw = peak_cols_max(g);
+    L.a_off = 0;
+    L.b_off = rh * rw;
+    L.cand_v = L.b_off + rh * rw;
+    L.cand_p = L.cand_v + CAND_CAP;
+    L.run_v = L.cand_p + CAND_CAP;
+    L.run_p = L.run_v + NB_MAX;
+    L.run_nn = L.run_p + NB_MAX;
+    L.red = L.run_nn + 9 * NB_MAX;  // 4 * 32 slots (min, max, sum as 2 floats each)
+    L.total = L.red + 4 * 32 + 8;   // + counters
+    return L;
+}
+
+__device__ __forceinline__ double np_interp_unit(double x, const double *fp, int n)
+{
+    // numpy.interp with xp = 0..n-1 (compiled_base.c: clamp outside, exact at nodes,
+    // slope*(x - xp[j]) + fp[j] inside)
+    if (!(x == x)) return x;
+    if (x < 0.0) return fp[0];
+    if (x > (double)(n - 1)) return fp[n - 1];
+    int j = (int)floor(x);
+    if (j >= n - 1) return fp[n - 1];
+    if ((double)j == x) return fp[j];
+    const double slope = fp[j + 1] - fp[j];
+    return slope * (x - (double)j) + fp[j];
+}
+
+__global__ __launch_bounds__(1024) void k_peaks(PeakArgs a)
+{
+    extern __shared__ __align__(16) float sm[];
+    const PeakGeo &g = a.g;
+    const PeakLds L = peak_lds_layout(g);
+    float *A = sm + L.a_off;
+    float *B = sm + L.b_off;
+    float *cand_v = sm + L.cand_v;
+    int *cand_p = reinterpret_cast<int *>(sm + L.cand_p);
+    float *run_v = sm + L.run_v;
+    int *run_p = reinterpret_cast<int *>(sm + L.run_p);
+    float *run_nn = sm + L.run_nn;
+    float *red = sm + L.red;
+    int *ctr = reinterpret_cast<int *>(sm + L.total - 8);
+
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int ckr = g.kr / 2, ckt = g.kt / 2;
+    const int CH = g.cr1 - g.cr0;
+    const int RH = CH + 2 * ckr;
+    const int nCells = g.nR * g.nT;
+
+    for (int64_t q = blockIdx.x; q < a.N; q += gridDim.x) {
+        const float *R = a.radon + q * (int64_t)nCells;
+        float smin = INFINITY, smax = -INFINITY;
+        double ssum = 0.0;
+        if (tid == 0) { ctr[0] = 0; ctr[1] = 0; ctr[2] = 0; }
+        __syncthreads();
+
+        for (int t = 0; t < g.ntiles; ++t) {
+            const int j0 = t * g.tile;
+            const int j1 = min(j0 + g.tile, g.nT);
+            const int hc0 = g.p1 + j0, hc1 = g.p1 + j1, HW = hc1 - hc0;
+            const int cc0 = max(hc0 - g.ht, 0), cc1 = min(hc1 + g.ht, g.nTp), CW = cc1 - cc0;
+            const int rb0 = g.cr0 - ckr;
+            const int cb0 = cc0 - ckt, RW = CW + 2 * ckt;
+
+            // ---- stage padded Radon tile (zero outside the padded array) ----
+            for (int idx = tid; idx < RH * RW; idx += nthr) {
+                const int y = idx / RW, x = idx - y * RW;
+                const int r = rb0 + y, c = cb0 + x;
+                float val = 0.0f;
+                if (r >= 0 && r < g.nRp && c >= 0 && c < g.nTp) val = R[pad_src(g, r, c)];
+                A[idx] = val;
+            }
+            __syncthreads();
+            // ---- theta pass: T[y][x] = sum_b v[b] * A[y][x + 2ckt - b] ----
+            for (int idx = tid; idx < RH * CW; idx += nthr) {
+                const int y = idx / CW, x = idx - y * CW;
+                const float *row = A + y * RW + x + 2 * ckt;
+                float s = 0.0f;
+                for (int b = 0; b < g.kt; ++b) s = fmaf(g.v[b], row[-b], s);
+                B[idx] = s;
+            }
+            __syncthreads();
+            // ---- rho pass: C[y][x] = sum_a u[a] * T[y + 2ckr - a][x] (into A) + stats ----
+            for (int idx = tid; idx < CH * CW; idx += nthr) {
+                const int y = idx / CW, x = idx - y * CW;
+                const float *col = B + (y + 2 * ckr) * CW + x;
+                float s = 0.0f;
+                for (int aa = 0; aa < g.kr; ++aa) s = fmaf(g.u[aa], col[-aa * CW], s);
+                A[idx] = s;
+                const int r = g.cr0 + y, c = cc0 + x;
+                if (r >= g.sr0 && r < g.sr1 && c >= hc0 && c < hc1) {
+                    smin = fminf(smin, s);
+                    smax = fmaxf(smax, s);
+                    ssum += (double)s;
+                }
+            }
+            __syncthreads();
+            // ---- theta dilation: H[y][x'] = max C[y][c-ht .. c+ht] (clamped) (into B) ----
+            for (int idx = tid; idx < CH * HW; idx += nthr) {
+                const int y = idx / HW, x = idx - y * HW;
+                const int c = hc0 + x;
+                const int c0 = max(c - g.ht, 0) - cc0, c1 = min(c + g.ht, g.nTp - 1) - cc0;
+                const float *row = A + y * CW;
+                float m = -INFINITY;
+                for (int k = c0; k <= c1; ++k) m = fmaxf(m, row[k]);
+                B[idx] = m;
+            }
+            __syncthreads();
+            // ---- rho dilation + equality under the mask -> candidates, merged per row
+            //      chunk; a chunk that overflows the candidate list (plateaus) is redone
+            //      in chunks small enough that every cell could be a candidate ----
+            int rstart = g.mr0;
+            int rstep = g.mr1 - g.mr0;
+            while (rstart < g.mr1) {
+            const int rend = min(rstart + rstep, g.mr1);
+            const int nrow = rend - rstart;
+            for (int idx = tid; idx < nrow * HW; idx += nthr) {
+                const int yy = idx / HW, x = idx - yy * HW;
+                const int r = rstart + yy, c = hc0 + x;
+                if (!a.mask[(r - g.p0) * g.nT + (c - g.p1)]) continue;
+                const int r0 = max(r - g.hr, 0) - g.cr0, r1 = min(r + g.hr, g.nRp - 1) - g.cr0;
+                float m = -INFINITY;
+                for (int k = r0; k <= r1; ++k) m = fmaxf(m, B[k * HW + x]);
+                const float cv = A[(r - g.cr0) * CW + (c - cc0)];
+                if (m == cv) {
+                    const int slot = atomicAdd(&ctr[1], 1);
+                    if (slot < CAND_CAP) {
+                        cand_v[slot] = cv;
+                        cand_p[slot] = r * g.nTp + c;
+                    }
+                }
+            }
+            __syncthreads();
+            const int found = ctr[1];
+            __syncthreads();
+            if (found > CAND_CAP) {
+                if (tid == 0) ctr[1] = 0;
+                __syncthreads();
+                rstep = max(1, CAND_CAP / HW);
+                continue;
+            }
+            // ---- merge the chunk's candidates into the running top-nB (wave 0) ----
+            if (wave == 0) {
+                const int nnew = min(ctr[1], CAND_CAP);
+                const int nrun = ctr[0];
+                const int ntot = nrun + nnew;
+                float lastv = INFINITY;
+                int lastp = 0x7fffffff;
+                bool havelast = false;
+                int mysrc = -1; // source index selected for slot == lane
+                int nsel = 0;
+                for (int k = 0; k < g.nB; ++k) {
+                    float bv = -INFINITY;
+                    int bp = -1, bi = -1;
+                    for (int i = lane; i < ntot; i += 64) {
+                        const float v = i < nrun ? run_v[i] : cand_v[i - nrun];
+                        const int p = i < nrun ? run_p[i] : cand_p[i - nrun];
+                        if (havelast && !key_gt(lastv, lastp, v, p)) continue;
+                        if (bi < 0 || key_gt(v, p, bv, bp)) { bv = v; bp = p; bi = i; }
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const float ov = __shfl_xor(bv, off, 64);
+                        const int op = __shfl_xor(bp, off, 64);
+                        const int oi = __shfl_xor(bi, off, 64);
+                        if (oi >= 0 && (bi < 0 || key_gt(ov, op, bv, bp))) { bv = ov; bp = op; bi = oi; }
+                    }
+                    if (bi < 0) break;
+                    if (lane == k) mysrc = bi;
+                    lastv = bv; lastp = bp; havelast = true;
+                    ++nsel;
+                }
+                // gather the selected entries (values, positions, 3x3 neighbourhoods)
+                float sv = 0.0f, snn[9];
+                int sp = 0;
+                if (lane < nsel) {
+                    if (mysrc < nrun) {
+                        sv = run_v[mysrc];
+                        sp = run_p[mysrc];
+                        for (int k = 0; k < 9; ++k) snn[k] = run_nn[mysrc * 9 + k];
+                    } else {
+                        sv = cand_v[mysrc - nrun];
+                        sp = cand_p[mysrc - nrun];
+                        const int r = sp / g.nTp, c = sp - (sp / g.nTp) * g.nTp;
+                        for (int dr = 0; dr < 3; ++dr)
+                            for (int dc = 0; dc < 3; ++dc)
+                                snn[dr * 3 + dc] = A[(r - 1 + dr - g.cr0) * CW + (c - 1 + dc - cc0)];
+                    }
+                }
+                if (lane < nsel) {
+                    run_v[lane] = sv;
+                    run_p[lane] = sp;
+                    for (int k = 0; k < 9; ++k) run_nn[lane * 9 + k] = snn[k];
+                }
+                if (lane == 0) {
+                    ctr[0] = nsel;
+                    ctr[2] += ctr[1];
+                    ctr[1] = 0;
+                }
+            }
+            __syncthreads();
+            rstart = rend;
+            }
+        }
+
+        // ---- stats reduction: min, max, sum over the work-group ----
+        for (int off = 32; off > 0; off >>= 1) {
+            smin = fminf(smin, __shfl_xor(smin, off, 64));
+            smax = fmaxf(smax, __shfl_xor(smax, off, 64));
+            ssum += __shfl_xor(ssum, off, 64);
+        }
+        double *redd = reinterpret_cast<double *>(red + 64);
+        if (lane == 0) {
+            red[wave] = smin;
+            red[32 + wave] = smax;
+            redd[wave] = ssum;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int nw = nthr >> 6;
+            float mn = red[0], mx = red[32];
+            double sum = redd[0];
+            for (int w = 1; w < nw; ++w) {
+                mn = fminf(mn, red[w]);
+                mx = fmaxf(mx, red[32 + w]);
+                sum += redd[w];
+            }
+            const double cnt = (double)(g.sr1 - g.sr0) * (double)(g.sc1 - g.sc0);
+            red[0] = mn;
+            red[1] = (float)(sum / cnt);
+            red[2] = mx;
+            a.out.d_stats[q * 3 + 0] = mn;
+            a.out.d_stats[q * 3 + 1] = red[1];
+            a.out.d_stats[q * 3 + 2] = mx;
+        }
+        __syncthreads();
+
+        // ---- finalize: band_detect.py:736-800, then :457-460 and :521-524 ----
+        if (tid < g.nB) {
+            const int i = tid;
+            const int64_t o = q * g.nB + i;
+            const float mn = red[0], ave = red[1];
+            float bmax = -2.0e6f, bavemax = -2.0e6f, width = 0.0f;
+            float ml_r = 0.0f, ml_c = 0.0f, al_r = 0.0f, al_c = 0.0f;
+            int8_t valid = 0;
+            if (i < ctr[0] && run_v[i] > 0.0f) {
+                const int pos = run_p[i];
+                const int r = pos / g.nTp, c = pos - (pos / g.nTp) * g.nTp;
+                float nn[9];
+                for (int k = 0; k < 9; ++k) nn[k] = run_nn[i * 9 + k];
+                const float mx = run_v[i];
+                ml_r = (float)r;
+                ml_c = (float)c;
+                bmax = mx;
+                // width (band_detect.py:748-753)
+                const float mnt = fminf(fminf(nn[1], nn[4]), nn[7]);
+                if (mnt > 0.0f) {
+                    const float lp = logf(nn[7]), lm = logf(nn[1]), l0 = logf(mx);
+                    const double aa = (double)(lp - lm) * 0.5 - (double)l0;
+                    if (aa < -1.e-8) width = (float)(2.0 * sqrt(log(2.0) / (-1.0 * aa)));
+                }
+                // 3x3 quadratic sub-pixel fit (band_detect.py:764-798)
+                float s = 0.0f;
+                for (int k = 0; k < 9; ++k) s += nn[k];
+                const double sumnn = s > 1e-12f ? (double)s : 1e-12;
+                for (int k = 0; k < 9; ++k) nn[k] = (float)((double)nn[k] / sumnn);
+                bavemax = (float)(sumnn / 9.0);
+                const double dx = 0.5 * (double)(nn[5] - nn[3]);
+                const double dy = 0.5 * (double)(nn[7] - nn[1]);
+                const double dxx = (double)(nn[5] + nn[3]) - 2.0 * (double)nn[4];
+                const double dyy = (double)(nn[7] + nn[1]) - 2.0 * (double)nn[4];
+                const double dxy = 0.25 * (double)(((nn[8] - nn[2]) - nn[6]) + nn[0]);
+                double det = dxx * dyy - dxy * dxy;
+                det = fabs(det) > 1e-12 ? det : 1.0e-12;
+                det = 1.0 / det;
+                double dc = (dyy * dx - dxy * dy) * det;
+                double rc = (dxx * dy - dxy * dx) * det;
+                if (fabs(dc) > 0.875 || fabs(rc) > 0.875) {
+                    det = dxx * dyy;
+                    det = fabs(det) > 1e-12 ? det : 1.0e-12;
+                    det = 1.0 / det;
+                    dc = (dyy * dx) * det;
+                    rc = (dxx * dy) * det;
+                    if (fabs(dc) > 0.875 || fabs(rc) > 0.875) { dc = 0.0; rc = 0.0; }
+                }
+                al_r = (float)((double)r - rc);
+                al_c = (float)((double)c - dc);
+                valid = 1;
+            }
+            // maxloc/aveloc -= padding (float32 - int64 in float64, stored as float32)
+            ml_r = (float)((double)ml_r - (double)g.p0);
+            ml_c = (float)((double)ml_c - (double)g.p1);
+            al_r = (float)((double)al_r - (double)g.p0);
+            al_c = (float)((double)al_c - (double)g.p1);
+            float d = ave - mn;
+            d = d < 1e-7f ? 1e-7f : d;
+            const float nm = (bmax - mn) / d;
+            const double *thg = a.grid, *rhg = a.grid + g.nT;
+            const double th = 3.141592653589793 - np_interp_unit((double)al_c, thg, g.nT) /
+                                                      (180.0 / 3.141592653589793);
+            const double rh = -1.0 * np_interp_unit((double)al_r, rhg, g.nR);
+            a.out.d_max[o] = bmax;
+            a.out.d_normmax[o] = nm;
+            a.out.d_avemax[o] = bavemax;
+            a.out.d_maxloc[2 * o + 0] = ml_r;
+            a.out.d_maxloc[2 * o + 1] = ml_c;
+            a.out.d_aveloc[2 * o + 0] = al_r;
+            a.out.d_aveloc[2 * o + 1] = al_c;
+            a.out.d_width[o] = width;
+            a.out.d_theta[o] = (float)th;
+            a.out.d_rho[o] = (float)rh;
+            a.out.d_valid[o] = valid;
+        }
+        __syncthreads();
+    }
+}
+
+// Convolved padded Radon, every padded cell, same arithmetic order as k_peaks.
+__global__ void k_conv_full(const float *radon, int64_t N, PeakGeo g, float *conv)
+{
+    const int64_t total = N * (int64_t)g.nRp * g.nTp;
+    const int ckr = g.kr / 2, ckt = g.kt / 2;
+    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+         id += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = id / ((int64_t)g.nRp * g.nTp);
+        const int rem = (int)(id - q * (int64_t)g.nRp * g.nTp);
+        const int r = rem / g.nTp, c = rem - (rem / g.nTp) * g.nTp;
+        const float *R = radon + q * (int64_t)g.nR * g.nT;
+        float s = 0.0f;
+        for (int aa = 0; aa < g.kr; ++aa) {
+            const int rr = r + ckr - aa;
+            float t = 0.0f;
+            for (int b = 0; b < g.kt; ++b) {
+                const int cc = c + ckt - b;
+                float val = 0.0f;
+                if (rr >= 0 && rr < g.nRp && cc >= 0 && cc < g.nTp) val = R[pad_src(g, rr, cc)];
+                t = fmaf(g.v[b], val, t);
+            }
+            s = fmaf(g.u[aa], t, s);
+        }
+        conv[id] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+template <typename T> static int dev_upload(T **dptr, const T *src, size_t n)
+{
+    if (n == 0) { *dptr = nullptr; return 0; }
+    HIPCHK(hipMalloc((void **)dptr, n * sizeof(T)));
+    HIPCHK(hipMemcpy(*dptr, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void plan_free(ebsd_plan *p)
+{
+    if (!p) return;
+    (void)hipFree(p->d_gidx);
+    (void)hipFree(p->d_bhdr);
+    (void)hipFree(p->d_lane);
+    (void)hipFree(p->d_linebg);
+    (void)hipFree(p->d_bg);
+    (void)hipFree(p->d_mask);
+    (void)hipFree(p->d_grid);
+    delete p;
+}
+
+// Rank-1 split of the (exactly symmetric, Gaussian-derivative) kernel:
+// u = centre column, v = centre row / centre value.
+static void split_kernel(const float *k, int kr, int kt, float *u, float *v)
+{
+    const int cr = kr / 2, ct = kt / 2;
+    const float kc = k[cr * kt + ct];
+    for (int a = 0; a < kr; ++a) u[a] = k[a * kt + ct];
+    for (int b = 0; b < kt; ++b) v[b] = (float)((double)k[cr * kt + b] / (double)kc);
+}
+
+static int geo_tiles(ebsd_plan *p)
+{
+    PeakGeo &g = p->g;
+    int tile = std::max(1, std::min(p->ptile, g.nT));
+    for (;;) {
+        g.tile = tile;
+        g.ntiles = (g.nT + tile - 1) / tile;
+        const size_t bytes = (size_t)peak_lds_layout(g).total * 4;
+        if ((int)bytes <= p->lds_max || tile == 1) break;
+        tile = std::max(1, tile * 3 / 4);
+    }
+    if ((size_t)peak_lds_layout(g).total * 4 > (size_t)p->lds_max)
+        return fail(-2, "Radon too large for the peak kernel's LDS tiling");
+    return 0;
+}
+
+extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_t *index_plan,
+                                int plan_len, const uint8_t *rdnmask, const float *kernel, int kr,
+                                int kt, int pad_r, int pad_t, int peak_r, int peak_t, int nBands,
+                                const float *theta_deg, const float *rho, ebsd_plan **out)
+{
+    if (!out || !index_plan || !rdnmask || !kernel || !theta_deg || !rho)
+        return fail(-1, "ebsd_plan_create: null argument");
+    *out = nullptr;
+    if (H <= 0 || W <= 0 || nTheta <= 0 || nRho <= 0 || plan_len <= 0)
+        return fail(-1, "ebsd_plan_create: non-positive size");
+    if ((int64_t)H * W + 1 > 65536)
+        return fail(-1, "ebsd_plan_create: patterns above 65535 pixels are not supported");
+    if (kr % 2 == 0 || kt % 2 == 0 || kr > KMAX || kt > KMAX)
+        return fail(-1, "ebsd_plan_create: kernel sizes must be odd and <= 16");
+    if (nBands <= 0 || nBands > NB_MAX) return fail(-1, "ebsd_plan_create: nBands must be in 1..64");
+    if (pad_t > nTheta) return fail(-1, "ebsd_plan_create: theta padding larger than nTheta");
+    if (peak_r % 2 == 0 || peak_t % 2 == 0)
+        return fail(-1, "ebsd_plan_create: peakPad must be odd");
+
+    ebsd_plan *p = new ebsd_plan();
+    int rc;
+    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return fail(-3, "hipGetDevice failed"); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) {
+        delete p;
+        return fail(-3, "hipGetDeviceProperties failed");
+    }
+    p->ncu = prop.multiProcessorCount;
+    p->lds_max = (int)prop.sharedMemPerBlock;
+    if (prop.maxSharedMemoryPerMultiProcessor > 0)
+        p->lds_max = std::max(p->lds_max, (int)std::min<size_t>(prop.maxSharedMemoryPerMultiProcessor, 160 * 1024));
+    p->H = H; p->W = W; p->nPx = H * W; p->nT = nTheta; p->nR = nRho; p->nCells = nTheta * nRho;
+
+    // ---- compile the gather plan ----
+    const int nPx = p->nPx, nCells = p->nCells;
+    p->h_cnt.assign(nCells, 0);
+    p->h_off.assign(nCells + 1, 0);
+    p->h_ent.clear();
+    p->h_ent.reserve((size_t)nCells * std::max(H, W));
+    for (int cell = 0; cell < nCells; ++cell) {
+        const int64_t *line = index_plan + (int64_t)cell * plan_len;
+        p->h_off[cell] = (int64_t)p->h_ent.size();
+        int n = 0;
+        for (int k = 0; k < plan_len; ++k) {
+            const int64_t id = line[k];
+            if (id >= nPx) break; // rdn_loops stops at the first out-of-range entry
+            if (id < 0) { plan_free(p); return fail(-1, "ebsd_plan_create: negative plan entry"); }
+            p->h_ent.push_back((uint16_t)id);
+            ++n;
+        }
+        p->h_cnt[cell] = n;
+    }
+    p->h_off[nCells] = (int64_t)p->h_ent.size();
+    std::vector<int> order(nCells);
+    for (int i = 0; i < nCells; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return p->h_cnt[x] > p->h_cnt[y]; });
+    p->nbatch = (nCells + 63) / 64;
+    std::vector<uint2> bhdr(p->nbatch);
+    std::vector<int2> lanes((size_t)p->nbatch * 64);
+    std::vector<uint16_t> gidx;
+    uint32_t blocks = 0;
+    for (int b = 0; b < p->nbatch; ++b) {
+        int mx = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int i = b * 64 + l;
+            if (i < nCells) mx = std::max(mx, p->h_cnt[order[i]]);
+        }
+        const int ngrp = (mx + 7) / 8;
+        bhdr[b] = make_uint2(blocks, (uint32_t)ngrp);
+        gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)nPx);
+        for (int l = 0; l < 64; ++l) {
+            const int i = b * 64 + l;
+            if (i >= nCells) { lanes[(size_t)b * 64 + l] = make_int2(-1, 0); continue; }
+            const int cell = order[i];
+            lanes[(size_t)b * 64 + l] = make_int2(cell, p->h_cnt[cell]);
+            for (int k = 0; k < p->h_cnt[cell]; ++k) {
+                const size_t at = ((size_t)(blocks + k / 8) * 64 + l) * 8 + (k % 8);
+                gidx[at] = p->h_ent[p->h_off[cell] + k];
+            }
+        }
+        blocks += (uint32_t)ngrp;
+    }
+    if (gidx.empty()) gidx.resize(64 * 8, (uint16_t)nPx);
+    if ((rc = dev_upload(reinterpret_cast<uint16_t **>(&p->d_gidx), gidx.data(), gidx.size()))) { plan_free(p); return rc; }
+    if ((rc = dev_upload(&p->d_bhdr, bhdr.data(), bhdr.size()))) { plan_free(p); return rc; }
+    if ((rc = dev_upload(&p->d_lane, lanes.data(), lanes.size()))) { plan_free(p); return rc; }
+
+    // ---- peak geometry ----
+    PeakGeo &g = p->g;
+    g.nR = nRho; g.nT = nTheta; g.p0 = pad_r; g.p1 = pad_t;
+    g.nRp = nRho + 2 * pad_r; g.nTp = nTheta + 2 * pad_t;
+    g.kr = kr; g.kt = kt; g.hr = peak_r / 2; g.ht = peak_t / 2;
+    g.nB = nBands;
+    split_kernel(kernel, kr, kt, g.u, g.v);
+    int mlo = nRho, mhi = -1;
+    for (int i = 0; i < nRho; ++i)
+        for (int j = 0; j < nTheta; ++j)
+            if (rdnmask[i * nTheta + j]) { mlo = std::min(mlo, i); mhi = std::max(mhi, i); }
+    if (mhi < 0) { mlo = 0; mhi = -1; }
+    g.mr0 = pad_r + mlo;
+    g.mr1 = pad_r + mhi + 1;
+    g.sr0 = pad_r;
+    g.sr1 = std::min(g.nRp, g.nTp - pad_r);
+    g.sc0 = pad_t;
+    g.sc1 = g.nTp - pad_t;
+    if (g.sr1 <= g.sr0 || g.sc1 <= g.sc0) { plan_free(p); return fail(-1, "ebsd_plan_create: empty stats region"); }
+    int lo = g.sr0, hi = g.sr1;
+    if (g.mr1 > g.mr0) {
+        lo = std::min(lo, std::max(g.mr0 - g.hr, 0));
+        hi = std::max(hi, std::min(g.mr1 + g.hr, g.nRp));
+        lo = std::min(lo, g.mr0 - 1);
+        hi = std::max(hi, g.mr1 + 1);
+    }
+    g.cr0 = std::max(lo, 0);
+    g.cr1 = std::min(hi, g.nRp);
+    if ((rc = geo_tiles(p))) { plan_free(p); return rc; }
+    if ((rc = dev_upload(&p->d_mask, rdnmask, (size_t)nCells))) { plan_free(p); return rc; }
+    std::vector<double> grid(nTheta + nRho);
+    for (int j = 0; j < nTheta; ++j) grid[j] = (double)theta_deg[j];
+    for (int i = 0; i < nRho; ++i) grid[nTheta + i] = (double)rho[i];
+    if ((rc = dev_upload(&p->d_grid, grid.data(), grid.size()))) { plan_free(p); return rc; }
+    *out = p;
+    return 0;
+}
+
+extern "C" int ebsd_plan_destroy(ebsd_plan *plan)
+{
+    plan_free(plan);
+    return 0;
+}
+
+extern "C" int ebsd_plan_set_background(ebsd_plan *p, const double *bg, int exact)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_background: null plan");
+    (void)hipFree(p->d_linebg);
+    (void)hipFree(p->d_bg);
+    p->d_linebg = nullptr;
+    p->d_bg = nullptr;
+    p->bg_mode = 0;
+    if (!bg) return 0;
+    int rc;
+    // the per-pixel background is always kept: float32 patterns stage float32(x - bg)
+    if ((rc = dev_upload(&p->d_bg, bg, (size_t)p->nPx))) return rc;
+    if (exact) {
+        p->bg_mode = 2;
+    } else {
+        std::vector<double> lb(p->nCells, 0.0);
+        for (int cell = 0; cell < p->nCells; ++cell) {
+            double s = 0.0;
+            for (int64_t k = p->h_off[cell]; k < p->h_off[cell + 1]; ++k) s += bg[p->h_ent[k]];
+            lb[cell] = s;
+        }
+        if ((rc = dev_upload(&p->d_linebg, lb.data(), lb.size()))) return rc;
+        p->bg_mode = 1;
+    }
+    return 0;
+}
+
+extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_threads, int peak_tile,
+                                    int peak_threads)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_tuning: null plan");
+    if (gather_pack > 0) {
+        // one value per element type, interpreted by the type-specific instantiation
+        p->gpack_u8 = std::min(16, std::max(4, gather_pack * 2));
+        p->gpack_u16 = std::min(8, std::max(2, gather_pack));
+        p->gpack_f32 = std::min(4, std::max(1, gather_pack / 2));
+    }
+    if (gather_threads > 0) {
+        if (gather_threads % 64 || gather_threads > 1024) return fail(-1, "gather_threads must be a multiple of 64 <= 1024");
+        p->gthreads = gather_threads;
+    }
+    if (peak_threads > 0) {
+        if (peak_threads % 64 || peak_threads > 1024) return fail(-1, "peak_threads must be a multiple of 64 <= 1024");
+        p->pthreads = peak_threads;
+    }
+    if (peak_tile > 0) {
+        p->ptile = peak_tile;
+        int rc = geo_tiles(p);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int ebsd_plan_info(const ebsd_plan *p, int64_t *o)
+{
+    if (!p || !o) return fail(-1, "ebsd_plan_info: null argument");
+    int64_t total = (int64_t)p->h_ent.size();
+    o[0] = p->nCells;
+    o[1] = total;                 // line-pixel pairs per pattern
+    o[2] = p->nbatch;
+    o[3] = p->g.tile;
+    o[4] = p->g.ntiles;
+    o[5] = (int64_t)peak_lds_layout(p->g).total * 4;
+    o[6] = p->bg_mode;
+    o[7] = p->ncu;
+    return 0;
+}
+
+extern "C" size_t ebsd_workspace_bytes(const ebsd_plan *p, int64_t chunk)
+{
+    if (!p || chunk <= 0) return 0;
+    return (size_t)chunk * (size_t)p->nCells * sizeof(float);
+}
+
+template <int IN, bool FLT, int P>
+static int launch_gather_t(ebsd_plan *p, const void *d_pats, int64_t N, float *d_out, bool use_bg,
+                           hipStream_t s)
+{
+    typedef GatherCfg<IN, FLT, P> C;
+    const size_t lds = (size_t)(p->nPx + 1) * C::WB;
+    if ((int)lds > p->lds_max) return 1; // caller falls back to a smaller pack
+    auto kern = k_radon_gather<IN, FLT, P>;
+    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    GatherArgs a;
+    a.pats = d_pats;
+    a.N = N;
+    a.nPx = p->nPx;
+    a.nbatch = p->nbatch;
+    a.gidx = p->d_gidx;
+    a.bhdr = p->d_bhdr;
+    a.lane = p->d_lane;
+    a.linebg = (use_bg && !FLT && p->bg_mode == 1) ? p->d_linebg : nullptr;
+    a.bg = (use_bg && FLT) ? p->d_bg : nullptr;
+    a.out = d_out;
+    a.nCells = p->nCells;
+    a.npacks = (N + P - 1) / P;
+    int occ = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->gthreads, lds));
+    if (occ < 1) return fail(-3, "radon gather kernel cannot be resident");
+    const int64_t grid = std::min<int64_t>(a.npacks, (int64_t)occ * p->ncu);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->gthreads), lds, s, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int64_t N, float *d_out,
+                         hipStream_t s)
+{
+    int rc = 1;
+    const int dtype = dtype_flags & 0xff;
+    const bool use_bg = !(dtype_flags & EBSD_FLAG_NOBG) && p->bg_mode != 0;
+    // float32 input (or the exact background mode) stages float32(x - bg) per pixel
+    const bool flt = (dtype == EBSD_F32) || (use_bg && p->bg_mode == 2);
+    if (flt) {
+        int P = p->gpack_f32;
+        for (; rc == 1 && P >= 1; P /= 2) {
+            if (dtype == EBSD_U8) {
+                if (P >= 4) rc = launch_gather_t<EBSD_U8, true, 4>(p, d_pats, N, d_out, use_bg, s);
+                else if (P >= 2) rc = launch_gather_t<EBSD_U8, true, 2>(p, d_pats, N, d_out, use_bg, s);
+                else rc = launch_gather_t<EBSD_U8, true, 1>(p, d_pats, N, d_out, use_bg, s);
+            } else if (dtype == EBSD_U16) {
+                if (P >= 4) rc = launch_gather_t<EBSD_U16, true, 4>(p, d_pats, N, d_out, use_bg, s);
+                else if (P >= 2) rc = launch_gather_t<EBSD_U16, true, 2>(p, d_pats, N, d_out, use_bg, s);
+                else rc = launch_gather_t<EBSD_U16, true, 1>(p, d_pats, N, d_out, use_bg, s);
+            } else {
+                if (P >= 4) rc = launch_gather_t<EBSD_F32, true, 4>(p, d_pats, N, d_out, use_bg, s);
+                else if (P >= 2) rc = launch_gather_t<EBSD_F32, true, 2>(p, d_pats, N, d_out, use_bg, s);
+                else rc = launch_gather_t<EBSD_F32, true, 1>(p, d_pats, N, d_out, use_bg, s);
+            }
+        }
+    } else if (dtype == EBSD_U8) {
+        for (int P = p->gpack_u8; rc == 1 && P >= 4; P /= 2) {
+            if (P >= 16) rc = launch_gather_t<EBSD_U8, false, 16>(p, d_pats, N, d_out, use_bg, s);
+            else if (P >= 8) rc = launch_gather_t<EBSD_U8, false, 8>(p, d_pats, N, d_out, use_bg, s);
+            else rc = launch_gather_t<EBSD_U8, false, 4>(p, d_pats, N, d_out, use_bg, s);
+        }
+    } else {
+        for (int P = p->gpack_u16; rc == 1 && P >= 2; P /= 2) {
+            if (P >= 8) rc = launch_gather_t<EBSD_U16, false, 8>(p, d_pats, N, d_out, use_bg, s);
+            else if (P >= 4) rc = launch_gather_t<EBSD_U16, false, 4>(p, d_pats, N, d_out, use_bg, s);
+            else rc = launch_gather_t<EBSD_U16, false, 2>(p, d_pats, N, d_out, use_bg, s);
+        }
+    }
+    if (rc == 1) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
+    return rc;
+}
+
+static int launch_peaks(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, hipStream_t s)
+{
+    const size_t lds = (size_t)peak_lds_layout(p->g).total * 4;
+    HIPCHK(hipFuncSetAttribute((const void *)k_peaks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    PeakArgs a;
+    a.radon = d_radon;
+    a.N = N;
+    a.mask = p->d_mask;
+    a.grid = p->d_grid;
+    a.out = *out;
+    a.g = p->g;
+    int occ = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_peaks, p->pthreads, lds));
+    if (occ < 1) return fail(-3, "peak kernel cannot be resident");
+    const int64_t grid = std::min<int64_t>(N, (int64_t)occ * p->ncu);
+    hipLaunchKernelGGL(k_peaks, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static ebsd_bands offset_bands(const ebsd_bands &o, int64_t q0, int nB)
+{
+    ebsd_bands r = o;
+    const int64_t e = q0 * nB;
+    r.d_max += e; r.d_normmax += e; r.d_avemax += e; r.d_width += e;
+    r.d_theta += e; r.d_rho += e; r.d_valid += e;
+    r.d_maxloc += 2 * e; r.d_aveloc += 2 * e;
+    r.d_stats += 3 * q0;
+    return r;
+}
+
+static bool bands_ok(const ebsd_bands *o)
+{
+    return o && o->d_max && o->d_normmax && o->d_avemax && o->d_maxloc && o->d_aveloc &&
+           o->d_width && o->d_theta && o->d_rho && o->d_valid && o->d_stats;
+}
+
+extern "C" int ebsd_find_bands(ebsd_plan *p, const void *d_pats, int dtype, int64_t N,
+                               const ebsd_bands *out, void *d_work, size_t ws_bytes, void *stream)
+{
+    if (!p) return fail(-1, "ebsd_find_bands: null plan");
+    if (N < 0) return fail(-1, "ebsd_find_bands: negative N");
+    if (N == 0) return 0;
+    if (!d_pats || !d_work) return fail(-1, "ebsd_find_bands: null device pointer");
+    if (!bands_ok(out)) return fail(-1, "ebsd_find_bands: null output pointer");
+    if ((dtype & 0xff) > EBSD_F32 || (dtype & ~(0xff | EBSD_FLAG_NOBG)))
+        return fail(-1, "ebsd_find_bands: bad dtype");
+    const size_t per = (size_t)p->nCells * sizeof(float);
+    const int64_t chunk = (int64_t)(ws_bytes / per);
+    if (chunk < 1) return fail(-1, "ebsd_find_bands: workspace smaller than one Radon");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t esz = (dtype & 0xff) == EBSD_U8 ? 1 : ((dtype & 0xff) == EBSD_U16 ? 2 : 4);
+    for (int64_t q0 = 0; q0 < N; q0 += chunk) {
+        const int64_t n = std::min(chunk, N - q0);
+        const void *src = (const char *)d_pats + (size_t)q0 * p->nPx * esz;
+        int rc = launch_gather(p, src, dtype, n, (float *)d_work, s);
+        if (rc) return rc;
+        ebsd_bands o = offset_bands(*out, q0, p->g.nB);
+        rc = launch_peaks(p, (const float *)d_work, n, &o, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int ebsd_radon(ebsd_plan *p, const void *d_pats, int dtype, int64_t N, float *d_radon, void *stream)
+{
+    if (!p) return fail(-1, "ebsd_radon: null plan");
+    if (N <= 0) return N == 0 ? 0 : fail(-1, "ebsd_radon: negative N");
+    if (!d_pats || !d_radon) return fail(-1, "ebsd_radon: null device pointer");
+    if ((dtype & 0xff) > EBSD_F32 || (dtype & ~(0xff | EBSD_FLAG_NOBG)))
+        return fail(-1, "ebsd_radon: bad dtype");
+    return launch_gather(p, d_pats, dtype, N, d_radon, (hipStream_t)stream);
+}
+
+extern "C" int ebsd_peaks(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, void *stream)
+{
+    if (!p) return fail(-1, "ebsd_peaks: null plan");
+    if (N <= 0) return N == 0 ? 0 : fail(-1, "ebsd_peaks: negative N");
+    if (!d_radon || !bands_ok(out)) return fail(-1, "ebsd_peaks: null device pointer");
+    return launch_peaks(p, d_radon, N, out, (hipStream_t)stream);
+}
+
+extern "C" int ebsd_conv(ebsd_plan *p, const float *d_radon, int64_t N, float *d_conv, void *stream)
+{
+    if (!p) return fail(-1, "ebsd_conv: null plan");
+    if (N <= 0) return N == 0 ? 0 : fail(-1, "ebsd_conv: negative N");
+    if (!d_radon || !d_conv) return fail(-1, "ebsd_conv: null device pointer");
+    const int64_t total = N * (int64_t)p->g.nRp * p->g.nTp;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_conv_full, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       d_radon, N, p->g, d_conv);
+    HIPCHK(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,66 @@
+"""Shared test helpers: parameter sets, oracle runs and near-tie-aware band comparison."""
+import numpy as np
+
+from oracle import fp64, ref_cpu
+
+# tSigma=1, rSigma=2 give the 9x5 kernel, peakPad (19, 11) and padding (19, 11) of the workloads
+STD = dict(nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9)
+
+
+def make_pair(shape, **kw):
+    """(product BandDetect, oracle RefBandDetect) with identical parameters."""
+    from pyebsdindex_amd import BandDetect
+    p = dict(STD)
+    p.update(kw)
+    bd = BandDetect(patDim=shape, **p)
+    ref = ref_cpu.RefBandDetect(shape, **{k: v for k, v in p.items() if k != 'background_mode'})
+    return bd, ref
+
+
+def oracle64(ref, pats, background=None):
+    return fp64.find_bands64(pats, ref.radonPlan.indexPlan, ref.kernel[0], ref.rdnmask, ref.padding,
+                             ref.peakPad, ref.nBands, ref.radonPlan.theta, ref.radonPlan.rho,
+                             background=background)
+
+
+def eps_local_max(C, r, c, hr, ht, eps):
+    """True if C[r,c] is within eps of the clamped-window maximum (a local max up to eps)."""
+    r0, r1 = max(r - hr, 0), min(r + hr, C.shape[0] - 1)
+    c0, c1 = max(c - ht, 0), min(c + ht, C.shape[1] - 1)
+    return C[r, c] >= C[r0:r1 + 1, c0:c1 + 1].max() - eps
+
+
+def compare_bands(got, res64, C64, padding, peakPad, rel_eps=2e-5, subpix_tol=1e-3):
+    """Compare device band output with the FP64 oracle.
+
+    Integer peak bins must be identical, except where the FP64 values of the two choices
+    differ by less than ``rel_eps * max|C|`` (an operational exact tie: the float32 paths
+    cannot order them).  Returns (n_slots_compared, n_tie_slots).  Sub-pixel positions of
+    matching slots must agree within ``subpix_tol`` bins."""
+    p0, p1 = int(padding[0]), int(padding[1])
+    hr, ht = int(peakPad[0]) // 2, int(peakPad[1]) // 2
+    N, nB = res64['max'].shape
+    ties = 0
+    for q in range(N):
+        C = C64[q]
+        eps = rel_eps * np.abs(C).max()
+        for i in range(nB):
+            gv, ov = int(got['valid'][q, i]), int(res64['valid'][q, i])
+            gl = got['maxloc'][q, i].astype(np.int64)
+            ol = res64['maxloc'][q, i].astype(np.int64)
+            if gv == ov == 1 and np.array_equal(gl, ol):
+                d = np.abs(got['aveloc'][q, i] - res64['aveloc'][q, i]).max()
+                assert d <= subpix_tol, (q, i, got['aveloc'][q, i], res64['aveloc'][q, i])
+                continue
+            if gv == 0 and ov == 0:
+                continue
+            # disagreement: must be an exact tie up to eps
+            gval = C[gl[0] + p0, gl[1] + p1] if gv else 0.0
+            oval = C[ol[0] + p0, ol[1] + p1] if ov else 0.0
+            ok = abs(gval - oval) <= eps
+            if gv:
+                ok = ok and eps_local_max(C, gl[0] + p0, gl[1] + p1, hr, ht, eps)
+            assert ok, (f"pattern {q} slot {i}: device {gl} valid {gv} C={gval}, "
+                        f"oracle {ol} valid {ov} C={oval}, eps={eps}")
+            ties += 1
+    return N * nB, ties

@@ -1,0 +1,129 @@
+"""Device parity: HIP kernels (through the C ABI) against the oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from helpers import compare_bands, make_pair, oracle64
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+
+def _pats(n, shape, dtype, seed, **kw):
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    return kikuchi_patterns(n, shape, dtype, seed=seed, **kw).numpy()
+
+
+def _dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.mark.parametrize("shape,dtype,n", [((60, 60), "uint8", 37), ((120, 120), "uint16", 19),
+                                           ((80, 80), "uint8", 5), ((100, 120), "uint16", 3),
+                                           ((60, 60), "float32", 9)])
+def test_radon_matches_oracle_bitwise(shape, dtype, n):
+    """u8/u16: integer line sums are exact, so float32(sum/count) is bit-identical to the
+    reference's float64 accumulation (radon_fast.py:288-310).  float32 input: the same
+    float64 sum in the same (sorted-plan) order -> also bit-identical."""
+    print(f"radon {shape} {dtype} n={n}", flush=True)
+    bd, ref = make_pair(shape)
+    pats = _pats(n, shape, dtype, seed=3)
+    rad = bd.device_plan().radon(_dev(pats)).cpu().numpy()
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    assert rad.shape == want.shape
+    assert np.array_equal(rad, want)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_radon_background(mode):
+    print(f"radon background {mode}", flush=True)
+    shape = (120, 120)
+    bd, ref = make_pair(shape, background_mode=mode)
+    pats = _pats(8, shape, "uint16", seed=4)
+    bg = bd.backsub_fit(pats.mean(axis=0))
+    bd.backgroundsub = bg
+    plan = bd.device_plan()
+    x, flags = bd._prepare(plan, _dev(pats))
+    rad = plan.radon(x, flags=flags).cpu().numpy()
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding, background=bg)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    if mode == "exact":
+        assert np.array_equal(rad, want)
+    else:
+        r64 = fp64.radon64(pats, ref.radonPlan.indexPlan, bg)
+        scale = np.abs(r64).max(axis=(1, 2), keepdims=True)
+        assert np.abs(rad - r64).max() <= 1e-6 * scale.max()
+        assert (np.abs(rad - want) / scale).max() <= 1e-5
+
+
+def test_conv_matches_fp64():
+    """Separable float32 convolution vs the FP64 direct sum with the exact 9x5 kernel:
+    within 1e-5 of max|C| (the reference's own float32 FFT is ~5e-7)."""
+    shape = (120, 120)
+    bd, ref = make_pair(shape)
+    pats = _pats(6, shape, "uint16", seed=5)
+    plan = bd.device_plan()
+    rad = plan.radon(_dev(pats))
+    conv = plan.conv(rad).cpu().numpy()
+    p0, p1 = ref.padding
+    C64 = fp64.conv64(fp64.pad64(rad.cpu().numpy().astype(np.float64), p0, p1), ref.kernel[0])
+    err = np.abs(conv - C64).max(axis=(1, 2)) / np.abs(C64).max(axis=(1, 2))
+    assert err.max() <= 1e-5, err
+
+
+@pytest.mark.parametrize("shape,dtype,n,seed", [((60, 60), "uint8", 40, 11), ((120, 120), "uint16", 24, 12),
+                                                ((160, 160), "uint16", 6, 13), ((100, 120), "uint8", 5, 14)])
+def test_find_bands_vs_fp64(shape, dtype, n, seed):
+    print(f"find_bands {shape} {dtype} n={n}", flush=True)
+    bd, ref = make_pair(shape)
+    pats = _pats(n, shape, dtype, seed=seed)
+    got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(_dev(pats)).items()}
+    res, st, C64, _ = oracle64(ref, pats)
+    total, ties = compare_bands(got, res, C64, ref.padding, ref.peakPad)
+    assert ties <= max(1, total // 100)
+    np.testing.assert_allclose(got['stats'][:, 0], st[0], rtol=0, atol=1e-5 * np.abs(st[0]).max())
+    np.testing.assert_allclose(got['stats'][:, 1], st[1], rtol=0, atol=1e-5 * np.abs(st[2]).max())
+    np.testing.assert_allclose(got['stats'][:, 2], st[2], rtol=1e-5)
+    same = (got['valid'] == 1) & (res['valid'] == 1) & np.all(got['maxloc'] == res['maxloc'], -1)
+    np.testing.assert_allclose(got['max'][same], res['max'][same], rtol=2e-5, atol=1e-5 * np.abs(res['max']).max())
+    np.testing.assert_allclose(got['theta'][same], res['theta'][same], atol=1e-3 * np.pi / 180)
+    np.testing.assert_allclose(got['rho'][same], res['rho'][same], atol=1e-3 * 2 * bd.rhoMax / bd.nRho)
+    np.testing.assert_allclose(got['width'][same], res['width'][same], rtol=1e-3, atol=1e-4)
+
+
+def test_find_bands_api_matches_reference_restatement(al_sim_pattern):
+    """BandDetect.find_bands (numpy in, structured array out) vs the reference-faithful CPU
+    restatement on the reference's own test pattern, plus synthetic 8-bit patterns."""
+    shape = al_sim_pattern.shape
+    bd, ref = make_pair(shape)
+    pats = np.concatenate([al_sim_pattern[None], _pats(7, shape, "uint8", seed=21)])
+    got = bd.find_bands(pats)
+    want = ref.find_bands(pats)
+    assert got.dtype == want.dtype
+    res, st, C64, _ = oracle64(ref, pats)
+    total, ties = compare_bands({k: got[k] for k in ('valid', 'maxloc', 'aveloc')}, res, C64,
+                                ref.padding, ref.peakPad)
+    assert ties <= 1
+    agree = np.all(got['maxloc'] == want['maxloc'], -1) & (got['valid'] == want['valid'])
+    assert agree.mean() >= 0.98
+    np.testing.assert_allclose(got['aveloc'][agree], want['aveloc'][agree], atol=1e-3)
+    np.testing.assert_array_equal(got['band_match_index'], want['band_match_index'])
+
+
+def test_edge_cases_constant_and_single():
+    """Plateaus (constant patterns make every masked cell a local max), N=1, N=0."""
+    import torch
+    shape = (60, 60)
+    bd, ref = make_pair(shape)
+    const = np.full((2,) + shape, 77, dtype=np.uint8)
+    got = bd.find_bands(const)
+    want = ref.find_bands(const)
+    assert np.array_equal(got['valid'], want['valid'])
+    one = _pats(1, shape, "uint8", seed=8)
+    g1 = bd.find_bands(one[0])
+    assert g1.shape == (1, 9)
+    res, st, C64, _ = oracle64(ref, one)
+    compare_bands({k: g1[k] for k in ('valid', 'maxloc', 'aveloc')}, res, C64, ref.padding, ref.peakPad)
+    out = bd.find_bands_device(torch.empty((0,) + shape, dtype=torch.uint8, device="cuda"))
+    assert out['max'].shape == (0, 9)
