@@ -67,11 +67,8 @@ class RefRadon:
         self.radon_plan_setup()
 
     def _masksetup(self):
-        # radon_fast.py:168-191
-        if self.mask is not None:
-            self.mask = np.array(self.mask).astype(int)
-        if self.maskindex is not None:
-            self.maskindex = np.array(self.maskindex).astype(np.int64)
+        # radon_fast.py:168-191, as called from radon_plan_setup (no arguments: a mask
+        # given at construction keeps its dtype; the plan truncates it with astype(int64))
         nPx = int(self.imDim[0]) * int(self.imDim[1])
         if self.mask is None:
             self.mask = np.ones(self.imDim)
