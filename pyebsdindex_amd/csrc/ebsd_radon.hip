@@ -82,7 +82,7 @@ struct ebsd_plan {
     uint8_t *d_mask = nullptr;   // nR*nT
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
     // tuning
-    int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 512;
+    int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 1024;
     int ptile = 60, pthreads = 256;
 };
 
@@ -104,10 +104,11 @@ struct GatherArgs {
     int64_t npacks;
 };
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <int WB> struct WordT;
 template <> struct WordT<4> { typedef uint32_t T; };
-template <> struct WordT<8> { typedef uint2 T; };
-template <> struct WordT<16> { typedef uint4 T; };
+template <> struct WordT<8> { typedef uint64_t T; };
+template <> struct WordT<16> { typedef u32x4 T; };
 
 template <int IN> struct InT;
 template <> struct InT<EBSD_U8> { typedef uint8_t T; };
@@ -120,20 +121,97 @@ template <int IN, bool FLT, int P> struct GatherCfg {
     static constexpr int EB = FLT ? 4 : (IN == EBSD_U8 ? 1 : 2); // bytes per element
     static constexpr int WB = EB * P;                             // bytes per LDS word
     static constexpr int ND = WB / 4;                             // dwords per word
+    static constexpr int VEC = 8 / (int)sizeof(typename InT<IN>::T); // pixels per 8-byte load
 };
 
-template <int IN, bool FLT, int P>
+// LDS byte address of the low / high 16-bit plan entry of a dword, times the word size:
+// v_mad_u32_u16 reads one 16-bit half (op_sel picks the high half) -- one VALU op per entry.
+// The LDS base of the staged words is folded into the same instruction.
+template <int WB> __device__ __forceinline__ uint32_t entry_addr_lo(uint32_t e, uint32_t base)
+{
+    uint32_t r;
+    asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(r) : "v"(e), "n"(WB), "s"(base));
+    return r;
+}
+template <int WB> __device__ __forceinline__ uint32_t entry_addr_hi(uint32_t e, uint32_t base)
+{
+    uint32_t r;
+    asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(r) : "v"(e), "n"(WB), "s"(base));
+    return r;
+}
+
+template <typename T> __device__ __forceinline__ T lds_load(uint32_t addr)
+{
+    return *reinterpret_cast<const __attribute__((address_space(3))) T *>((size_t)addr);
+}
+
+// acc += field SEL of d (SEL 0..3: byte, 4/5: low/high 16 bits) -- one SDWA add.
+template <int SEL> __device__ __forceinline__ void add_field(uint32_t &acc, uint32_t d)
+{
+    if constexpr (SEL == 0)
+        asm("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0 src1_sel:DWORD" : "+v"(acc) : "v"(d));
+    else if constexpr (SEL == 1)
+        asm("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 src1_sel:DWORD" : "+v"(acc) : "v"(d));
+    else if constexpr (SEL == 2)
+        asm("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2 src1_sel:DWORD" : "+v"(acc) : "v"(d));
+    else if constexpr (SEL == 3)
+        asm("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_3 src1_sel:DWORD" : "+v"(acc) : "v"(d));
+    else if constexpr (SEL == 4)
+        asm("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_0 src1_sel:DWORD" : "+v"(acc) : "v"(d));
+    else
+        asm("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:DWORD" : "+v"(acc) : "v"(d));
+}
+
+template <int IN, bool FLT, int P, int K = 0>
 __device__ __forceinline__ void accumulate_word(const uint32_t *d, uint32_t *acc, double *accd)
 {
-    if constexpr (FLT) {
+    if constexpr (K < P) {
+        if constexpr (FLT) {
+            accd[K] += (double)__uint_as_float(d[K]);
+        } else if constexpr (IN == EBSD_U8) {
+            add_field<K & 3>(acc[K], d[K >> 2]);
+        } else {
+            add_field<4 + (K & 1)>(acc[K], d[K >> 1]);
+        }
+        accumulate_word<IN, FLT, P, K + 1>(d, acc, accd);
+    }
+}
+
+template <int IN, bool FLT, int P>
+__device__ __forceinline__ void gather_group(uint32_t base, const uint4 &e, uint32_t *acc, double *accd)
+{
+    typedef GatherCfg<IN, FLT, P> C;
+    typedef typename WordT<C::WB>::T Word;
+    const uint32_t ew[4] = {e.x, e.y, e.z, e.w};
+    Word w[8];
 #pragma unroll
-        for (int p = 0; p < P; ++p) accd[p] += (double)__uint_as_float(d[p]);
-    } else if constexpr (IN == EBSD_U8) {
+    for (int k = 0; k < 4; ++k) {
+        w[2 * k] = lds_load<Word>(entry_addr_lo<C::WB>(ew[k], base));
+        w[2 * k + 1] = lds_load<Word>(entry_addr_hi<C::WB>(ew[k], base));
+    }
 #pragma unroll
-        for (int p = 0; p < P; ++p) acc[p] += (d[p >> 2] >> (8 * (p & 3))) & 0xffu;
-    } else {
+    for (int k = 0; k < 8; ++k)
+        accumulate_word<IN, FLT, P>(reinterpret_cast<const uint32_t *>(&w[k]), acc, accd);
+}
+
+// Build the LDS word of pixel px from element values x[p] (p < P), as packed integers or
+// float32(x - bg).
+template <int IN, bool FLT, int P>
+__device__ __forceinline__ void pack_word(uint32_t *d, const typename InT<IN>::T *x, const double *bg, int px)
+{
+    typedef GatherCfg<IN, FLT, P> C;
 #pragma unroll
-        for (int p = 0; p < P; ++p) acc[p] += (d[p >> 1] >> (16 * (p & 1))) & 0xffffu;
+    for (int k = 0; k < C::ND; ++k) d[k] = 0u;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if constexpr (FLT) {
+            const float f = bg ? (float)((double)x[p] - bg[px]) : (float)x[p];
+            d[p] = __float_as_uint(f);
+        } else if constexpr (IN == EBSD_U8) {
+            d[p >> 2] |= (uint32_t)x[p] << (8 * (p & 3));
+        } else {
+            d[p >> 1] |= (uint32_t)x[p] << (16 * (p & 1));
+        }
     }
 }
 
@@ -143,6 +221,7 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
     typedef GatherCfg<IN, FLT, P> C;
     typedef typename WordT<C::WB>::T Word;
     typedef typename InT<IN>::T Tin;
+    constexpr int VEC = C::VEC;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Word *lds = reinterpret_cast<Word *>(lds_raw);
 
@@ -152,32 +231,47 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
     const int wave = tid >> 6;
     const int nwaves = blockDim.x >> 6;
     const int nPx = a.nPx;
+    const int nb = a.nbatch;
+    const bool vec_ok = (nPx % VEC) == 0 && ((uintptr_t)a.pats % 8) == 0;
+    const int nvec = vec_ok ? nPx / VEC : 0;
+    const uint4 *gbase = a.gidx + lane;
+    // LDS address of the staged words (the entries are word indices)
+    const uint32_t lbase = (uint32_t)(size_t)(const __attribute__((address_space(3))) unsigned char *)(lds_raw);
 
     for (int64_t pack = blockIdx.x; pack < a.npacks; pack += gridDim.x) {
         const int64_t q0 = pack * P;
-        // ---- stage P patterns, pixel-major, plus one zero word at index nPx ----
-        for (int px = tid; px <= nPx; px += blockDim.x) {
+        // ---- stage P patterns pixel-major (8-byte loads per pattern), zero word at nPx ----
+        for (int v = tid; v < nvec; v += blockDim.x) {
+            Tin x[P][VEC];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                uint2 raw = make_uint2(0u, 0u);
+                if (q0 + p < a.N)
+                    raw = *reinterpret_cast<const uint2 *>(pats + (q0 + p) * (int64_t)nPx + (int64_t)v * VEC);
+                __builtin_memcpy(&x[p][0], &raw, 8);
+            }
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) {
+                Tin xp[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) xp[p] = x[p][j];
+                uint32_t d[C::ND];
+                pack_word<IN, FLT, P>(d, xp, a.bg, v * VEC + j);
+                Word w;
+                __builtin_memcpy(&w, d, sizeof(Word));
+                lds[v * VEC + j] = w;
+            }
+        }
+        for (int px = nvec * VEC + tid; px <= nPx; px += blockDim.x) {
+            Tin xp[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                xp[p] = (px < nPx && q0 + p < a.N) ? pats[(q0 + p) * (int64_t)nPx + px] : (Tin)0;
             uint32_t d[C::ND];
+            pack_word<IN, FLT, P>(d, xp, px < nPx ? a.bg : nullptr, px);
+            if (px == nPx) {
 #pragma unroll
-            for (int k = 0; k < C::ND; ++k) d[k] = 0u;
-            if (px < nPx) {
-#pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    const int64_t q = q0 + p;
-                    if (q < a.N) {
-                        Tin x = pats[q * (int64_t)nPx + px];
-                        if constexpr (FLT) {
-                            float f;
-                            if (a.bg) f = (float)((double)x - a.bg[px]);
-                            else f = (float)x;
-                            d[p] = __float_as_uint(f);
-                        } else if constexpr (IN == EBSD_U8) {
-                            d[p >> 2] |= (uint32_t)x << (8 * (p & 3));
-                        } else {
-                            d[p >> 1] |= (uint32_t)x << (16 * (p & 1));
-                        }
-                    }
-                }
+                for (int k = 0; k < C::ND; ++k) d[k] = 0u;
             }
             Word w;
             __builtin_memcpy(&w, d, sizeof(Word));
@@ -185,47 +279,55 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
         }
         __syncthreads();
 
-        // ---- gather: each lane owns one line of the current 64-line batch ----
-        for (int b = wave; b < a.nbatch; b += nwaves) {
-            const uint2 h = a.bhdr[b];
-            const uint4 *g = a.gidx + (size_t)h.x * 64 + lane;
-            const int ngrp = (int)h.y;
-            uint32_t acc[P];
-            double accd[P];
+        // ---- gather: each lane owns one line of the wave's current 64-line batch; the
+        //      next plan group (or the next batch's first group) is always in flight ----
+        int b = wave;
+        if (b < nb) {
+            uint2 h = a.bhdr[b];
+            int bn = b + nwaves;
+            uint2 hn = a.bhdr[bn < nb ? bn : nb - 1];
+            int2 li = a.lane[(size_t)b * 64 + lane];
+            uint4 e = gbase[(size_t)h.x * 64];
+            for (;;) {
+                const int ngrp = (int)h.y;
+                int2 lin = a.lane[(size_t)(bn < nb ? bn : b) * 64 + lane];
+                uint32_t acc[P];
+                double accd[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
-            uint4 e = ngrp > 0 ? g[0] : make_uint4(0, 0, 0, 0);
-            for (int gi = 0; gi < ngrp; ++gi) {
-                uint4 en = (gi + 1 < ngrp) ? g[(size_t)(gi + 1) * 64] : e;
-                const uint32_t ew[4] = {e.x, e.y, e.z, e.w};
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint32_t idx = (ew[k >> 1] >> (16 * (k & 1))) & 0xffffu;
-                    Word w = lds[idx];
-                    accumulate_word<IN, FLT, P>(reinterpret_cast<const uint32_t *>(&w), acc, accd);
+                for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+                for (int gi = 0; gi < ngrp; ++gi) {
+                    const size_t nxt = (gi + 1 < ngrp) ? (size_t)(h.x + gi + 1) : (size_t)hn.x;
+                    const uint4 en = gbase[nxt * 64];
+                    gather_group<IN, FLT, P>(lbase, e, acc, accd);
+                    e = en;
                 }
-                e = en;
-            }
-            const int2 li = a.lane[(size_t)b * 64 + lane];
-            if (li.x >= 0) {
-                const double cnt = (double)li.y;
+                if (ngrp == 0) e = gbase[(size_t)hn.x * 64];
+                if (li.x >= 0) {
+                    const double cnt = (double)li.y;
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    const int64_t q = q0 + p;
-                    if (q < a.N) {
-                        float r;
-                        if (li.y == 0) {
-                            r = -1.0f;
-                        } else if constexpr (FLT) {
-                            r = (float)(accd[p] / cnt);
-                        } else {
-                            double s = (double)acc[p];
-                            if (a.linebg) s -= a.linebg[li.x];
-                            r = (float)(s / cnt);
+                    for (int p = 0; p < P; ++p) {
+                        const int64_t q = q0 + p;
+                        if (q < a.N) {
+                            float r;
+                            if (li.y == 0) {
+                                r = -1.0f;
+                            } else if constexpr (FLT) {
+                                r = (float)(accd[p] / cnt);
+                            } else {
+                                double s = (double)acc[p];
+                                if (a.linebg) s -= a.linebg[li.x];
+                                r = (float)(s / cnt);
+                            }
+                            a.out[q * (int64_t)a.nCells + li.x] = r;
                         }
-                        a.out[q * (int64_t)a.nCells + li.x] = r;
                     }
                 }
+                b = bn;
+                if (b >= nb) break;
+                h = hn;
+                li = lin;
+                bn = b + nwaves;
+                hn = a.bhdr[bn < nb ? bn : nb - 1];
             }
         }
         __syncthreads();
@@ -764,7 +866,8 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
         }
         blocks += (uint32_t)ngrp;
     }
-    if (gidx.empty()) gidx.resize(64 * 8, (uint16_t)nPx);
+    // one filler block past the end: every batch (even an empty one) has a readable first block
+    gidx.resize(gidx.size() + 64 * 8, (uint16_t)nPx);
     if ((rc = dev_upload(reinterpret_cast<uint16_t **>(&p->d_gidx), gidx.data(), gidx.size()))) { plan_free(p); return rc; }
     if ((rc = dev_upload(&p->d_bhdr, bhdr.data(), bhdr.size()))) { plan_free(p); return rc; }
     if ((rc = dev_upload(&p->d_lane, lanes.data(), lanes.size()))) { plan_free(p); return rc; }
