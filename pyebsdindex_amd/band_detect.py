@@ -539,8 +539,9 @@ class BandDetect:
             x, flags = self._prepare(plan, pats)
             out = plan.find_bands(x, chunk=chunk, flags=flags)
             host = {k: v.cpu().numpy() for k, v in out.items()}
+        # band_detect.py:425-426 fills band_match_index with -100, but :466 then assigns
+        # whole records of the chunk array (np.zeros, :668), so the reference returns 0
         bandData = np.zeros((N, self.nBands), dtype=self.dataType)
-        bandData['band_match_index'] = -100
         for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid"):
             bandData[k] = host[k]
         if verbose > 0:
