@@ -72,7 +72,7 @@ int ebsd_plan_set_background(ebsd_plan *plan, const double *bg, int exact);
  *   gather_pack:    patterns interleaved per Radon work-group (u8: 4/8/16, u16: 2/4/8, f32: 1/2/4)
  *   gather_threads: threads per Radon work-group (multiple of 64, <= 1024)
  *   peak_tile:      theta columns per tile of the convolution/peak kernel
- *   peak_threads:   threads per convolution/peak work-group (multiple of 64, <= 1024) */
+ *   peak_threads:   threads per convolution/peak work-group (multiple of 64, <= 512) */
 int ebsd_plan_set_tuning(ebsd_plan *plan, int gather_pack, int gather_threads, int peak_tile,
                          int peak_threads);
 

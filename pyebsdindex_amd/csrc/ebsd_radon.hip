@@ -48,7 +48,7 @@ extern "C" const char *ebsd_last_error(void) { return g_err.c_str(); }
 // plan
 // ---------------------------------------------------------------------------------------
 static constexpr int KMAX = 16;      // max separable taps per axis
-static constexpr int CAND_CAP = 1024; // candidates per tile pass held in LDS
+static constexpr int CAND_CAP = 256;  // candidates per tile pass held in LDS
 static constexpr int NB_MAX = 64;     // max bands per pattern
 
 struct PeakGeo {
@@ -58,6 +58,7 @@ struct PeakGeo {
     int mr0, mr1;         // rows that can hold a candidate
     int sr0, sr1, sc0, sc1; // stats region
     int tile, ntiles;
+    int rh, apitch, bpitch, maskw; // LDS tile rows, pitches of the two tile buffers, mask words per row
     int nB;
     float u[KMAX], v[KMAX]; // C = sum_a u[a] sum_b v[b] Rpad[r + kr/2 - a][c + kt/2 - b]
 };
@@ -376,30 +377,27 @@ __device__ __forceinline__ bool key_gt(float av, int ap, float bv, int bp)
     return av > bv || (av == bv && ap > bp);
 }
 
-// LDS layout of k_peaks (all offsets in floats)
+// LDS layout of k_peaks (offsets in 4-byte words).  A holds the staged padded-Radon tile
+// and later the convolved tile; B holds the theta-convolved tile and later the theta
+// dilation; the pattern-independent rho mask is kept as bits.
 struct PeakLds {
-    int a_off, b_off, cand_v, cand_p, run_v, run_p, run_nn, red, total;
+    int a_off, b_off, mask_off, cand_v, cand_p, run_v, run_p, run_nn, red, total;
 };
-
-__host__ __device__ inline int peak_rows(const PeakGeo &g) { return g.cr1 - g.cr0 + 2 * (g.kr / 2); }
-__host__ __device__ inline int peak_cols_max(const PeakGeo &g)
-{
-    return g.tile + 2 * g.ht + 2 * (g.kt / 2);
-}
 
 __host__ __device__ inline PeakLds peak_lds_layout(const PeakGeo &g)
 {
     PeakLds L;
-    const int rh = peak_rows(g), rw = peak_cols_max(g);
     L.a_off = 0;
-    L.b_off = rh * rw;
-    L.cand_v = L.b_off + rh * rw;
+    L.b_off = g.rh * g.apitch;
+    L.mask_off = L.b_off + g.rh * g.bpitch;
+    L.cand_v = L.mask_off + g.nR * g.maskw;
     L.cand_p = L.cand_v + CAND_CAP;
     L.run_v = L.cand_p + CAND_CAP;
     L.run_p = L.run_v + NB_MAX;
     L.run_nn = L.run_p + NB_MAX;
-    L.red = L.run_nn + 9 * NB_MAX;  // 4 * 32 slots (min, max, sum as 2 floats each)
-    L.total = L.red + 4 * 32 + 8;   // + counters
+    L.red = L.run_nn + 9 * NB_MAX; // min[32], max[32], sum (double)[32]
+    L.red += L.red & 1;            // 8-byte alignment of the double sums
+    L.total = L.red + 4 * 32 + 8;  // + counters
     return L;
 }
 
@@ -417,13 +415,33 @@ __device__ __forceinline__ double np_interp_unit(double x, const double *fp, int
     return slope * (x - (double)j) + fp[j];
 }
 
-__global__ __launch_bounds__(1024) void k_peaks(PeakArgs a)
+// n / d for n, d < 2^16 by multiply-high (m = ceil(2^32 / d)).
+struct FDiv {
+    uint32_t d, m;
+};
+__device__ __forceinline__ FDiv fdiv_make(uint32_t d)
+{
+    FDiv f;
+    f.d = d;
+    f.m = d > 1 ? 0xFFFFFFFFu / d + 1u : 0u;
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv_q(uint32_t n, FDiv f) { return f.d > 1 ? __umulhi(n, f.m) : n; }
+
+constexpr int CSTRIP = 8; // rows per thread in the rho passes
+constexpr int CGRP = 4;   // columns per thread in the theta passes
+
+// KR/KT > 0: compile-time kernel sizes (dilation half-windows equal to them, as
+// band_detect.py:287-288 derives peakPad = 2*ksz+1); 0: run-time sizes.
+template <int KR, int KT>
+__global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
 {
     extern __shared__ __align__(16) float sm[];
     const PeakGeo &g = a.g;
     const PeakLds L = peak_lds_layout(g);
     float *A = sm + L.a_off;
     float *B = sm + L.b_off;
+    uint32_t *mbits = reinterpret_cast<uint32_t *>(sm + L.mask_off);
     float *cand_v = sm + L.cand_v;
     int *cand_p = reinterpret_cast<int *>(sm + L.cand_p);
     float *run_v = sm + L.run_v;
@@ -434,10 +452,27 @@ __global__ __launch_bounds__(1024) void k_peaks(PeakArgs a)
 
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int ckr = g.kr / 2, ckt = g.kt / 2;
+    const int kr = KR > 0 ? KR : g.kr, kt = KT > 0 ? KT : g.kt;
+    const int hr = KR > 0 ? KR : g.hr, ht = KT > 0 ? KT : g.ht;
+    const int ckr = kr / 2, ckt = kt / 2;
     const int CH = g.cr1 - g.cr0;
-    const int RH = CH + 2 * ckr;
+    const int RH = g.rh;
+    const int AP = g.apitch, BP = g.bpitch;
     const int nCells = g.nR * g.nT;
+
+    // ---- rho mask as bits (same for every pattern) ----
+    {
+        const FDiv fw = fdiv_make(g.maskw);
+        for (int it = tid; it < g.nR * g.maskw; it += nthr) {
+            const int i = (int)fdiv_q(it, fw), wd = it - i * g.maskw;
+            uint32_t bits = 0u;
+            for (int b = 0; b < 32; ++b) {
+                const int j = wd * 32 + b;
+                if (j < g.nT && a.mask[i * g.nT + j]) bits |= 1u << b;
+            }
+            mbits[it] = bits;
+        }
+    }
 
     for (int64_t q = blockIdx.x; q < a.N; q += gridDim.x) {
         const float *R = a.radon + q * (int64_t)nCells;
@@ -450,147 +485,276 @@ __global__ __launch_bounds__(1024) void k_peaks(PeakArgs a)
             const int j0 = t * g.tile;
             const int j1 = min(j0 + g.tile, g.nT);
             const int hc0 = g.p1 + j0, hc1 = g.p1 + j1, HW = hc1 - hc0;
-            const int cc0 = max(hc0 - g.ht, 0), cc1 = min(hc1 + g.ht, g.nTp), CW = cc1 - cc0;
+            const int cc0 = max(hc0 - ht, 0), cc1 = min(hc1 + ht, g.nTp), CW = cc1 - cc0;
             const int rb0 = g.cr0 - ckr;
             const int cb0 = cc0 - ckt, RW = CW + 2 * ckt;
+            const int hoff = hc0 - ht - cc0; // 0 unless the theta window is clipped
 
             // ---- stage padded Radon tile (zero outside the padded array) ----
-            for (int idx = tid; idx < RH * RW; idx += nthr) {
-                const int y = idx / RW, x = idx - y * RW;
-                const int r = rb0 + y, c = cb0 + x;
-                float val = 0.0f;
-                if (r >= 0 && r < g.nRp && c >= 0 && c < g.nTp) val = R[pad_src(g, r, c)];
-                A[idx] = val;
-            }
-            __syncthreads();
-            // ---- theta pass: T[y][x] = sum_b v[b] * A[y][x + 2ckt - b] ----
-            for (int idx = tid; idx < RH * CW; idx += nthr) {
-                const int y = idx / CW, x = idx - y * CW;
-                const float *row = A + y * RW + x + 2 * ckt;
-                float s = 0.0f;
-                for (int b = 0; b < g.kt; ++b) s = fmaf(g.v[b], row[-b], s);
-                B[idx] = s;
-            }
-            __syncthreads();
-            // ---- rho pass: C[y][x] = sum_a u[a] * T[y + 2ckr - a][x] (into A) + stats ----
-            for (int idx = tid; idx < CH * CW; idx += nthr) {
-                const int y = idx / CW, x = idx - y * CW;
-                const float *col = B + (y + 2 * ckr) * CW + x;
-                float s = 0.0f;
-                for (int aa = 0; aa < g.kr; ++aa) s = fmaf(g.u[aa], col[-aa * CW], s);
-                A[idx] = s;
-                const int r = g.cr0 + y, c = cc0 + x;
-                if (r >= g.sr0 && r < g.sr1 && c >= hc0 && c < hc1) {
-                    smin = fminf(smin, s);
-                    smax = fmaxf(smax, s);
-                    ssum += (double)s;
+            {
+                const FDiv f = fdiv_make(RW);
+                for (int it = tid; it < RH * RW; it += nthr) {
+                    const int y = (int)fdiv_q(it, f), x = it - y * RW;
+                    const int r = rb0 + y, c = cb0 + x;
+                    float val = 0.0f;
+                    if (r >= 0 && r < g.nRp && c >= 0 && c < g.nTp) val = R[pad_src(g, r, c)];
+                    A[y * AP + x] = val;
                 }
             }
             __syncthreads();
-            // ---- theta dilation: H[y][x'] = max C[y][c-ht .. c+ht] (clamped) (into B) ----
-            for (int idx = tid; idx < CH * HW; idx += nthr) {
-                const int y = idx / HW, x = idx - y * HW;
-                const int c = hc0 + x;
-                const int c0 = max(c - g.ht, 0) - cc0, c1 = min(c + g.ht, g.nTp - 1) - cc0;
-                const float *row = A + y * CW;
-                float m = -INFINITY;
-                for (int k = c0; k <= c1; ++k) m = fmaxf(m, row[k]);
-                B[idx] = m;
+            // ---- theta pass: T[y][x] = sum_b v[b] * A[y][x + kt - 1 - b], 4 columns/thread ----
+            {
+                const int G = (CW + CGRP - 1) / CGRP;
+                const FDiv f = fdiv_make(G);
+                for (int it = tid; it < RH * G; it += nthr) {
+                    const int y = (int)fdiv_q(it, f), x0 = (it - y * G) * CGRP;
+                    const float *row = A + y * AP + x0;
+                    float o[CGRP];
+                    if constexpr (KT > 0) {
+                        float w[CGRP + KT - 1];
+#pragma unroll
+                        for (int k = 0; k < CGRP + KT - 1; ++k) w[k] = row[k];
+#pragma unroll
+                        for (int c = 0; c < CGRP; ++c) {
+                            float s = 0.0f;
+#pragma unroll
+                            for (int b = 0; b < KT; ++b) s = fmaf(g.v[b], w[c + KT - 1 - b], s);
+                            o[c] = s;
+                        }
+                    } else {
+                        for (int c = 0; c < CGRP; ++c) {
+                            float s = 0.0f;
+                            for (int b = 0; b < kt; ++b) s = fmaf(g.v[b], row[c + kt - 1 - b], s);
+                            o[c] = s;
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < CGRP; ++c)
+                        if (x0 + c < CW) B[y * BP + x0 + c] = o[c];
+                }
             }
             __syncthreads();
+            // ---- rho pass: C[y][x] = sum_a u[a] * T[y + kr - 1 - a][x], 8 rows/thread (into A) ----
+            {
+                const int NS = (CH + CSTRIP - 1) / CSTRIP;
+                const FDiv f = fdiv_make(CW);
+                for (int it = tid; it < NS * CW; it += nthr) {
+                    const int s = (int)fdiv_q(it, f), x = it - s * CW;
+                    const int y0 = s * CSTRIP;
+                    float o[CSTRIP];
+                    if constexpr (KR > 0) {
+                        float w[CSTRIP + KR - 1];
+#pragma unroll
+                        for (int k = 0; k < CSTRIP + KR - 1; ++k)
+                            w[k] = (y0 + k < RH) ? B[(y0 + k) * BP + x] : 0.0f;
+#pragma unroll
+                        for (int c = 0; c < CSTRIP; ++c) {
+                            float acc = 0.0f;
+#pragma unroll
+                            for (int aa = 0; aa < KR; ++aa) acc = fmaf(g.u[aa], w[c + KR - 1 - aa], acc);
+                            o[c] = acc;
+                        }
+                    } else {
+                        for (int c = 0; c < CSTRIP; ++c) {
+                            float acc = 0.0f;
+                            if (y0 + c < CH)
+                                for (int aa = 0; aa < kr; ++aa)
+                                    acc = fmaf(g.u[aa], B[(y0 + c + kr - 1 - aa) * BP + x], acc);
+                            o[c] = acc;
+                        }
+                    }
+                    const int cglob = cc0 + x;
+                    const bool scol = cglob >= hc0 && cglob < hc1;
+#pragma unroll
+                    for (int c = 0; c < CSTRIP; ++c) {
+                        if (y0 + c < CH) {
+                            A[(y0 + c) * AP + x] = o[c];
+                            const int r = g.cr0 + y0 + c;
+                            if (scol && r >= g.sr0 && r < g.sr1) {
+                                smin = fminf(smin, o[c]);
+                                smax = fmaxf(smax, o[c]);
+                                ssum += (double)o[c];
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- theta dilation: H[y][x] = max C[y][x + hoff .. x + hoff + 2ht], 4 columns/thread (into B) ----
+            {
+                const int G = (HW + CGRP - 1) / CGRP;
+                const FDiv f = fdiv_make(G);
+                for (int it = tid; it < CH * G; it += nthr) {
+                    const int y = (int)fdiv_q(it, f), x0 = (it - y * G) * CGRP;
+                    const float *row = A + y * AP + x0 + hoff;
+                    float o[CGRP];
+                    if constexpr (KT > 0) {
+                        // windows w[c .. c + 2KT]; all four share w[3 .. 2KT]
+                        float w[CGRP + 2 * KT];
+#pragma unroll
+                        for (int k = 0; k < CGRP + 2 * KT; ++k) w[k] = row[k];
+                        float mid = w[CGRP - 1];
+#pragma unroll
+                        for (int k = CGRP; k <= 2 * KT; ++k) mid = fmaxf(mid, w[k]);
+                        const float l2 = w[2], l1 = fmaxf(w[1], l2), l0 = fmaxf(w[0], l1);
+                        const float r1 = w[2 * KT + 1], r2 = fmaxf(r1, w[2 * KT + 2]), r3 = fmaxf(r2, w[2 * KT + 3]);
+                        o[0] = fmaxf(mid, l0);
+                        o[1] = fmaxf(mid, fmaxf(l1, r1));
+                        o[2] = fmaxf(mid, fmaxf(l2, r2));
+                        o[3] = fmaxf(mid, r3);
+                    } else {
+                        for (int c = 0; c < CGRP; ++c) {
+                            const int cg = hc0 + x0 + c;
+                            const int lo = max(cg - ht, 0) - cc0, hi = min(cg + ht, g.nTp - 1) - cc0;
+                            float m = -INFINITY;
+                            for (int k = lo; k <= hi && cg < hc1; ++k) m = fmaxf(m, A[y * AP + k]);
+                            o[c] = m;
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < CGRP; ++c)
+                        if (x0 + c < HW) B[y * BP + x0 + c] = o[c];
+                }
+            }
+            __syncthreads();
+
             // ---- rho dilation + equality under the mask -> candidates, merged per row
             //      chunk; a chunk that overflows the candidate list (plateaus) is redone
             //      in chunks small enough that every cell could be a candidate ----
             int rstart = g.mr0;
             int rstep = g.mr1 - g.mr0;
             while (rstart < g.mr1) {
-            const int rend = min(rstart + rstep, g.mr1);
-            const int nrow = rend - rstart;
-            for (int idx = tid; idx < nrow * HW; idx += nthr) {
-                const int yy = idx / HW, x = idx - yy * HW;
-                const int r = rstart + yy, c = hc0 + x;
-                if (!a.mask[(r - g.p0) * g.nT + (c - g.p1)]) continue;
-                const int r0 = max(r - g.hr, 0) - g.cr0, r1 = min(r + g.hr, g.nRp - 1) - g.cr0;
-                float m = -INFINITY;
-                for (int k = r0; k <= r1; ++k) m = fmaxf(m, B[k * HW + x]);
-                const float cv = A[(r - g.cr0) * CW + (c - cc0)];
-                if (m == cv) {
-                    const int slot = atomicAdd(&ctr[1], 1);
-                    if (slot < CAND_CAP) {
-                        cand_v[slot] = cv;
-                        cand_p[slot] = r * g.nTp + c;
+                const int rend = min(rstart + rstep, g.mr1);
+                {
+                    const int NS = (rend - rstart + CSTRIP - 1) / CSTRIP;
+                    const FDiv f = fdiv_make(HW);
+                    for (int it = tid; it < NS * HW; it += nthr) {
+                        const int s = (int)fdiv_q(it, f), x = it - s * HW;
+                        const int r0 = rstart + s * CSTRIP;
+                        const int jc = hc0 + x - g.p1;
+                        uint32_t any = 0u;
+#pragma unroll
+                        for (int c = 0; c < CSTRIP; ++c) {
+                            const int r = r0 + c;
+                            if (r < rend)
+                                any |= ((mbits[(r - g.p0) * g.maskw + (jc >> 5)] >> (jc & 31)) & 1u) << c;
+                        }
+                        if (!any) continue;
+                        float d[CSTRIP];
+                        if constexpr (KR > 0) {
+                            float w[CSTRIP + 2 * KR];
+#pragma unroll
+                            for (int k = 0; k < CSTRIP + 2 * KR; ++k) {
+                                const int row = r0 - KR + k;
+                                w[k] = (row >= 0 && row < g.nRp && row < g.cr1) ? B[(row - g.cr0) * BP + x] : -INFINITY;
+                            }
+                            // windows w[c .. c + 2KR]; all eight share w[7 .. 2KR]
+                            float mid = w[CSTRIP - 1];
+#pragma unroll
+                            for (int k = CSTRIP; k <= 2 * KR; ++k) mid = fmaxf(mid, w[k]);
+                            float lft[CSTRIP], rgt[CSTRIP];
+                            lft[CSTRIP - 1] = -INFINITY;
+#pragma unroll
+                            for (int c = CSTRIP - 2; c >= 0; --c) lft[c] = fmaxf(w[c], lft[c + 1]);
+                            rgt[0] = -INFINITY;
+#pragma unroll
+                            for (int c = 1; c < CSTRIP; ++c) rgt[c] = fmaxf(rgt[c - 1], w[2 * KR + c]);
+#pragma unroll
+                            for (int c = 0; c < CSTRIP; ++c) d[c] = fmaxf(mid, fmaxf(lft[c], rgt[c]));
+                        } else {
+                            for (int c = 0; c < CSTRIP; ++c) {
+                                const int r = r0 + c;
+                                const int lo = max(r - hr, 0) - g.cr0, hi = min(r + hr, g.nRp - 1) - g.cr0;
+                                float m = -INFINITY;
+                                if (r < rend)
+                                    for (int k = lo; k <= hi; ++k) m = fmaxf(m, B[k * BP + x]);
+                                d[c] = m;
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < CSTRIP; ++c) {
+                            if (!((any >> c) & 1u)) continue;
+                            const int r = r0 + c;
+                            const float cv = A[(r - g.cr0) * AP + (hc0 + x - cc0)];
+                            if (d[c] == cv) {
+                                const int slot = atomicAdd(&ctr[1], 1);
+                                if (slot < CAND_CAP) {
+                                    cand_v[slot] = cv;
+                                    cand_p[slot] = r * g.nTp + hc0 + x;
+                                }
+                            }
+                        }
                     }
                 }
-            }
-            __syncthreads();
-            const int found = ctr[1];
-            __syncthreads();
-            if (found > CAND_CAP) {
-                if (tid == 0) ctr[1] = 0;
                 __syncthreads();
-                rstep = max(1, CAND_CAP / HW);
-                continue;
-            }
-            // ---- merge the chunk's candidates into the running top-nB (wave 0) ----
-            if (wave == 0) {
-                const int nnew = min(ctr[1], CAND_CAP);
-                const int nrun = ctr[0];
-                const int ntot = nrun + nnew;
-                float lastv = INFINITY;
-                int lastp = 0x7fffffff;
-                bool havelast = false;
-                int mysrc = -1; // source index selected for slot == lane
-                int nsel = 0;
-                for (int k = 0; k < g.nB; ++k) {
-                    float bv = -INFINITY;
-                    int bp = -1, bi = -1;
-                    for (int i = lane; i < ntot; i += 64) {
-                        const float v = i < nrun ? run_v[i] : cand_v[i - nrun];
-                        const int p = i < nrun ? run_p[i] : cand_p[i - nrun];
-                        if (havelast && !key_gt(lastv, lastp, v, p)) continue;
-                        if (bi < 0 || key_gt(v, p, bv, bp)) { bv = v; bp = p; bi = i; }
+                const int found = ctr[1];
+                __syncthreads();
+                if (found > CAND_CAP) {
+                    if (tid == 0) ctr[1] = 0;
+                    __syncthreads();
+                    rstep = max(1, CAND_CAP / HW);
+                    continue;
+                }
+                // ---- merge the chunk's candidates into the running top-nB (wave 0) ----
+                if (wave == 0) {
+                    const int nnew = min(ctr[1], CAND_CAP);
+                    const int nrun = ctr[0];
+                    const int ntot = nrun + nnew;
+                    float lastv = INFINITY;
+                    int lastp = 0x7fffffff;
+                    bool havelast = false;
+                    int mysrc = -1; // source index selected for slot == lane
+                    int nsel = 0;
+                    for (int k = 0; k < g.nB; ++k) {
+                        float bv = -INFINITY;
+                        int bp = -1, bi = -1;
+                        for (int i = lane; i < ntot; i += 64) {
+                            const float v = i < nrun ? run_v[i] : cand_v[i - nrun];
+                            const int pp = i < nrun ? run_p[i] : cand_p[i - nrun];
+                            if (havelast && !key_gt(lastv, lastp, v, pp)) continue;
+                            if (bi < 0 || key_gt(v, pp, bv, bp)) { bv = v; bp = pp; bi = i; }
+                        }
+                        for (int off = 32; off > 0; off >>= 1) {
+                            const float ov = __shfl_xor(bv, off, 64);
+                            const int op = __shfl_xor(bp, off, 64);
+                            const int oi = __shfl_xor(bi, off, 64);
+                            if (oi >= 0 && (bi < 0 || key_gt(ov, op, bv, bp))) { bv = ov; bp = op; bi = oi; }
+                        }
+                        if (bi < 0) break;
+                        if (lane == k) mysrc = bi;
+                        lastv = bv; lastp = bp; havelast = true;
+                        ++nsel;
                     }
-                    for (int off = 32; off > 0; off >>= 1) {
-                        const float ov = __shfl_xor(bv, off, 64);
-                        const int op = __shfl_xor(bp, off, 64);
-                        const int oi = __shfl_xor(bi, off, 64);
-                        if (oi >= 0 && (bi < 0 || key_gt(ov, op, bv, bp))) { bv = ov; bp = op; bi = oi; }
+                    // gather the selected entries (values, positions, 3x3 neighbourhoods)
+                    float sv = 0.0f, snn[9];
+                    int sp = 0;
+                    if (lane < nsel) {
+                        if (mysrc < nrun) {
+                            sv = run_v[mysrc];
+                            sp = run_p[mysrc];
+                            for (int k = 0; k < 9; ++k) snn[k] = run_nn[mysrc * 9 + k];
+                        } else {
+                            sv = cand_v[mysrc - nrun];
+                            sp = cand_p[mysrc - nrun];
+                            const int r = sp / g.nTp, c = sp - (sp / g.nTp) * g.nTp;
+                            for (int dr = 0; dr < 3; ++dr)
+                                for (int dc = 0; dc < 3; ++dc)
+                                    snn[dr * 3 + dc] = A[(r - 1 + dr - g.cr0) * AP + (c - 1 + dc - cc0)];
+                        }
                     }
-                    if (bi < 0) break;
-                    if (lane == k) mysrc = bi;
-                    lastv = bv; lastp = bp; havelast = true;
-                    ++nsel;
-                }
-                // gather the selected entries (values, positions, 3x3 neighbourhoods)
-                float sv = 0.0f, snn[9];
-                int sp = 0;
-                if (lane < nsel) {
-                    if (mysrc < nrun) {
-                        sv = run_v[mysrc];
-                        sp = run_p[mysrc];
-                        for (int k = 0; k < 9; ++k) snn[k] = run_nn[mysrc * 9 + k];
-                    } else {
-                        sv = cand_v[mysrc - nrun];
-                        sp = cand_p[mysrc - nrun];
-                        const int r = sp / g.nTp, c = sp - (sp / g.nTp) * g.nTp;
-                        for (int dr = 0; dr < 3; ++dr)
-                            for (int dc = 0; dc < 3; ++dc)
-                                snn[dr * 3 + dc] = A[(r - 1 + dr - g.cr0) * CW + (c - 1 + dc - cc0)];
+                    if (lane < nsel) {
+                        run_v[lane] = sv;
+                        run_p[lane] = sp;
+                        for (int k = 0; k < 9; ++k) run_nn[lane * 9 + k] = snn[k];
+                    }
+                    if (lane == 0) {
+                        ctr[0] = nsel;
+                        ctr[2] += ctr[1];
+                        ctr[1] = 0;
                     }
                 }
-                if (lane < nsel) {
-                    run_v[lane] = sv;
-                    run_p[lane] = sp;
-                    for (int k = 0; k < 9; ++k) run_nn[lane * 9 + k] = snn[k];
-                }
-                if (lane == 0) {
-                    ctr[0] = nsel;
-                    ctr[2] += ctr[1];
-                    ctr[1] = 0;
-                }
-            }
-            __syncthreads();
-            rstart = rend;
+                __syncthreads();
+                rstart = rend;
             }
         }
 
@@ -771,9 +935,14 @@ static int geo_tiles(ebsd_plan *p)
 {
     PeakGeo &g = p->g;
     int tile = std::max(1, std::min(p->ptile, g.nT));
+    g.rh = g.cr1 - g.cr0 + 2 * (g.kr / 2);
+    g.maskw = (g.nT + 31) / 32;
     for (;;) {
         g.tile = tile;
         g.ntiles = (g.nT + tile - 1) / tile;
+        const int cw = tile + 2 * g.ht;
+        g.apitch = ((cw + 2 * (g.kt / 2) + 3) / 4) * 4 + 4;
+        g.bpitch = ((cw + 3) / 4) * 4;
         const size_t bytes = (size_t)peak_lds_layout(g).total * 4;
         if ((int)bytes <= p->lds_max || tile == 1) break;
         tile = std::max(1, tile * 3 / 4);
@@ -958,7 +1127,7 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
         p->gthreads = gather_threads;
     }
     if (peak_threads > 0) {
-        if (peak_threads % 64 || peak_threads > 1024) return fail(-1, "peak_threads must be a multiple of 64 <= 1024");
+        if (peak_threads % 64 || peak_threads > 512) return fail(-1, "peak_threads must be a multiple of 64 <= 512");
         p->pthreads = peak_threads;
     }
     if (peak_tile > 0) {
@@ -1063,10 +1232,12 @@ static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int6
     return rc;
 }
 
-static int launch_peaks(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, hipStream_t s)
+template <int KR, int KT>
+static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, hipStream_t s)
 {
     const size_t lds = (size_t)peak_lds_layout(p->g).total * 4;
-    HIPCHK(hipFuncSetAttribute((const void *)k_peaks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    auto kern = k_peaks<KR, KT>;
+    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     PeakArgs a;
     a.radon = d_radon;
     a.N = N;
@@ -1075,12 +1246,21 @@ static int launch_peaks(ebsd_plan *p, const float *d_radon, int64_t N, const ebs
     a.out = *out;
     a.g = p->g;
     int occ = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_peaks, p->pthreads, lds));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->pthreads, lds));
     if (occ < 1) return fail(-3, "peak kernel cannot be resident");
     const int64_t grid = std::min<int64_t>(N, (int64_t)occ * p->ncu);
-    hipLaunchKernelGGL(k_peaks, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+static int launch_peaks(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, hipStream_t s)
+{
+    const PeakGeo &g = p->g;
+    // compile-time kernel sizes when the dilation half-windows equal them (peakPad = 2*ksz+1)
+    if (g.kr == 9 && g.kt == 5 && g.hr == 9 && g.ht == 5) return launch_peaks_t<9, 5>(p, d_radon, N, out, s);
+    if (g.kr == 5 && g.kt == 5 && g.hr == 5 && g.ht == 5) return launch_peaks_t<5, 5>(p, d_radon, N, out, s);
+    return launch_peaks_t<0, 0>(p, d_radon, N, out, s);
 }
 
 static ebsd_bands offset_bands(const ebsd_bands &o, int64_t q0, int nB)
