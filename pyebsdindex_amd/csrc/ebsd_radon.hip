@@ -84,7 +84,7 @@ struct ebsd_plan {
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
     // tuning
     int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 1024;
-    int ptile = 60, pthreads = 256;
+    int ptile = 45, pthreads = 512;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -381,7 +381,8 @@ __device__ __forceinline__ bool key_gt(float av, int ap, float bv, int bp)
 // and later the convolved tile; B holds the theta-convolved tile and later the theta
 // dilation; the pattern-independent rho mask is kept as bits.
 struct PeakLds {
-    int a_off, b_off, mask_off, cand_v, cand_p, run_v, run_p, run_nn, red, total;
+    int a_off, b_off, mask_off, cand_v, cand_p, cand_nn, run_v, run_p, run_nn, sel_v, sel_p, sel_nn, red,
+        total;
 };
 
 __host__ __device__ inline PeakLds peak_lds_layout(const PeakGeo &g)
@@ -392,10 +393,14 @@ __host__ __device__ inline PeakLds peak_lds_layout(const PeakGeo &g)
     L.mask_off = L.b_off + g.rh * g.bpitch;
     L.cand_v = L.mask_off + g.nR * g.maskw;
     L.cand_p = L.cand_v + CAND_CAP;
-    L.run_v = L.cand_p + CAND_CAP;
+    L.cand_nn = L.cand_p + CAND_CAP;
+    L.run_v = L.cand_nn + 9 * CAND_CAP;
     L.run_p = L.run_v + NB_MAX;
     L.run_nn = L.run_p + NB_MAX;
-    L.red = L.run_nn + 9 * NB_MAX; // min[32], max[32], sum (double)[32]
+    L.sel_v = L.run_nn + 9 * NB_MAX;
+    L.sel_p = L.sel_v + NB_MAX;
+    L.sel_nn = L.sel_p + NB_MAX;
+    L.red = L.sel_nn + 9 * NB_MAX; // min[32], max[32], sum (double)[32]
     L.red += L.red & 1;            // 8-byte alignment of the double sums
     L.total = L.red + 4 * 32 + 8;  // + counters
     return L;
@@ -428,6 +433,48 @@ __device__ __forceinline__ FDiv fdiv_make(uint32_t d)
 }
 __device__ __forceinline__ uint32_t fdiv_q(uint32_t n, FDiv f) { return f.d > 1 ? __umulhi(n, f.m) : n; }
 
+// Top-nB of (running list U candidate list) by (value, position) descending -- a total
+// order, so ranks are unique: each entry counts the entries above it and, if its rank is
+// below nB, writes itself to that slot.  Result replaces the running list; the candidate
+// list is emptied.  All threads of the work-group call it.
+__device__ __noinline__ void select_top(int tid, int nthr, int nB, int *ctr, float *run_v, int *run_p,
+                                        float *run_nn, const float *cand_v, const int *cand_p,
+                                        const float *cand_nn, float *sel_v, int *sel_p, float *sel_nn)
+{
+    __syncthreads();
+    const int nrun = ctr[0];
+    const int ncand = min(ctr[1], CAND_CAP);
+    const int ntot = nrun + ncand;
+    for (int i = tid; i < ntot; i += nthr) {
+        const bool ir = i < nrun;
+        const float v = ir ? run_v[i] : cand_v[i - nrun];
+        const int p = ir ? run_p[i] : cand_p[i - nrun];
+        int rank = 0;
+        for (int j = 0; j < nrun; ++j) rank += key_gt(run_v[j], run_p[j], v, p);
+        for (int j = 0; j < ncand && rank < nB; ++j) rank += key_gt(cand_v[j], cand_p[j], v, p);
+        if (rank < nB) {
+            sel_v[rank] = v;
+            sel_p[rank] = p;
+            const float *nn = ir ? run_nn + 9 * i : cand_nn + 9 * (i - nrun);
+            for (int k = 0; k < 9; ++k) sel_nn[9 * rank + k] = nn[k];
+        }
+    }
+    __syncthreads();
+    const int nsel = min(nB, ntot);
+    for (int i = tid; i < nsel * 11; i += nthr) {
+        const int e = i / 11, k = i - e * 11;
+        if (k == 0) run_v[e] = sel_v[e];
+        else if (k == 1) run_p[e] = sel_p[e];
+        else run_nn[9 * e + k - 2] = sel_nn[9 * e + k - 2];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        ctr[0] = nsel;
+        ctr[1] = 0;
+    }
+    __syncthreads();
+}
+
 constexpr int CSTRIP = 8; // rows per thread in the rho passes
 constexpr int CGRP = 4;   // columns per thread in the theta passes
 
@@ -444,11 +491,15 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
     uint32_t *mbits = reinterpret_cast<uint32_t *>(sm + L.mask_off);
     float *cand_v = sm + L.cand_v;
     int *cand_p = reinterpret_cast<int *>(sm + L.cand_p);
+    float *cand_nn = sm + L.cand_nn;
     float *run_v = sm + L.run_v;
     int *run_p = reinterpret_cast<int *>(sm + L.run_p);
     float *run_nn = sm + L.run_nn;
+    float *sel_v = sm + L.sel_v;
+    int *sel_p = reinterpret_cast<int *>(sm + L.sel_p);
+    float *sel_nn = sm + L.sel_nn;
     float *red = sm + L.red;
-    int *ctr = reinterpret_cast<int *>(sm + L.total - 8);
+    int *ctr = reinterpret_cast<int *>(sm + L.total - 8); // [0] running count, [1] candidates
 
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -478,7 +529,7 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
         const float *R = a.radon + q * (int64_t)nCells;
         float smin = INFINITY, smax = -INFINITY;
         double ssum = 0.0;
-        if (tid == 0) { ctr[0] = 0; ctr[1] = 0; ctr[2] = 0; }
+        if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
         __syncthreads();
 
         for (int t = 0; t < g.ntiles; ++t) {
@@ -490,15 +541,27 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
             const int cb0 = cc0 - ckt, RW = CW + 2 * ckt;
             const int hoff = hc0 - ht - cc0; // 0 unless the theta window is clipped
 
-            // ---- stage padded Radon tile (zero outside the padded array) ----
+            // ---- stage padded Radon tile (zero outside the padded array); 8 loads in
+            //      flight per thread ----
             {
                 const FDiv f = fdiv_make(RW);
-                for (int it = tid; it < RH * RW; it += nthr) {
-                    const int y = (int)fdiv_q(it, f), x = it - y * RW;
-                    const int r = rb0 + y, c = cb0 + x;
-                    float val = 0.0f;
-                    if (r >= 0 && r < g.nRp && c >= 0 && c < g.nTp) val = R[pad_src(g, r, c)];
-                    A[y * AP + x] = val;
+                constexpr int U = 8;
+                for (int it0 = tid; it0 < RH * RW; it0 += U * nthr) {
+                    float val[U];
+                    int dst[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int it = it0 + u * nthr;
+                        const int y = (int)fdiv_q(it, f), x = it - y * RW;
+                        const int r = rb0 + y, c = cb0 + x;
+                        dst[u] = it < RH * RW ? y * AP + x : -1;
+                        val[u] = 0.0f;
+                        if (it < RH * RW && r >= 0 && r < g.nRp && c >= 0 && c < g.nTp)
+                            val[u] = __builtin_nontemporal_load(R + pad_src(g, r, c));
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (dst[u] >= 0) A[dst[u]] = val[u];
                 }
             }
             __syncthreads();
@@ -618,11 +681,14 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
             }
             __syncthreads();
 
-            // ---- rho dilation + equality under the mask -> candidates, merged per row
-            //      chunk; a chunk that overflows the candidate list (plateaus) is redone
-            //      in chunks small enough that every cell could be a candidate ----
+            // ---- rho dilation + equality under the mask -> candidates (with their 3x3
+            //      neighbourhoods) appended to the pattern's list.  If a tile overflows the
+            //      list (plateaus), its appends are dropped, the list is merged into the
+            //      running top-nB and the tile is redone in row chunks that cannot overflow ----
+            const int pre = ctr[1];
             int rstart = g.mr0;
             int rstep = g.mr1 - g.mr0;
+            bool chunked = false;
             while (rstart < g.mr1) {
                 const int rend = min(rstart + rstep, g.mr1);
                 {
@@ -675,12 +741,18 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
                         for (int c = 0; c < CSTRIP; ++c) {
                             if (!((any >> c) & 1u)) continue;
                             const int r = r0 + c;
-                            const float cv = A[(r - g.cr0) * AP + (hc0 + x - cc0)];
+                            const int ax = hc0 + x - cc0;
+                            const float cv = A[(r - g.cr0) * AP + ax];
                             if (d[c] == cv) {
                                 const int slot = atomicAdd(&ctr[1], 1);
                                 if (slot < CAND_CAP) {
                                     cand_v[slot] = cv;
                                     cand_p[slot] = r * g.nTp + hc0 + x;
+#pragma unroll
+                                    for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+                                        for (int dc = 0; dc < 3; ++dc)
+                                            cand_nn[slot * 9 + dr * 3 + dc] = A[(r - 1 + dr - g.cr0) * AP + (ax - 1 + dc)];
                                 }
                             }
                         }
@@ -690,73 +762,22 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
                 const int found = ctr[1];
                 __syncthreads();
                 if (found > CAND_CAP) {
-                    if (tid == 0) ctr[1] = 0;
+                    // drop this pass's appends, fold the earlier ones into the running list
+                    if (tid == 0) ctr[1] = chunked ? 0 : pre;
                     __syncthreads();
+                    select_top(tid, nthr, g.nB, ctr, run_v, run_p, run_nn, cand_v, cand_p, cand_nn, sel_v,
+                               sel_p, sel_nn);
+                    chunked = true;
                     rstep = max(1, CAND_CAP / HW);
                     continue;
                 }
-                // ---- merge the chunk's candidates into the running top-nB (wave 0) ----
-                if (wave == 0) {
-                    const int nnew = min(ctr[1], CAND_CAP);
-                    const int nrun = ctr[0];
-                    const int ntot = nrun + nnew;
-                    float lastv = INFINITY;
-                    int lastp = 0x7fffffff;
-                    bool havelast = false;
-                    int mysrc = -1; // source index selected for slot == lane
-                    int nsel = 0;
-                    for (int k = 0; k < g.nB; ++k) {
-                        float bv = -INFINITY;
-                        int bp = -1, bi = -1;
-                        for (int i = lane; i < ntot; i += 64) {
-                            const float v = i < nrun ? run_v[i] : cand_v[i - nrun];
-                            const int pp = i < nrun ? run_p[i] : cand_p[i - nrun];
-                            if (havelast && !key_gt(lastv, lastp, v, pp)) continue;
-                            if (bi < 0 || key_gt(v, pp, bv, bp)) { bv = v; bp = pp; bi = i; }
-                        }
-                        for (int off = 32; off > 0; off >>= 1) {
-                            const float ov = __shfl_xor(bv, off, 64);
-                            const int op = __shfl_xor(bp, off, 64);
-                            const int oi = __shfl_xor(bi, off, 64);
-                            if (oi >= 0 && (bi < 0 || key_gt(ov, op, bv, bp))) { bv = ov; bp = op; bi = oi; }
-                        }
-                        if (bi < 0) break;
-                        if (lane == k) mysrc = bi;
-                        lastv = bv; lastp = bp; havelast = true;
-                        ++nsel;
-                    }
-                    // gather the selected entries (values, positions, 3x3 neighbourhoods)
-                    float sv = 0.0f, snn[9];
-                    int sp = 0;
-                    if (lane < nsel) {
-                        if (mysrc < nrun) {
-                            sv = run_v[mysrc];
-                            sp = run_p[mysrc];
-                            for (int k = 0; k < 9; ++k) snn[k] = run_nn[mysrc * 9 + k];
-                        } else {
-                            sv = cand_v[mysrc - nrun];
-                            sp = cand_p[mysrc - nrun];
-                            const int r = sp / g.nTp, c = sp - (sp / g.nTp) * g.nTp;
-                            for (int dr = 0; dr < 3; ++dr)
-                                for (int dc = 0; dc < 3; ++dc)
-                                    snn[dr * 3 + dc] = A[(r - 1 + dr - g.cr0) * AP + (c - 1 + dc - cc0)];
-                        }
-                    }
-                    if (lane < nsel) {
-                        run_v[lane] = sv;
-                        run_p[lane] = sp;
-                        for (int k = 0; k < 9; ++k) run_nn[lane * 9 + k] = snn[k];
-                    }
-                    if (lane == 0) {
-                        ctr[0] = nsel;
-                        ctr[2] += ctr[1];
-                        ctr[1] = 0;
-                    }
-                }
-                __syncthreads();
+                if (chunked) select_top(tid, nthr, g.nB, ctr, run_v, run_p, run_nn, cand_v, cand_p, cand_nn,
+                                        sel_v, sel_p, sel_nn);
                 rstart = rend;
             }
         }
+        // ---- top-nB of the running list and all remaining candidates ----
+        select_top(tid, nthr, g.nB, ctr, run_v, run_p, run_nn, cand_v, cand_p, cand_nn, sel_v, sel_p, sel_nn);
 
         // ---- stats reduction: min, max, sum over the work-group ----
         for (int off = 32; off > 0; off >>= 1) {
@@ -934,7 +955,9 @@ static void split_kernel(const float *k, int kr, int kt, float *u, float *v)
 static int geo_tiles(ebsd_plan *p)
 {
     PeakGeo &g = p->g;
-    int tile = std::max(1, std::min(p->ptile, g.nT));
+    // a tile row never holds more cells than the candidate list (the overflow fallback
+    // processes one row at a time)
+    int tile = std::max(1, std::min(std::min(p->ptile, g.nT), CAND_CAP));
     g.rh = g.cr1 - g.cr0 + 2 * (g.kr / 2);
     g.maskw = (g.nT + 31) / 32;
     for (;;) {

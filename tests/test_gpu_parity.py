@@ -120,6 +120,11 @@ def test_edge_cases_constant_and_single():
     got = bd.find_bands(const)
     want = ref.find_bands(const)
     assert np.array_equal(got['valid'], want['valid'])
+    # every masked cell ties exactly (FP64 and the device's separable float32 alike), which
+    # overflows the candidate list and exercises the chunked fallback; ties resolve to the
+    # highest row-major positions in both
+    res, st, C64, _ = oracle64(ref, const)
+    assert np.array_equal(got['maxloc'], res['maxloc'].astype(np.float32))
     one = _pats(1, shape, "uint8", seed=8)
     g1 = bd.find_bands(one[0])
     assert g1.shape == (1, 9)
