@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=5)
+    ap.add_argument("--chunk", type=int, default=2048,
+                    help="patterns per Radon+peak pass (the Radon of a chunk stays in the MALL)")
     return ap.parse_args()
 
 
@@ -82,15 +84,16 @@ def main():
     info = plan.info()
     N = pats.shape[0]
     out = plan.alloc_out(N)
-    work = plan.workspace(N, N)
+    chunk = max(1, min(args.chunk, N))
+    work = plan.workspace(N, chunk)
     print(f"[bench] plan {info}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
-        bd.find_bands_device(pats, out=out, work=work, chunk=N)
+        bd.find_bands_device(pats, out=out, work=work, chunk=chunk)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        bd.find_bands_device(pats, out=out, work=work, chunk=N)
+        bd.find_bands_device(pats, out=out, work=work, chunk=chunk)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
@@ -99,7 +102,7 @@ def main():
 
     # ---- per-kernel durations with HIP events on the launch stream ----
     s = torch.cuda.current_stream(dev)
-    rad = work[:N * bd.nRho * bd.nTheta].view(N, bd.nRho, bd.nTheta)
+    rad = torch.empty((N, bd.nRho, bd.nTheta), dtype=torch.float32, device=dev)
     x, flags = bd._prepare(plan, pats)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     plan.radon(x, flags=flags, out=rad)
@@ -144,7 +147,7 @@ def main():
         ("u8" if esz == 1 else "f32"), "data": "synthetic",
         "config": {"workload": f"plain find_bands, {N} x {H}x{W} {args.dtype} FCC patterns, "
                                f"{'background subtraction, ' if bg is not None else ''}Radon 180x90, "
-                               f"kernel 9x5, 9 bands", "global_batch": N, "parallelism": "none"},
+                               f"kernel 9x5, 9 bands", "global_batch": N, "chunk": chunk, "parallelism": "none"},
         "roofline": {"bound": "lds", "kernel": "k_radon_gather", "achieved": round(lds_gbs, 1),
                      "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": lds_bytes,

@@ -135,7 +135,7 @@ class DevicePlan:
         nbytes = int(self.lib.ebsd_workspace_bytes(self.handle, n))
         return torch.empty(nbytes // 4, dtype=torch.float32, device=self.device)
 
-    def find_bands(self, pats, out=None, work=None, chunk=8192, stream=None, flags=0):
+    def find_bands(self, pats, out=None, work=None, chunk=2048, stream=None, flags=0):
         """``ebsd_find_bands`` on device patterns; returns the dict of device outputs."""
         import torch
         pats, code = self._check_pats(pats)
@@ -506,7 +506,7 @@ class BandDetect:
             x = x - torch.as_tensor(np.asarray(bg, dtype=np.float64), device=pats.device)
         return x.to(torch.float32), _hip.EBSD_FLAG_NOBG
 
-    def find_bands_device(self, pats, gpu_id=None, out=None, work=None, chunk=8192):
+    def find_bands_device(self, pats, gpu_id=None, out=None, work=None, chunk=2048):
         """find_bands on a device tensor (N,H,W); returns a dict of device tensors."""
         plan = self.device_plan(gpu_id)
         pats = pats.unsqueeze(0) if pats.dim() == 2 else pats
@@ -534,7 +534,7 @@ class BandDetect:
         if pats.dim() == 2:
             pats = pats.unsqueeze(0)
         N = pats.shape[0]
-        chunk = chunksize if chunksize and chunksize > 0 else 8192
+        chunk = chunksize if chunksize and chunksize > 0 else 2048
         with torch.cuda.device(plan.device):
             x, flags = self._prepare(plan, pats)
             out = plan.find_bands(x, chunk=chunk, flags=flags)

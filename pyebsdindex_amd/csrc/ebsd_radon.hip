@@ -475,13 +475,93 @@ __device__ __noinline__ void select_top(int tid, int nthr, int nB, int *ctr, flo
     __syncthreads();
 }
 
+// Geometry of theta tile t: interior columns [hc0, hc1) of the padded Radon, convolved
+// columns [cc0, cc1) (dilation halo), staged columns starting at cb0 (convolution halo).
+struct TileGeo {
+    int hc0, hc1, HW, cc0, cc1, CW, cb0, RW, hoff;
+};
+__device__ __forceinline__ TileGeo tile_geo(const PeakGeo &g, int t, int ht, int ckt)
+{
+    TileGeo T;
+    const int j0 = t * g.tile, j1 = min(j0 + g.tile, g.nT);
+    T.hc0 = g.p1 + j0;
+    T.hc1 = g.p1 + j1;
+    T.HW = T.hc1 - T.hc0;
+    T.cc0 = max(T.hc0 - ht, 0);
+    T.cc1 = min(T.hc1 + ht, g.nTp);
+    T.CW = T.cc1 - T.cc0;
+    T.cb0 = T.cc0 - ckt;
+    T.RW = T.CW + 2 * ckt;
+    T.hoff = T.hc0 - ht - T.cc0;
+    return T;
+}
+
+constexpr int PF_REGS = 16; // staged elements per thread held in registers across a tile
+
+// Padded Radon value of staged element `it` of a tile (zero outside the padded array).
+__device__ __forceinline__ float tile_elem(const PeakGeo &g, const float *R, const TileGeo &T, int y, int x)
+{
+    const int r = g.cr0 - g.kr / 2 + y, c = T.cb0 + x;
+    if (r >= 0 && r < g.nRp && c >= 0 && c < g.nTp) return __builtin_nontemporal_load(R + pad_src(g, r, c));
+    return 0.0f;
+}
+
+__device__ __forceinline__ void tile_fetch(const PeakGeo &g, const float *R, const TileGeo &T, int RH, int tid,
+                                           int nthr, float *pf)
+{
+    const FDiv f = fdiv_make(T.RW);
+#pragma unroll
+    for (int u = 0; u < PF_REGS; ++u) {
+        const int it = tid + u * nthr;
+        pf[u] = 0.0f;
+        if (it < RH * T.RW) {
+            const int y = (int)fdiv_q(it, f), x = it - y * T.RW;
+            pf[u] = tile_elem(g, R, T, y, x);
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_commit(const PeakGeo &g, const float *R, const TileGeo &T, int RH, int tid,
+                                            int nthr, const float *pf, float *A, int AP)
+{
+    const FDiv f = fdiv_make(T.RW);
+#pragma unroll
+    for (int u = 0; u < PF_REGS; ++u) {
+        const int it = tid + u * nthr;
+        if (it < RH * T.RW) {
+            const int y = (int)fdiv_q(it, f), x = it - y * T.RW;
+            A[y * AP + x] = pf[u];
+        }
+    }
+    // elements beyond the register window (large Radons), 8 loads in flight
+    constexpr int U = 8;
+    for (int it0 = tid + PF_REGS * nthr; it0 < RH * T.RW; it0 += U * nthr) {
+        float val[U];
+        int dst[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int it = it0 + u * nthr;
+            dst[u] = -1;
+            val[u] = 0.0f;
+            if (it < RH * T.RW) {
+                const int y = (int)fdiv_q(it, f), x = it - y * T.RW;
+                dst[u] = y * AP + x;
+                val[u] = tile_elem(g, R, T, y, x);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (dst[u] >= 0) A[dst[u]] = val[u];
+    }
+}
+
 constexpr int CSTRIP = 8; // rows per thread in the rho passes
 constexpr int CGRP = 4;   // columns per thread in the theta passes
 
 // KR/KT > 0: compile-time kernel sizes (dilation half-windows equal to them, as
 // band_detect.py:287-288 derives peakPad = 2*ksz+1); 0: run-time sizes.
 template <int KR, int KT>
-__global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_peaks(PeakArgs a)
 {
     extern __shared__ __align__(16) float sm[];
     const PeakGeo &g = a.g;
@@ -525,6 +605,12 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
         }
     }
 
+    // the padded-Radon block of the next tile is loaded into registers while the current
+    // tile is processed (the Radon of a pattern comes from HBM/MALL)
+    float pf[PF_REGS];
+    if ((int64_t)blockIdx.x < a.N)
+        tile_fetch(g, a.radon + (int64_t)blockIdx.x * nCells, tile_geo(g, 0, ht, ckt), RH, tid, nthr, pf);
+
     for (int64_t q = blockIdx.x; q < a.N; q += gridDim.x) {
         const float *R = a.radon + q * (int64_t)nCells;
         float smin = INFINITY, smax = -INFINITY;
@@ -533,38 +619,20 @@ __global__ __launch_bounds__(512) void k_peaks(PeakArgs a)
         __syncthreads();
 
         for (int t = 0; t < g.ntiles; ++t) {
-            const int j0 = t * g.tile;
-            const int j1 = min(j0 + g.tile, g.nT);
-            const int hc0 = g.p1 + j0, hc1 = g.p1 + j1, HW = hc1 - hc0;
-            const int cc0 = max(hc0 - ht, 0), cc1 = min(hc1 + ht, g.nTp), CW = cc1 - cc0;
-            const int rb0 = g.cr0 - ckr;
-            const int cb0 = cc0 - ckt, RW = CW + 2 * ckt;
-            const int hoff = hc0 - ht - cc0; // 0 unless the theta window is clipped
+            const TileGeo TG = tile_geo(g, t, ht, ckt);
+            const int hc0 = TG.hc0, hc1 = TG.hc1, HW = TG.HW;
+            const int cc0 = TG.cc0, CW = TG.CW;
+            const int hoff = TG.hoff; // 0 unless the theta window is clipped
 
-            // ---- stage padded Radon tile (zero outside the padded array); 8 loads in
-            //      flight per thread ----
-            {
-                const FDiv f = fdiv_make(RW);
-                constexpr int U = 8;
-                for (int it0 = tid; it0 < RH * RW; it0 += U * nthr) {
-                    float val[U];
-                    int dst[U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int it = it0 + u * nthr;
-                        const int y = (int)fdiv_q(it, f), x = it - y * RW;
-                        const int r = rb0 + y, c = cb0 + x;
-                        dst[u] = it < RH * RW ? y * AP + x : -1;
-                        val[u] = 0.0f;
-                        if (it < RH * RW && r >= 0 && r < g.nRp && c >= 0 && c < g.nTp)
-                            val[u] = __builtin_nontemporal_load(R + pad_src(g, r, c));
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        if (dst[u] >= 0) A[dst[u]] = val[u];
-                }
-            }
+            // ---- stage the prefetched padded Radon tile, then start fetching the next ----
+            tile_commit(g, R, TG, RH, tid, nthr, pf, A, AP);
             __syncthreads();
+            {
+                int tn = t + 1;
+                int64_t qn = q;
+                if (tn == g.ntiles) { tn = 0; qn = q + gridDim.x; }
+                if (qn < a.N) tile_fetch(g, a.radon + qn * (int64_t)nCells, tile_geo(g, tn, ht, ckt), RH, tid, nthr, pf);
+            }
             // ---- theta pass: T[y][x] = sum_b v[b] * A[y][x + kt - 1 - b], 4 columns/thread ----
             {
                 const int G = (CW + CGRP - 1) / CGRP;
