@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=5)
+    ap.add_argument("--pmc-driver", action="store_true",
+                    help="short fixed launch sequence for rocprofv3 --pmc: gather x3, peaks x3 on one "
+                         "chunk, then a 256 MiB device copy x3 (the counter calibration)")
     ap.add_argument("--chunk", type=int, default=2048,
                     help="patterns per Radon+peak pass (the Radon of a chunk stays in the MALL)")
     return ap.parse_args()
@@ -87,6 +90,24 @@ def main():
     chunk = max(1, min(args.chunk, N))
     work = plan.workspace(N, chunk)
     print(f"[bench] plan {info}", file=sys.stderr, flush=True)
+
+    if args.pmc_driver:
+        x, flags = bd._prepare(plan, pats[:chunk])
+        rad = torch.empty((chunk, bd.nRho, bd.nTheta), dtype=torch.float32, device=dev)
+        o2 = plan.alloc_out(chunk)
+        for _ in range(3):
+            plan.radon(x, flags=flags, out=rad)
+        for _ in range(3):
+            plan.peaks(rad, out=o2)
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_driver": True, "chunk": chunk, "radon_bytes_per_gather": rad.numel() * 4,
+                          "input_bytes_per_gather": x.numel() * x.element_size(),
+                          "copy_bytes": src.numel() * 4}), flush=True)
+        return
 
     for _ in range(args.warmup):
         bd.find_bands_device(pats, out=out, work=work, chunk=chunk)
