@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--pmc-driver", action="store_true",
                     help="short fixed launch sequence for rocprofv3 --pmc: gather x3, peaks x3 on one "
                          "chunk, then a 256 MiB device copy x3 (the counter calibration)")
+    ap.add_argument("--tune", default="", help="gather_pack,gather_threads,peak_tile,peak_threads (0 = keep)")
     ap.add_argument("--chunk", type=int, default=2048,
                     help="patterns per Radon+peak pass (the Radon of a chunk stays in the MALL)")
     return ap.parse_args()
@@ -84,6 +85,8 @@ def main():
         bd.collect_background(patsIn=sample.astype(np.float64), nsample=1024)
         bg = bd.backgroundsub
     plan = bd.device_plan()
+    if args.tune:
+        plan.set_tuning(*[int(v) for v in args.tune.split(",")])
     info = plan.info()
     N = pats.shape[0]
     out = plan.alloc_out(N)

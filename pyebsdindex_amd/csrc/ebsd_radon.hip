@@ -74,6 +74,10 @@ struct ebsd_plan {
     std::vector<int> h_cnt;      // per cell entry count
     std::vector<int64_t> h_off;  // per cell first entry in h_ent
     std::vector<uint16_t> h_ent; // per cell valid entries, plan order
+    std::vector<uint16_t> h_groups; // batch-major plan groups [block][lane][8]
+    std::vector<uint2> h_bhdr;   // per batch: first block in h_groups, number of blocks
+    int stream_waves = 0;        // waves per work-group the device streams are laid out for
+    int *d_wofs = nullptr, *d_wcnt = nullptr;
     // background
     int bg_mode = 0;             // 0 none, 1 per-line sums, 2 exact per-pixel
     double *d_linebg = nullptr;  // nCells
@@ -95,9 +99,11 @@ struct GatherArgs {
     int64_t N;
     int nPx;
     int nbatch;
-    const uint4 *gidx;
-    const uint2 *bhdr;
-    const int2 *lane;
+    const uint4 *gidx;    // per-wave streams of plan groups
+    const uint2 *bhdr;    // per batch: (unused, groups)
+    const int2 *lane;     // per batch lane: (cell, count)
+    const int *wofs;      // per wave: first group of its stream
+    const int *wcnt;      // per wave: groups in its stream
     const double *linebg; // fast background mode
     const double *bg;     // exact background mode
     float *out;           // (N, nCells)
@@ -281,54 +287,57 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
         __syncthreads();
 
         // ---- gather: each lane owns one line of the wave's current 64-line batch; the
-        //      next plan group (or the next batch's first group) is always in flight ----
+        //      wave walks its stream of plan groups with three groups in flight ----
         int b = wave;
         if (b < nb) {
-            uint2 h = a.bhdr[b];
-            int bn = b + nwaves;
-            uint2 hn = a.bhdr[bn < nb ? bn : nb - 1];
+            const uint4 *S = a.gidx + (size_t)a.wofs[wave] * 64 + lane;
+            const int G = a.wcnt[wave];
+            uint4 e0 = S[0], e1 = S[64], e2 = S[128]; // filler past the end keeps these in bounds
+            int bend = (int)a.bhdr[b].y;
             int2 li = a.lane[(size_t)b * 64 + lane];
-            uint4 e = gbase[(size_t)h.x * 64];
-            for (;;) {
-                const int ngrp = (int)h.y;
-                int2 lin = a.lane[(size_t)(bn < nb ? bn : b) * 64 + lane];
-                uint32_t acc[P];
-                double accd[P];
+            int bn = b + nwaves;
+            int2 lin = li;
+            int ngn = 0;
+            if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
+            uint32_t acc[P];
+            double accd[P];
 #pragma unroll
-                for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
-                for (int gi = 0; gi < ngrp; ++gi) {
-                    const size_t nxt = (gi + 1 < ngrp) ? (size_t)(h.x + gi + 1) : (size_t)hn.x;
-                    const uint4 en = gbase[nxt * 64];
-                    gather_group<IN, FLT, P>(lbase, e, acc, accd);
-                    e = en;
-                }
-                if (ngrp == 0) e = gbase[(size_t)hn.x * 64];
-                if (li.x >= 0) {
-                    const double cnt = (double)li.y;
+            for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+            for (int gi = 0; gi < G; ++gi) {
+                const uint4 e3 = S[(size_t)(gi + 3) * 64];
+                gather_group<IN, FLT, P>(lbase, e0, acc, accd);
+                e0 = e1;
+                e1 = e2;
+                e2 = e3;
+                if (gi + 1 == bend) {
+                    if (li.x >= 0) {
+                        const double cnt = (double)li.y;
 #pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        const int64_t q = q0 + p;
-                        if (q < a.N) {
-                            float r;
-                            if (li.y == 0) {
-                                r = -1.0f;
-                            } else if constexpr (FLT) {
-                                r = (float)(accd[p] / cnt);
-                            } else {
-                                double s = (double)acc[p];
-                                if (a.linebg) s -= a.linebg[li.x];
-                                r = (float)(s / cnt);
+                        for (int p = 0; p < P; ++p) {
+                            const int64_t q = q0 + p;
+                            if (q < a.N) {
+                                float r;
+                                if (li.y == 0) {
+                                    r = -1.0f;
+                                } else if constexpr (FLT) {
+                                    r = (float)(accd[p] / cnt);
+                                } else {
+                                    double sv = (double)acc[p];
+                                    if (a.linebg) sv -= a.linebg[li.x];
+                                    r = (float)(sv / cnt);
+                                }
+                                a.out[q * (int64_t)a.nCells + li.x] = r;
                             }
-                            a.out[q * (int64_t)a.nCells + li.x] = r;
                         }
                     }
+#pragma unroll
+                    for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+                    b = bn;
+                    li = lin;
+                    bend += ngn;
+                    bn = b + nwaves;
+                    if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
                 }
-                b = bn;
-                if (b >= nb) break;
-                h = hn;
-                li = lin;
-                bn = b + nwaves;
-                hn = a.bhdr[bn < nb ? bn : nb - 1];
             }
         }
         __syncthreads();
@@ -1003,6 +1012,8 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_gidx);
     (void)hipFree(p->d_bhdr);
     (void)hipFree(p->d_lane);
+    (void)hipFree(p->d_wofs);
+    (void)hipFree(p->d_wcnt);
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     (void)hipFree(p->d_mask);
@@ -1040,6 +1051,43 @@ static int geo_tiles(ebsd_plan *p)
     }
     if ((size_t)peak_lds_layout(g).total * 4 > (size_t)p->lds_max)
         return fail(-2, "Radon too large for the peak kernel's LDS tiling");
+    return 0;
+}
+
+// Lay the plan groups out as one contiguous stream per wave slot: wave w of a work-group
+// with nw waves walks batches w, w + nw, w + 2nw, ... back to back, so its next groups can
+// be prefetched several iterations ahead without looking at batch boundaries.
+static int layout_streams(ebsd_plan *p, int nw)
+{
+    if (nw == p->stream_waves && p->d_gidx) return 0;
+    std::vector<uint16_t> st;
+    std::vector<int> wofs(nw), wcnt(nw);
+    st.reserve(p->h_groups.size() + (size_t)(nw + 1) * 64 * 8);
+    int ofs = 0;
+    for (int w = 0; w < nw; ++w) {
+        wofs[w] = ofs;
+        int cnt = 0;
+        for (int b = w; b < p->nbatch; b += nw) {
+            const uint2 h = p->h_bhdr[b];
+            const size_t from = (size_t)h.x * 64 * 8;
+            st.insert(st.end(), p->h_groups.begin() + from, p->h_groups.begin() + from + (size_t)h.y * 64 * 8);
+            cnt += (int)h.y;
+        }
+        wcnt[w] = cnt;
+        ofs += cnt;
+    }
+    // filler groups past the end keep the prefetch queue's reads in bounds
+    st.resize(st.size() + 4 * 64 * 8, (uint16_t)p->nPx);
+    (void)hipFree(p->d_gidx);
+    (void)hipFree(p->d_wofs);
+    (void)hipFree(p->d_wcnt);
+    p->d_gidx = nullptr;
+    p->d_wofs = p->d_wcnt = nullptr;
+    int rc;
+    if ((rc = dev_upload(reinterpret_cast<uint16_t **>(&p->d_gidx), st.data(), st.size()))) return rc;
+    if ((rc = dev_upload(&p->d_wofs, wofs.data(), wofs.size()))) return rc;
+    if ((rc = dev_upload(&p->d_wcnt, wcnt.data(), wcnt.size()))) return rc;
+    p->stream_waves = nw;
     return 0;
 }
 
@@ -1111,7 +1159,7 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
             const int i = b * 64 + l;
             if (i < nCells) mx = std::max(mx, p->h_cnt[order[i]]);
         }
-        const int ngrp = (mx + 7) / 8;
+        const int ngrp = std::max(1, (mx + 7) / 8); // an empty batch still writes its -1 cells
         bhdr[b] = make_uint2(blocks, (uint32_t)ngrp);
         gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)nPx);
         for (int l = 0; l < 64; ++l) {
@@ -1126,9 +1174,9 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
         }
         blocks += (uint32_t)ngrp;
     }
-    // one filler block past the end: every batch (even an empty one) has a readable first block
-    gidx.resize(gidx.size() + 64 * 8, (uint16_t)nPx);
-    if ((rc = dev_upload(reinterpret_cast<uint16_t **>(&p->d_gidx), gidx.data(), gidx.size()))) { plan_free(p); return rc; }
+    p->h_groups.swap(gidx);
+    p->h_bhdr = bhdr;
+    if ((rc = layout_streams(p, p->gthreads / 64))) { plan_free(p); return rc; }
     if ((rc = dev_upload(&p->d_bhdr, bhdr.data(), bhdr.size()))) { plan_free(p); return rc; }
     if ((rc = dev_upload(&p->d_lane, lanes.data(), lanes.size()))) { plan_free(p); return rc; }
 
@@ -1216,6 +1264,8 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
     if (gather_threads > 0) {
         if (gather_threads % 64 || gather_threads > 1024) return fail(-1, "gather_threads must be a multiple of 64 <= 1024");
         p->gthreads = gather_threads;
+        int rc = layout_streams(p, gather_threads / 64);
+        if (rc) return rc;
     }
     if (peak_threads > 0) {
         if (peak_threads % 64 || peak_threads > 512) return fail(-1, "peak_threads must be a multiple of 64 <= 512");
@@ -1267,6 +1317,9 @@ static int launch_gather_t(ebsd_plan *p, const void *d_pats, int64_t N, float *d
     a.gidx = p->d_gidx;
     a.bhdr = p->d_bhdr;
     a.lane = p->d_lane;
+    a.wofs = p->d_wofs;
+    a.wcnt = p->d_wcnt;
+    if (p->stream_waves != p->gthreads / 64) return fail(-3, "plan streams laid out for another work-group size");
     a.linebg = (use_bg && !FLT && p->bg_mode == 1) ? p->d_linebg : nullptr;
     a.bg = (use_bg && FLT) ? p->d_bg : nullptr;
     a.out = d_out;
