@@ -184,18 +184,26 @@ __device__ __forceinline__ void accumulate_word(const uint32_t *d, uint32_t *acc
     }
 }
 
+// The LDS words of one plan group (8 entries of a line) are issued one group ahead of
+// their accumulation, so the LDS and VALU work of consecutive groups overlap in a wave.
 template <int IN, bool FLT, int P>
-__device__ __forceinline__ void gather_group(uint32_t base, const uint4 &e, uint32_t *acc, double *accd)
+__device__ __forceinline__ void group_issue(uint32_t base, const uint4 &e,
+                                            typename WordT<GatherCfg<IN, FLT, P>::WB>::T *w)
 {
     typedef GatherCfg<IN, FLT, P> C;
     typedef typename WordT<C::WB>::T Word;
     const uint32_t ew[4] = {e.x, e.y, e.z, e.w};
-    Word w[8];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         w[2 * k] = lds_load<Word>(entry_addr_lo<C::WB>(ew[k], base));
         w[2 * k + 1] = lds_load<Word>(entry_addr_hi<C::WB>(ew[k], base));
     }
+}
+
+template <int IN, bool FLT, int P>
+__device__ __forceinline__ void group_accumulate(const typename WordT<GatherCfg<IN, FLT, P>::WB>::T *w,
+                                                 uint32_t *acc, double *accd)
+{
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         accumulate_word<IN, FLT, P>(reinterpret_cast<const uint32_t *>(&w[k]), acc, accd);
@@ -303,9 +311,14 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
             double accd[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+            Word wc[8], wn[8];
+            group_issue<IN, FLT, P>(lbase, e0, wc);
             for (int gi = 0; gi < G; ++gi) {
                 const uint4 e3 = S[(size_t)(gi + 3) * 64];
-                gather_group<IN, FLT, P>(lbase, e0, acc, accd);
+                group_issue<IN, FLT, P>(lbase, e1, wn); // next group's words (filler past the end)
+                group_accumulate<IN, FLT, P>(wc, acc, accd);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) wc[k] = wn[k];
                 e0 = e1;
                 e1 = e2;
                 e2 = e3;
@@ -651,9 +664,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     const float *row = A + y * AP + x0;
                     float o[CGRP];
                     if constexpr (KT > 0) {
-                        float w[CGRP + KT - 1];
+                        // 16-byte aligned reads (AP and x0 are multiples of 4): conflict-free
+                        constexpr int NV = (CGRP + KT - 1 + 3) / 4;
+                        float w[4 * NV];
 #pragma unroll
-                        for (int k = 0; k < CGRP + KT - 1; ++k) w[k] = row[k];
+                        for (int k = 0; k < NV; ++k) {
+                            const float4 v4 = reinterpret_cast<const float4 *>(row)[k];
+                            w[4 * k] = v4.x; w[4 * k + 1] = v4.y; w[4 * k + 2] = v4.z; w[4 * k + 3] = v4.w;
+                        }
 #pragma unroll
                         for (int c = 0; c < CGRP; ++c) {
                             float s = 0.0f;
@@ -729,10 +747,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     const float *row = A + y * AP + x0 + hoff;
                     float o[CGRP];
                     if constexpr (KT > 0) {
-                        // windows w[c .. c + 2KT]; all four share w[3 .. 2KT]
-                        float w[CGRP + 2 * KT];
+                        // (hoff is 0: the padding p1 > ht keeps theta windows inside the array)
+                        // windows w[c .. c + 2KT]; all four share w[3 .. 2KT]; 16-byte reads
+                        constexpr int NV = (CGRP + 2 * KT + 3) / 4;
+                        float w[4 * NV];
 #pragma unroll
-                        for (int k = 0; k < CGRP + 2 * KT; ++k) w[k] = row[k];
+                        for (int k = 0; k < NV; ++k) {
+                            const float4 v4 = reinterpret_cast<const float4 *>(row)[k];
+                            w[4 * k] = v4.x; w[4 * k + 1] = v4.y; w[4 * k + 2] = v4.z; w[4 * k + 3] = v4.w;
+                        }
                         float mid = w[CGRP - 1];
 #pragma unroll
                         for (int k = CGRP; k <= 2 * KT; ++k) mid = fmaxf(mid, w[k]);
@@ -1043,7 +1066,8 @@ static int geo_tiles(ebsd_plan *p)
         g.tile = tile;
         g.ntiles = (g.nT + tile - 1) / tile;
         const int cw = tile + 2 * g.ht;
-        g.apitch = ((cw + 2 * (g.kt / 2) + 3) / 4) * 4 + 4;
+        // the theta passes read whole 16-byte groups past the row end: 2*4 floats of slack
+        g.apitch = ((cw + 2 * (g.kt / 2) + 3) / 4) * 4 + 8;
         g.bpitch = ((cw + 3) / 4) * 4;
         const size_t bytes = (size_t)peak_lds_layout(g).total * 4;
         if ((int)bytes <= p->lds_max || tile == 1) break;
