@@ -19,6 +19,7 @@ HIPCC_FLAGS = [
     # float contraction only where the source asks for fmaf(): keeps the f64 parity
     # arithmetic (sub-pixel fit, np.interp restatement) free of fused operations
     "-ffp-contract=off",
+    "-fno-strict-aliasing",
     # no packed-FP32 VALU instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]

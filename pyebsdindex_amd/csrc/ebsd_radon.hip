@@ -204,9 +204,13 @@ template <int IN, bool FLT, int P>
 __device__ __forceinline__ void group_accumulate(const typename WordT<GatherCfg<IN, FLT, P>::WB>::T *w,
                                                  uint32_t *acc, double *accd)
 {
+    typedef GatherCfg<IN, FLT, P> C;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        accumulate_word<IN, FLT, P>(reinterpret_cast<const uint32_t *>(&w[k]), acc, accd);
+    for (int k = 0; k < 8; ++k) {
+        uint32_t d[C::ND];
+        __builtin_memcpy(d, &w[k], sizeof(d)); // no type punning through pointers
+        accumulate_word<IN, FLT, P>(d, acc, accd);
+    }
 }
 
 // Build the LDS word of pixel px from element values x[p] (p < P), as packed integers or
