@@ -304,7 +304,8 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
         if (b < nb) {
             const uint4 *S = a.gidx + (size_t)a.wofs[wave] * 64 + lane;
             const int G = a.wcnt[wave];
-            uint4 e0 = S[0], e1 = S[64], e2 = S[128]; // filler past the end keeps these in bounds
+            // four plan groups in flight; each register is refilled right after use
+            uint4 eq[4] = {S[0], S[64], S[128], S[192]}; // filler past the end keeps these in bounds
             int bend = (int)a.bhdr[b].y;
             int2 li = a.lane[(size_t)b * 64 + lane];
             int bn = b + nwaves;
@@ -315,45 +316,45 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
             double accd[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
-            Word wc[8], wn[8];
-            group_issue<IN, FLT, P>(lbase, e0, wc);
-            for (int gi = 0; gi < G; ++gi) {
-                const uint4 e3 = S[(size_t)(gi + 3) * 64];
-                group_issue<IN, FLT, P>(lbase, e1, wn); // next group's words (filler past the end)
-                group_accumulate<IN, FLT, P>(wc, acc, accd);
+            for (int g0 = 0; g0 < G; g0 += 4) {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) wc[k] = wn[k];
-                e0 = e1;
-                e1 = e2;
-                e2 = e3;
-                if (gi + 1 == bend) {
-                    if (li.x >= 0) {
-                        const double cnt = (double)li.y;
+                for (int u = 0; u < 4; ++u) {
+                    const int gi = g0 + u;
+                    if (gi < G) {
+                        Word w[8];
+                        group_issue<IN, FLT, P>(lbase, eq[u], w);
+                        eq[u] = S[(size_t)(gi + 4) * 64];
+                        group_accumulate<IN, FLT, P>(w, acc, accd);
+                        if (gi + 1 == bend) {
+                            if (li.x >= 0) {
+                                const double cnt = (double)li.y;
 #pragma unroll
-                        for (int p = 0; p < P; ++p) {
-                            const int64_t q = q0 + p;
-                            if (q < a.N) {
-                                float r;
-                                if (li.y == 0) {
-                                    r = -1.0f;
-                                } else if constexpr (FLT) {
-                                    r = (float)(accd[p] / cnt);
-                                } else {
-                                    double sv = (double)acc[p];
-                                    if (a.linebg) sv -= a.linebg[li.x];
-                                    r = (float)(sv / cnt);
+                                for (int p = 0; p < P; ++p) {
+                                    const int64_t q = q0 + p;
+                                    if (q < a.N) {
+                                        float r;
+                                        if (li.y == 0) {
+                                            r = -1.0f;
+                                        } else if constexpr (FLT) {
+                                            r = (float)(accd[p] / cnt);
+                                        } else {
+                                            double sv = (double)acc[p];
+                                            if (a.linebg) sv -= a.linebg[li.x];
+                                            r = (float)(sv / cnt);
+                                        }
+                                        a.out[q * (int64_t)a.nCells + li.x] = r;
+                                    }
                                 }
-                                a.out[q * (int64_t)a.nCells + li.x] = r;
                             }
+#pragma unroll
+                            for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+                            b = bn;
+                            li = lin;
+                            bend += ngn;
+                            bn = b + nwaves;
+                            if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
                         }
                     }
-#pragma unroll
-                    for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
-                    b = bn;
-                    li = lin;
-                    bend += ngn;
-                    bn = b + nwaves;
-                    if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
                 }
             }
         }
@@ -1105,7 +1106,7 @@ static int layout_streams(ebsd_plan *p, int nw)
         ofs += cnt;
     }
     // filler groups past the end keep the prefetch queue's reads in bounds
-    st.resize(st.size() + 4 * 64 * 8, (uint16_t)p->nPx);
+    st.resize(st.size() + 8 * 64 * 8, (uint16_t)p->nPx);
     (void)hipFree(p->d_gidx);
     (void)hipFree(p->d_wofs);
     (void)hipFree(p->d_wcnt);
