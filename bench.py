@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--no-background", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--roofline-reps", type=int, default=5)
+    ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--pmc-driver", action="store_true",
                     help="short fixed launch sequence for rocprofv3 --pmc: gather x3, peaks x3 on one "
                          "chunk, then a 256 MiB device copy x3 (the counter calibration)")
@@ -125,9 +125,11 @@ def main():
     print(f"[bench] find_bands {ms:.3f} ms/step, {value:.1f} patterns/s", file=sys.stderr, flush=True)
 
     # ---- per-kernel durations with HIP events on the launch stream ----
+    # one launch = one chunk, as inside find_bands
     s = torch.cuda.current_stream(dev)
-    rad = torch.empty((N, bd.nRho, bd.nTheta), dtype=torch.float32, device=dev)
-    x, flags = bd._prepare(plan, pats)
+    rad = torch.empty((chunk, bd.nRho, bd.nTheta), dtype=torch.float32, device=dev)
+    x, flags = bd._prepare(plan, pats[:chunk])
+    out_c = plan.alloc_out(chunk)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     plan.radon(x, flags=flags, out=rad)
     torch.cuda.synchronize()
@@ -136,13 +138,13 @@ def main():
         plan.radon(x, flags=flags, out=rad)
     ev[1].record(s)
     for _ in range(args.roofline_reps):
-        plan.peaks(rad, out=out)
+        plan.peaks(rad, out=out_c)
     ev[2].record(s)
     torch.cuda.synchronize()
     gather_ms = ev[0].elapsed_time(ev[1]) / args.roofline_reps
     peaks_ms = ev[1].elapsed_time(ev[2]) / args.roofline_reps
     pairs = int(info["pairs"])
-    lds_bytes = 4.0 * pairs * N                       # algorithmic LDS bytes per gather launch
+    lds_bytes = 4.0 * pairs * chunk                   # algorithmic LDS bytes per gather launch
     lds_gbs = lds_bytes / (gather_ms * 1e-3) / 1e9
     esz = {"uint8": 1, "uint16": 2, "float32": 4}[args.dtype]
     hbm_bytes = N * (H * W * esz + 9 * 28 + 12)       # input + band output per launch
@@ -178,7 +180,8 @@ def main():
                      "avg_launch_ms": round(gather_ms, 4), "pairs_per_pattern": pairs},
         "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_step": hbm_bytes},
-        "kernels_ms": {"k_radon_gather": round(gather_ms, 4), "k_peaks": round(peaks_ms, 4)},
+        "kernels_ms_per_chunk": {"k_radon_gather": round(gather_ms, 4), "k_peaks": round(peaks_ms, 4),
+                                 "chunk": chunk},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
