@@ -150,10 +150,12 @@ def main():
     hbm_bytes = N * (H * W * esz + 9 * 28 + 12)       # input + band output per launch
     hbm_gbs = hbm_bytes / (ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_radon_gather_{H}x{W}_{args.dtype}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_radon_gather_{H}x{W}_{args.dtype}.json")  # 2,048-pattern launches
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("patterns_per_launch") == chunk:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
