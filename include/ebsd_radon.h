@@ -99,6 +99,21 @@ int ebsd_peaks(ebsd_plan *plan, const float *d_radon, int64_t N, const ebsd_band
  * Replaces: the rdnConv array of rdn_conv (band_detect.py:562-637). */
 int ebsd_conv(ebsd_plan *plan, const float *d_radon, int64_t N, float *d_conv, void *stream);
 
+/* Batches of mixed pattern sizes (each set with its own plan): one persistent Radon
+ * launch and one persistent peak launch over heterogeneous work lists.  The batch binds the
+ * given device pointers; ebsd_batch_graph_launch captures both launches into a hipGraph on
+ * its first call (on a non-default stream) and replays it afterwards.
+ * Replaces: calling find_bands once per pattern size (one BandDetect/plan per size, as
+ * EBSDIndexer re-plans on a size change, pyebsdindex/_ebsd_index_single.py:696-700). */
+typedef struct ebsd_batch ebsd_batch;
+size_t ebsd_batch_workspace_bytes(int nsets, ebsd_plan *const *plans, const int64_t *N);
+int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void *const *d_pats, const int *dtypes,
+                      const int64_t *N, const ebsd_bands *outs, void *d_work, size_t ws_bytes,
+                      ebsd_batch **out);
+int ebsd_batch_run(ebsd_batch *batch, void *stream);
+int ebsd_batch_graph_launch(ebsd_batch *batch, void *stream);
+int ebsd_batch_destroy(ebsd_batch *batch);
+
 /* Plan geometry queries. */
 int ebsd_plan_info(const ebsd_plan *plan, int64_t *out8);
 

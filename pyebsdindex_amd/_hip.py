@@ -16,7 +16,8 @@ EBSD_U8, EBSD_U16, EBSD_F32 = 0, 1, 2
 SYMBOLS = (
     "ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background", "ebsd_plan_set_tuning",
     "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
-    "ebsd_plan_info", "ebsd_last_error",
+    "ebsd_plan_info", "ebsd_last_error", "ebsd_batch_workspace_bytes", "ebsd_batch_create",
+    "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy",
 )
 
 
@@ -61,11 +62,18 @@ def load(path: str = LIB_PATH):
         L.ebsd_peaks.argtypes = [P, P, I64, ctypes.POINTER(EbsdBands), P]
         L.ebsd_conv.argtypes = [P, P, I64, P, P]
         L.ebsd_plan_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
+        L.ebsd_batch_workspace_bytes.argtypes = [I, P, P]
+        L.ebsd_batch_workspace_bytes.restype = ctypes.c_size_t
+        L.ebsd_batch_create.argtypes = [I, P, P, P, P, P, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.ebsd_batch_run.argtypes = [P, P]
+        L.ebsd_batch_graph_launch.argtypes = [P, P]
+        L.ebsd_batch_destroy.argtypes = [P]
         L.ebsd_last_error.argtypes = []
         L.ebsd_last_error.restype = ctypes.c_char_p
         for name in ("ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background",
                      "ebsd_plan_set_tuning", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks",
-                     "ebsd_conv", "ebsd_plan_info"):
+                     "ebsd_conv", "ebsd_plan_info", "ebsd_batch_create", "ebsd_batch_run",
+                     "ebsd_batch_graph_launch", "ebsd_batch_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
         return L

@@ -113,6 +113,7 @@ struct GatherArgs {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <int WB> struct WordT;
+template <> struct WordT<2> { typedef uint16_t T; };
 template <> struct WordT<4> { typedef uint32_t T; };
 template <> struct WordT<8> { typedef uint64_t T; };
 template <> struct WordT<16> { typedef u32x4 T; };
@@ -127,7 +128,7 @@ template <> struct InT<EBSD_F32> { typedef float T; };
 template <int IN, bool FLT, int P> struct GatherCfg {
     static constexpr int EB = FLT ? 4 : (IN == EBSD_U8 ? 1 : 2); // bytes per element
     static constexpr int WB = EB * P;                             // bytes per LDS word
-    static constexpr int ND = WB / 4;                             // dwords per word
+    static constexpr int ND = WB >= 4 ? WB / 4 : 1;               // dwords per word (a 2-byte word in 1)
     static constexpr int VEC = 8 / (int)sizeof(typename InT<IN>::T); // pixels per 8-byte load
 };
 
@@ -205,10 +206,11 @@ __device__ __forceinline__ void group_accumulate(const typename WordT<GatherCfg<
                                                  uint32_t *acc, double *accd)
 {
     typedef GatherCfg<IN, FLT, P> C;
+    typedef typename WordT<C::WB>::T Word;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        uint32_t d[C::ND];
-        __builtin_memcpy(d, &w[k], sizeof(d)); // no type punning through pointers
+        uint32_t d[C::ND] = {};
+        __builtin_memcpy(d, &w[k], sizeof(Word)); // no type punning through pointers
         accumulate_word<IN, FLT, P>(d, acc, accd);
     }
 }
@@ -234,14 +236,15 @@ __device__ __forceinline__ void pack_word(uint32_t *d, const typename InT<IN>::T
     }
 }
 
+// One pack of P patterns: stage them in LDS, then every wave walks its stream of plan
+// groups and writes the P Radon values of each of its lines.
 template <int IN, bool FLT, int P>
-__global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
+__device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, unsigned char *lds_raw)
 {
     typedef GatherCfg<IN, FLT, P> C;
     typedef typename WordT<C::WB>::T Word;
     typedef typename InT<IN>::T Tin;
     constexpr int VEC = C::VEC;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     Word *lds = reinterpret_cast<Word *>(lds_raw);
 
     const Tin *pats = reinterpret_cast<const Tin *>(a.pats);
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
     // LDS address of the staged words (the entries are word indices)
     const uint32_t lbase = (uint32_t)(size_t)(const __attribute__((address_space(3))) unsigned char *)(lds_raw);
 
-    for (int64_t pack = blockIdx.x; pack < a.npacks; pack += gridDim.x) {
+    {
         const int64_t q0 = pack * P;
         // ---- stage P patterns pixel-major (8-byte loads per pattern), zero word at nPx ----
         for (int v = tid; v < nvec; v += blockDim.x) {
@@ -362,10 +365,55 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
     }
 }
 
+template <int IN, bool FLT, int P>
+__global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    for (int64_t pack = blockIdx.x; pack < a.npacks; pack += gridDim.x) gather_pack<IN, FLT, P>(a, pack, lds_raw);
+}
+
+// Variants of the gather a batch may mix (element type, float staging, patterns per word).
+#define EBSD_GATHER_VARIANTS(X)                                                                    \
+    X(0, EBSD_U8, false, 16) X(1, EBSD_U8, false, 8) X(2, EBSD_U8, false, 4) X(3, EBSD_U8, false, 2)  \
+    X(4, EBSD_U16, false, 8) X(5, EBSD_U16, false, 4) X(6, EBSD_U16, false, 2) X(7, EBSD_U16, false, 1) \
+    X(8, EBSD_U8, true, 4) X(9, EBSD_U8, true, 2) X(10, EBSD_U8, true, 1)                          \
+    X(11, EBSD_U16, true, 4) X(12, EBSD_U16, true, 2) X(13, EBSD_U16, true, 1)                     \
+    X(14, EBSD_F32, true, 4) X(15, EBSD_F32, true, 2) X(16, EBSD_F32, true, 1)
+
+template <int IN, bool FLT, int P> struct GatherVariant;
+#define EBSD_GV_ID(ID, IN, FLT, P) \
+    template <> struct GatherVariant<IN, FLT, P> { static constexpr int id = ID; };
+EBSD_GATHER_VARIANTS(EBSD_GV_ID)
+#undef EBSD_GV_ID
+
+struct GatherMultiArgs {
+    const GatherArgs *sets; // per set
+    const int *variant;     // per set
+    const int2 *items;      // (set, pack), largest patterns first
+    int64_t nitems;
+};
+
+// Persistent gather over a heterogeneous work list: each item is one pack of one set.
+__global__ __launch_bounds__(1024) void k_radon_gather_multi(GatherMultiArgs m)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    for (int64_t it = blockIdx.x; it < m.nitems; it += gridDim.x) {
+        const int2 item = m.items[it];
+        const GatherArgs &a = m.sets[item.x];
+        switch (m.variant[item.x]) {
+#define EBSD_GV_CASE(ID, IN, FLT, P) \
+    case ID: gather_pack<IN, FLT, P>(a, item.y, lds_raw); break;
+            EBSD_GATHER_VARIANTS(EBSD_GV_CASE)
+#undef EBSD_GV_CASE
+        default: break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // k_peaks
 // ---------------------------------------------------------------------------------------
-struct PeakArgs {
+struct PeakSet {
     const float *radon; // (N, nR, nT)
     int64_t N;
     const uint8_t *mask; // (nR, nT)
@@ -432,6 +480,19 @@ __host__ __device__ inline PeakLds peak_lds_layout(const PeakGeo &g)
     L.total = L.red + 4 * 32 + 8;  // + counters
     return L;
 }
+
+struct PeakArgs {
+    PeakSet s;
+    PeakLds L;
+};
+
+// Heterogeneous batch: items are (set, pattern); all sets share the kernel sizes.
+struct PeakMultiArgs {
+    const PeakSet *sets;
+    const int2 *items;
+    int64_t nitems;
+    PeakLds L;
+};
 
 __device__ __forceinline__ double np_interp_unit(double x, const double *fp, int n)
 {
@@ -586,13 +647,13 @@ constexpr int CSTRIP = 8; // rows per thread in the rho passes
 constexpr int CGRP = 4;   // columns per thread in the theta passes
 
 // KR/KT > 0: compile-time kernel sizes (dilation half-windows equal to them, as
-// band_detect.py:287-288 derives peakPad = 2*ksz+1); 0: run-time sizes.
-template <int KR, int KT>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_peaks(PeakArgs a)
+// band_detect.py:287-288 derives peakPad = 2*ksz+1); 0: run-time sizes.  MULTI walks a
+// heterogeneous (set, pattern) work list, otherwise the patterns of one set.
+template <int KR, int KT, bool MULTI>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_peaks(PeakArgs a, PeakMultiArgs m)
 {
     extern __shared__ __align__(16) float sm[];
-    const PeakGeo &g = a.g;
-    const PeakLds L = peak_lds_layout(g);
+    const PeakLds L = MULTI ? m.L : a.L;
     float *A = sm + L.a_off;
     float *B = sm + L.b_off;
     uint32_t *mbits = reinterpret_cast<uint32_t *>(sm + L.mask_off);
@@ -610,36 +671,51 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int kr = KR > 0 ? KR : g.kr, kt = KT > 0 ? KT : g.kt;
-    const int hr = KR > 0 ? KR : g.hr, ht = KT > 0 ? KT : g.ht;
-    const int ckr = kr / 2, ckt = kt / 2;
-    const int CH = g.cr1 - g.cr0;
-    const int RH = g.rh;
-    const int AP = g.apitch, BP = g.bpitch;
-    const int nCells = g.nR * g.nT;
-
-    // ---- rho mask as bits (same for every pattern) ----
-    {
-        const FDiv fw = fdiv_make(g.maskw);
-        for (int it = tid; it < g.nR * g.maskw; it += nthr) {
-            const int i = (int)fdiv_q(it, fw), wd = it - i * g.maskw;
-            uint32_t bits = 0u;
-            for (int b = 0; b < 32; ++b) {
-                const int j = wd * 32 + b;
-                if (j < g.nT && a.mask[i * g.nT + j]) bits |= 1u << b;
-            }
-            mbits[it] = bits;
-        }
-    }
+    const int64_t nitems = MULTI ? m.nitems : a.s.N;
+    int cur_set = -1;
 
     // the padded-Radon block of the next tile is loaded into registers while the current
     // tile is processed (the Radon of a pattern comes from HBM/MALL)
     float pf[PF_REGS];
-    if ((int64_t)blockIdx.x < a.N)
-        tile_fetch(g, a.radon + (int64_t)blockIdx.x * nCells, tile_geo(g, 0, ht, ckt), RH, tid, nthr, pf);
+    if ((int64_t)blockIdx.x < nitems) {
+        const int64_t it0 = blockIdx.x;
+        const PeakSet &S0 = MULTI ? m.sets[m.items[it0].x] : a.s;
+        const int64_t q0 = MULTI ? (int64_t)m.items[it0].y : it0;
+        const int ht0 = KT > 0 ? KT : S0.g.ht, ckt0 = (KT > 0 ? KT : S0.g.kt) / 2;
+        tile_fetch(S0.g, S0.radon + q0 * (int64_t)(S0.g.nR * S0.g.nT), tile_geo(S0.g, 0, ht0, ckt0), S0.g.rh,
+                   tid, nthr, pf);
+    }
 
-    for (int64_t q = blockIdx.x; q < a.N; q += gridDim.x) {
-        const float *R = a.radon + q * (int64_t)nCells;
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int sidx = MULTI ? m.items[it].x : 0;
+        const PeakSet &S = MULTI ? m.sets[sidx] : a.s;
+        const int64_t q = MULTI ? (int64_t)m.items[it].y : it;
+        const PeakGeo &g = S.g;
+        const int kr = KR > 0 ? KR : g.kr, kt = KT > 0 ? KT : g.kt;
+        const int hr = KR > 0 ? KR : g.hr, ht = KT > 0 ? KT : g.ht;
+        const int ckr = kr / 2, ckt = kt / 2;
+        const int CH = g.cr1 - g.cr0;
+        const int RH = g.rh;
+        const int AP = g.apitch, BP = g.bpitch;
+        const int nCells = g.nR * g.nT;
+        (void)ckr;
+
+        // ---- rho mask as bits, reloaded when the set changes ----
+        if (sidx != cur_set) {
+            const FDiv fw = fdiv_make(g.maskw);
+            for (int i2 = tid; i2 < g.nR * g.maskw; i2 += nthr) {
+                const int i = (int)fdiv_q(i2, fw), wd = i2 - i * g.maskw;
+                uint32_t bits = 0u;
+                for (int b = 0; b < 32; ++b) {
+                    const int j = wd * 32 + b;
+                    if (j < g.nT && S.mask[i * g.nT + j]) bits |= 1u << b;
+                }
+                mbits[i2] = bits;
+            }
+            cur_set = sidx;
+        }
+
+        const float *R = S.radon + q * (int64_t)nCells;
         float smin = INFINITY, smax = -INFINITY;
         double ssum = 0.0;
         if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
@@ -654,11 +730,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             // ---- stage the prefetched padded Radon tile, then start fetching the next ----
             tile_commit(g, R, TG, RH, tid, nthr, pf, A, AP);
             __syncthreads();
-            {
-                int tn = t + 1;
-                int64_t qn = q;
-                if (tn == g.ntiles) { tn = 0; qn = q + gridDim.x; }
-                if (qn < a.N) tile_fetch(g, a.radon + qn * (int64_t)nCells, tile_geo(g, tn, ht, ckt), RH, tid, nthr, pf);
+            if (t + 1 < g.ntiles) {
+                tile_fetch(g, R, tile_geo(g, t + 1, ht, ckt), RH, tid, nthr, pf);
+            } else if (it + gridDim.x < nitems) {
+                const int64_t itn = it + gridDim.x;
+                const PeakSet &Sn = MULTI ? m.sets[m.items[itn].x] : a.s;
+                const int64_t qn = MULTI ? (int64_t)m.items[itn].y : itn;
+                const int htn = KT > 0 ? KT : Sn.g.ht, cktn = (KT > 0 ? KT : Sn.g.kt) / 2;
+                tile_fetch(Sn.g, Sn.radon + qn * (int64_t)(Sn.g.nR * Sn.g.nT), tile_geo(Sn.g, 0, htn, cktn),
+                           Sn.g.rh, tid, nthr, pf);
             }
             // ---- theta pass: T[y][x] = sum_b v[b] * A[y][x + kt - 1 - b], 4 columns/thread ----
             {
@@ -910,9 +990,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             red[0] = mn;
             red[1] = (float)(sum / cnt);
             red[2] = mx;
-            a.out.d_stats[q * 3 + 0] = mn;
-            a.out.d_stats[q * 3 + 1] = red[1];
-            a.out.d_stats[q * 3 + 2] = mx;
+            S.out.d_stats[q * 3 + 0] = mn;
+            S.out.d_stats[q * 3 + 1] = red[1];
+            S.out.d_stats[q * 3 + 2] = mx;
         }
         __syncthreads();
 
@@ -976,21 +1056,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             float d = ave - mn;
             d = d < 1e-7f ? 1e-7f : d;
             const float nm = (bmax - mn) / d;
-            const double *thg = a.grid, *rhg = a.grid + g.nT;
+            const double *thg = S.grid, *rhg = S.grid + g.nT;
             const double th = 3.141592653589793 - np_interp_unit((double)al_c, thg, g.nT) /
                                                       (180.0 / 3.141592653589793);
             const double rh = -1.0 * np_interp_unit((double)al_r, rhg, g.nR);
-            a.out.d_max[o] = bmax;
-            a.out.d_normmax[o] = nm;
-            a.out.d_avemax[o] = bavemax;
-            a.out.d_maxloc[2 * o + 0] = ml_r;
-            a.out.d_maxloc[2 * o + 1] = ml_c;
-            a.out.d_aveloc[2 * o + 0] = al_r;
-            a.out.d_aveloc[2 * o + 1] = al_c;
-            a.out.d_width[o] = width;
-            a.out.d_theta[o] = (float)th;
-            a.out.d_rho[o] = (float)rh;
-            a.out.d_valid[o] = valid;
+            S.out.d_max[o] = bmax;
+            S.out.d_normmax[o] = nm;
+            S.out.d_avemax[o] = bavemax;
+            S.out.d_maxloc[2 * o + 0] = ml_r;
+            S.out.d_maxloc[2 * o + 1] = ml_c;
+            S.out.d_aveloc[2 * o + 0] = al_r;
+            S.out.d_aveloc[2 * o + 1] = al_c;
+            S.out.d_width[o] = width;
+            S.out.d_theta[o] = (float)th;
+            S.out.d_rho[o] = (float)rh;
+            S.out.d_valid[o] = valid;
         }
         __syncthreads();
     }
@@ -1286,8 +1366,8 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
     if (!p) return fail(-1, "ebsd_plan_set_tuning: null plan");
     if (gather_pack > 0) {
         // one value per element type, interpreted by the type-specific instantiation
-        p->gpack_u8 = std::min(16, std::max(4, gather_pack * 2));
-        p->gpack_u16 = std::min(8, std::max(2, gather_pack));
+        p->gpack_u8 = std::min(16, std::max(2, gather_pack * 2));
+        p->gpack_u16 = std::min(8, std::max(1, gather_pack));
         p->gpack_f32 = std::min(4, std::max(1, gather_pack / 2));
     }
     if (gather_threads > 0) {
@@ -1389,16 +1469,18 @@ static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int6
             }
         }
     } else if (dtype == EBSD_U8) {
-        for (int P = p->gpack_u8; rc == 1 && P >= 4; P /= 2) {
+        for (int P = p->gpack_u8; rc == 1 && P >= 2; P /= 2) {
             if (P >= 16) rc = launch_gather_t<EBSD_U8, false, 16>(p, d_pats, N, d_out, use_bg, s);
             else if (P >= 8) rc = launch_gather_t<EBSD_U8, false, 8>(p, d_pats, N, d_out, use_bg, s);
-            else rc = launch_gather_t<EBSD_U8, false, 4>(p, d_pats, N, d_out, use_bg, s);
+            else if (P >= 4) rc = launch_gather_t<EBSD_U8, false, 4>(p, d_pats, N, d_out, use_bg, s);
+            else rc = launch_gather_t<EBSD_U8, false, 2>(p, d_pats, N, d_out, use_bg, s);
         }
     } else {
-        for (int P = p->gpack_u16; rc == 1 && P >= 2; P /= 2) {
+        for (int P = p->gpack_u16; rc == 1 && P >= 1; P /= 2) {
             if (P >= 8) rc = launch_gather_t<EBSD_U16, false, 8>(p, d_pats, N, d_out, use_bg, s);
             else if (P >= 4) rc = launch_gather_t<EBSD_U16, false, 4>(p, d_pats, N, d_out, use_bg, s);
-            else rc = launch_gather_t<EBSD_U16, false, 2>(p, d_pats, N, d_out, use_bg, s);
+            else if (P >= 2) rc = launch_gather_t<EBSD_U16, false, 2>(p, d_pats, N, d_out, use_bg, s);
+            else rc = launch_gather_t<EBSD_U16, false, 1>(p, d_pats, N, d_out, use_bg, s);
         }
     }
     if (rc == 1) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
@@ -1408,21 +1490,23 @@ static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int6
 template <int KR, int KT>
 static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, hipStream_t s)
 {
-    const size_t lds = (size_t)peak_lds_layout(p->g).total * 4;
-    auto kern = k_peaks<KR, KT>;
-    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     PeakArgs a;
-    a.radon = d_radon;
-    a.N = N;
-    a.mask = p->d_mask;
-    a.grid = p->d_grid;
-    a.out = *out;
-    a.g = p->g;
+    a.s.radon = d_radon;
+    a.s.N = N;
+    a.s.mask = p->d_mask;
+    a.s.grid = p->d_grid;
+    a.s.out = *out;
+    a.s.g = p->g;
+    a.L = peak_lds_layout(p->g);
+    PeakMultiArgs m{};
+    const size_t lds = (size_t)a.L.total * 4;
+    auto kern = k_peaks<KR, KT, false>;
+    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int occ = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->pthreads, lds));
     if (occ < 1) return fail(-3, "peak kernel cannot be resident");
     const int64_t grid = std::min<int64_t>(N, (int64_t)occ * p->ncu);
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a, m);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1508,5 +1592,256 @@ extern "C" int ebsd_conv(ebsd_plan *p, const float *d_radon, int64_t N, float *d
     hipLaunchKernelGGL(k_conv_full, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        d_radon, N, p->g, d_conv);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// batches of mixed pattern sizes: one persistent gather launch and one persistent peak
+// launch over heterogeneous work lists, optionally captured once into a hipGraph
+// ---------------------------------------------------------------------------------------
+struct VariantInfo {
+    int id, IN, P, WB;
+    bool FLT;
+};
+
+static const VariantInfo kVariants[] = {
+#define EBSD_GV_INFO(ID, IN, FLT, P) {ID, IN, P, GatherCfg<IN, FLT, P>::WB, FLT},
+    EBSD_GATHER_VARIANTS(EBSD_GV_INFO)
+#undef EBSD_GV_INFO
+};
+
+// The variant the single-set launcher would pick: the largest pack (within the plan's
+// tuning) whose pixel-major LDS words fit.
+static int choose_variant(const ebsd_plan *p, int dtype, bool flt, VariantInfo *out)
+{
+    int pmax = flt ? p->gpack_f32 : (dtype == EBSD_U8 ? p->gpack_u8 : p->gpack_u16);
+    int best = -1;
+    for (const VariantInfo &v : kVariants) {
+        if (v.IN != dtype || v.FLT != flt || v.P > pmax) continue;
+        if ((size_t)(p->nPx + 1) * v.WB > (size_t)p->lds_max) continue;
+        if (best < 0 || v.P > kVariants[best].P) best = (int)(&v - kVariants);
+    }
+    if (best < 0) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
+    *out = kVariants[best];
+    return 0;
+}
+
+struct ebsd_batch {
+    int nsets = 0, device = 0, ncu = 0;
+    int gthreads = 1024, pthreads = 512;
+    int kr = 0, kt = 0; // 0 = run-time kernel sizes in the peak kernel
+    size_t lds_gather = 0, lds_peaks = 0;
+    PeakLds L{};
+    GatherArgs *d_gsets = nullptr;
+    int *d_variant = nullptr;
+    int2 *d_gitems = nullptr;
+    int64_t ngitems = 0;
+    PeakSet *d_psets = nullptr;
+    int2 *d_pitems = nullptr;
+    int64_t npitems = 0;
+    int grid_g = 0, grid_p = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void batch_free(ebsd_batch *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    (void)hipFree(b->d_gsets);
+    (void)hipFree(b->d_variant);
+    (void)hipFree(b->d_gitems);
+    (void)hipFree(b->d_psets);
+    (void)hipFree(b->d_pitems);
+    delete b;
+}
+
+extern "C" size_t ebsd_batch_workspace_bytes(int nsets, ebsd_plan *const *plans, const int64_t *N)
+{
+    if (nsets <= 0 || !plans || !N) return 0;
+    size_t total = 0;
+    for (int i = 0; i < nsets; ++i) {
+        if (!plans[i] || N[i] < 0) return 0;
+        total += (((size_t)N[i] * plans[i]->nCells * sizeof(float) + 255) / 256) * 256;
+    }
+    return total;
+}
+
+extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void *const *d_pats,
+                                 const int *dtypes, const int64_t *N, const ebsd_bands *outs, void *d_work,
+                                 size_t ws_bytes, ebsd_batch **out)
+{
+    if (!out || nsets <= 0 || !plans || !d_pats || !dtypes || !N || !outs || !d_work)
+        return fail(-1, "ebsd_batch_create: null argument");
+    *out = nullptr;
+    if (ws_bytes < ebsd_batch_workspace_bytes(nsets, plans, N))
+        return fail(-1, "ebsd_batch_create: workspace smaller than ebsd_batch_workspace_bytes");
+    ebsd_batch *b = new ebsd_batch();
+    b->nsets = nsets;
+    b->device = plans[0]->device;
+    b->ncu = plans[0]->ncu;
+    std::vector<GatherArgs> gs(nsets);
+    std::vector<int> var(nsets);
+    std::vector<PeakSet> ps(nsets);
+    std::vector<std::pair<int, int64_t>> gcost; // (set, packs) for ordering
+    PeakGeo gmax{};
+    char *work = (char *)d_work;
+    int rc;
+    b->kr = plans[0]->g.kr;
+    b->kt = plans[0]->g.kt;
+    for (int i = 0; i < nsets; ++i) {
+        ebsd_plan *p = plans[i];
+        if (!p || N[i] < 0) { batch_free(b); return fail(-1, "ebsd_batch_create: bad set"); }
+        if (p->device != b->device) { batch_free(b); return fail(-1, "ebsd_batch_create: plans on different devices"); }
+        if (!bands_ok(&outs[i]) || (N[i] > 0 && !d_pats[i])) { batch_free(b); return fail(-1, "ebsd_batch_create: null set pointer"); }
+        const int dtype = dtypes[i] & 0xff;
+        if (dtype > EBSD_F32) { batch_free(b); return fail(-1, "ebsd_batch_create: bad dtype"); }
+        const bool use_bg = !(dtypes[i] & EBSD_FLAG_NOBG) && p->bg_mode != 0;
+        const bool flt = dtype == EBSD_F32 || (use_bg && p->bg_mode == 2);
+        if ((rc = layout_streams(p, b->gthreads / 64))) { batch_free(b); return rc; }
+        p->gthreads = b->gthreads;
+        VariantInfo v;
+        if ((rc = choose_variant(p, dtype, flt, &v))) { batch_free(b); return rc; }
+        var[i] = v.id;
+        b->lds_gather = std::max(b->lds_gather, (size_t)(p->nPx + 1) * v.WB);
+        float *radon = (float *)work;
+        work += (((size_t)N[i] * p->nCells * sizeof(float) + 255) / 256) * 256;
+        GatherArgs &a = gs[i];
+        a.pats = d_pats[i];
+        a.N = N[i];
+        a.nPx = p->nPx;
+        a.nbatch = p->nbatch;
+        a.gidx = p->d_gidx;
+        a.bhdr = p->d_bhdr;
+        a.lane = p->d_lane;
+        a.wofs = p->d_wofs;
+        a.wcnt = p->d_wcnt;
+        a.linebg = (use_bg && !flt && p->bg_mode == 1) ? p->d_linebg : nullptr;
+        a.bg = (use_bg && flt) ? p->d_bg : nullptr;
+        a.out = radon;
+        a.nCells = p->nCells;
+        a.npacks = (N[i] + v.P - 1) / v.P;
+        gcost.push_back({i, a.npacks});
+        PeakSet &S = ps[i];
+        S.radon = radon;
+        S.N = N[i];
+        S.mask = p->d_mask;
+        S.grid = p->d_grid;
+        S.out = outs[i];
+        S.g = p->g;
+        if (p->g.kr != b->kr || p->g.kt != b->kt || p->g.hr != p->g.kr || p->g.ht != p->g.kt) b->kr = b->kt = 0;
+        gmax.rh = std::max(gmax.rh, p->g.rh);
+        gmax.apitch = std::max(gmax.apitch, p->g.apitch);
+        gmax.bpitch = std::max(gmax.bpitch, p->g.bpitch);
+        gmax.nR = std::max(gmax.nR, p->g.nR);
+        gmax.maskw = std::max(gmax.maskw, p->g.maskw);
+    }
+    if (b->kr == 9 && b->kt == 5) {
+    } else if (b->kr == 5 && b->kt == 5) {
+    } else {
+        b->kr = b->kt = 0;
+    }
+    // gather items: every pack of every set, the largest patterns first, so the static
+    // round-robin over persistent work-groups ends with the cheapest packs
+    std::vector<int2> gitems;
+    std::vector<int> order(nsets);
+    for (int i = 0; i < nsets; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return plans[x]->nPx > plans[y]->nPx; });
+    for (int i : order)
+        for (int64_t k = 0; k < gs[i].npacks; ++k) gitems.push_back(make_int2(i, (int)k));
+    std::vector<int2> pitems;
+    for (int i : order)
+        for (int64_t q = 0; q < N[i]; ++q) pitems.push_back(make_int2(i, (int)q));
+    b->ngitems = (int64_t)gitems.size();
+    b->npitems = (int64_t)pitems.size();
+    b->L = peak_lds_layout(gmax);
+    b->lds_peaks = (size_t)b->L.total * 4;
+    if ((rc = dev_upload(&b->d_gsets, gs.data(), gs.size()))) { batch_free(b); return rc; }
+    if ((rc = dev_upload(&b->d_variant, var.data(), var.size()))) { batch_free(b); return rc; }
+    if ((rc = dev_upload(&b->d_gitems, gitems.data(), gitems.size()))) { batch_free(b); return rc; }
+    if ((rc = dev_upload(&b->d_psets, ps.data(), ps.size()))) { batch_free(b); return rc; }
+    if ((rc = dev_upload(&b->d_pitems, pitems.data(), pitems.size()))) { batch_free(b); return rc; }
+    // occupancy-derived persistent grids
+    int occ = 0;
+    if (hipFuncSetAttribute((const void *)k_radon_gather_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)b->lds_gather) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_radon_gather_multi, b->gthreads, b->lds_gather) !=
+            hipSuccess || occ < 1) {
+        batch_free(b);
+        return fail(-3, "batched gather kernel cannot be resident");
+    }
+    b->grid_g = (int)std::min<int64_t>(b->ngitems, (int64_t)occ * b->ncu);
+    const void *pk = b->kr == 9 ? (const void *)k_peaks<9, 5, true>
+                   : b->kr == 5 ? (const void *)k_peaks<5, 5, true> : (const void *)k_peaks<0, 0, true>;
+    occ = 0;
+    if (hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds_peaks) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pk, b->pthreads, b->lds_peaks) != hipSuccess ||
+        occ < 1) {
+        batch_free(b);
+        return fail(-3, "batched peak kernel cannot be resident");
+    }
+    b->grid_p = (int)std::min<int64_t>(b->npitems, (int64_t)occ * b->ncu);
+    *out = b;
+    return 0;
+}
+
+static int batch_launch(ebsd_batch *b, hipStream_t s)
+{
+    if (b->ngitems > 0) {
+        GatherMultiArgs gm;
+        gm.sets = b->d_gsets;
+        gm.variant = b->d_variant;
+        gm.items = b->d_gitems;
+        gm.nitems = b->ngitems;
+        hipLaunchKernelGGL(k_radon_gather_multi, dim3(b->grid_g), dim3(b->gthreads), b->lds_gather, s, gm);
+        HIPCHK(hipGetLastError());
+    }
+    if (b->npitems > 0) {
+        PeakArgs a{};
+        PeakMultiArgs m;
+        m.sets = b->d_psets;
+        m.items = b->d_pitems;
+        m.nitems = b->npitems;
+        m.L = b->L;
+        if (b->kr == 9)
+            hipLaunchKernelGGL((k_peaks<9, 5, true>), dim3(b->grid_p), dim3(b->pthreads), b->lds_peaks, s, a, m);
+        else if (b->kr == 5)
+            hipLaunchKernelGGL((k_peaks<5, 5, true>), dim3(b->grid_p), dim3(b->pthreads), b->lds_peaks, s, a, m);
+        else
+            hipLaunchKernelGGL((k_peaks<0, 0, true>), dim3(b->grid_p), dim3(b->pthreads), b->lds_peaks, s, a, m);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int ebsd_batch_run(ebsd_batch *b, void *stream)
+{
+    if (!b) return fail(-1, "ebsd_batch_run: null batch");
+    return batch_launch(b, (hipStream_t)stream);
+}
+
+extern "C" int ebsd_batch_graph_launch(ebsd_batch *b, void *stream)
+{
+    if (!b) return fail(-1, "ebsd_batch_graph_launch: null batch");
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(-1, "ebsd_batch_graph_launch: graphs are captured on a non-default stream");
+    if (!b->exec) {
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_launch(b, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->exec, s));
+    return 0;
+}
+
+extern "C" int ebsd_batch_destroy(ebsd_batch *b)
+{
+    batch_free(b);
     return 0;
 }

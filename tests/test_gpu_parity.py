@@ -132,3 +132,22 @@ def test_edge_cases_constant_and_single():
     compare_bands({k: g1[k] for k in ('valid', 'maxloc', 'aveloc')}, res, C64, ref.padding, ref.peakPad)
     out = bd.find_bands_device(torch.empty((0,) + shape, dtype=torch.uint8, device="cuda"))
     assert out['max'].shape == (0, 9)
+
+
+@pytest.mark.parametrize("dtype", ["uint16", "uint8"])
+def test_scan_scale_geometry_240(dtype):
+    """240x240 patterns with a 360x180 Radon and 12 bands (the streaming workload's shape):
+    2-byte LDS words in the gather and a 218x382 padded Radon in the peak kernel."""
+    print(f"240x240 {dtype}", flush=True)
+    shape = (240, 240)
+    bd, ref = make_pair(shape, nTheta=360, nRho=180, nBands=12)
+    pats = _pats(3, shape, dtype, seed=31, family="HCP")
+    rad = bd.device_plan().radon(_dev(pats)).cpu().numpy()
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    assert np.array_equal(rad, want)
+    got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(_dev(pats)).items()}
+    res, st, C64, _ = oracle64(ref, pats)
+    total, ties = compare_bands(got, res, C64, ref.padding, ref.peakPad)
+    assert ties <= 1
+    np.testing.assert_allclose(got['stats'][:, 2], st[2], rtol=1e-5)
