@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--no-background", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the mixed-size batched measurement")
     ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--pmc-driver", action="store_true",
                     help="short fixed launch sequence for rocprofv3 --pmc: gather x3, peaks x3 on one "
@@ -59,6 +60,80 @@ def cpu_baseline(pats_host, bg, params, target_s):
         off = (off + n) % pats_host.shape[0]
     dt = time.perf_counter() - t0
     return done / dt, done, dt
+
+
+def batched_measure(dev, steps, warmup, npats=32768, seed=7, autotune_sets=True):
+    """Mixed sizes {60,80,120,160}^2 x {uint8, uint16} (equal shares), BCC bands, one pattern
+    centre per size (x*, y* ~ U(.45,.55), z* ~ U(.55,.7)), a plan per size: per-size plain
+    launches vs one batched persistent launch pair replayed as a hipGraph."""
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.autotune import autotune
+    from pyebsdindex_amd.batch import BatchBandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    rng = np.random.default_rng(seed)
+    sizes = [60, 80, 120, 160]
+    dtypes = ["uint8", "uint16"]
+    per = npats // (len(sizes) * len(dtypes))
+    params = dict(nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9)
+    dets, pats, tuned = [], [], {}
+    for si, sz in enumerate(sizes):
+        pc = (rng.uniform(0.45, 0.55), rng.uniform(0.45, 0.55), rng.uniform(0.55, 0.7))
+        for di, dt in enumerate(dtypes):
+            x = kikuchi_patterns(per, (sz, sz), dt, seed=seed + 10 * si + di, family="BCC", pc=pc,
+                                 device=dev, chunk=1024)
+            bd = BandDetect(patDim=(sz, sz), **params)
+            if autotune_sets:
+                tuned[f"{sz}x{sz} {dt}"] = autotune(bd, x, sample=1024)
+            dets.append(bd)
+            pats.append(x)
+    torch.cuda.synchronize()
+    outs = [bd.device_plan().alloc_out(x.shape[0]) for bd, x in zip(dets, pats)]
+    works = [bd.device_plan().workspace(x.shape[0], 2048) for bd, x in zip(dets, pats)]
+
+    def plain():
+        for bd, x, o, w in zip(dets, pats, outs, works):
+            bd.find_bands_device(x, out=o, work=w, chunk=2048)
+
+    for _ in range(warmup):
+        plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain()
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / steps
+    batch = BatchBandDetect(dets, pats)
+    gs = torch.cuda.Stream(dev)
+    for _ in range(warmup):
+        batch.graph_launch(gs)
+    gs.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.graph_launch(gs)
+    gs.synchronize()
+    graph_s = (time.perf_counter() - t0) / steps
+    for _ in range(warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    torch.cuda.synchronize()
+    run_s = (time.perf_counter() - t0) / steps
+    # the batched bins and sub-pixel positions against the per-size launches
+    dmax = 0.0
+    same = True
+    for o, ob in zip(outs, batch.outs):
+        same &= bool(torch.equal(o["maxloc"], ob["maxloc"]))
+        dmax = max(dmax, float((o["aveloc"] - ob["aveloc"]).abs().max()))
+    n = sum(int(x.shape[0]) for x in pats)
+    return {"workload": f"{n} patterns, {{60,80,120,160}}^2 x {{uint8,uint16}} in equal shares, BCC, "
+                        f"one pattern centre and plan per size, Radon 180x90, 9 bands",
+            "plain_patterns_per_s": round(n / plain_s, 1), "batched_patterns_per_s": round(n / run_s, 1),
+            "graph_patterns_per_s": round(n / graph_s, 1), "graph_vs_plain": round(plain_s / graph_s, 3),
+            "plain_ms": round(plain_s * 1e3, 4), "graph_ms": round(graph_s * 1e3, 4),
+            "peak_bins_identical": same, "max_subpixel_diff_bins": dmax, "autotuned": tuned}
 
 
 def main():
@@ -157,6 +232,13 @@ def main():
         if tj.get("patterns_per_launch") == chunk:
             traffic = tj.get("hbm_bytes_per_launch")
 
+    batched = None
+    if not args.no_batched:
+        print("[bench] mixed-size batched workload", file=sys.stderr, flush=True)
+        batched = batched_measure(dev, args.steps, args.warmup)
+        print(f"[bench] batched {batched['graph_patterns_per_s']:.1f} patterns/s, plain per-size "
+              f"{batched['plain_patterns_per_s']:.1f} patterns/s", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle restatement of the reference CPU path)", file=sys.stderr, flush=True)
@@ -185,6 +267,7 @@ def main():
         "kernels_ms_per_chunk": {"k_radon_gather": round(gather_ms, 4), "k_peaks": round(peaks_ms, 4),
                                  "chunk": chunk},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

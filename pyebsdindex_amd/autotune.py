@@ -1,0 +1,78 @@
+"""Autotuner for the launch shape of one plan (pattern size × dtype × background mode).
+
+Sweeps the gather's patterns per LDS word and work-group size, and the peak kernel's
+theta tile width and work-group size, timing each on a sample of the caller's patterns
+with HIP events; the fastest shape is applied to the plan (``ebsd_plan_set_tuning``) and
+remembered per (device, H, W, nTheta, nRho, dtype, background mode).
+"""
+import itertools
+
+_CACHE = {}
+
+
+def _key(bd, plan, dtype):
+    import torch
+    name = torch.cuda.get_device_name(plan.device)
+    return (name, plan.H, plan.W, plan.nT, plan.nR, str(dtype), bd.background_mode,
+            bd.backgroundsub is not None)
+
+
+def _time(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 8),
+             gather_threads=(512, 1024), peak_tiles=(24, 30, 36, 45, 60, 90), peak_threads=(256, 512),
+             verbose=False):
+    """Tune ``bd``'s device plan on (a sample of) the device tensor ``pats``; returns
+    ``{"gather_pack", "gather_threads", "peak_tile", "peak_threads", "gather_ms", "peak_ms"}``."""
+    import torch
+    from . import _hip
+    plan = bd.device_plan(gpu_id)
+    x = pats[:sample]
+    key = _key(bd, plan, x.dtype)
+    if key in _CACHE:
+        best = _CACHE[key]
+        plan.set_tuning(best["gather_pack"], best["gather_threads"], best["peak_tile"], best["peak_threads"])
+        return dict(best)
+    xx, flags = bd._prepare(plan, x)
+    rad = torch.empty((xx.shape[0], plan.nR, plan.nT), dtype=torch.float32, device=plan.device)
+    out = plan.alloc_out(xx.shape[0])
+    best_g = None
+    for gp, gt in itertools.product(gather_packs, gather_threads):
+        try:
+            plan.set_tuning(gp, gt, 0, 0)
+            ms = _time(lambda: plan.radon(xx, flags=flags, out=rad), reps)
+        except _hip.HipLibraryError:
+            continue
+        if verbose:
+            print(f"  gather pack={gp} threads={gt}: {ms:.4f} ms")
+        if best_g is None or ms < best_g[0]:
+            best_g = (ms, gp, gt)
+    plan.set_tuning(best_g[1], best_g[2], 0, 0)
+    plan.radon(xx, flags=flags, out=rad)
+    best_p = None
+    for pt, pth in itertools.product(peak_tiles, peak_threads):
+        try:
+            plan.set_tuning(0, 0, pt, pth)
+            ms = _time(lambda: plan.peaks(rad, out=out), reps)
+        except _hip.HipLibraryError:
+            continue
+        if verbose:
+            print(f"  peaks tile={pt} threads={pth}: {ms:.4f} ms")
+        if best_p is None or ms < best_p[0]:
+            best_p = (ms, pt, pth)
+    plan.set_tuning(0, 0, best_p[1], best_p[2])
+    best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "peak_tile": best_p[1],
+            "peak_threads": best_p[2], "gather_ms": best_g[0], "peak_ms": best_p[0]}
+    _CACHE[key] = best
+    return dict(best)
