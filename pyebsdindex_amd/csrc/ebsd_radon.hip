@@ -1626,20 +1626,30 @@ static int choose_variant(const ebsd_plan *p, int dtype, bool flt, VariantInfo *
     return 0;
 }
 
+// Sets whose staged patterns need similar LDS share one persistent gather launch, so small
+// patterns keep the occupancy their own launches would get.
+struct GatherClass {
+    int threads = 1024;
+    size_t lds = 0;
+    int64_t item0 = 0, nitems = 0; // slice of the batch's gather work list
+    int grid = 0;
+};
+
 struct ebsd_batch {
     int nsets = 0, device = 0, ncu = 0;
-    int gthreads = 1024, pthreads = 512;
+    int pthreads = 512;
     int kr = 0, kt = 0; // 0 = run-time kernel sizes in the peak kernel
-    size_t lds_gather = 0, lds_peaks = 0;
+    size_t lds_peaks = 0;
     PeakLds L{};
     GatherArgs *d_gsets = nullptr;
     int *d_variant = nullptr;
     int2 *d_gitems = nullptr;
     int64_t ngitems = 0;
+    std::vector<GatherClass> classes;
     PeakSet *d_psets = nullptr;
     int2 *d_pitems = nullptr;
     int64_t npitems = 0;
-    int grid_g = 0, grid_p = 0;
+    int grid_p = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1690,6 +1700,9 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     int rc;
     b->kr = plans[0]->g.kr;
     b->kt = plans[0]->g.kt;
+    // LDS occupancy class of each set: work-groups per CU its staged pack allows (1..4);
+    // classes of 2+ work-groups per CU run 512-thread work-groups, class 1 runs 1024
+    std::vector<int> cls(nsets);
     for (int i = 0; i < nsets; ++i) {
         ebsd_plan *p = plans[i];
         if (!p || N[i] < 0) { batch_free(b); return fail(-1, "ebsd_batch_create: bad set"); }
@@ -1699,12 +1712,15 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         if (dtype > EBSD_F32) { batch_free(b); return fail(-1, "ebsd_batch_create: bad dtype"); }
         const bool use_bg = !(dtypes[i] & EBSD_FLAG_NOBG) && p->bg_mode != 0;
         const bool flt = dtype == EBSD_F32 || (use_bg && p->bg_mode == 2);
-        if ((rc = layout_streams(p, b->gthreads / 64))) { batch_free(b); return rc; }
-        p->gthreads = b->gthreads;
         VariantInfo v;
         if ((rc = choose_variant(p, dtype, flt, &v))) { batch_free(b); return rc; }
         var[i] = v.id;
-        b->lds_gather = std::max(b->lds_gather, (size_t)(p->nPx + 1) * v.WB);
+        const size_t lds = (size_t)(p->nPx + 1) * v.WB;
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)p->lds_max / std::max<size_t>(lds, 1)));
+        cls[i] = per_cu;
+        const int threads = per_cu >= 2 ? 512 : 1024;
+        if ((rc = layout_streams(p, threads / 64))) { batch_free(b); return rc; }
+        p->gthreads = threads;
         float *radon = (float *)work;
         work += (((size_t)N[i] * p->nCells * sizeof(float) + 255) / 256) * 256;
         GatherArgs &a = gs[i];
@@ -1722,7 +1738,6 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         a.out = radon;
         a.nCells = p->nCells;
         a.npacks = (N[i] + v.P - 1) / v.P;
-        gcost.push_back({i, a.npacks});
         PeakSet &S = ps[i];
         S.radon = radon;
         S.N = N[i];
@@ -1742,14 +1757,24 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     } else {
         b->kr = b->kt = 0;
     }
-    // gather items: every pack of every set, the largest patterns first, so the static
-    // round-robin over persistent work-groups ends with the cheapest packs
+    // gather items, class by class (4 work-groups per CU first); inside a class the largest
+    // patterns first, so the static round-robin over persistent work-groups ends cheap
     std::vector<int2> gitems;
     std::vector<int> order(nsets);
     for (int i = 0; i < nsets; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return plans[x]->nPx > plans[y]->nPx; });
-    for (int i : order)
-        for (int64_t k = 0; k < gs[i].npacks; ++k) gitems.push_back(make_int2(i, (int)k));
+    for (int c = 4; c >= 1; --c) {
+        GatherClass gc;
+        gc.item0 = (int64_t)gitems.size();
+        for (int i : order) {
+            if (cls[i] != c) continue;
+            gc.threads = plans[i]->gthreads;
+            gc.lds = std::max(gc.lds, (size_t)(plans[i]->nPx + 1) * kVariants[var[i]].WB);
+            for (int64_t k = 0; k < gs[i].npacks; ++k) gitems.push_back(make_int2(i, (int)k));
+        }
+        gc.nitems = (int64_t)gitems.size() - gc.item0;
+        if (gc.nitems > 0) b->classes.push_back(gc);
+    }
     std::vector<int2> pitems;
     for (int i : order)
         for (int64_t q = 0; q < N[i]; ++q) pitems.push_back(make_int2(i, (int)q));
@@ -1757,6 +1782,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     b->npitems = (int64_t)pitems.size();
     b->L = peak_lds_layout(gmax);
     b->lds_peaks = (size_t)b->L.total * 4;
+    (void)gcost;
     if ((rc = dev_upload(&b->d_gsets, gs.data(), gs.size()))) { batch_free(b); return rc; }
     if ((rc = dev_upload(&b->d_variant, var.data(), var.size()))) { batch_free(b); return rc; }
     if ((rc = dev_upload(&b->d_gitems, gitems.data(), gitems.size()))) { batch_free(b); return rc; }
@@ -1764,14 +1790,25 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     if ((rc = dev_upload(&b->d_pitems, pitems.data(), pitems.size()))) { batch_free(b); return rc; }
     // occupancy-derived persistent grids
     int occ = 0;
-    if (hipFuncSetAttribute((const void *)k_radon_gather_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)b->lds_gather) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_radon_gather_multi, b->gthreads, b->lds_gather) !=
-            hipSuccess || occ < 1) {
-        batch_free(b);
-        return fail(-3, "batched gather kernel cannot be resident");
+    for (GatherClass &gc : b->classes) {
+        occ = 0;
+        if (hipFuncSetAttribute((const void *)k_radon_gather_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)gc.lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_radon_gather_multi, gc.threads, gc.lds) !=
+                hipSuccess || occ < 1) {
+            batch_free(b);
+            return fail(-3, "batched gather kernel cannot be resident");
+        }
+        gc.grid = (int)std::min<int64_t>(gc.nitems, (int64_t)occ * b->ncu);
     }
-    b->grid_g = (int)std::min<int64_t>(b->ngitems, (int64_t)occ * b->ncu);
+    // the dynamic-LDS limit of the gather kernel covers every class (set once, not per launch)
+    size_t lds_all = 0;
+    for (const GatherClass &gc : b->classes) lds_all = std::max(lds_all, gc.lds);
+    if (lds_all > 0 && hipFuncSetAttribute((const void *)k_radon_gather_multi,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_all) != hipSuccess) {
+        batch_free(b);
+        return fail(-3, "batched gather kernel LDS limit");
+    }
     const void *pk = b->kr == 9 ? (const void *)k_peaks<9, 5, true>
                    : b->kr == 5 ? (const void *)k_peaks<5, 5, true> : (const void *)k_peaks<0, 0, true>;
     occ = 0;
@@ -1788,13 +1825,13 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
 
 static int batch_launch(ebsd_batch *b, hipStream_t s)
 {
-    if (b->ngitems > 0) {
+    for (const GatherClass &gc : b->classes) {
         GatherMultiArgs gm;
         gm.sets = b->d_gsets;
         gm.variant = b->d_variant;
-        gm.items = b->d_gitems;
-        gm.nitems = b->ngitems;
-        hipLaunchKernelGGL(k_radon_gather_multi, dim3(b->grid_g), dim3(b->gthreads), b->lds_gather, s, gm);
+        gm.items = b->d_gitems + gc.item0;
+        gm.nitems = gc.nitems;
+        hipLaunchKernelGGL(k_radon_gather_multi, dim3(gc.grid), dim3(gc.threads), gc.lds, s, gm);
         HIPCHK(hipGetLastError());
     }
     if (b->npitems > 0) {
