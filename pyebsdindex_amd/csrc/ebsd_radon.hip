@@ -413,13 +413,13 @@ __global__ __launch_bounds__(1024) void k_radon_gather_multi(GatherMultiArgs m)
 // ---------------------------------------------------------------------------------------
 // k_peaks
 // ---------------------------------------------------------------------------------------
-struct PeakSet {
+// Pointers of one pattern set of the peak kernel.
+struct PeakPtrs {
     const float *radon; // (N, nR, nT)
     int64_t N;
     const uint8_t *mask; // (nR, nT)
     const double *grid;  // theta_deg[nT], rho[nR]
     ebsd_bands out;
-    PeakGeo g;
 };
 
 // Source cell of padded Radon cell (r, c) in the unpadded (nR, nT) Radon: theta padding
@@ -481,17 +481,16 @@ __host__ __device__ inline PeakLds peak_lds_layout(const PeakGeo &g)
     return L;
 }
 
+// Kernel arguments of k_peaks.  The geometry and LDS layout are launch-wide and travel by
+// value (kernel arguments stay invariant across barriers); a heterogeneous launch adds a
+// (set, pattern) work list over sets that share that geometry.
 struct PeakArgs {
-    PeakSet s;
-    PeakLds L;
-};
-
-// Heterogeneous batch: items are (set, pattern); all sets share the kernel sizes.
-struct PeakMultiArgs {
-    const PeakSet *sets;
-    const int2 *items;
+    PeakPtrs s;             // the one set of a plain launch
+    const PeakPtrs *sets;   // heterogeneous launch: per-set pointers
+    const int2 *items;      //   (set, pattern) work list
     int64_t nitems;
     PeakLds L;
+    PeakGeo g;
 };
 
 __device__ __forceinline__ double np_interp_unit(double x, const double *fp, int n)
@@ -650,10 +649,10 @@ constexpr int CGRP = 4;   // columns per thread in the theta passes
 // band_detect.py:287-288 derives peakPad = 2*ksz+1); 0: run-time sizes.  MULTI walks a
 // heterogeneous (set, pattern) work list, otherwise the patterns of one set.
 template <int KR, int KT, bool MULTI>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_peaks(PeakArgs a, PeakMultiArgs m)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_peaks(PeakArgs a)
 {
     extern __shared__ __align__(16) float sm[];
-    const PeakLds L = MULTI ? m.L : a.L;
+    const PeakLds &L = a.L;
     float *A = sm + L.a_off;
     float *B = sm + L.b_off;
     uint32_t *mbits = reinterpret_cast<uint32_t *>(sm + L.mask_off);
@@ -671,7 +670,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int64_t nitems = MULTI ? m.nitems : a.s.N;
+    const int64_t nitems = MULTI ? a.nitems : a.s.N;
     int cur_set = -1;
 
     // the padded-Radon block of the next tile is loaded into registers while the current
@@ -679,18 +678,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     float pf[PF_REGS];
     if ((int64_t)blockIdx.x < nitems) {
         const int64_t it0 = blockIdx.x;
-        const PeakSet &S0 = MULTI ? m.sets[m.items[it0].x] : a.s;
-        const int64_t q0 = MULTI ? (int64_t)m.items[it0].y : it0;
-        const int ht0 = KT > 0 ? KT : S0.g.ht, ckt0 = (KT > 0 ? KT : S0.g.kt) / 2;
-        tile_fetch(S0.g, S0.radon + q0 * (int64_t)(S0.g.nR * S0.g.nT), tile_geo(S0.g, 0, ht0, ckt0), S0.g.rh,
-                   tid, nthr, pf);
+        const PeakGeo &g0 = a.g;
+        const float *R0 = MULTI ? a.sets[a.items[it0].x].radon : a.s.radon;
+        const int64_t q0 = MULTI ? (int64_t)a.items[it0].y : it0;
+        const int ht0 = KT > 0 ? KT : g0.ht, ckt0 = (KT > 0 ? KT : g0.kt) / 2;
+        tile_fetch(g0, R0 + q0 * (int64_t)(g0.nR * g0.nT), tile_geo(g0, 0, ht0, ckt0), g0.rh, tid, nthr, pf);
     }
 
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int sidx = MULTI ? m.items[it].x : 0;
-        const PeakSet &S = MULTI ? m.sets[sidx] : a.s;
-        const int64_t q = MULTI ? (int64_t)m.items[it].y : it;
-        const PeakGeo &g = S.g;
+        const int sidx = MULTI ? a.items[it].x : 0;
+        const PeakPtrs &S = MULTI ? a.sets[sidx] : a.s;
+        const int64_t q = MULTI ? (int64_t)a.items[it].y : it;
+        const PeakGeo &g = a.g;
         const int kr = KR > 0 ? KR : g.kr, kt = KT > 0 ? KT : g.kt;
         const int hr = KR > 0 ? KR : g.hr, ht = KT > 0 ? KT : g.ht;
         const int ckr = kr / 2, ckt = kt / 2;
@@ -734,11 +733,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 tile_fetch(g, R, tile_geo(g, t + 1, ht, ckt), RH, tid, nthr, pf);
             } else if (it + gridDim.x < nitems) {
                 const int64_t itn = it + gridDim.x;
-                const PeakSet &Sn = MULTI ? m.sets[m.items[itn].x] : a.s;
-                const int64_t qn = MULTI ? (int64_t)m.items[itn].y : itn;
-                const int htn = KT > 0 ? KT : Sn.g.ht, cktn = (KT > 0 ? KT : Sn.g.kt) / 2;
-                tile_fetch(Sn.g, Sn.radon + qn * (int64_t)(Sn.g.nR * Sn.g.nT), tile_geo(Sn.g, 0, htn, cktn),
-                           Sn.g.rh, tid, nthr, pf);
+                const float *Rn = MULTI ? a.sets[a.items[itn].x].radon : a.s.radon;
+                const int64_t qn = MULTI ? (int64_t)a.items[itn].y : itn;
+                tile_fetch(g, Rn + qn * (int64_t)nCells, tile_geo(g, 0, ht, ckt), RH, tid, nthr, pf);
             }
             // ---- theta pass: T[y][x] = sum_b v[b] * A[y][x + kt - 1 - b], 4 columns/thread ----
             {
@@ -1496,9 +1493,8 @@ static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const e
     a.s.mask = p->d_mask;
     a.s.grid = p->d_grid;
     a.s.out = *out;
-    a.s.g = p->g;
+    a.g = p->g;
     a.L = peak_lds_layout(p->g);
-    PeakMultiArgs m{};
     const size_t lds = (size_t)a.L.total * 4;
     auto kern = k_peaks<KR, KT, false>;
     HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1506,7 +1502,7 @@ static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const e
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->pthreads, lds));
     if (occ < 1) return fail(-3, "peak kernel cannot be resident");
     const int64_t grid = std::min<int64_t>(N, (int64_t)occ * p->ncu);
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a, m);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->pthreads), lds, s, a);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1635,21 +1631,29 @@ struct GatherClass {
     int grid = 0;
 };
 
+// Sets whose Radon geometry is identical share one persistent peak launch; the geometry
+// travels in that launch's kernel arguments.
+struct PeakClass {
+    PeakGeo g{};
+    PeakLds L{};
+    size_t lds = 0;
+    int kr = 0, kt = 0; // 0 = run-time kernel sizes in the peak kernel
+    int threads = 512;
+    int64_t item0 = 0, nitems = 0; // slice of the batch's peak work list
+    int grid = 0;
+};
+
 struct ebsd_batch {
     int nsets = 0, device = 0, ncu = 0;
-    int pthreads = 512;
-    int kr = 0, kt = 0; // 0 = run-time kernel sizes in the peak kernel
-    size_t lds_peaks = 0;
-    PeakLds L{};
     GatherArgs *d_gsets = nullptr;
     int *d_variant = nullptr;
     int2 *d_gitems = nullptr;
     int64_t ngitems = 0;
     std::vector<GatherClass> classes;
-    PeakSet *d_psets = nullptr;
+    PeakPtrs *d_psets = nullptr;
     int2 *d_pitems = nullptr;
     int64_t npitems = 0;
-    int grid_p = 0;
+    std::vector<PeakClass> pclasses;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1693,13 +1697,9 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     b->ncu = plans[0]->ncu;
     std::vector<GatherArgs> gs(nsets);
     std::vector<int> var(nsets);
-    std::vector<PeakSet> ps(nsets);
-    std::vector<std::pair<int, int64_t>> gcost; // (set, packs) for ordering
-    PeakGeo gmax{};
+    std::vector<PeakPtrs> ps(nsets);
     char *work = (char *)d_work;
     int rc;
-    b->kr = plans[0]->g.kr;
-    b->kt = plans[0]->g.kt;
     // LDS occupancy class of each set: work-groups per CU its staged pack allows (1..4);
     // classes of 2+ work-groups per CU run 512-thread work-groups, class 1 runs 1024
     std::vector<int> cls(nsets);
@@ -1738,24 +1738,12 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         a.out = radon;
         a.nCells = p->nCells;
         a.npacks = (N[i] + v.P - 1) / v.P;
-        PeakSet &S = ps[i];
+        PeakPtrs &S = ps[i];
         S.radon = radon;
         S.N = N[i];
         S.mask = p->d_mask;
         S.grid = p->d_grid;
         S.out = outs[i];
-        S.g = p->g;
-        if (p->g.kr != b->kr || p->g.kt != b->kt || p->g.hr != p->g.kr || p->g.ht != p->g.kt) b->kr = b->kt = 0;
-        gmax.rh = std::max(gmax.rh, p->g.rh);
-        gmax.apitch = std::max(gmax.apitch, p->g.apitch);
-        gmax.bpitch = std::max(gmax.bpitch, p->g.bpitch);
-        gmax.nR = std::max(gmax.nR, p->g.nR);
-        gmax.maskw = std::max(gmax.maskw, p->g.maskw);
-    }
-    if (b->kr == 9 && b->kt == 5) {
-    } else if (b->kr == 5 && b->kt == 5) {
-    } else {
-        b->kr = b->kt = 0;
     }
     // gather items, class by class (4 work-groups per CU first); inside a class the largest
     // patterns first, so the static round-robin over persistent work-groups ends cheap
@@ -1775,14 +1763,31 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         gc.nitems = (int64_t)gitems.size() - gc.item0;
         if (gc.nitems > 0) b->classes.push_back(gc);
     }
+    // peak items, one class per distinct geometry, in first-seen order
     std::vector<int2> pitems;
-    for (int i : order)
-        for (int64_t q = 0; q < N[i]; ++q) pitems.push_back(make_int2(i, (int)q));
+    std::vector<int> done(nsets, 0);
+    for (int i0 = 0; i0 < nsets; ++i0) {
+        if (done[i0]) continue;
+        const PeakGeo &g = plans[i0]->g;
+        PeakClass pc;
+        pc.g = g;
+        pc.L = peak_lds_layout(g);
+        pc.lds = (size_t)pc.L.total * 4;
+        pc.threads = plans[i0]->pthreads;
+        const bool fixed = g.hr == g.kr && g.ht == g.kt;
+        pc.kr = fixed && g.kt == 5 && (g.kr == 9 || g.kr == 5) ? g.kr : 0;
+        pc.kt = pc.kr ? 5 : 0;
+        pc.item0 = (int64_t)pitems.size();
+        for (int i : order) {
+            if (done[i] || std::memcmp(&plans[i]->g, &g, sizeof(PeakGeo)) != 0) continue;
+            done[i] = 1;
+            for (int64_t q = 0; q < N[i]; ++q) pitems.push_back(make_int2(i, (int)q));
+        }
+        pc.nitems = (int64_t)pitems.size() - pc.item0;
+        if (pc.nitems > 0) b->pclasses.push_back(pc);
+    }
     b->ngitems = (int64_t)gitems.size();
     b->npitems = (int64_t)pitems.size();
-    b->L = peak_lds_layout(gmax);
-    b->lds_peaks = (size_t)b->L.total * 4;
-    (void)gcost;
     if ((rc = dev_upload(&b->d_gsets, gs.data(), gs.size()))) { batch_free(b); return rc; }
     if ((rc = dev_upload(&b->d_variant, var.data(), var.size()))) { batch_free(b); return rc; }
     if ((rc = dev_upload(&b->d_gitems, gitems.data(), gitems.size()))) { batch_free(b); return rc; }
@@ -1809,16 +1814,29 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         batch_free(b);
         return fail(-3, "batched gather kernel LDS limit");
     }
-    const void *pk = b->kr == 9 ? (const void *)k_peaks<9, 5, true>
-                   : b->kr == 5 ? (const void *)k_peaks<5, 5, true> : (const void *)k_peaks<0, 0, true>;
-    occ = 0;
-    if (hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds_peaks) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pk, b->pthreads, b->lds_peaks) != hipSuccess ||
-        occ < 1) {
-        batch_free(b);
-        return fail(-3, "batched peak kernel cannot be resident");
+    size_t lds_pk[3] = {0, 0, 0};
+    for (PeakClass &pc : b->pclasses) {
+        const int k = pc.kr == 9 ? 0 : pc.kr == 5 ? 1 : 2;
+        const void *pk = k == 0 ? (const void *)k_peaks<9, 5, true>
+                       : k == 1 ? (const void *)k_peaks<5, 5, true> : (const void *)k_peaks<0, 0, true>;
+        occ = 0;
+        if (hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pc.lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pk, pc.threads, pc.lds) != hipSuccess || occ < 1) {
+            batch_free(b);
+            return fail(-3, "batched peak kernel cannot be resident");
+        }
+        pc.grid = (int)std::min<int64_t>(pc.nitems, (int64_t)occ * b->ncu);
+        lds_pk[k] = std::max(lds_pk[k], pc.lds);
     }
-    b->grid_p = (int)std::min<int64_t>(b->npitems, (int64_t)occ * b->ncu);
+    // each instantiation's dynamic-LDS limit covers all of its classes
+    const void *pks[3] = {(const void *)k_peaks<9, 5, true>, (const void *)k_peaks<5, 5, true>,
+                          (const void *)k_peaks<0, 0, true>};
+    for (int k = 0; k < 3; ++k)
+        if (lds_pk[k] > 0 &&
+            hipFuncSetAttribute(pks[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pk[k]) != hipSuccess) {
+            batch_free(b);
+            return fail(-3, "batched peak kernel LDS limit");
+        }
     *out = b;
     return 0;
 }
@@ -1834,19 +1852,19 @@ static int batch_launch(ebsd_batch *b, hipStream_t s)
         hipLaunchKernelGGL(k_radon_gather_multi, dim3(gc.grid), dim3(gc.threads), gc.lds, s, gm);
         HIPCHK(hipGetLastError());
     }
-    if (b->npitems > 0) {
+    for (const PeakClass &pc : b->pclasses) {
         PeakArgs a{};
-        PeakMultiArgs m;
-        m.sets = b->d_psets;
-        m.items = b->d_pitems;
-        m.nitems = b->npitems;
-        m.L = b->L;
-        if (b->kr == 9)
-            hipLaunchKernelGGL((k_peaks<9, 5, true>), dim3(b->grid_p), dim3(b->pthreads), b->lds_peaks, s, a, m);
-        else if (b->kr == 5)
-            hipLaunchKernelGGL((k_peaks<5, 5, true>), dim3(b->grid_p), dim3(b->pthreads), b->lds_peaks, s, a, m);
+        a.sets = b->d_psets;
+        a.items = b->d_pitems + pc.item0;
+        a.nitems = pc.nitems;
+        a.L = pc.L;
+        a.g = pc.g;
+        if (pc.kr == 9)
+            hipLaunchKernelGGL((k_peaks<9, 5, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
+        else if (pc.kr == 5)
+            hipLaunchKernelGGL((k_peaks<5, 5, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
         else
-            hipLaunchKernelGGL((k_peaks<0, 0, true>), dim3(b->grid_p), dim3(b->pthreads), b->lds_peaks, s, a, m);
+            hipLaunchKernelGGL((k_peaks<0, 0, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
         HIPCHK(hipGetLastError());
     }
     return 0;
