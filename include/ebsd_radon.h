@@ -76,6 +76,12 @@ int ebsd_plan_set_background(ebsd_plan *plan, const double *bg, int exact);
 int ebsd_plan_set_tuning(ebsd_plan *plan, int gather_pack, int gather_threads, int peak_tile,
                          int peak_threads);
 
+/* Which convolution/peak kernel runs the stages of band_detect.py:562-809 (same results):
+ *   0 automatic (the streamed kernel where it applies), 1 theta-tiled work-group kernel,
+ *   2 streamed kernel (one wavefront per block of theta columns; needs a 9x5 or 5x5 kernel
+ *     with peakPad = 2*ksz+1 and at most 16 wavefronts a pattern). */
+int ebsd_plan_set_peak_kernel(ebsd_plan *plan, int kind);
+
 /* Workspace (device bytes) that ebsd_find_bands needs to process `chunk` patterns per pass. */
 size_t ebsd_workspace_bytes(const ebsd_plan *plan, int64_t chunk);
 

@@ -60,6 +60,7 @@ struct PeakGeo {
     int tile, ntiles;
     int rh, apitch, bpitch, maskw; // LDS tile rows, pitches of the two tile buffers, mask words per row
     int nB;
+    int sbw, mcw;         // streamed peak kernel: theta columns per wavefront (0: not used), mask words
     float u[KMAX], v[KMAX]; // C = sum_a u[a] sum_b v[b] Rpad[r + kr/2 - a][c + kt/2 - b]
 };
 
@@ -85,10 +86,13 @@ struct ebsd_plan {
     // peaks
     PeakGeo g{};
     uint8_t *d_mask = nullptr;   // nR*nT
+    uint32_t *d_mcol = nullptr;  // rho-mask bits per theta column (PeakGeo::mcw words each)
+    uint32_t *d_mrow = nullptr;  // rho-mask bits per rho row (PeakGeo::maskw words each)
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
     // tuning
     int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 1024;
     int ptile = 45, pthreads = 512;
+    int pkind = 0; // peak kernel: 0 automatic, 1 tiled (k_peaks), 2 streamed (k_peaks_stream)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -419,6 +423,8 @@ struct PeakPtrs {
     int64_t N;
     const uint8_t *mask; // (nR, nT)
     const double *grid;  // theta_deg[nT], rho[nR]
+    const uint32_t *mcol; // rho-mask bits per theta column, rows -32 .. (mcw words a column)
+    const uint32_t *mrow; // rho-mask bits per rho row (maskw words a row)
     ebsd_bands out;
 };
 
@@ -593,18 +599,33 @@ __device__ __forceinline__ float tile_elem(const PeakGeo &g, const float *R, con
     return 0.0f;
 }
 
+// Raw load for staged element (y, x), clamped into the padded array so that the address is
+// always valid; tile_commit zeroes what lies outside.  No branches: the loads of a whole
+// prefetch stay in flight together.
+__device__ __forceinline__ float tile_load(const PeakGeo &g, const float *R, const TileGeo &T, int y, int x)
+{
+    int r = g.cr0 - g.kr / 2 + y, c = T.cb0 + x;
+    r = r < 0 ? 0 : (r > g.nRp - 1 ? g.nRp - 1 : r);
+    c = c < 0 ? 0 : (c > g.nTp - 1 ? g.nTp - 1 : c);
+    return __builtin_nontemporal_load(R + pad_src(g, r, c));
+}
+
+__device__ __forceinline__ bool tile_inside(const PeakGeo &g, const TileGeo &T, int y, int x)
+{
+    const int r = g.cr0 - g.kr / 2 + y, c = T.cb0 + x;
+    return r >= 0 && r < g.nRp && c >= 0 && c < g.nTp;
+}
+
 __device__ __forceinline__ void tile_fetch(const PeakGeo &g, const float *R, const TileGeo &T, int RH, int tid,
                                            int nthr, float *pf)
 {
     const FDiv f = fdiv_make(T.RW);
+    const int last = RH * T.RW - 1;
 #pragma unroll
     for (int u = 0; u < PF_REGS; ++u) {
-        const int it = tid + u * nthr;
-        pf[u] = 0.0f;
-        if (it < RH * T.RW) {
-            const int y = (int)fdiv_q(it, f), x = it - y * T.RW;
-            pf[u] = tile_elem(g, R, T, y, x);
-        }
+        const int it = min(tid + u * nthr, last);
+        const int y = (int)fdiv_q(it, f), x = it - y * T.RW;
+        pf[u] = tile_load(g, R, T, y, x);
     }
 }
 
@@ -617,7 +638,7 @@ __device__ __forceinline__ void tile_commit(const PeakGeo &g, const float *R, co
         const int it = tid + u * nthr;
         if (it < RH * T.RW) {
             const int y = (int)fdiv_q(it, f), x = it - y * T.RW;
-            A[y * AP + x] = pf[u];
+            A[y * AP + x] = tile_inside(g, T, y, x) ? pf[u] : 0.0f;
         }
     }
     // elements beyond the register window (large Radons), 8 loads in flight
@@ -640,6 +661,86 @@ __device__ __forceinline__ void tile_commit(const PeakGeo &g, const float *R, co
         for (int u = 0; u < U; ++u)
             if (dst[u] >= 0) A[dst[u]] = val[u];
     }
+}
+
+// One band slot of one pattern: width, 3x3 quadratic sub-pixel fit, normmax, theta/rho
+// (band_detect.py:736-800, then :457-460 and :521-524).  Slot i of the top-nB list
+// (v, p, nn) sorted by (value, position) descending; nsel entries are filled.
+__device__ __forceinline__ void finalize_band(const PeakGeo &g, const PeakPtrs &S, int64_t q, int i, int nsel,
+                                           const float *run_v, const int *run_p, const float *run_nn, float mn,
+                                           float ave)
+{
+    const int64_t o = q * g.nB + i;
+    float bmax = -2.0e6f, bavemax = -2.0e6f, width = 0.0f;
+    float ml_r = 0.0f, ml_c = 0.0f, al_r = 0.0f, al_c = 0.0f;
+    int8_t valid = 0;
+    if (i < nsel && run_v[i] > 0.0f) {
+        const int pos = run_p[i];
+        const int r = pos / g.nTp, c = pos - (pos / g.nTp) * g.nTp;
+        float nn[9];
+        for (int k = 0; k < 9; ++k) nn[k] = run_nn[i * 9 + k];
+        const float mx = run_v[i];
+        ml_r = (float)r;
+        ml_c = (float)c;
+        bmax = mx;
+        // width (band_detect.py:748-753)
+        const float mnt = fminf(fminf(nn[1], nn[4]), nn[7]);
+        if (mnt > 0.0f) {
+            const float lp = logf(nn[7]), lm = logf(nn[1]), l0 = logf(mx);
+            const double aa = (double)(lp - lm) * 0.5 - (double)l0;
+            if (aa < -1.e-8) width = (float)(2.0 * sqrt(log(2.0) / (-1.0 * aa)));
+        }
+        // 3x3 quadratic sub-pixel fit (band_detect.py:764-798)
+        float s = 0.0f;
+        for (int k = 0; k < 9; ++k) s += nn[k];
+        const double sumnn = s > 1e-12f ? (double)s : 1e-12;
+        for (int k = 0; k < 9; ++k) nn[k] = (float)((double)nn[k] / sumnn);
+        bavemax = (float)(sumnn / 9.0);
+        const double dx = 0.5 * (double)(nn[5] - nn[3]);
+        const double dy = 0.5 * (double)(nn[7] - nn[1]);
+        const double dxx = (double)(nn[5] + nn[3]) - 2.0 * (double)nn[4];
+        const double dyy = (double)(nn[7] + nn[1]) - 2.0 * (double)nn[4];
+        const double dxy = 0.25 * (double)(((nn[8] - nn[2]) - nn[6]) + nn[0]);
+        double det = dxx * dyy - dxy * dxy;
+        det = fabs(det) > 1e-12 ? det : 1.0e-12;
+        det = 1.0 / det;
+        double dc = (dyy * dx - dxy * dy) * det;
+        double rc = (dxx * dy - dxy * dx) * det;
+        if (fabs(dc) > 0.875 || fabs(rc) > 0.875) {
+            det = dxx * dyy;
+            det = fabs(det) > 1e-12 ? det : 1.0e-12;
+            det = 1.0 / det;
+            dc = (dyy * dx) * det;
+            rc = (dxx * dy) * det;
+            if (fabs(dc) > 0.875 || fabs(rc) > 0.875) { dc = 0.0; rc = 0.0; }
+        }
+        al_r = (float)((double)r - rc);
+        al_c = (float)((double)c - dc);
+        valid = 1;
+    }
+    // maxloc/aveloc -= padding (float32 - int64 in float64, stored as float32)
+    ml_r = (float)((double)ml_r - (double)g.p0);
+    ml_c = (float)((double)ml_c - (double)g.p1);
+    al_r = (float)((double)al_r - (double)g.p0);
+    al_c = (float)((double)al_c - (double)g.p1);
+    float d = ave - mn;
+    d = d < 1e-7f ? 1e-7f : d;
+    const float nm = (bmax - mn) / d;
+    const double *thg = S.grid, *rhg = S.grid + g.nT;
+    const double th = 3.141592653589793 - np_interp_unit((double)al_c, thg, g.nT) /
+                                              (180.0 / 3.141592653589793);
+    const double rh = -1.0 * np_interp_unit((double)al_r, rhg, g.nR);
+    S.out.d_max[o] = bmax;
+    S.out.d_normmax[o] = nm;
+    S.out.d_avemax[o] = bavemax;
+    S.out.d_maxloc[2 * o + 0] = ml_r;
+    S.out.d_maxloc[2 * o + 1] = ml_c;
+    S.out.d_aveloc[2 * o + 0] = al_r;
+    S.out.d_aveloc[2 * o + 1] = al_c;
+    S.out.d_width[o] = width;
+    S.out.d_theta[o] = (float)th;
+    S.out.d_rho[o] = (float)rh;
+    S.out.d_valid[o] = valid;
 }
 
 constexpr int CSTRIP = 8; // rows per thread in the rho passes
@@ -701,16 +802,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 
         // ---- rho mask as bits, reloaded when the set changes ----
         if (sidx != cur_set) {
-            const FDiv fw = fdiv_make(g.maskw);
-            for (int i2 = tid; i2 < g.nR * g.maskw; i2 += nthr) {
-                const int i = (int)fdiv_q(i2, fw), wd = i2 - i * g.maskw;
-                uint32_t bits = 0u;
-                for (int b = 0; b < 32; ++b) {
-                    const int j = wd * 32 + b;
-                    if (j < g.nT && S.mask[i * g.nT + j]) bits |= 1u << b;
-                }
-                mbits[i2] = bits;
-            }
+            for (int i2 = tid; i2 < g.nR * g.maskw; i2 += nthr) mbits[i2] = S.mrow[i2];
             cur_set = sidx;
         }
 
@@ -994,83 +1086,359 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         __syncthreads();
 
         // ---- finalize: band_detect.py:736-800, then :457-460 and :521-524 ----
-        if (tid < g.nB) {
-            const int i = tid;
-            const int64_t o = q * g.nB + i;
-            const float mn = red[0], ave = red[1];
-            float bmax = -2.0e6f, bavemax = -2.0e6f, width = 0.0f;
-            float ml_r = 0.0f, ml_c = 0.0f, al_r = 0.0f, al_c = 0.0f;
-            int8_t valid = 0;
-            if (i < ctr[0] && run_v[i] > 0.0f) {
-                const int pos = run_p[i];
-                const int r = pos / g.nTp, c = pos - (pos / g.nTp) * g.nTp;
-                float nn[9];
-                for (int k = 0; k < 9; ++k) nn[k] = run_nn[i * 9 + k];
-                const float mx = run_v[i];
-                ml_r = (float)r;
-                ml_c = (float)c;
-                bmax = mx;
-                // width (band_detect.py:748-753)
-                const float mnt = fminf(fminf(nn[1], nn[4]), nn[7]);
-                if (mnt > 0.0f) {
-                    const float lp = logf(nn[7]), lm = logf(nn[1]), l0 = logf(mx);
-                    const double aa = (double)(lp - lm) * 0.5 - (double)l0;
-                    if (aa < -1.e-8) width = (float)(2.0 * sqrt(log(2.0) / (-1.0 * aa)));
-                }
-                // 3x3 quadratic sub-pixel fit (band_detect.py:764-798)
-                float s = 0.0f;
-                for (int k = 0; k < 9; ++k) s += nn[k];
-                const double sumnn = s > 1e-12f ? (double)s : 1e-12;
-                for (int k = 0; k < 9; ++k) nn[k] = (float)((double)nn[k] / sumnn);
-                bavemax = (float)(sumnn / 9.0);
-                const double dx = 0.5 * (double)(nn[5] - nn[3]);
-                const double dy = 0.5 * (double)(nn[7] - nn[1]);
-                const double dxx = (double)(nn[5] + nn[3]) - 2.0 * (double)nn[4];
-                const double dyy = (double)(nn[7] + nn[1]) - 2.0 * (double)nn[4];
-                const double dxy = 0.25 * (double)(((nn[8] - nn[2]) - nn[6]) + nn[0]);
-                double det = dxx * dyy - dxy * dxy;
-                det = fabs(det) > 1e-12 ? det : 1.0e-12;
-                det = 1.0 / det;
-                double dc = (dyy * dx - dxy * dy) * det;
-                double rc = (dxx * dy - dxy * dx) * det;
-                if (fabs(dc) > 0.875 || fabs(rc) > 0.875) {
-                    det = dxx * dyy;
-                    det = fabs(det) > 1e-12 ? det : 1.0e-12;
-                    det = 1.0 / det;
-                    dc = (dyy * dx) * det;
-                    rc = (dxx * dy) * det;
-                    if (fabs(dc) > 0.875 || fabs(rc) > 0.875) { dc = 0.0; rc = 0.0; }
-                }
-                al_r = (float)((double)r - rc);
-                al_c = (float)((double)c - dc);
-                valid = 1;
-            }
-            // maxloc/aveloc -= padding (float32 - int64 in float64, stored as float32)
-            ml_r = (float)((double)ml_r - (double)g.p0);
-            ml_c = (float)((double)ml_c - (double)g.p1);
-            al_r = (float)((double)al_r - (double)g.p0);
-            al_c = (float)((double)al_c - (double)g.p1);
-            float d = ave - mn;
-            d = d < 1e-7f ? 1e-7f : d;
-            const float nm = (bmax - mn) / d;
-            const double *thg = S.grid, *rhg = S.grid + g.nT;
-            const double th = 3.141592653589793 - np_interp_unit((double)al_c, thg, g.nT) /
-                                                      (180.0 / 3.141592653589793);
-            const double rh = -1.0 * np_interp_unit((double)al_r, rhg, g.nR);
-            S.out.d_max[o] = bmax;
-            S.out.d_normmax[o] = nm;
-            S.out.d_avemax[o] = bavemax;
-            S.out.d_maxloc[2 * o + 0] = ml_r;
-            S.out.d_maxloc[2 * o + 1] = ml_c;
-            S.out.d_aveloc[2 * o + 0] = al_r;
-            S.out.d_aveloc[2 * o + 1] = al_c;
-            S.out.d_width[o] = width;
-            S.out.d_theta[o] = (float)th;
-            S.out.d_rho[o] = (float)rh;
-            S.out.d_valid[o] = valid;
-        }
+        if (tid < g.nB) finalize_band(g, S, q, tid, ctr[0], run_v, run_p, run_nn, red[0], red[1]);
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_peaks_stream: the same stages as k_peaks, streamed down the rho axis in registers.
+// One work-group per pattern; wavefront w owns a block of `sbw` interior theta columns
+// (one padded column per lane, with a halo of KT/2 + HT lanes each side).  Per strip of
+// SROWS rows a lane stages its padded-Radon column, convolves along theta with DPP lane
+// shifts and along rho from a register ring, dilates along theta with DPP and along rho
+// from a second ring, and tests D == C against the rho mask.  Only the candidate lists,
+// the per-wave statistics and the final top-nB pass through LDS; the work-group
+// synchronises twice per pattern.
+// ---------------------------------------------------------------------------------------
+constexpr int SROWS = 8; // rows per strip
+constexpr int WCAP = 64; // candidates per wavefront list (>= the useful lanes of one row)
+
+__device__ __forceinline__ float lane_next(float x) // lane i receives lane i+1 (column c+1)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_prev(float x) // lane i receives lane i-1 (column c-1)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+
+// LDS layout of k_peaks_stream (4-byte words), per wavefront w of nw:
+//   list: v[WCAP], p[WCAP], nn[9*WCAP]; run: v[nB], p[nB], nn[9*nB]; sel: same as run;
+//   red: min, max, sum (double); cnt: running-list count; mask: [mcw][64] rho-mask words
+// then the final list (v, p, nn) of nB and the pattern statistics.
+struct StreamLds {
+    int per_wave, list_v, list_p, list_nn, run_v, run_p, run_nn, sel_v, sel_p, sel_nn, red, cnt, mask, fin_v,
+        fin_p, fin_nn, stats, total;
+};
+__host__ __device__ inline StreamLds stream_lds_layout(int nw, int nB, int mcw)
+{
+    StreamLds L;
+    L.list_v = 0;
+    L.list_p = WCAP;
+    L.list_nn = 2 * WCAP;
+    L.run_v = 11 * WCAP;
+    L.run_p = L.run_v + nB;
+    L.run_nn = L.run_p + nB;
+    L.sel_v = L.run_nn + 9 * nB;
+    L.sel_p = L.sel_v + nB;
+    L.sel_nn = L.sel_p + nB;
+    L.red = L.sel_nn + 9 * nB;
+    L.red += L.red & 1;        // 8-byte alignment of the double sum
+    L.cnt = L.red + 4;         // running-list count
+    L.mask = L.cnt + 2;        // rho-mask words of the wavefront's columns, [mcw][64]
+    L.per_wave = L.mask + 64 * mcw; // even: every wave's red stays 8-byte aligned
+    L.fin_v = nw * L.per_wave;
+    L.fin_p = L.fin_v + nB;
+    L.fin_nn = L.fin_p + nB;
+    L.stats = L.fin_nn + 9 * nB;
+    L.total = L.stats + 4;
+    return L;
+}
+
+// Wave-local: the top-nB of (running list U candidate list) by (value, position) replace
+// the running list; returns the new running count.  The whole wavefront calls it.
+__device__ __forceinline__ int wave_merge(int lane, int nB, int nrun, int ncand, float *run_v, int *run_p,
+                                       float *run_nn, const float *lst_v, const int *lst_p, const float *lst_nn,
+                                       float *sel_v, int *sel_p, float *sel_nn)
+{
+    const int ntot = nrun + ncand;
+    for (int i = lane; i < ntot; i += 64) {
+        const bool ir = i < nrun;
+        const float v = ir ? run_v[i] : lst_v[i - nrun];
+        const int p = ir ? run_p[i] : lst_p[i - nrun];
+        int rank = 0;
+        for (int j = 0; j < nrun; ++j) rank += key_gt(run_v[j], run_p[j], v, p);
+        for (int j = 0; j < ncand && rank < nB; ++j) rank += key_gt(lst_v[j], lst_p[j], v, p);
+        if (rank < nB) {
+            sel_v[rank] = v;
+            sel_p[rank] = p;
+            const float *nn = ir ? run_nn + 9 * i : lst_nn + 9 * (i - nrun);
+            for (int k = 0; k < 9; ++k) sel_nn[9 * rank + k] = nn[k];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int nsel = min(nB, ntot);
+    for (int i = lane; i < nsel * 11; i += 64) {
+        const int e = i / 11, k = i - e * 11;
+        if (k == 0) run_v[e] = sel_v[e];
+        else if (k == 1) run_p[e] = sel_p[e];
+        else run_nn[9 * e + k - 2] = sel_nn[9 * e + k - 2];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return nsel;
+}
+
+template <int KR, int KT, bool MULTI>
+__global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
+{
+    constexpr int CKR = KR / 2, CKT = KT / 2, HR = KR, HT = KT; // dilation half-windows = kernel sizes
+    constexpr int HALO = CKT + HT;
+    constexpr int TRING = SROWS + 2 * CKR;            // theta-convolved rows r0-CKR .. r0+SROWS-1+CKR
+    constexpr int CRING = SROWS + HR + 1;             // convolved rows r0-HR-1 .. r0+SROWS-1
+    constexpr int HRING = SROWS + 2 * HR;             // theta-dilated rows r0-2HR .. r0+SROWS-1
+    static_assert(2 * CKR <= SROWS && SROWS - 1 <= 2 * HR, "strip shorter than the rho windows");
+    extern __shared__ __align__(16) float sm[];
+    const PeakGeo &g = a.g;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    const int64_t it = blockIdx.x;
+    const int sidx = MULTI ? a.items[it].x : 0;
+    const PeakPtrs &S = MULTI ? a.sets[sidx] : a.s;
+    const int64_t q = MULTI ? (int64_t)a.items[it].y : it;
+    const int nB = g.nB;
+    const StreamLds L = stream_lds_layout(nw, nB, g.mcw);
+    float *wl = sm + wave * L.per_wave;
+    float *lst_v = wl + L.list_v, *lst_nn = wl + L.list_nn;
+    int *lst_p = reinterpret_cast<int *>(wl + L.list_p);
+    float *run_v = wl + L.run_v, *run_nn = wl + L.run_nn;
+    int *run_p = reinterpret_cast<int *>(wl + L.run_p);
+    float *sel_v = wl + L.sel_v, *sel_nn = wl + L.sel_nn;
+    int *sel_p = reinterpret_cast<int *>(wl + L.sel_p);
+
+    // ---- this lane's padded column and its source in the unpadded Radon ----
+    const int c = g.p1 + wave * g.sbw - HALO + lane;
+    const bool cval = c >= 0 && c < g.nTp;
+    const int jint = c - g.p1;                              // interior column index
+    const bool useful = lane >= HALO && lane < HALO + g.sbw && jint < g.nT;
+    int jc = jint;
+    bool flip = false;
+    if (c < g.p1) { jc = g.nT - g.p1 + c; flip = true; }
+    else if (c >= g.p1 + g.nT) { jc = c - g.p1 - g.nT; flip = true; }
+    if (!cval) { jc = 0; flip = false; }
+    const float *Rq = S.radon + q * (int64_t)(g.nR * g.nT) + jc;
+    const uint32_t *mcol = S.mcol + (size_t)(useful ? jint : 0) * g.mcw;
+
+    // staged value of padded row r: issued unconditionally (the row is clamped into the
+    // array, so the address is always valid) and masked where it is consumed, which keeps
+    // the loads of a whole strip in flight together
+    auto stage_load = [&](int r) -> float {
+        int i = r - g.p0;
+        i = i < 0 ? 0 : (i > g.nR - 1 ? g.nR - 1 : i);
+        const int i2 = g.nR - 1 - i;
+        return __builtin_nontemporal_load(Rq + (flip ? i2 : i) * g.nT);
+    };
+
+    // this wavefront's mask columns, lane-major in LDS (read without touching the vector
+    // memory counter that the prefetched Radon rows are waited on with)
+    uint32_t *wmask = reinterpret_cast<uint32_t *>(wl + L.mask);
+    for (int w = 0; w < g.mcw; ++w) {
+        const uint32_t mv = mcol[w];
+        wmask[w * 64 + lane] = useful ? mv : 0u;
+    }
+
+    float Tr[TRING], Cr[CRING], Hr[HRING], pf[SROWS];
+#pragma unroll
+    for (int k = 0; k < TRING; ++k) Tr[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < CRING; ++k) Cr[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < HRING; ++k) Hr[k] = -INFINITY;
+    float smin = INFINITY, smax = -INFINITY;
+    double ssum = 0.0;
+    int nrun = 0, ncand = 0;
+
+    const int rbeg = g.cr0 - SROWS, rend = max(g.cr1, g.mr1 + HR);
+#pragma unroll
+    for (int k = 0; k < SROWS; ++k) pf[k] = stage_load(rbeg + CKR + k);
+
+    for (int r0 = rbeg; r0 < rend; r0 += SROWS) {
+        // the rdnmask bits of this strip's dilation rows d0 .. d0+7 (words cover rows -32 ..)
+        const int d0 = r0 - HR;
+        const int mi0 = min(max(d0 - g.p0 + 32, 0), 32 * (g.mcw - 1) - 1); // word mi0>>5 <= mcw-2
+        const uint32_t mlo = wmask[(mi0 >> 5) * 64 + lane], mhi = wmask[((mi0 >> 5) + 1) * 64 + lane];
+        // ---- stage rows r0+CKR .. r0+CKR+SROWS-1 (zero outside the padded array); fetch
+        //      the next strip's ----
+        float x[SROWS];
+#pragma unroll
+        for (int k = 0; k < SROWS; ++k) {
+            const int r = r0 + CKR + k;
+            x[k] = (cval && r >= 0 && r < g.nRp) ? pf[k] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < SROWS; ++k) pf[k] = stage_load(r0 + SROWS + CKR + k);
+        // ---- theta convolution: T[c] = sum_b v[b] S[c + CKT - b] (fma order of k_peaks) ----
+#pragma unroll
+        for (int k = 0; k < SROWS; ++k) {
+            float sh[KT];
+            sh[CKT] = x[k];
+#pragma unroll
+            for (int d = 1; d <= CKT; ++d) {
+                sh[CKT + d] = lane_next(sh[CKT + d - 1]);
+                sh[CKT - d] = lane_prev(sh[CKT - d + 1]);
+            }
+            float s = 0.0f;
+#pragma unroll
+            for (int b = 0; b < KT; ++b) s = fmaf(g.v[b], sh[KT - 1 - b], s);
+            Tr[2 * CKR + k] = s;
+        }
+        // ---- rho convolution: C[r] = sum_a u[a] T[r + CKR - a]; statistics; theta dilation ----
+#pragma unroll
+        for (int k = 0; k < SROWS; ++k) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int aa = 0; aa < KR; ++aa) acc = fmaf(g.u[aa], Tr[k + 2 * CKR - aa], acc);
+            const int r = r0 + k;
+            Cr[HR + 1 + k] = acc;
+            if (useful && r >= g.sr0 && r < g.sr1) {
+                smin = fminf(smin, acc);
+                smax = fmaxf(smax, acc);
+                ssum += (double)acc;
+            }
+            const float cm = (cval && r >= g.cr0 && r < g.cr1) ? acc : -INFINITY;
+            float lft = cm, rgt = cm;
+#pragma unroll
+            for (int d = 0; d < HT; ++d) {
+                lft = fmaxf(cm, lane_prev(lft));
+                rgt = fmaxf(cm, lane_next(rgt));
+            }
+            Hr[2 * HR + k] = fmaxf(lft, rgt);
+        }
+        // ---- rho dilation of rows d = r0 - HR + k and the candidate test ----
+        if (d0 + SROWS > g.mr0 && d0 < g.mr1) {
+            // (in this branch d0 - p0 + 32 >= 25: mi0 is not clamped)
+            const uint64_t v64 = (uint64_t)mlo | ((uint64_t)mhi << 32);
+            const uint32_t bits = useful ? (uint32_t)(v64 >> (mi0 & 31)) & ((1u << SROWS) - 1u) : 0u;
+            // windows Hr[k .. k + 2HR]; all strips rows share Hr[SROWS-1 .. 2HR]
+            float mid = Hr[SROWS - 1];
+#pragma unroll
+            for (int k = SROWS; k <= 2 * HR; ++k) mid = fmaxf(mid, Hr[k]);
+            float lf[SROWS], rg[SROWS];
+            lf[SROWS - 1] = -INFINITY;
+#pragma unroll
+            for (int k = SROWS - 2; k >= 0; --k) lf[k] = fmaxf(Hr[k], lf[k + 1]);
+            rg[0] = -INFINITY;
+#pragma unroll
+            for (int k = 1; k < SROWS; ++k) rg[k] = fmaxf(rg[k - 1], Hr[2 * HR + k]);
+            uint32_t cand = 0u;
+#pragma unroll
+            for (int k = 0; k < SROWS; ++k) {
+                const float dk = fmaxf(mid, fmaxf(lf[k], rg[k]));
+                cand |= (((bits >> k) & 1u) && dk == Cr[k + 1]) ? (1u << k) : 0u;
+            }
+            if (__ballot(cand != 0u)) {
+#pragma unroll
+                for (int k = 0; k < SROWS; ++k) {
+                    const bool ck = (cand >> k) & 1u;
+                    const uint64_t bal = __ballot(ck);
+                    if (bal == 0) continue;
+                    const int n = __popcll(bal);
+                    if (ncand + n > WCAP) {
+                        nrun = wave_merge(lane, nB, nrun, ncand, run_v, run_p, run_nn, lst_v, lst_p, lst_nn, sel_v,
+                                          sel_p, sel_nn);
+                        ncand = 0;
+                    }
+                    // 3x3 neighbourhood: rows d-1, d, d+1 = Cr[k], Cr[k+1], Cr[k+2]; columns c-1, c, c+1
+                    float nn[9];
+#pragma unroll
+                    for (int dr = 0; dr < 3; ++dr) {
+                        const float cv = Cr[k + dr];
+                        nn[dr * 3 + 0] = lane_prev(cv);
+                        nn[dr * 3 + 1] = cv;
+                        nn[dr * 3 + 2] = lane_next(cv);
+                    }
+                    if (ck) {
+                        const int slot = ncand + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                        lst_v[slot] = nn[4];
+                        lst_p[slot] = (d0 + k) * g.nTp + c;
+#pragma unroll
+                        for (int e = 0; e < 9; ++e) lst_nn[9 * slot + e] = nn[e];
+                    }
+                    ncand += n;
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        // ---- advance the rings by one strip ----
+#pragma unroll
+        for (int k = 0; k < TRING - SROWS; ++k) Tr[k] = Tr[k + SROWS];
+#pragma unroll
+        for (int k = 0; k < CRING - SROWS; ++k) Cr[k] = Cr[k + SROWS];
+#pragma unroll
+        for (int k = 0; k < HRING - SROWS; ++k) Hr[k] = Hr[k + SROWS];
+    }
+    // ---- per wavefront: top-nB of its candidates, statistics ----
+    nrun = wave_merge(lane, nB, nrun, ncand, run_v, run_p, run_nn, lst_v, lst_p, lst_nn, sel_v, sel_p, sel_nn);
+    for (int off = 32; off > 0; off >>= 1) {
+        smin = fminf(smin, __shfl_xor(smin, off, 64));
+        smax = fmaxf(smax, __shfl_xor(smax, off, 64));
+        ssum += __shfl_xor(ssum, off, 64);
+    }
+    if (lane == 0) {
+        wl[L.red + 0] = smin;
+        wl[L.red + 1] = smax;
+        *reinterpret_cast<double *>(wl + L.red + 2) = ssum;
+        reinterpret_cast<int *>(wl)[L.cnt] = nrun;
+    }
+    __syncthreads();
+    // ---- top-nB over the wavefronts' lists (wave 0), statistics (thread 0) ----
+    float *fin_v = sm + L.fin_v, *fin_nn = sm + L.fin_nn, *stats = sm + L.stats;
+    int *fin_p = reinterpret_cast<int *>(sm + L.fin_p);
+    if (wave == 0) {
+        int ntot = 0;
+        for (int w = 0; w < nw; ++w) ntot += reinterpret_cast<const int *>(sm + w * L.per_wave)[L.cnt];
+        for (int i = lane; i < ntot; i += 64) {
+            int w = 0, e = i;
+            for (;;) {
+                const int nr = reinterpret_cast<const int *>(sm + w * L.per_wave)[L.cnt];
+                if (e < nr) break;
+                e -= nr;
+                ++w;
+            }
+            const float *wv = sm + w * L.per_wave + L.run_v;
+            const int *wp = reinterpret_cast<const int *>(sm + w * L.per_wave + L.run_p);
+            const float v = wv[e];
+            const int p = wp[e];
+            int rank = 0;
+            for (int w2 = 0; w2 < nw && rank < nB; ++w2) {
+                const float *v2 = sm + w2 * L.per_wave + L.run_v;
+                const int *p2 = reinterpret_cast<const int *>(sm + w2 * L.per_wave + L.run_p);
+                const int nr2 = reinterpret_cast<const int *>(sm + w2 * L.per_wave)[L.cnt];
+                for (int j = 0; j < nr2; ++j) rank += key_gt(v2[j], p2[j], v, p);
+            }
+            if (rank < nB) {
+                fin_v[rank] = v;
+                fin_p[rank] = p;
+                const float *nn = sm + w * L.per_wave + L.run_nn + 9 * e;
+                for (int k = 0; k < 9; ++k) fin_nn[9 * rank + k] = nn[k];
+            }
+        }
+        if (lane == 0) {
+            reinterpret_cast<int *>(stats)[3] = min(ntot, nB);
+            float mn = sm[L.red], mx = sm[L.red + 1];
+            double sum = *reinterpret_cast<const double *>(sm + L.red + 2);
+            for (int w = 1; w < nw; ++w) {
+                const float *r = sm + w * L.per_wave + L.red;
+                mn = fminf(mn, r[0]);
+                mx = fmaxf(mx, r[1]);
+                sum += *reinterpret_cast<const double *>(r + 2);
+            }
+            const double cnt = (double)(g.sr1 - g.sr0) * (double)(g.sc1 - g.sc0);
+            stats[0] = mn;
+            stats[1] = (float)(sum / cnt);
+            stats[2] = mx;
+            S.out.d_stats[q * 3 + 0] = mn;
+            S.out.d_stats[q * 3 + 1] = stats[1];
+            S.out.d_stats[q * 3 + 2] = mx;
+        }
+    }
+    __syncthreads();
+    if (tid < nB)
+        finalize_band(g, S, q, tid, reinterpret_cast<const int *>(stats)[3], fin_v, fin_p, fin_nn, stats[0],
+                      stats[1]);
 }
 
 // Convolved padded Radon, every padded cell, same arithmetic order as k_peaks.
@@ -1122,6 +1490,8 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     (void)hipFree(p->d_mask);
+    (void)hipFree(p->d_mcol);
+    (void)hipFree(p->d_mrow);
     (void)hipFree(p->d_grid);
     delete p;
 }
@@ -1135,6 +1505,25 @@ static void split_kernel(const float *k, int kr, int kt, float *u, float *v)
     for (int a = 0; a < kr; ++a) u[a] = k[a * kt + ct];
     for (int b = 0; b < kt; ++b) v[b] = (float)((double)k[cr * kt + b] / (double)kc);
 }
+
+// Streamed peak kernel: one wavefront per block of sbw theta columns, each lane one padded
+// column, with KT/2 + HT halo lanes on either side; sbw = 0 where it does not apply
+// (kernel sizes without an instantiation, or more than 16 wavefronts a pattern).
+static void geo_stream(PeakGeo &g)
+{
+    g.mcw = ((g.nR + 31) >> 5) + 2;
+    g.sbw = 0;
+    const bool fixed = g.hr == g.kr && g.ht == g.kt && ((g.kr == 9 && g.kt == 5) || (g.kr == 5 && g.kt == 5));
+    if (!fixed) return;
+    const int usable = 64 - 2 * (g.kt / 2 + g.ht);
+    const int nblk = (g.nT + usable - 1) / usable;
+    if (nblk > 16) return;
+    g.sbw = (g.nT + nblk - 1) / nblk;
+}
+
+static int stream_waves(const PeakGeo &g) { return g.sbw > 0 ? (g.nT + g.sbw - 1) / g.sbw : 0; }
+
+static bool use_stream(const ebsd_plan *p) { return p->g.sbw > 0 && p->pkind != 1; }
 
 static int geo_tiles(ebsd_plan *p)
 {
@@ -1315,7 +1704,22 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
     g.cr0 = std::max(lo, 0);
     g.cr1 = std::min(hi, g.nRp);
     if ((rc = geo_tiles(p))) { plan_free(p); return rc; }
+    geo_stream(g);
     if ((rc = dev_upload(&p->d_mask, rdnmask, (size_t)nCells))) { plan_free(p); return rc; }
+    {
+        // column-major mask bits; bit (i + 32) of column j is rdnmask[i][j]
+        std::vector<uint32_t> mcol((size_t)nTheta * g.mcw, 0u);
+        for (int j = 0; j < nTheta; ++j)
+            for (int i = 0; i < nRho; ++i)
+                if (rdnmask[i * nTheta + j]) mcol[(size_t)j * g.mcw + ((i + 32) >> 5)] |= 1u << ((i + 32) & 31);
+        if ((rc = dev_upload(&p->d_mcol, mcol.data(), mcol.size()))) { plan_free(p); return rc; }
+        // row-major mask bits; bit b of word w of row i is rdnmask[i][32 w + b]
+        std::vector<uint32_t> mrow((size_t)nRho * g.maskw, 0u);
+        for (int i = 0; i < nRho; ++i)
+            for (int j = 0; j < nTheta; ++j)
+                if (rdnmask[i * nTheta + j]) mrow[(size_t)i * g.maskw + (j >> 5)] |= 1u << (j & 31);
+        if ((rc = dev_upload(&p->d_mrow, mrow.data(), mrow.size()))) { plan_free(p); return rc; }
+    }
     std::vector<double> grid(nTheta + nRho);
     for (int j = 0; j < nTheta; ++j) grid[j] = (double)theta_deg[j];
     for (int i = 0; i < nRho; ++i) grid[nTheta + i] = (double)rho[i];
@@ -1354,6 +1758,15 @@ extern "C" int ebsd_plan_set_background(ebsd_plan *p, const double *bg, int exac
         if ((rc = dev_upload(&p->d_linebg, lb.data(), lb.size()))) return rc;
         p->bg_mode = 1;
     }
+    return 0;
+}
+
+extern "C" int ebsd_plan_set_peak_kernel(ebsd_plan *p, int kind)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_peak_kernel: null plan");
+    if (kind < 0 || kind > 2) return fail(-1, "ebsd_plan_set_peak_kernel: kind is 0, 1 or 2");
+    if (kind == 2 && p->g.sbw <= 0) return fail(-1, "ebsd_plan_set_peak_kernel: the streamed kernel does not apply to this plan");
+    p->pkind = kind;
     return 0;
 }
 
@@ -1492,6 +1905,8 @@ static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const e
     a.s.N = N;
     a.s.mask = p->d_mask;
     a.s.grid = p->d_grid;
+    a.s.mcol = p->d_mcol;
+    a.s.mrow = p->d_mrow;
     a.s.out = *out;
     a.g = p->g;
     a.L = peak_lds_layout(p->g);
@@ -1507,9 +1922,36 @@ static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const e
     return 0;
 }
 
+template <int KR, int KT>
+static int launch_peaks_stream_t(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out,
+                                 hipStream_t s)
+{
+    PeakArgs a{};
+    a.s.radon = d_radon;
+    a.s.N = N;
+    a.s.mask = p->d_mask;
+    a.s.grid = p->d_grid;
+    a.s.mcol = p->d_mcol;
+    a.s.mrow = p->d_mrow;
+    a.s.out = *out;
+    a.g = p->g;
+    const int nw = stream_waves(p->g);
+    const size_t lds = (size_t)stream_lds_layout(nw, p->g.nB, p->g.mcw).total * 4;
+    auto kern = k_peaks_stream<KR, KT, false>;
+    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)N), dim3(64 * nw), lds, s, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int launch_peaks(ebsd_plan *p, const float *d_radon, int64_t N, const ebsd_bands *out, hipStream_t s)
 {
     const PeakGeo &g = p->g;
+    if (N <= 0) return 0;
+    if (use_stream(p)) {
+        if (g.kr == 9) return launch_peaks_stream_t<9, 5>(p, d_radon, N, out, s);
+        return launch_peaks_stream_t<5, 5>(p, d_radon, N, out, s);
+    }
     // compile-time kernel sizes when the dilation half-windows equal them (peakPad = 2*ksz+1)
     if (g.kr == 9 && g.kt == 5 && g.hr == 9 && g.ht == 5) return launch_peaks_t<9, 5>(p, d_radon, N, out, s);
     if (g.kr == 5 && g.kt == 5 && g.hr == 5 && g.ht == 5) return launch_peaks_t<5, 5>(p, d_radon, N, out, s);
@@ -1639,6 +2081,7 @@ struct PeakClass {
     size_t lds = 0;
     int kr = 0, kt = 0; // 0 = run-time kernel sizes in the peak kernel
     int threads = 512;
+    bool stream = false; // k_peaks_stream (one work-group per item) instead of k_peaks
     int64_t item0 = 0, nitems = 0; // slice of the batch's peak work list
     int grid = 0;
 };
@@ -1743,6 +2186,8 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         S.N = N[i];
         S.mask = p->d_mask;
         S.grid = p->d_grid;
+        S.mcol = p->d_mcol;
+        S.mrow = p->d_mrow;
         S.out = outs[i];
     }
     // gather items, class by class (4 work-groups per CU first); inside a class the largest
@@ -1777,9 +2222,15 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         const bool fixed = g.hr == g.kr && g.ht == g.kt;
         pc.kr = fixed && g.kt == 5 && (g.kr == 9 || g.kr == 5) ? g.kr : 0;
         pc.kt = pc.kr ? 5 : 0;
+        pc.stream = use_stream(plans[i0]);
+        if (pc.stream) {
+            pc.threads = 64 * stream_waves(g);
+            pc.lds = (size_t)stream_lds_layout(stream_waves(g), g.nB, g.mcw).total * 4;
+        }
         pc.item0 = (int64_t)pitems.size();
         for (int i : order) {
-            if (done[i] || std::memcmp(&plans[i]->g, &g, sizeof(PeakGeo)) != 0) continue;
+            if (done[i] || use_stream(plans[i]) != pc.stream || std::memcmp(&plans[i]->g, &g, sizeof(PeakGeo)) != 0)
+                continue;
             done[i] = 1;
             for (int64_t q = 0; q < N[i]; ++q) pitems.push_back(make_int2(i, (int)q));
         }
@@ -1814,8 +2265,14 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         batch_free(b);
         return fail(-3, "batched gather kernel LDS limit");
     }
-    size_t lds_pk[3] = {0, 0, 0};
+    size_t lds_pk[3] = {0, 0, 0}, lds_st[2] = {0, 0};
     for (PeakClass &pc : b->pclasses) {
+        if (pc.stream) {
+            pc.grid = (int)pc.nitems;
+            const int k = pc.kr == 9 ? 0 : 1;
+            lds_st[k] = std::max(lds_st[k], pc.lds);
+            continue;
+        }
         const int k = pc.kr == 9 ? 0 : pc.kr == 5 ? 1 : 2;
         const void *pk = k == 0 ? (const void *)k_peaks<9, 5, true>
                        : k == 1 ? (const void *)k_peaks<5, 5, true> : (const void *)k_peaks<0, 0, true>;
@@ -1834,6 +2291,13 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     for (int k = 0; k < 3; ++k)
         if (lds_pk[k] > 0 &&
             hipFuncSetAttribute(pks[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pk[k]) != hipSuccess) {
+            batch_free(b);
+            return fail(-3, "batched peak kernel LDS limit");
+        }
+    const void *sks[2] = {(const void *)k_peaks_stream<9, 5, true>, (const void *)k_peaks_stream<5, 5, true>};
+    for (int k = 0; k < 2; ++k)
+        if (lds_st[k] > 0 &&
+            hipFuncSetAttribute(sks[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_st[k]) != hipSuccess) {
             batch_free(b);
             return fail(-3, "batched peak kernel LDS limit");
         }
@@ -1859,7 +2323,11 @@ static int batch_launch(ebsd_batch *b, hipStream_t s)
         a.nitems = pc.nitems;
         a.L = pc.L;
         a.g = pc.g;
-        if (pc.kr == 9)
+        if (pc.stream && pc.kr == 9)
+            hipLaunchKernelGGL((k_peaks_stream<9, 5, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
+        else if (pc.stream)
+            hipLaunchKernelGGL((k_peaks_stream<5, 5, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
+        else if (pc.kr == 9)
             hipLaunchKernelGGL((k_peaks<9, 5, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
         else if (pc.kr == 5)
             hipLaunchKernelGGL((k_peaks<5, 5, true>), dim3(pc.grid), dim3(pc.threads), pc.lds, s, a);
