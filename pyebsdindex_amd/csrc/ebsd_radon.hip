@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1096,10 +1097,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // One work-group per pattern; wavefront w owns a block of `sbw` interior theta columns
 // (one padded column per lane, with a halo of KT/2 + HT lanes each side).  Per strip of
 // SROWS rows a lane stages its padded-Radon column, convolves along theta with DPP lane
-// shifts and along rho from a register ring, dilates along theta with DPP and along rho
-// from a second ring, and tests D == C against the rho mask.  Only the candidate lists,
-// the per-wave statistics and the final top-nB pass through LDS; the work-group
-// synchronises twice per pattern.
+// shifts and along rho from a register ring, then -- on rows that can hold a candidate --
+// dilates along rho from a second ring and along theta with DPP (the separable max in the
+// other order: the same exact result) and tests D == C against the rho mask.  Only the
+// candidate lists, the per-wave statistics and the final top-nB pass through LDS; the
+// work-group synchronises twice per pattern.
 // ---------------------------------------------------------------------------------------
 constexpr int SROWS = 8; // rows per strip
 constexpr int WCAP = 64; // candidates per wavefront list (>= the useful lanes of one row)
@@ -1112,6 +1114,20 @@ __device__ __forceinline__ float lane_prev(float x) // lane i receives lane i-1 
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
 }
+__device__ __forceinline__ int lane_next_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ int lane_prev_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true); }
+
+// Order-preserving integer key of a float (an involution): the dilation maxima and the
+// D == C test run on keys, exact and without the canonicalising moves float max needs
+// for values that arrive through lane shifts.  Keys of -0 and +0 differ; only cells with
+// C > 0 can become bands, so that never changes a result.
+__device__ __forceinline__ int fkey(float x)
+{
+    const int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float fkey_inv(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+constexpr int KEY_MASKED = INT_MIN; // below the key of every float
 
 // LDS layout of k_peaks_stream (4-byte words), per wavefront w of nw:
 //   list: v[WCAP], p[WCAP], nn[9*WCAP]; run: v[nB], p[nB], nn[9*nB]; sel: same as run;
@@ -1187,8 +1203,7 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     constexpr int CKR = KR / 2, CKT = KT / 2, HR = KR, HT = KT; // dilation half-windows = kernel sizes
     constexpr int HALO = CKT + HT;
     constexpr int TRING = SROWS + 2 * CKR;            // theta-convolved rows r0-CKR .. r0+SROWS-1+CKR
-    constexpr int CRING = SROWS + HR + 1;             // convolved rows r0-HR-1 .. r0+SROWS-1
-    constexpr int HRING = SROWS + 2 * HR;             // theta-dilated rows r0-2HR .. r0+SROWS-1
+    constexpr int MRING = SROWS + 2 * HR;             // convolved rows r0-2HR .. r0+SROWS-1
     static_assert(2 * CKR <= SROWS && SROWS - 1 <= 2 * HR, "strip shorter than the rho windows");
     extern __shared__ __align__(16) float sm[];
     const PeakGeo &g = a.g;
@@ -1238,13 +1253,12 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
         wmask[w * 64 + lane] = useful ? mv : 0u;
     }
 
-    float Tr[TRING], Cr[CRING], Hr[HRING], pf[SROWS];
+    float Tr[TRING], pf[SROWS];
+    int Mr[MRING]; // keys of the convolved rows (KEY_MASKED outside the array and the kept rows)
 #pragma unroll
     for (int k = 0; k < TRING; ++k) Tr[k] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < CRING; ++k) Cr[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < HRING; ++k) Hr[k] = -INFINITY;
+    for (int k = 0; k < MRING; ++k) Mr[k] = KEY_MASKED;
     float smin = INFINITY, smax = -INFINITY;
     double ssum = 0.0;
     int nrun = 0, ncand = 0;
@@ -1283,49 +1297,50 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
             for (int b = 0; b < KT; ++b) s = fmaf(g.v[b], sh[KT - 1 - b], s);
             Tr[2 * CKR + k] = s;
         }
-        // ---- rho convolution: C[r] = sum_a u[a] T[r + CKR - a]; statistics; theta dilation ----
+        // ---- rho convolution: C[r] = sum_a u[a] T[r + CKR - a]; statistics; the ring keeps C
+        //      with -inf outside the array and the kept rows (the dilation windows' clamp) ----
 #pragma unroll
         for (int k = 0; k < SROWS; ++k) {
             float acc = 0.0f;
 #pragma unroll
             for (int aa = 0; aa < KR; ++aa) acc = fmaf(g.u[aa], Tr[k + 2 * CKR - aa], acc);
             const int r = r0 + k;
-            Cr[HR + 1 + k] = acc;
-            if (useful && r >= g.sr0 && r < g.sr1) {
+            if (r >= g.sr0 && r < g.sr1) { // every lane; lanes outside the block are dropped below
                 smin = fminf(smin, acc);
                 smax = fmaxf(smax, acc);
                 ssum += (double)acc;
             }
-            const float cm = (cval && r >= g.cr0 && r < g.cr1) ? acc : -INFINITY;
-            float lft = cm, rgt = cm;
-#pragma unroll
-            for (int d = 0; d < HT; ++d) {
-                lft = fmaxf(cm, lane_prev(lft));
-                rgt = fmaxf(cm, lane_next(rgt));
-            }
-            Hr[2 * HR + k] = fmaxf(lft, rgt);
+            Mr[2 * HR + k] = (cval && r >= g.cr0 && r < g.cr1) ? fkey(acc) : KEY_MASKED;
         }
-        // ---- rho dilation of rows d = r0 - HR + k and the candidate test ----
+        // ---- rows d = d0 + k: rho dilation (windows Mr[k .. k + 2HR]), theta dilation,
+        //      D == C under the mask -> candidates ----
         if (d0 + SROWS > g.mr0 && d0 < g.mr1) {
             // (in this branch d0 - p0 + 32 >= 25: mi0 is not clamped)
             const uint64_t v64 = (uint64_t)mlo | ((uint64_t)mhi << 32);
             const uint32_t bits = useful ? (uint32_t)(v64 >> (mi0 & 31)) & ((1u << SROWS) - 1u) : 0u;
-            // windows Hr[k .. k + 2HR]; all strips rows share Hr[SROWS-1 .. 2HR]
-            float mid = Hr[SROWS - 1];
+            // all eight windows share Mr[SROWS-1 .. 2HR]
+            int mid = Mr[SROWS - 1];
 #pragma unroll
-            for (int k = SROWS; k <= 2 * HR; ++k) mid = fmaxf(mid, Hr[k]);
-            float lf[SROWS], rg[SROWS];
-            lf[SROWS - 1] = -INFINITY;
+            for (int k = SROWS; k <= 2 * HR; ++k) mid = max(mid, Mr[k]);
+            int lf[SROWS], rg[SROWS];
+            lf[SROWS - 1] = KEY_MASKED;
 #pragma unroll
-            for (int k = SROWS - 2; k >= 0; --k) lf[k] = fmaxf(Hr[k], lf[k + 1]);
-            rg[0] = -INFINITY;
+            for (int k = SROWS - 2; k >= 0; --k) lf[k] = max(Mr[k], lf[k + 1]);
+            rg[0] = KEY_MASKED;
 #pragma unroll
-            for (int k = 1; k < SROWS; ++k) rg[k] = fmaxf(rg[k - 1], Hr[2 * HR + k]);
+            for (int k = 1; k < SROWS; ++k) rg[k] = max(rg[k - 1], Mr[2 * HR + k]);
             uint32_t cand = 0u;
 #pragma unroll
             for (int k = 0; k < SROWS; ++k) {
-                const float dk = fmaxf(mid, fmaxf(lf[k], rg[k]));
-                cand |= (((bits >> k) & 1u) && dk == Cr[k + 1]) ? (1u << k) : 0u;
+                const int vk = max(mid, max(lf[k], rg[k]));
+                int lft = vk, rgt = vk;
+#pragma unroll
+                for (int d = 0; d < HT; ++d) {
+                    lft = max(vk, lane_prev_i(lft));
+                    rgt = max(vk, lane_next_i(rgt));
+                }
+                const int dk = max(lft, rgt);
+                cand |= (((bits >> k) & 1u) && dk == Mr[k + HR]) ? (1u << k) : 0u;
             }
             if (__ballot(cand != 0u)) {
 #pragma unroll
@@ -1339,11 +1354,12 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
                                           sel_p, sel_nn);
                         ncand = 0;
                     }
-                    // 3x3 neighbourhood: rows d-1, d, d+1 = Cr[k], Cr[k+1], Cr[k+2]; columns c-1, c, c+1
+                    // 3x3 neighbourhood: rows d-1, d, d+1 = Mr[k+HR-1 .. k+HR+1] (inside the kept
+                    // rows and the array for every candidate); columns c-1, c, c+1
                     float nn[9];
 #pragma unroll
                     for (int dr = 0; dr < 3; ++dr) {
-                        const float cv = Cr[k + dr];
+                        const float cv = fkey_inv(Mr[k + HR - 1 + dr]);
                         nn[dr * 3 + 0] = lane_prev(cv);
                         nn[dr * 3 + 1] = cv;
                         nn[dr * 3 + 2] = lane_next(cv);
@@ -1366,12 +1382,15 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
 #pragma unroll
         for (int k = 0; k < TRING - SROWS; ++k) Tr[k] = Tr[k + SROWS];
 #pragma unroll
-        for (int k = 0; k < CRING - SROWS; ++k) Cr[k] = Cr[k + SROWS];
-#pragma unroll
-        for (int k = 0; k < HRING - SROWS; ++k) Hr[k] = Hr[k + SROWS];
+        for (int k = 0; k < MRING - SROWS; ++k) Mr[k] = Mr[k + SROWS];
     }
     // ---- per wavefront: top-nB of its candidates, statistics ----
     nrun = wave_merge(lane, nB, nrun, ncand, run_v, run_p, run_nn, lst_v, lst_p, lst_nn, sel_v, sel_p, sel_nn);
+    if (!useful) {
+        smin = INFINITY;
+        smax = -INFINITY;
+        ssum = 0.0;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         smin = fminf(smin, __shfl_xor(smin, off, 64));
         smax = fmaxf(smax, __shfl_xor(smax, off, 64));
