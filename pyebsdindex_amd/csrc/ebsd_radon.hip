@@ -1222,17 +1222,21 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     float *sel_v = wl + L.sel_v, *sel_nn = wl + L.sel_nn;
     int *sel_p = reinterpret_cast<int *>(wl + L.sel_p);
 
-    // ---- this lane's padded column and its source in the unpadded Radon ----
+    // the pattern's Radon as a buffer resource (word 3: raw 32-bit access on gfx9), formed
+    // before any lane-dependent control flow so that it stays in scalar registers; each
+    // load's row, column and flip form a 32-bit lane offset
+    const __amdgpu_buffer_rsrc_t Rrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(S.radon + q * (int64_t)(g.nR * g.nT)), (short)0, g.nR * g.nT * 4, 0x00020000);
+
+    // ---- this lane's padded column and its source in the unpadded Radon (selects only) ----
     const int c = g.p1 + wave * g.sbw - HALO + lane;
     const bool cval = c >= 0 && c < g.nTp;
     const int jint = c - g.p1;                              // interior column index
     const bool useful = lane >= HALO && lane < HALO + g.sbw && jint < g.nT;
-    int jc = jint;
-    bool flip = false;
-    if (c < g.p1) { jc = g.nT - g.p1 + c; flip = true; }
-    else if (c >= g.p1 + g.nT) { jc = c - g.p1 - g.nT; flip = true; }
-    if (!cval) { jc = 0; flip = false; }
-    const float *Rq = S.radon + q * (int64_t)(g.nR * g.nT) + jc;
+    const bool left = c < g.p1, right = c >= g.p1 + g.nT;
+    const int jc = !cval ? 0 : (left ? g.nT - g.p1 + c : (right ? c - g.p1 - g.nT : jint));
+    const bool flip = cval && (left || right);
+    const int jc4 = jc * 4;
     const uint32_t *mcol = S.mcol + (size_t)(useful ? jint : 0) * g.mcw;
 
     // staged value of padded row r: issued unconditionally (the row is clamped into the
@@ -1241,8 +1245,9 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     auto stage_load = [&](int r) -> float {
         int i = r - g.p0;
         i = i < 0 ? 0 : (i > g.nR - 1 ? g.nR - 1 : i);
-        const int i2 = g.nR - 1 - i;
-        return __builtin_nontemporal_load(Rq + (flip ? i2 : i) * g.nT);
+        // the whole offset in the lane register (the range check ignores the scalar offset)
+        const int row = flip ? g.nR - 1 - i : i;
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Rrs, row * g.nT * 4 + jc4, 0, 0));
     };
 
     // this wavefront's mask columns, lane-major in LDS (read without touching the vector
@@ -1259,7 +1264,7 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     for (int k = 0; k < TRING; ++k) Tr[k] = 0.0f;
 #pragma unroll
     for (int k = 0; k < MRING; ++k) Mr[k] = KEY_MASKED;
-    float smin = INFINITY, smax = -INFINITY;
+    int kmin = INT_MAX, kmax = KEY_MASKED; // statistics min/max as keys
     double ssum = 0.0;
     int nrun = 0, ncand = 0;
 
@@ -1305,12 +1310,13 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
 #pragma unroll
             for (int aa = 0; aa < KR; ++aa) acc = fmaf(g.u[aa], Tr[k + 2 * CKR - aa], acc);
             const int r = r0 + k;
+            const int key = fkey(acc);
             if (r >= g.sr0 && r < g.sr1) { // every lane; lanes outside the block are dropped below
-                smin = fminf(smin, acc);
-                smax = fmaxf(smax, acc);
+                kmin = min(kmin, key);
+                kmax = max(kmax, key);
                 ssum += (double)acc;
             }
-            Mr[2 * HR + k] = (cval && r >= g.cr0 && r < g.cr1) ? fkey(acc) : KEY_MASKED;
+            Mr[2 * HR + k] = (cval && r >= g.cr0 && r < g.cr1) ? key : KEY_MASKED;
         }
         // ---- rows d = d0 + k: rho dilation (windows Mr[k .. k + 2HR]), theta dilation,
         //      D == C under the mask -> candidates ----
@@ -1387,18 +1393,18 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     // ---- per wavefront: top-nB of its candidates, statistics ----
     nrun = wave_merge(lane, nB, nrun, ncand, run_v, run_p, run_nn, lst_v, lst_p, lst_nn, sel_v, sel_p, sel_nn);
     if (!useful) {
-        smin = INFINITY;
-        smax = -INFINITY;
+        kmin = INT_MAX;
+        kmax = KEY_MASKED;
         ssum = 0.0;
     }
     for (int off = 32; off > 0; off >>= 1) {
-        smin = fminf(smin, __shfl_xor(smin, off, 64));
-        smax = fmaxf(smax, __shfl_xor(smax, off, 64));
+        kmin = min(kmin, __shfl_xor(kmin, off, 64));
+        kmax = max(kmax, __shfl_xor(kmax, off, 64));
         ssum += __shfl_xor(ssum, off, 64);
     }
     if (lane == 0) {
-        wl[L.red + 0] = smin;
-        wl[L.red + 1] = smax;
+        wl[L.red + 0] = fkey_inv(kmin); // every wavefront has a column in the block
+        wl[L.red + 1] = fkey_inv(kmax);
         *reinterpret_cast<double *>(wl + L.red + 2) = ssum;
         reinterpret_cast<int *>(wl)[L.cnt] = nrun;
     }
