@@ -510,12 +510,13 @@ class BandDetect:
             x = x - torch.as_tensor(np.asarray(bg, dtype=np.float64), device=pats.device)
         return x.to(torch.float32), _hip.EBSD_FLAG_NOBG
 
-    def find_bands_device(self, pats, gpu_id=None, out=None, work=None, chunk=2048):
-        """find_bands on a device tensor (N,H,W); returns a dict of device tensors."""
+    def find_bands_device(self, pats, gpu_id=None, out=None, work=None, chunk=2048, stream=None):
+        """find_bands on a device tensor (N,H,W); returns a dict of device tensors.  The
+        kernels run on ``stream`` (a torch.cuda.Stream; default: the current stream)."""
         plan = self.device_plan(gpu_id)
         pats = pats.unsqueeze(0) if pats.dim() == 2 else pats
         x, flags = self._prepare(plan, pats)
-        return plan.find_bands(x, out=out, work=work, chunk=chunk, flags=flags)
+        return plan.find_bands(x, out=out, work=work, chunk=chunk, flags=flags, stream=stream)
 
     def find_bands(self, patternsIn, verbose=0, chunksize=-1, clparams=None, gpu_id=None, **kwargs):
         """band_detect.py:411-526 on the GPU.  ``chunksize`` > 0 bounds the patterns per
