@@ -123,6 +123,46 @@ int ebsd_batch_destroy(ebsd_batch *batch);
 /* Plan geometry queries. */
 int ebsd_plan_info(const ebsd_plan *plan, int64_t *out8);
 
+/* ---- adjoints: gradients of the band positions --------------------------------------
+ * The reference has no derivatives; these are the adjoints of its forward stages, so that
+ * a loss on the band (rho, theta) can be differentiated w.r.t. the patterns and the
+ * pattern centre.  All outputs are written in full (no zeroing by the caller) and are
+ * deterministic (float64 gathers in a fixed order). */
+
+/* dL/dC of the convolved padded Radon (N, nRp, nTp) from dL/drho, dL/dtheta (N, nB) of
+ * the bands ebsd_find_bands returned (maxloc, aveloc, valid): the Jacobian of the 3x3
+ * sub-pixel fit of band_detect.py:764-798 (with the branch the forward took), the slope of
+ * the rho/theta interpolation of band_detect.py:521-524.  d_radon: the (recomputed)
+ * unpadded Radon (N, nR, nT). */
+int ebsd_band_grad(ebsd_plan *plan, const float *d_radon, int64_t N, const float *d_maxloc,
+                   const float *d_aveloc, const int8_t *d_valid, const float *d_grad_rho,
+                   const float *d_grad_theta, float *d_grad_conv, void *stream);
+
+/* Adjoint of padding + 'same' convolution (band_detect.py:573-580, 605-606):
+ * dR (N, nR, nT) from dC (N, nRp, nTp). */
+int ebsd_conv_transpose(ebsd_plan *plan, const float *d_grad_conv, int64_t N, float *d_grad_radon,
+                        void *stream);
+
+/* Adjoint of the normalised Radon (radon_fast.py:288-310), the backprojection:
+ * d(pixels) (N, H, W) float32 from dR (N, nR, nT). */
+int ebsd_backproject(ebsd_plan *plan, const float *d_grad_radon, int64_t N, float *d_grad_pats,
+                     void *stream);
+
+/* Predicted bands: the inverse of radon2pole (radon_fast.py:314-397).  For detector-frame
+ * plane normals (N, M, 3) and pattern centres (3, or N x 3 when pc_per_pattern), the band
+ * (rho, theta) (N, M) and d rho / d PC (N, M, 3); theta does not depend on PC.
+ * vendor: 0 EDAX, 1 OXFORD, 2 BRUKER/KIKUCHIPY (radon_fast.py:361-370). */
+int ebsd_gnomonic_bands(int64_t N, int M, const double *d_normals, const double *d_pc,
+                        int pc_per_pattern, int H, int W, int vendor, double *d_rho,
+                        double *d_theta, double *d_drho_dpc, void *stream);
+
+/* Radon.radon2pole (radon_fast.py:314-397) on the device: band (theta, rho) (N, nB) ->
+ * unit plane normals (N, nB, 3) float32 (float64 arithmetic); d_dn_dpc (N, nB, 3, 3) or
+ * NULL: the Jacobian of each normal w.r.t. the pattern centre. */
+int ebsd_radon2pole(int64_t N, int nB, const float *d_theta, const float *d_rho,
+                    const double *d_pc, int pc_per_pattern, int H, int W, int vendor,
+                    float *d_normals, double *d_dn_dpc, void *stream);
+
 const char *ebsd_last_error(void);
 
 #ifdef __cplusplus

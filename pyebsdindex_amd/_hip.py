@@ -17,7 +17,8 @@ SYMBOLS = (
     "ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background", "ebsd_plan_set_tuning",
     "ebsd_plan_set_peak_kernel", "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
     "ebsd_plan_info", "ebsd_last_error", "ebsd_batch_workspace_bytes", "ebsd_batch_create",
-    "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy",
+    "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy", "ebsd_band_grad",
+    "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands", "ebsd_radon2pole",
 )
 
 
@@ -69,12 +70,19 @@ def load(path: str = LIB_PATH):
         L.ebsd_batch_run.argtypes = [P, P]
         L.ebsd_batch_graph_launch.argtypes = [P, P]
         L.ebsd_batch_destroy.argtypes = [P]
+        L.ebsd_band_grad.argtypes = [P, P, I64, P, P, P, P, P, P, P]
+        L.ebsd_conv_transpose.argtypes = [P, P, I64, P, P]
+        L.ebsd_backproject.argtypes = [P, P, I64, P, P]
+        L.ebsd_gnomonic_bands.argtypes = [I64, I, P, P, I, I, I, I, P, P, P, P]
+        L.ebsd_radon2pole.argtypes = [I64, I, P, P, P, I, I, I, I, P, P, P]
         L.ebsd_last_error.argtypes = []
         L.ebsd_last_error.restype = ctypes.c_char_p
         for name in ("ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background",
                      "ebsd_plan_set_tuning", "ebsd_plan_set_peak_kernel", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks",
                      "ebsd_conv", "ebsd_plan_info", "ebsd_batch_create", "ebsd_batch_run",
-                     "ebsd_batch_graph_launch", "ebsd_batch_destroy"):
+                     "ebsd_batch_graph_launch", "ebsd_batch_destroy",
+                     "ebsd_band_grad", "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands",
+                     "ebsd_radon2pole"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
         return L

@@ -89,6 +89,10 @@ struct ebsd_plan {
     uint8_t *d_mask = nullptr;   // nR*nT
     uint32_t *d_mcol = nullptr;  // rho-mask bits per theta column (PeakGeo::mcw words each)
     uint32_t *d_mrow = nullptr;  // rho-mask bits per rho row (PeakGeo::maskw words each)
+    // adjoint structures, built on first use
+    int *d_px_off = nullptr, *d_px_cell = nullptr; // pixel -> lines through it (CSR)
+    int *d_cnt = nullptr;                          // pixels per line
+    int *d_pre_off = nullptr, *d_pre = nullptr;    // Radon cell -> padded cells copying it (CSR)
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
     // tuning
     int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 1024;
@@ -1517,6 +1521,11 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_mask);
     (void)hipFree(p->d_mcol);
     (void)hipFree(p->d_mrow);
+    (void)hipFree(p->d_px_off);
+    (void)hipFree(p->d_px_cell);
+    (void)hipFree(p->d_cnt);
+    (void)hipFree(p->d_pre_off);
+    (void)hipFree(p->d_pre);
     (void)hipFree(p->d_grid);
     delete p;
 }
@@ -2054,6 +2063,427 @@ extern "C" int ebsd_conv(ebsd_plan *p, const float *d_radon, int64_t N, float *d
     const int64_t blocks = std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(k_conv_full, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        d_radon, N, p->g, d_conv);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Adjoints (differentiation of the band positions).  The band's sub-pixel position is a
+// function of its 3x3 convolved-Radon neighbourhood (band_detect.py:764-798); the
+// convolution is linear in the padded Radon (band_detect.py:605-606), the padding an index
+// map (:573-580) and the Radon linear in the pixels (radon_fast.py:288-310).  The backward
+// pass therefore runs: band-fit Jacobian -> dC (3x3 per band) -> transposed convolution
+// and padding adjoint -> dR -> backprojection -> d(pixels).  All three are gathers with
+// float64 sums in a fixed order: the gradients are deterministic.
+// ---------------------------------------------------------------------------------------
+struct BandGradArgs {
+    const float *radon;    // (N, nR, nT), recomputed by the caller
+    int64_t N;
+    const float *maxloc;   // (N, nB, 2) integer peak bins, padding removed (find_bands output)
+    const float *aveloc;   // (N, nB, 2) sub-pixel positions (find_bands output)
+    const int8_t *valid;   // (N, nB)
+    const float *g_rho;    // (N, nB) dL/drho
+    const float *g_theta;  // (N, nB) dL/dtheta
+    const double *grid;    // theta_deg[nT], rho[nR]
+    float *dconv;          // (N, nRp, nTp) out: dL/dC, zero except at the bands' neighbourhoods
+    PeakGeo g;
+};
+
+// Convolved padded Radon at (y, x) with the arithmetic of the peak kernels (theta taps
+// then rho taps, fmaf chains from zero), so that the neighbourhood equals the forward's.
+__device__ __forceinline__ float conv_at(const PeakGeo &g, const float *R, int y, int x)
+{
+    const int ckr = g.kr / 2, ckt = g.kt / 2;
+    float acc = 0.0f;
+    for (int aa = 0; aa < g.kr; ++aa) {
+        const int yy = y + ckr - aa;
+        float t = 0.0f;
+        for (int b = 0; b < g.kt; ++b) {
+            const int xx = x + ckt - b;
+            const float s = (yy >= 0 && yy < g.nRp && xx >= 0 && xx < g.nTp) ? R[pad_src(g, yy, xx)] : 0.0f;
+            t = fmaf(g.v[b], s, t);
+        }
+        acc = fmaf(g.u[aa], t, acc);
+    }
+    return acc;
+}
+
+// Slope of numpy.interp(x, 0..n-1, fp) (0 where it clamps).
+__device__ __forceinline__ double np_interp_slope(double x, const double *fp, int n)
+{
+    if (!(x >= 0.0) || x > (double)(n - 1) || n < 2) return 0.0;
+    int j = (int)floor(x);
+    j = j > n - 2 ? n - 2 : j;
+    return fp[j + 1] - fp[j];
+}
+
+// dL/d(nn) of one band: the Jacobian of band_detect.py:764-798 (including which branch
+// -- full Hessian, diagonal fallback, zero -- the forward took) applied to
+// dL/d(aveloc).  nn: raw 3x3 neighbourhood (row-major, rows along rho).
+__device__ void band_fit_grad(const float *nn, double gr_al, double gc_al, double *dnn)
+{
+    // forward-identical normalisation and branch decisions (float32 sum, as the peak kernels)
+    float s = 0.0f;
+    for (int k = 0; k < 9; ++k) s += nn[k];
+    const bool sclamp = !(s > 1e-12f);
+    const double S = sclamp ? 1e-12 : (double)s;
+    float nf[9];
+    for (int k = 0; k < 9; ++k) nf[k] = (float)((double)nn[k] / S);
+    auto terms = [](const double *n, double &dx, double &dy, double &dxx, double &dyy, double &dxy) {
+        dx = 0.5 * (n[5] - n[3]);
+        dy = 0.5 * (n[7] - n[1]);
+        dxx = (n[5] + n[3]) - 2.0 * n[4];
+        dyy = (n[7] + n[1]) - 2.0 * n[4];
+        dxy = 0.25 * (((n[8] - n[2]) - n[6]) + n[0]);
+    };
+    double nd[9];
+    for (int k = 0; k < 9; ++k) nd[k] = (double)nf[k];
+    double dx, dy, dxx, dyy, dxy;
+    terms(nd, dx, dy, dxx, dyy, dxy);
+    int branch = 0; // 0 full, 1 diagonal, 2 zero
+    bool dclamp = false;
+    {
+        double det = dxx * dyy - dxy * dxy;
+        const bool cl0 = !(fabs(det) > 1e-12);
+        det = cl0 ? 1.0e-12 : det;
+        const double dc = (dyy * dx - dxy * dy) / det, rc = (dxx * dy - dxy * dx) / det;
+        dclamp = cl0;
+        if (fabs(dc) > 0.875 || fabs(rc) > 0.875) {
+            double d2 = dxx * dyy;
+            const bool cl1 = !(fabs(d2) > 1e-12);
+            d2 = cl1 ? 1.0e-12 : d2;
+            dclamp = cl1;
+            const double dc2 = (dyy * dx) / d2, rc2 = (dxx * dy) / d2;
+            branch = (fabs(dc2) > 0.875 || fabs(rc2) > 0.875) ? 2 : 1;
+        }
+    }
+    for (int k = 0; k < 9; ++k) dnn[k] = 0.0;
+    if (branch == 2) return;
+    // derivatives at the exactly normalised neighbourhood
+    double n64[9];
+    for (int k = 0; k < 9; ++k) n64[k] = (double)nn[k] / S;
+    terms(n64, dx, dy, dxx, dyy, dxy);
+    // q = (dx, dy, dxx, dyy, dxy); dL/dq from dL/drc = -gr_al, dL/ddc = -gc_al
+    const double Lrc = -gr_al, Ldc = -gc_al;
+    double Lq[5];
+    if (branch == 0) {
+        const double D = dclamp ? 1.0e-12 : dxx * dyy - dxy * dxy;
+        const double Nc = dyy * dx - dxy * dy, Nr = dxx * dy - dxy * dx;
+        const double iD = 1.0 / D, iD2 = iD * iD;
+        const double dD[5] = {0.0, 0.0, dclamp ? 0.0 : dyy, dclamp ? 0.0 : dxx, dclamp ? 0.0 : -2.0 * dxy};
+        const double dNc[5] = {dyy, -dxy, 0.0, dx, -dy};
+        const double dNr[5] = {-dxy, dxx, dy, 0.0, -dx};
+        for (int i = 0; i < 5; ++i)
+            Lq[i] = Ldc * (dNc[i] * iD - Nc * dD[i] * iD2) + Lrc * (dNr[i] * iD - Nr * dD[i] * iD2);
+    } else {
+        const double D = dclamp ? 1.0e-12 : dxx * dyy;
+        const double Nc = dyy * dx, Nr = dxx * dy;
+        const double iD = 1.0 / D, iD2 = iD * iD;
+        const double dD[5] = {0.0, 0.0, dclamp ? 0.0 : dyy, dclamp ? 0.0 : dxx, 0.0};
+        const double dNc[5] = {dyy, 0.0, 0.0, dx, 0.0};
+        const double dNr[5] = {0.0, dxx, dy, 0.0, 0.0};
+        for (int i = 0; i < 5; ++i)
+            Lq[i] = Ldc * (dNc[i] * iD - Nc * dD[i] * iD2) + Lrc * (dNr[i] * iD - Nr * dD[i] * iD2);
+    }
+    // q as linear forms of the normalised neighbourhood
+    double gn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    gn[5] += 0.5 * Lq[0]; gn[3] -= 0.5 * Lq[0];
+    gn[7] += 0.5 * Lq[1]; gn[1] -= 0.5 * Lq[1];
+    gn[5] += Lq[2]; gn[3] += Lq[2]; gn[4] -= 2.0 * Lq[2];
+    gn[7] += Lq[3]; gn[1] += Lq[3]; gn[4] -= 2.0 * Lq[3];
+    gn[8] += 0.25 * Lq[4]; gn[2] -= 0.25 * Lq[4]; gn[6] -= 0.25 * Lq[4]; gn[0] += 0.25 * Lq[4];
+    // through the normalisation n / S(n)
+    double dot = 0.0;
+    for (int k = 0; k < 9; ++k) dot += gn[k] * n64[k];
+    for (int k = 0; k < 9; ++k) dnn[k] = sclamp ? gn[k] / S : (gn[k] - dot) / S;
+}
+
+// One work-group per pattern: zero its dC, then one thread per band writes dL/dC at the
+// band's 3x3 neighbourhood (atomically: plateau candidates may be adjacent).
+__global__ __launch_bounds__(256) void k_band_grad(BandGradArgs a)
+{
+    const PeakGeo &g = a.g;
+    const int64_t q = blockIdx.x;
+    const int nC = g.nRp * g.nTp;
+    float *dC = a.dconv + q * (int64_t)nC;
+    for (int i = threadIdx.x; i < nC; i += blockDim.x) dC[i] = 0.0f;
+    __syncthreads();
+    const float *R = a.radon + q * (int64_t)(g.nR * g.nT);
+    for (int b = threadIdx.x; b < g.nB; b += blockDim.x) {
+        const int64_t o = q * g.nB + b;
+        if (!a.valid[o]) continue;
+        const int r = (int)a.maxloc[2 * o] + g.p0, c = (int)a.maxloc[2 * o + 1] + g.p1;
+        float nn[9];
+        for (int dr = 0; dr < 3; ++dr)
+            for (int dc = 0; dc < 3; ++dc) nn[dr * 3 + dc] = conv_at(g, R, r - 1 + dr, c - 1 + dc);
+        // the forward's sub-pixel location selects the interpolation segments
+        const double *thg = a.grid, *rhg = a.grid + g.nT;
+        const double al_r = (double)a.aveloc[2 * o], al_c = (double)a.aveloc[2 * o + 1];
+        double dnn0[9];
+        // theta = pi - interp(al_c)/RADEG, rho = -interp(al_r)  (band_detect.py:521-524)
+        const double gc_al = (double)a.g_theta[o] * (-np_interp_slope(al_c, thg, g.nT) / (180.0 / 3.141592653589793));
+        const double gr_al = (double)a.g_rho[o] * (-np_interp_slope(al_r, rhg, g.nR));
+        band_fit_grad(nn, gr_al, gc_al, dnn0);
+        for (int dr = 0; dr < 3; ++dr)
+            for (int dc = 0; dc < 3; ++dc)
+                atomicAdd(&dC[(r - 1 + dr) * g.nTp + (c - 1 + dc)], (float)dnn0[dr * 3 + dc]);
+    }
+}
+
+// dR = padding adjoint of the transposed convolution of dC: for every Radon cell, the sum
+// over the padded cells that copy it (interior, replicated rho rows, mirrored theta
+// columns) of sum_{a,b} u[a] v[b] dC[y - kr/2 + a][x - kt/2 + b].
+__global__ __launch_bounds__(256) void k_conv_transpose(const float *dconv, int64_t N, PeakGeo g, const int *pre_off,
+                                                        const int *pre, float *dradon)
+{
+    const int nCells = g.nR * g.nT, nC = g.nRp * g.nTp;
+    const int ckr = g.kr / 2, ckt = g.kt / 2;
+    const int64_t total = N * (int64_t)nCells;
+    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+         id += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = id / nCells;
+        const int cell = (int)(id - q * nCells);
+        const float *dC = dconv + q * (int64_t)nC;
+        double acc = 0.0;
+        for (int k = pre_off[cell]; k < pre_off[cell + 1]; ++k) {
+            const int y = pre[k] / g.nTp, x = pre[k] - (pre[k] / g.nTp) * g.nTp;
+            for (int aa = 0; aa < g.kr; ++aa) {
+                const int yy = y - ckr + aa;
+                if (yy < 0 || yy >= g.nRp) continue;
+                double t = 0.0;
+                for (int b = 0; b < g.kt; ++b) {
+                    const int xx = x - ckt + b;
+                    if (xx >= 0 && xx < g.nTp) t += (double)g.v[b] * (double)dC[yy * g.nTp + xx];
+                }
+                acc += (double)g.u[aa] * t;
+            }
+        }
+        dradon[id] = (float)acc;
+    }
+}
+
+// Radon adjoint: d(pixel) = sum over the lines through the pixel of dR[line] / count[line]
+// (radon_fast.py:288-310 divides each line sum by its pixel count).
+__global__ __launch_bounds__(256) void k_backproject(const float *dradon, int64_t N, int nPx, int nCells,
+                                                     const int *px_off, const int *px_cell, const int *cnt,
+                                                     float *dpats)
+{
+    const int64_t total = N * (int64_t)nPx;
+    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+         id += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = id / nPx;
+        const int px = (int)(id - q * nPx);
+        const float *dR = dradon + q * (int64_t)nCells;
+        double acc = 0.0;
+        for (int k = px_off[px]; k < px_off[px + 1]; ++k) {
+            const int cell = px_cell[k];
+            acc += (double)dR[cell] / (double)cnt[cell];
+        }
+        dpats[id] = (float)acc;
+    }
+}
+
+// Detector-frame translation t = a * PC + b of the gnomonic geometry (radon_fast.py:361-370).
+__device__ __forceinline__ bool vendor_t(int vendor, int H, int W, double *a, double *b)
+{
+    const double h = H, w = W, mn = H < W ? H : W;
+    b[0] = b[1] = b[2] = 0.0;
+    if (vendor == 0) { a[0] = w; a[1] = h; a[2] = -mn; return true; }          // EDAX
+    if (vendor == 1) { a[0] = w; a[1] = w; a[2] = -w; return true; }           // OXFORD
+    if (vendor == 2) { a[0] = w; a[1] = -h; a[2] = -h; b[1] = h; return true; } // BRUKER / KIKUCHIPY
+    return false;
+}
+
+// Predicted band of a detector-frame plane normal n at pattern centre PC: the trace of the
+// plane through t with normal n on the detector, in the (rho, theta) of radon2pole
+// (radon_fast.py:379-396): cos(th) (x - W/2) + sin(th) (y - H/2) = rho, theta in (0, pi].
+// d rho / d PC is returned; theta does not depend on PC.
+__global__ void k_gnomonic_bands(int64_t N, int M, const double *normals, const double *pc, int pc_per_pattern,
+                                 int H, int W, int vendor, double *rho, double *theta, double *drho_dpc)
+{
+    const int64_t total = N * (int64_t)M;
+    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+         id += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = id / M;
+        const double *n = normals + 3 * id;
+        const double *P = pc + (pc_per_pattern ? 3 * q : 0);
+        double va[3], vb[3];
+        vendor_t(vendor, H, W, va, vb);
+        const double t0 = va[0] * P[0] + vb[0], t1 = va[1] * P[1] + vb[1], t2 = va[2] * P[2] + vb[2];
+        const double m = sqrt(n[0] * n[0] + n[1] * n[1]);
+        double sg = n[1] > 0.0 ? 1.0 : (n[1] < 0.0 ? -1.0 : (n[0] < 0.0 ? 1.0 : -1.0));
+        if (!(m > 0.0)) {
+            rho[id] = NAN;
+            theta[id] = NAN;
+            drho_dpc[3 * id] = drho_dpc[3 * id + 1] = drho_dpc[3 * id + 2] = 0.0;
+            continue;
+        }
+        const double d = n[0] * t0 + n[1] * t1 + n[2] * t2;
+        rho[id] = sg * (d - n[0] * 0.5 * W - n[1] * 0.5 * H) / m;
+        theta[id] = atan2(sg * n[1], sg * n[0]);
+        drho_dpc[3 * id + 0] = sg * n[0] * va[0] / m;
+        drho_dpc[3 * id + 1] = sg * n[1] * va[1] / m;
+        drho_dpc[3 * id + 2] = sg * n[2] * va[2] / m;
+    }
+}
+
+// radon2pole (radon_fast.py:314-397): band (theta, rho) -> unit plane normal in the detector
+// frame, float64 arithmetic, float32 result; dn/dPC (3x3 per band) when d_dn_dpc is set.
+__global__ void k_radon2pole(int64_t N, int nB, const float *theta, const float *rho, const double *pc,
+                             int pc_per_pattern, int H, int W, int vendor, float *normals, double *dn_dpc)
+{
+    const int64_t total = N * (int64_t)nB;
+    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+         id += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = id / nB;
+        const double *P = pc + (pc_per_pattern ? 3 * q : 0);
+        double va[3], vb[3];
+        vendor_t(vendor, H, W, va, vb);
+        const double t[3] = {va[0] * P[0] + vb[0], va[1] * P[1] + vb[1], va[2] * P[2] + vb[2]};
+        const double th = theta[id], rh = rho[id];
+        const double st = sin(th), ct = cos(th);
+        const double r[3] = {-st, ct, 0.0};
+        const double p[3] = {rh * ct + 0.5 * W, rh * st + 0.5 * H, 0.0};
+        const double d[3] = {p[0] - t[0], p[1] - t[1], p[2] - t[2]};
+        const double n[3] = {r[1] * d[2] - r[2] * d[1], r[2] * d[0] - r[0] * d[2], r[0] * d[1] - r[1] * d[0]};
+        double nrm = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        const bool clip = !(nrm > 1e-12);
+        nrm = clip ? 1e-12 : nrm;
+        for (int k = 0; k < 3; ++k) normals[3 * id + k] = (float)(n[k] / nrm);
+        if (dn_dpc) {
+            // n = r x (p - t): dn/dt_j = -(r x e_j); normalised: (I - u u^T) dn / |n|
+            const double u[3] = {n[0] / nrm, n[1] / nrm, n[2] / nrm};
+            for (int j = 0; j < 3; ++j) {
+                double e[3] = {0.0, 0.0, 0.0};
+                e[j] = 1.0;
+                const double dn[3] = {-(r[1] * e[2] - r[2] * e[1]), -(r[2] * e[0] - r[0] * e[2]),
+                                      -(r[0] * e[1] - r[1] * e[0])};
+                const double ud = clip ? 0.0 : u[0] * dn[0] + u[1] * dn[1] + u[2] * dn[2];
+                for (int k = 0; k < 3; ++k) dn_dpc[9 * id + 3 * k + j] = (dn[k] - u[k] * ud) / nrm * va[j];
+            }
+        }
+    }
+}
+
+static int ensure_adjoint(ebsd_plan *p)
+{
+    if (p->d_px_off) return 0;
+    const int nPx = p->nPx, nCells = p->nCells;
+    // pixel -> lines through it, by ascending line index
+    std::vector<int> px_off(nPx + 1, 0), px_cell(p->h_ent.size()), cnt(nCells);
+    for (size_t k = 0; k < p->h_ent.size(); ++k) ++px_off[p->h_ent[k] + 1];
+    for (int i = 0; i < nPx; ++i) px_off[i + 1] += px_off[i];
+    std::vector<int> fill(px_off.begin(), px_off.end() - 1);
+    for (int cell = 0; cell < nCells; ++cell) {
+        cnt[cell] = p->h_cnt[cell];
+        for (int64_t k = p->h_off[cell]; k < p->h_off[cell + 1]; ++k) px_cell[fill[p->h_ent[k]]++] = cell;
+    }
+    // Radon cell -> padded cells that copy it (the padding index map inverted)
+    const PeakGeo &g = p->g;
+    const int nC = g.nRp * g.nTp;
+    std::vector<int> pre_off(nCells + 1, 0), pre(nC), src(nC);
+    for (int y = 0; y < g.nRp; ++y)
+        for (int x = 0; x < g.nTp; ++x) {
+            int i = y - g.p0;
+            i = i < 0 ? 0 : (i > g.nR - 1 ? g.nR - 1 : i);
+            int j;
+            if (x < g.p1) { j = g.nT - g.p1 + x; i = g.nR - 1 - i; }
+            else if (x >= g.p1 + g.nT) { j = x - g.p1 - g.nT; i = g.nR - 1 - i; }
+            else j = x - g.p1;
+            src[y * g.nTp + x] = i * g.nT + j;
+            ++pre_off[i * g.nT + j + 1];
+        }
+    for (int i = 0; i < nCells; ++i) pre_off[i + 1] += pre_off[i];
+    std::vector<int> pf(pre_off.begin(), pre_off.end() - 1);
+    for (int k = 0; k < nC; ++k) pre[pf[src[k]]++] = k;
+    int rc;
+    if ((rc = dev_upload(&p->d_px_off, px_off.data(), px_off.size()))) return rc;
+    if ((rc = dev_upload(&p->d_px_cell, px_cell.data(), px_cell.size()))) return rc;
+    if ((rc = dev_upload(&p->d_cnt, cnt.data(), cnt.size()))) return rc;
+    if ((rc = dev_upload(&p->d_pre_off, pre_off.data(), pre_off.size()))) return rc;
+    if ((rc = dev_upload(&p->d_pre, pre.data(), pre.size()))) return rc;
+    return 0;
+}
+
+static unsigned grid_for(int64_t total)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+}
+
+extern "C" int ebsd_band_grad(ebsd_plan *p, const float *d_radon, int64_t N, const float *d_maxloc,
+                              const float *d_aveloc, const int8_t *d_valid, const float *d_grad_rho,
+                              const float *d_grad_theta, float *d_grad_conv, void *stream)
+{
+    if (!p) return fail(-1, "ebsd_band_grad: null plan");
+    if (N <= 0) return N == 0 ? 0 : fail(-1, "ebsd_band_grad: negative N");
+    if (!d_radon || !d_maxloc || !d_aveloc || !d_valid || !d_grad_rho || !d_grad_theta || !d_grad_conv)
+        return fail(-1, "ebsd_band_grad: null device pointer");
+    BandGradArgs a;
+    a.radon = d_radon;
+    a.N = N;
+    a.maxloc = d_maxloc;
+    a.aveloc = d_aveloc;
+    a.valid = d_valid;
+    a.g_rho = d_grad_rho;
+    a.g_theta = d_grad_theta;
+    a.grid = p->d_grid;
+    a.dconv = d_grad_conv;
+    a.g = p->g;
+    hipLaunchKernelGGL(k_band_grad, dim3((unsigned)N), dim3(256), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_conv_transpose(ebsd_plan *p, const float *d_grad_conv, int64_t N, float *d_grad_radon,
+                                   void *stream)
+{
+    if (!p) return fail(-1, "ebsd_conv_transpose: null plan");
+    if (N <= 0) return N == 0 ? 0 : fail(-1, "ebsd_conv_transpose: negative N");
+    if (!d_grad_conv || !d_grad_radon) return fail(-1, "ebsd_conv_transpose: null device pointer");
+    int rc;
+    if ((rc = ensure_adjoint(p))) return rc;
+    hipLaunchKernelGGL(k_conv_transpose, dim3(grid_for(N * (int64_t)p->nCells)), dim3(256), 0, (hipStream_t)stream,
+                       d_grad_conv, N, p->g, p->d_pre_off, p->d_pre, d_grad_radon);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_backproject(ebsd_plan *p, const float *d_grad_radon, int64_t N, float *d_grad_pats,
+                                void *stream)
+{
+    if (!p) return fail(-1, "ebsd_backproject: null plan");
+    if (N <= 0) return N == 0 ? 0 : fail(-1, "ebsd_backproject: negative N");
+    if (!d_grad_radon || !d_grad_pats) return fail(-1, "ebsd_backproject: null device pointer");
+    int rc;
+    if ((rc = ensure_adjoint(p))) return rc;
+    hipLaunchKernelGGL(k_backproject, dim3(grid_for(N * (int64_t)p->nPx)), dim3(256), 0, (hipStream_t)stream,
+                       d_grad_radon, N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell, p->d_cnt, d_grad_pats);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_gnomonic_bands(int64_t N, int M, const double *d_normals, const double *d_pc,
+                                   int pc_per_pattern, int H, int W, int vendor, double *d_rho, double *d_theta,
+                                   double *d_drho_dpc, void *stream)
+{
+    if (N <= 0 || M <= 0) return (N == 0 || M == 0) ? 0 : fail(-1, "ebsd_gnomonic_bands: negative size");
+    if (!d_normals || !d_pc || !d_rho || !d_theta || !d_drho_dpc) return fail(-1, "ebsd_gnomonic_bands: null pointer");
+    if (vendor < 0 || vendor > 2) return fail(-1, "ebsd_gnomonic_bands: vendor is 0 (EDAX), 1 (OXFORD) or 2 (BRUKER)");
+    hipLaunchKernelGGL(k_gnomonic_bands, dim3(grid_for(N * (int64_t)M)), dim3(256), 0, (hipStream_t)stream, N, M,
+                       d_normals, d_pc, pc_per_pattern, H, W, vendor, d_rho, d_theta, d_drho_dpc);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_radon2pole(int64_t N, int nB, const float *d_theta, const float *d_rho, const double *d_pc,
+                               int pc_per_pattern, int H, int W, int vendor, float *d_normals, double *d_dn_dpc,
+                               void *stream)
+{
+    if (N <= 0 || nB <= 0) return (N == 0 || nB == 0) ? 0 : fail(-1, "ebsd_radon2pole: negative size");
+    if (!d_theta || !d_rho || !d_pc || !d_normals) return fail(-1, "ebsd_radon2pole: null pointer");
+    if (vendor < 0 || vendor > 2) return fail(-1, "ebsd_radon2pole: vendor is 0 (EDAX), 1 (OXFORD) or 2 (BRUKER)");
+    hipLaunchKernelGGL(k_radon2pole, dim3(grid_for(N * (int64_t)nB)), dim3(256), 0, (hipStream_t)stream, N, nB,
+                       d_theta, d_rho, d_pc, pc_per_pattern, H, W, vendor, d_normals, d_dn_dpc);
     HIPCHK(hipGetLastError());
     return 0;
 }
