@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-size batched measurement")
     ap.add_argument("--no-streaming", action="store_true", help="skip the pinned-host streaming measurement")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation measurement")
     ap.add_argument("--stream-patterns", type=int, default=1048576,
                     help="patterns streamed in the scan-scale measurement")
     ap.add_argument("--roofline-reps", type=int, default=20)
@@ -89,10 +90,10 @@ def streaming_measure(dev, total=1048576, batch=8192, ring=4, seed=11):
     torch.cuda.synchronize()
     nb = max(1, total // batch)
     seq = [host[j % ring] for j in range(nb)]
-    res = sb.pinned_results(nb * batch)
-    sb.run(seq[:2], results={k: v[:2 * batch] for k, v in res.items()})  # warm-up
+    blocks = sb.pinned_blocks(nb)
+    sb.run(seq[:2], blocks=blocks[:2])  # warm-up
     t0 = time.perf_counter()
-    sb.run(seq, results=res)
+    res = sb.run(seq, blocks=blocks)
     dt = time.perf_counter() - t0
     n = nb * batch
     # the same batches resident in HBM (no copy) and a plain pinned host->device copy
@@ -114,8 +115,7 @@ def streaming_measure(dev, total=1048576, batch=8192, ring=4, seed=11):
     h2d = 4 * host[0].numel() * 2 / (time.perf_counter() - t2) / 1e9
     bytes_per = shape[0] * shape[1] * 2
     achieved_h2d = n * bytes_per / dt / 1e9
-    same = all(bool(torch.equal(res[k][:batch], res[k][ring * batch: (ring + 1) * batch])) for k in ("maxloc", "valid")) \
-        if nb > ring else None
+    same = all(bool(torch.equal(res[0][k], res[ring][k])) for k in ("maxloc", "valid")) if nb > ring else None
     return {"workload": f"{n} patterns of 240x240 uint16 HCP (c/a 1.587), Radon 360x180, 12 bands, background "
                         f"subtraction; {nb} batches of {batch} from pinned host memory (a ring of {ring} "
                         f"distinct batches), H2D copy on a side stream overlapped with band detection",
@@ -127,6 +127,56 @@ def streaming_measure(dev, total=1048576, batch=8192, ring=4, seed=11):
                          "peak": round(h2d, 2), "unit": "GB/s", "frac": round(achieved_h2d / h2d, 4),
                          "peak_kind": "plain pinned host->device copy measured in the same run"},
             "repeat_batches_identical": same}
+
+
+def differentiation_measure(dev, n=256, reps=5, seed=13):
+    """256 float32 120x120 FCC patterns at a hidden pattern centre: forward (Radon,
+    convolution, 9-band sub-pixel peaks) + backward (band-fit Jacobian, transposed
+    convolution, backprojection, Radon recomputed) throughput, the PC gradient of the
+    band-position loss, and Gauss-Newton refinement from a centre perturbed by 0.01."""
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.differentiable import BandPeaks, refine_pc
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    shape = (120, 120)
+    bd = BandDetect(patDim=shape, nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9)
+    true = np.array([0.512, 0.487, 0.634])
+    pats, meta = kikuchi_patterns(n, shape, "float32", seed=seed, family="FCC", pc=true, device=dev,
+                                  return_meta=True)
+    normals = torch.as_tensor(meta["g_det"], device=dev)
+    w = torch.randn((n, 9), device=dev, generator=torch.Generator(device=dev).manual_seed(seed))
+
+    def step():
+        x = pats.detach().requires_grad_(True)
+        rho, theta, valid = BandPeaks.apply(x, bd)
+        ((w * rho).sum() + (w * theta).sum()).backward()
+        return x.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    fb = reps * n / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        with torch.no_grad():
+            BandPeaks.apply(pats, bd)
+    torch.cuda.synchronize()
+    fw = reps * n / (time.perf_counter() - t0)
+    pc0 = true + np.array([0.01, -0.01, 0.01])
+    t0 = time.perf_counter()
+    pc, hist = refine_pc(bd, pats, normals, pc0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pc = pc.cpu().numpy()
+    return {"workload": f"{n} float32 120x120 FCC patterns at a hidden pattern centre, Radon 180x90, 9 bands",
+            "forward_backward_patterns_per_s": round(fb, 1), "forward_patterns_per_s": round(fw, 1),
+            "pc_true": true.tolist(), "pc_start": pc0.tolist(), "pc_refined": [round(float(v), 6) for v in pc],
+            "pc_error_start": float(np.abs(pc0 - true).max()), "pc_error_refined": float(np.abs(pc - true).max()),
+            "refine_iterations": len(hist) - 1, "refine_seconds": round(dt, 4),
+            "loss_start": hist[0][1], "loss_refined": hist[-1][1]}
 
 
 def batched_measure(dev, steps, warmup, npats=32768, seed=7, autotune_sets=True):
@@ -299,6 +349,18 @@ def main():
         if tj.get("patterns_per_launch") == chunk:
             traffic = tj.get("hbm_bytes_per_launch")
 
+    # the streamed measurement runs before the batched one: after the 32,768-pattern batched
+    # workload in the same process its uploads stopped overlapping band detection (about
+    # half the rate), which a fresh process does not show
+    streaming = None
+    if not args.no_streaming:
+        print("[bench] scan-scale streaming workload", file=sys.stderr, flush=True)
+        streaming = streaming_measure(dev, total=args.stream_patterns)
+        print(f"[bench] streaming {streaming['patterns_per_s']:.1f} patterns/s "
+              f"(resident {streaming['resident_patterns_per_s']:.1f}, H2D "
+              f"{streaming['roofline']['achieved']:.1f} of {streaming['roofline']['peak']:.1f} GB/s)",
+              file=sys.stderr, flush=True)
+
     batched = None
     if not args.no_batched:
         print("[bench] mixed-size batched workload", file=sys.stderr, flush=True)
@@ -312,14 +374,12 @@ def main():
         print(f"[bench] small mixed batches: graph {small['graph_patterns_per_s']:.1f} vs plain "
               f"{small['plain_patterns_per_s']:.1f} patterns/s", file=sys.stderr, flush=True)
 
-    streaming = None
-    if not args.no_streaming:
-        print("[bench] scan-scale streaming workload", file=sys.stderr, flush=True)
-        streaming = streaming_measure(dev, total=args.stream_patterns)
-        print(f"[bench] streaming {streaming['patterns_per_s']:.1f} patterns/s "
-              f"(resident {streaming['resident_patterns_per_s']:.1f}, H2D "
-              f"{streaming['roofline']['achieved']:.1f} of {streaming['roofline']['peak']:.1f} GB/s)",
-              file=sys.stderr, flush=True)
+    diff = None
+    if not args.no_diff:
+        print("[bench] differentiation workload", file=sys.stderr, flush=True)
+        diff = differentiation_measure(dev)
+        print(f"[bench] forward+backward {diff['forward_backward_patterns_per_s']:.1f} patterns/s, PC error "
+              f"{diff['pc_error_start']:.4f} -> {diff['pc_error_refined']:.5f}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -351,6 +411,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "streaming": streaming,
+        "differentiation": diff,
     }
     print(json.dumps(line), flush=True)
 

@@ -6,11 +6,17 @@ each chunk to ``find_bands`` (pyebsdindex/_ebsd_index_single.py:692-707, chunk l
 ``index_pats``); the OpenCL backend then uploads the chunk and blocks on the result
 (opencl/band_detect_cl.py:95-199).  Here each batch is copied with an asynchronous
 host-to-device copy on a copy-in stream into one of ``depth`` device slots while the
-compute stream runs ``ebsd_find_bands`` on the slot filled before; band results return to
-pinned host memory on a separate copy-out stream, so that the next batch's upload never
-queues behind the previous batch's results.  Events order the three: a slot is refilled
-only after the band detection that read it, and its result buffers are overwritten only
-after their copy-back.  All streams keep the default priority.
+compute stream runs ``ebsd_find_bands`` on a slot filled before.  A slot's band results
+live in one device block (all fields) that returns to pinned host memory in a single copy
+on a copy-out stream.  Events order the streams: a slot is refilled only after the band
+detection that read it, and its result block is overwritten only after its copy-back.
+
+The host enqueues uploads ``depth - 1`` batches ahead of band detection.  Streams can share
+a hardware queue (HIP spreads a process's streams over a few queues), and a wait in an
+in-order queue holds everything behind it: an upload enqueued after a batch's band
+detection would then start only when that detection ends.  Enqueued ahead, an upload
+only ever waits behind the detection that last used its slot, which is the dependency it
+has anyway.  All streams keep the default priority.
 """
 import numpy as np
 
@@ -24,7 +30,7 @@ class StreamingBandDetect:
 
     FIELDS = ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid", "stats")
 
-    def __init__(self, bd, batch=8192, dtype="uint16", gpu_id=None, depth=2, chunk=2048):
+    def __init__(self, bd, batch=8192, dtype="uint16", gpu_id=None, depth=3, chunk=2048):
         import torch
         if depth < 2:
             raise ValueError("depth must be >= 2 (one slot copying while another computes)")
@@ -36,10 +42,23 @@ class StreamingBandDetect:
         self.chunk = int(chunk)
         self.tdtype = {"uint8": torch.uint8, "uint16": torch.uint16, "float32": torch.float32}[str(np.dtype(dtype))]
         H, W = self.plan.H, self.plan.W
+        nB = self.plan.nB
+        # field layout of one result block: (name, torch dtype, per-pattern shape), 256-byte aligned
+        self.layout = []
+        off = 0
+        for k in self.FIELDS:
+            shp = {"maxloc": (nB, 2), "aveloc": (nB, 2), "stats": (3,)}.get(k, (nB,))
+            dt = torch.int8 if k == "valid" else torch.float32
+            nbytes = self.batch * int(np.prod(shp)) * (1 if k == "valid" else 4)
+            self.layout.append((k, dt, shp, off, nbytes))
+            off += (nbytes + 255) // 256 * 256
+        self.block_bytes = off
         with torch.cuda.device(self.device):
             self.slots = [torch.empty((self.batch, H, W), dtype=self.tdtype, device=self.device)
                           for _ in range(self.depth)]
-            self.outs = [self.plan.alloc_out(self.batch) for _ in range(self.depth)]
+            self.blocks = [torch.empty(self.block_bytes, dtype=torch.uint8, device=self.device)
+                           for _ in range(self.depth)]
+            self.outs = [self._views(b, self.batch) for b in self.blocks]
             self.work = self.plan.workspace(self.batch, self.chunk)
             self.copy_stream = torch.cuda.Stream(self.device)      # host -> device batches
             self.compute_stream = torch.cuda.Stream(self.device)
@@ -49,65 +68,80 @@ class StreamingBandDetect:
             self.returned = [torch.cuda.Event() for _ in range(self.depth)]
         self._used = [False] * self.depth
 
+    def _views(self, block, n):
+        """Typed views of the first ``n`` patterns of every field of a result block."""
+        out = {}
+        for k, dt, shp, off, nbytes in self.layout:
+            v = block[off:off + nbytes].view(dt).view((self.batch,) + shp)
+            out[k] = v[:n]
+        return out
+
     def pinned_batch(self, n=None):
         """A pinned host tensor shaped for one batch (``n`` patterns, default ``batch``)."""
         import torch
         n = self.batch if n is None else int(n)
         return torch.empty((n, self.plan.H, self.plan.W), dtype=self.tdtype, pin_memory=True)
 
-    def pinned_results(self, n):
-        """Pinned host tensors for ``n`` patterns' band results (keys of ``FIELDS``)."""
+    def pinned_blocks(self, nbatches):
+        """Pinned host result blocks for ``nbatches`` batches."""
         import torch
-        nB = self.plan.nB
-        f = dict(dtype=torch.float32, pin_memory=True)
-        shapes = {"maxloc": (n, nB, 2), "aveloc": (n, nB, 2), "stats": (n, 3)}
-        res = {}
-        for k in self.FIELDS:
-            shp = shapes.get(k, (n, nB))
-            res[k] = torch.empty(shp, dtype=torch.int8 if k == "valid" else torch.float32, pin_memory=True)
-        return res
+        return [torch.empty(self.block_bytes, dtype=torch.uint8, pin_memory=True) for _ in range(nbatches)]
 
-    def run(self, batches, results=None):
-        """Stream ``batches`` (an iterable of pinned host tensors of at most ``batch``
-        patterns) through band detection.  Returns pinned host result tensors holding every
-        pattern in order (``results`` may pass them in, e.g. from ``pinned_results``)."""
+    def run(self, batches, blocks=None):
+        """Stream ``batches`` (pinned host tensors of at most ``batch`` patterns each)
+        through band detection.  Returns, per batch, a dict of pinned host tensors (views of
+        one result block per batch; ``blocks`` may pass the blocks in, e.g. from
+        ``pinned_blocks``, to reuse them)."""
         import torch
         batches = list(batches)
-        total = sum(int(b.shape[0]) for b in batches)
-        if results is None:
-            results = self.pinned_results(total)
-        x_flags = None
-        off = 0
+        for hb in batches:
+            if int(hb.shape[0]) > self.batch or tuple(hb.shape[1:]) != (self.plan.H, self.plan.W):
+                raise ValueError("batch larger than the pipeline's batch size or of another pattern size")
+            if hb.dtype != self.tdtype:
+                raise TypeError(f"batch dtype {hb.dtype}, pipeline dtype {self.tdtype}")
+        if blocks is None:
+            blocks = self.pinned_blocks(len(batches))
+        if len(blocks) < len(batches):
+            raise ValueError("fewer result blocks than batches")
+        cs, ks, rs = self.copy_stream, self.compute_stream, self.return_stream
+        D = self.depth
+
+        def upload(j):
+            s, hb = j % D, batches[j]
+            if self._used[s]:
+                cs.wait_event(self.computed[s])              # slot s's last band detection has read it
+            with torch.cuda.stream(cs):
+                self.slots[s][:hb.shape[0]].copy_(hb, non_blocking=True)
+            self.copied[s].record(cs)
+
+        results = []
+        ahead = D - 1
         with torch.cuda.device(self.device):
+            for j in range(min(ahead, len(batches))):
+                upload(j)
             for j, hb in enumerate(batches):
-                n = int(hb.shape[0])
-                if n > self.batch or tuple(hb.shape[1:]) != (self.plan.H, self.plan.W):
-                    raise ValueError("batch larger than the pipeline's batch size or of another pattern size")
-                if hb.dtype != self.tdtype:
-                    raise TypeError(f"batch dtype {hb.dtype}, pipeline dtype {self.tdtype}")
-                s = j % self.depth
-                cs, ks = self.copy_stream, self.compute_stream
-                if self._used[s]:
-                    cs.wait_event(self.computed[s])          # slot s's last band detection has read it
-                with torch.cuda.stream(cs):
-                    self.slots[s][:n].copy_(hb, non_blocking=True)
-                self.copied[s].record(cs)
+                if j + ahead < len(batches):
+                    upload(j + ahead)                        # its slot's last reader is batch j - 1
+                s, n = j % D, int(hb.shape[0])
                 ks.wait_event(self.copied[s])
                 if self._used[s]:
-                    ks.wait_event(self.returned[s])          # slot s's last results have been copied back
-                out = self.outs[s]
-                sub = {k: v[:n] for k, v in out.items()}
+                    ks.wait_event(self.returned[s])          # slot s's last results are back on the host
+                sub = {k: v[:n] for k, v in self.outs[s].items()}
                 self.bd.find_bands_device(self.slots[s][:n], gpu_id=self.device.index, out=sub, work=self.work,
                                           chunk=self.chunk, stream=ks)
                 self.computed[s].record(ks)
-                rs = self.return_stream
+                self._used[s] = True
                 rs.wait_event(self.computed[s])
                 with torch.cuda.stream(rs):
-                    for k in self.FIELDS:
-                        results[k][off:off + n].copy_(sub[k], non_blocking=True)
+                    blocks[j].copy_(self.blocks[s], non_blocking=True)
                 self.returned[s].record(rs)
-                self._used[s] = True
-                off += n
-            self.return_stream.synchronize()
-            self.copy_stream.synchronize()
+                results.append(self._views(blocks[j], n))
+            rs.synchronize()
+            cs.synchronize()
         return results
+
+    @staticmethod
+    def concat(results):
+        """Per-field host tensors of all patterns from ``run``'s per-batch results."""
+        import torch
+        return {k: torch.cat([r[k] for r in results]) for k in results[0]} if results else {}

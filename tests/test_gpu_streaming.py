@@ -29,11 +29,11 @@ def test_streaming_equals_resident(shape, dtype, kw, sizes, batch):
         t.copy_(pats[off:off + n])
         hb.append(t)
         off += n
-    got = sb.run(hb)
+    got = sb.concat(sb.run(hb))
     torch.cuda.synchronize()
     for k in want:
         assert np.array_equal(got[k].numpy(), want[k]), k
     # a second pass through the same pipeline (slots reused after their events)
-    got2 = sb.run(hb)
+    got2 = sb.concat(sb.run(hb))
     for k in want:
         assert np.array_equal(got2[k].numpy(), want[k]), k
