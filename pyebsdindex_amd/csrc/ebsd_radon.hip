@@ -91,7 +91,7 @@ struct ebsd_plan {
     uint32_t *d_mrow = nullptr;  // rho-mask bits per rho row (PeakGeo::maskw words each)
     // adjoint structures, built on first use
     int *d_px_off = nullptr, *d_px_cell = nullptr; // pixel -> lines through it (CSR)
-    int *d_cnt = nullptr;                          // pixels per line
+    double *d_icnt = nullptr;                      // 1 / pixels per line
     int *d_pre_off = nullptr, *d_pre = nullptr;    // Radon cell -> padded cells copying it (CSR)
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
     // tuning
@@ -1523,7 +1523,7 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_mrow);
     (void)hipFree(p->d_px_off);
     (void)hipFree(p->d_px_cell);
-    (void)hipFree(p->d_cnt);
+    (void)hipFree(p->d_icnt);
     (void)hipFree(p->d_pre_off);
     (void)hipFree(p->d_pre);
     (void)hipFree(p->d_grid);
@@ -2232,18 +2232,16 @@ __global__ __launch_bounds__(256) void k_band_grad(BandGradArgs a)
 
 // dR = padding adjoint of the transposed convolution of dC: for every Radon cell, the sum
 // over the padded cells that copy it (interior, replicated rho rows, mirrored theta
-// columns) of sum_{a,b} u[a] v[b] dC[y - kr/2 + a][x - kt/2 + b].
-__global__ __launch_bounds__(256) void k_conv_transpose(const float *dconv, int64_t N, PeakGeo g, const int *pre_off,
-                                                        const int *pre, float *dradon)
+// columns) of sum_{a,b} u[a] v[b] dC[y - kr/2 + a][x - kt/2 + b].  LDS: one work-group per
+// pattern stages its dC; GLOBAL: dC read from memory (padded Radons too large for LDS).
+template <bool LDS>
+__global__ __launch_bounds__(1024) void k_conv_transpose(const float *dconv, int64_t N, PeakGeo g, const int *pre_off,
+                                                         const int *pre, float *dradon)
 {
+    extern __shared__ __align__(16) float sdc[];
     const int nCells = g.nR * g.nT, nC = g.nRp * g.nTp;
     const int ckr = g.kr / 2, ckt = g.kt / 2;
-    const int64_t total = N * (int64_t)nCells;
-    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
-         id += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t q = id / nCells;
-        const int cell = (int)(id - q * nCells);
-        const float *dC = dconv + q * (int64_t)nC;
+    auto cell_value = [&](const float *dC, int cell) -> double {
         double acc = 0.0;
         for (int k = pre_off[cell]; k < pre_off[cell + 1]; ++k) {
             const int y = pre[k] / g.nTp, x = pre[k] - (pre[k] / g.nTp) * g.nTp;
@@ -2258,28 +2256,65 @@ __global__ __launch_bounds__(256) void k_conv_transpose(const float *dconv, int6
                 acc += (double)g.u[aa] * t;
             }
         }
-        dradon[id] = (float)acc;
+        return acc;
+    };
+    if constexpr (LDS) {
+        const int64_t q = blockIdx.x;
+        const float *src = dconv + q * (int64_t)nC;
+        for (int i = threadIdx.x; i < nC; i += blockDim.x) sdc[i] = src[i];
+        __syncthreads();
+        for (int cell = threadIdx.x; cell < nCells; cell += blockDim.x)
+            dradon[q * (int64_t)nCells + cell] = (float)cell_value(sdc, cell);
+    } else {
+        const int64_t total = N * (int64_t)nCells;
+        for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+             id += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t q = id / nCells;
+            const int cell = (int)(id - q * nCells);
+            dradon[id] = (float)cell_value(dconv + q * (int64_t)nC, cell);
+        }
     }
 }
 
 // Radon adjoint: d(pixel) = sum over the lines through the pixel of dR[line] / count[line]
-// (radon_fast.py:288-310 divides each line sum by its pixel count).
-__global__ __launch_bounds__(256) void k_backproject(const float *dradon, int64_t N, int nPx, int nCells,
-                                                     const int *px_off, const int *px_cell, const int *cnt,
-                                                     float *dpats)
+// (radon_fast.py:288-310 divides each line sum by its pixel count).  LDS: one work-group
+// per pattern stages dR[line] / count[line] in float64 once; each wavefront takes a pixel
+// at a time, its lanes read the pixel's lines with coalesced loads and the float64 sum
+// closes with a fixed butterfly (deterministic).  GLOBAL: one thread per pixel reads the
+// lines from memory (Radons too large for LDS).
+template <bool LDS>
+__global__ __launch_bounds__(1024) void k_backproject(const float *dradon, int64_t N, int nPx, int nCells,
+                                                      const int *px_off, const int *px_cell, const double *icnt,
+                                                      float *dpats)
 {
-    const int64_t total = N * (int64_t)nPx;
-    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
-         id += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t q = id / nPx;
-        const int px = (int)(id - q * nPx);
+    if constexpr (LDS) {
+        extern __shared__ __align__(16) double sdr[];
+        const int64_t q = blockIdx.x;
         const float *dR = dradon + q * (int64_t)nCells;
-        double acc = 0.0;
-        for (int k = px_off[px]; k < px_off[px + 1]; ++k) {
-            const int cell = px_cell[k];
-            acc += (double)dR[cell] / (double)cnt[cell];
+        for (int c = threadIdx.x; c < nCells; c += blockDim.x) sdr[c] = (double)dR[c] * icnt[c];
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int px = wave; px < nPx; px += nw) {
+            const int k0 = px_off[px], k1 = px_off[px + 1];
+            double acc = 0.0;
+            for (int k = k0 + lane; k < k1; k += 64) acc += sdr[px_cell[k]];
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (lane == 0) dpats[q * (int64_t)nPx + px] = (float)acc;
         }
-        dpats[id] = (float)acc;
+    } else {
+        const int64_t total = N * (int64_t)nPx;
+        for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < total;
+             id += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t q = id / nPx;
+            const int px = (int)(id - q * nPx);
+            const float *dR = dradon + q * (int64_t)nCells;
+            double acc = 0.0;
+            for (int k = px_off[px]; k < px_off[px + 1]; ++k) {
+                const int cell = px_cell[k];
+                acc = fma((double)dR[cell], icnt[cell], acc);
+            }
+            dpats[id] = (float)acc;
+        }
     }
 }
 
@@ -2370,12 +2405,13 @@ static int ensure_adjoint(ebsd_plan *p)
     if (p->d_px_off) return 0;
     const int nPx = p->nPx, nCells = p->nCells;
     // pixel -> lines through it, by ascending line index
-    std::vector<int> px_off(nPx + 1, 0), px_cell(p->h_ent.size()), cnt(nCells);
+    std::vector<int> px_off(nPx + 1, 0), px_cell(p->h_ent.size());
+    std::vector<double> icnt(nCells);
     for (size_t k = 0; k < p->h_ent.size(); ++k) ++px_off[p->h_ent[k] + 1];
     for (int i = 0; i < nPx; ++i) px_off[i + 1] += px_off[i];
     std::vector<int> fill(px_off.begin(), px_off.end() - 1);
     for (int cell = 0; cell < nCells; ++cell) {
-        cnt[cell] = p->h_cnt[cell];
+        icnt[cell] = p->h_cnt[cell] > 0 ? 1.0 / (double)p->h_cnt[cell] : 0.0;
         for (int64_t k = p->h_off[cell]; k < p->h_off[cell + 1]; ++k) px_cell[fill[p->h_ent[k]]++] = cell;
     }
     // Radon cell -> padded cells that copy it (the padding index map inverted)
@@ -2399,7 +2435,7 @@ static int ensure_adjoint(ebsd_plan *p)
     int rc;
     if ((rc = dev_upload(&p->d_px_off, px_off.data(), px_off.size()))) return rc;
     if ((rc = dev_upload(&p->d_px_cell, px_cell.data(), px_cell.size()))) return rc;
-    if ((rc = dev_upload(&p->d_cnt, cnt.data(), cnt.size()))) return rc;
+    if ((rc = dev_upload(&p->d_icnt, icnt.data(), icnt.size()))) return rc;
     if ((rc = dev_upload(&p->d_pre_off, pre_off.data(), pre_off.size()))) return rc;
     if ((rc = dev_upload(&p->d_pre, pre.data(), pre.size()))) return rc;
     return 0;
@@ -2442,8 +2478,16 @@ extern "C" int ebsd_conv_transpose(ebsd_plan *p, const float *d_grad_conv, int64
     if (!d_grad_conv || !d_grad_radon) return fail(-1, "ebsd_conv_transpose: null device pointer");
     int rc;
     if ((rc = ensure_adjoint(p))) return rc;
-    hipLaunchKernelGGL(k_conv_transpose, dim3(grid_for(N * (int64_t)p->nCells)), dim3(256), 0, (hipStream_t)stream,
-                       d_grad_conv, N, p->g, p->d_pre_off, p->d_pre, d_grad_radon);
+    const size_t lds = (size_t)p->g.nRp * p->g.nTp * 4;
+    if ((int)lds <= p->lds_max) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_conv_transpose<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(k_conv_transpose<true>, dim3((unsigned)N), dim3(1024), lds, (hipStream_t)stream,
+                           d_grad_conv, N, p->g, p->d_pre_off, p->d_pre, d_grad_radon);
+    } else {
+        hipLaunchKernelGGL(k_conv_transpose<false>, dim3(grid_for(N * (int64_t)p->nCells)), dim3(256), 0,
+                           (hipStream_t)stream, d_grad_conv, N, p->g, p->d_pre_off, p->d_pre, d_grad_radon);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -2456,8 +2500,17 @@ extern "C" int ebsd_backproject(ebsd_plan *p, const float *d_grad_radon, int64_t
     if (!d_grad_radon || !d_grad_pats) return fail(-1, "ebsd_backproject: null device pointer");
     int rc;
     if ((rc = ensure_adjoint(p))) return rc;
-    hipLaunchKernelGGL(k_backproject, dim3(grid_for(N * (int64_t)p->nPx)), dim3(256), 0, (hipStream_t)stream,
-                       d_grad_radon, N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell, p->d_cnt, d_grad_pats);
+    const size_t lds = (size_t)p->nCells * 8;
+    if ((int)lds <= p->lds_max) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_backproject<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(k_backproject<true>, dim3((unsigned)N), dim3(1024), lds, (hipStream_t)stream, d_grad_radon,
+                           N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell, p->d_icnt, d_grad_pats);
+    } else {
+        hipLaunchKernelGGL(k_backproject<false>, dim3(grid_for(N * (int64_t)p->nPx)), dim3(256), 0,
+                           (hipStream_t)stream, d_grad_radon, N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell,
+                           p->d_icnt, d_grad_pats);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }
