@@ -367,12 +367,16 @@ def main():
         batched = batched_measure(dev, args.steps, args.warmup)
         print(f"[bench] batched {batched['graph_patterns_per_s']:.1f} patterns/s, plain per-size "
               f"{batched['plain_patterns_per_s']:.1f} patterns/s", file=sys.stderr, flush=True)
-        small = batched_measure(dev, args.steps, args.warmup, npats=2048, autotune_sets=False)
-        batched["small_batches"] = {k: small[k] for k in ("workload", "plain_patterns_per_s", "graph_patterns_per_s",
-                                                          "graph_vs_plain", "plain_ms", "graph_ms",
-                                                          "peak_bins_identical")}
-        print(f"[bench] small mixed batches: graph {small['graph_patterns_per_s']:.1f} vs plain "
-              f"{small['plain_patterns_per_s']:.1f} patterns/s", file=sys.stderr, flush=True)
+        sweep = []
+        for n in (512, 2048, 8192):
+            small = batched_measure(dev, args.steps, args.warmup, npats=n, autotune_sets=False)
+            sweep.append({k: small[k] for k in ("plain_patterns_per_s", "graph_patterns_per_s", "graph_vs_plain",
+                                                "plain_ms", "graph_ms", "peak_bins_identical")})
+            sweep[-1]["patterns"] = n
+            print(f"[bench] {n} mixed patterns: graph {small['graph_patterns_per_s']:.1f} vs plain "
+                  f"{small['plain_patterns_per_s']:.1f} patterns/s", file=sys.stderr, flush=True)
+        batched["smaller_batches"] = {"workload": "the same mixture at fewer patterns (launch defaults, no autotune)",
+                                      "sweep": sweep}
 
     diff = None
     if not args.no_diff:

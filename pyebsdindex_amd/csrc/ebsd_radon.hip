@@ -2716,12 +2716,18 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         gc.nitems = (int64_t)gitems.size() - gc.item0;
         if (gc.nitems > 0) b->classes.push_back(gc);
     }
-    // peak items, one class per distinct geometry, in first-seen order
+    // peak items, one class per distinct geometry, in first-seen order (the streamed kernel
+    // ignores the tiled kernel's tile shape, so sets that differ only there share a launch)
+    auto geo_key = [](const ebsd_plan *p) {
+        PeakGeo k = p->g;
+        if (use_stream(p)) k.tile = k.ntiles = k.rh = k.apitch = k.bpitch = 0;
+        return k;
+    };
     std::vector<int2> pitems;
     std::vector<int> done(nsets, 0);
     for (int i0 = 0; i0 < nsets; ++i0) {
         if (done[i0]) continue;
-        const PeakGeo &g = plans[i0]->g;
+        const PeakGeo g = geo_key(plans[i0]);
         PeakClass pc;
         pc.g = g;
         pc.L = peak_lds_layout(g);
@@ -2737,8 +2743,9 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         }
         pc.item0 = (int64_t)pitems.size();
         for (int i : order) {
-            if (done[i] || use_stream(plans[i]) != pc.stream || std::memcmp(&plans[i]->g, &g, sizeof(PeakGeo)) != 0)
-                continue;
+            if (done[i] || use_stream(plans[i]) != pc.stream) continue;
+            const PeakGeo gi = geo_key(plans[i]);
+            if (std::memcmp(&gi, &g, sizeof(PeakGeo)) != 0) continue;
             done[i] = 1;
             for (int64_t q = 0; q < N[i]; ++q) pitems.push_back(make_int2(i, (int)q));
         }
