@@ -76,6 +76,10 @@ int ebsd_plan_set_background(ebsd_plan *plan, const double *bg, int exact);
 int ebsd_plan_set_tuning(ebsd_plan *plan, int gather_pack, int gather_threads, int peak_tile,
                          int peak_threads);
 
+/* LDS words per staged pattern row in the Radon gather (>= W; <= 0 restores W).  Padding the
+ * rows changes which LDS banks the pixels of a line fall in; results are identical. */
+int ebsd_plan_set_stage_pitch(ebsd_plan *plan, int pitch);
+
 /* Which convolution/peak kernel runs the stages of band_detect.py:562-809 (same results):
  *   0 automatic (the streamed kernel where it applies), 1 theta-tiled work-group kernel,
  *   2 streamed kernel (one wavefront per block of theta columns; needs a 9x5 or 5x5 kernel

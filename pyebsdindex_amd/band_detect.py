@@ -96,6 +96,10 @@ class DevicePlan:
         _hip.check(self.lib.ebsd_plan_set_tuning(self.handle, int(gather_pack), int(gather_threads),
                                                  int(peak_tile), int(peak_threads)))
 
+    def set_stage_pitch(self, pitch):
+        """LDS words per staged pattern row in the Radon gather (0 = the pattern width)."""
+        _hip.check(self.lib.ebsd_plan_set_stage_pitch(self.handle, int(pitch)))
+
     def set_peak_kernel(self, kind):
         """0 automatic, 1 theta-tiled work-group kernel, 2 streamed wavefront kernel."""
         _hip.check(self.lib.ebsd_plan_set_peak_kernel(self.handle, int(kind)))

@@ -70,7 +70,8 @@ struct ebsd_plan {
     int H = 0, W = 0, nPx = 0, nT = 0, nR = 0, nCells = 0;
     // compiled gather plan
     int nbatch = 0;
-    uint4 *d_gidx = nullptr;     // [block][lane] groups of 8 u16 pixel ids
+    int spitch = 0;              // LDS words per staged pattern row (>= W); plan entries use it
+    int nwords = 0;              // H * spitch: the zero word's index, one less than the words staged
     uint2 *d_bhdr = nullptr;     // per batch: first block, number of blocks
     int2 *d_lane = nullptr;      // per batch lane: cell id (-1 = none), entry count
     std::vector<int> h_cnt;      // per cell entry count
@@ -78,8 +79,14 @@ struct ebsd_plan {
     std::vector<uint16_t> h_ent; // per cell valid entries, plan order
     std::vector<uint16_t> h_groups; // batch-major plan groups [block][lane][8]
     std::vector<uint2> h_bhdr;   // per batch: first block in h_groups, number of blocks
-    int stream_waves = 0;        // waves per work-group the device streams are laid out for
-    int *d_wofs = nullptr, *d_wcnt = nullptr;
+    // device streams, one layout per count of (virtual) wave slots: a work-group of nw waves
+    // that splits each pack's lines S ways walks them as nw * S slots
+    struct Layout {
+        int nv;
+        uint4 *gidx;
+        int *wofs, *wcnt;
+    };
+    std::vector<Layout> layouts;
     // background
     int bg_mode = 0;             // 0 none, 1 per-line sums, 2 exact per-pixel
     double *d_linebg = nullptr;  // nCells
@@ -100,6 +107,19 @@ struct ebsd_plan {
     int pkind = 0; // peak kernel: 0 automatic, 1 tiled (k_peaks), 2 streamed (k_peaks_stream)
 };
 
+// n / d for n, d < 2^16 by multiply-high (m = ceil(2^32 / d)).
+struct FDiv {
+    uint32_t d, m;
+};
+__device__ __forceinline__ FDiv fdiv_make(uint32_t d)
+{
+    FDiv f;
+    f.d = d;
+    f.m = d > 1 ? 0xFFFFFFFFu / d + 1u : 0u;
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv_q(uint32_t n, FDiv f) { return f.d > 1 ? __umulhi(n, f.m) : n; }
+
 // ---------------------------------------------------------------------------------------
 // k_radon_gather
 // ---------------------------------------------------------------------------------------
@@ -107,12 +127,14 @@ struct GatherArgs {
     const void *pats;
     int64_t N;
     int nPx;
+    int W, pitch;         // pattern row length; LDS words per staged row (>= W)
     int nbatch;
     const uint4 *gidx;    // per-wave streams of plan groups
     const uint2 *bhdr;    // per batch: (unused, groups)
     const int2 *lane;     // per batch lane: (cell, count)
-    const int *wofs;      // per wave: first group of its stream
-    const int *wcnt;      // per wave: groups in its stream
+    const int *wofs;      // per wave slot: first group of its stream
+    const int *wcnt;      // per wave slot: groups in its stream
+    int nslice;           // work-groups sharing a pack's lines (slot = slice * waves + wave)
     const double *linebg; // fast background mode
     const double *bg;     // exact background mode
     float *out;           // (N, nCells)
@@ -248,7 +270,7 @@ __device__ __forceinline__ void pack_word(uint32_t *d, const typename InT<IN>::T
 // One pack of P patterns: stage them in LDS, then every wave walks its stream of plan
 // groups and writes the P Radon values of each of its lines.
 template <int IN, bool FLT, int P>
-__device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, unsigned char *lds_raw)
+__device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, int slice, unsigned char *lds_raw)
 {
     typedef GatherCfg<IN, FLT, P> C;
     typedef typename WordT<C::WB>::T Word;
@@ -263,8 +285,11 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, u
     const int nwaves = blockDim.x >> 6;
     const int nPx = a.nPx;
     const int nb = a.nbatch;
-    const bool vec_ok = (nPx % VEC) == 0 && ((uintptr_t)a.pats % 8) == 0;
+    const bool vec_ok = (a.W % VEC) == 0 && ((uintptr_t)a.pats % 8) == 0;
     const int nvec = vec_ok ? nPx / VEC : 0;
+    // pixel px is staged at word (px / W) * pitch + px % W (the plan's entries use the same map)
+    const FDiv fW = fdiv_make(a.W);
+    auto word_of = [&](int px) { const int y = (int)fdiv_q(px, fW); return y * a.pitch + (px - y * a.W); };
     const uint4 *gbase = a.gidx + lane;
     // LDS address of the staged words (the entries are word indices)
     const uint32_t lbase = (uint32_t)(size_t)(const __attribute__((address_space(3))) unsigned char *)(lds_raw);
@@ -281,6 +306,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, u
                     raw = *reinterpret_cast<const uint2 *>(pats + (q0 + p) * (int64_t)nPx + (int64_t)v * VEC);
                 __builtin_memcpy(&x[p][0], &raw, 8);
             }
+            const int w0 = word_of(v * VEC); // VEC divides W: a vector stays in one row
 #pragma unroll
             for (int j = 0; j < VEC; ++j) {
                 Tin xp[P];
@@ -290,7 +316,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, u
                 pack_word<IN, FLT, P>(d, xp, a.bg, v * VEC + j);
                 Word w;
                 __builtin_memcpy(&w, d, sizeof(Word));
-                lds[v * VEC + j] = w;
+                lds[w0 + j] = w;
             }
         }
         for (int px = nvec * VEC + tid; px <= nPx; px += blockDim.x) {
@@ -306,21 +332,22 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, u
             }
             Word w;
             __builtin_memcpy(&w, d, sizeof(Word));
-            lds[px] = w;
+            lds[word_of(px)] = w; // px == nPx: the zero word H * pitch
         }
         __syncthreads();
 
         // ---- gather: each lane owns one line of the wave's current 64-line batch; the
         //      wave walks its stream of plan groups with three groups in flight ----
-        int b = wave;
+        const int nslot = nwaves * a.nslice, slot = slice * nwaves + wave;
+        int b = slot;
         if (b < nb) {
-            const uint4 *S = a.gidx + (size_t)a.wofs[wave] * 64 + lane;
-            const int G = a.wcnt[wave];
+            const uint4 *S = a.gidx + (size_t)a.wofs[slot] * 64 + lane;
+            const int G = a.wcnt[slot];
             // four plan groups in flight; each register is refilled right after use
             uint4 eq[4] = {S[0], S[64], S[128], S[192]}; // filler past the end keeps these in bounds
             int bend = (int)a.bhdr[b].y;
             int2 li = a.lane[(size_t)b * 64 + lane];
-            int bn = b + nwaves;
+            int bn = b + nslot;
             int2 lin = li;
             int ngn = 0;
             if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
@@ -363,7 +390,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, u
                             b = bn;
                             li = lin;
                             bend += ngn;
-                            bn = b + nwaves;
+                            bn = b + nslot;
                             if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
                         }
                     }
@@ -378,7 +405,9 @@ template <int IN, bool FLT, int P>
 __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    for (int64_t pack = blockIdx.x; pack < a.npacks; pack += gridDim.x) gather_pack<IN, FLT, P>(a, pack, lds_raw);
+    const int64_t nitems = a.npacks * a.nslice;
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x)
+        gather_pack<IN, FLT, P>(a, it / a.nslice, (int)(it % a.nslice), lds_raw);
 }
 
 // Variants of the gather a batch may mix (element type, float staging, patterns per word).
@@ -411,7 +440,7 @@ __global__ __launch_bounds__(1024) void k_radon_gather_multi(GatherMultiArgs m)
         const GatherArgs &a = m.sets[item.x];
         switch (m.variant[item.x]) {
 #define EBSD_GV_CASE(ID, IN, FLT, P) \
-    case ID: gather_pack<IN, FLT, P>(a, item.y, lds_raw); break;
+    case ID: gather_pack<IN, FLT, P>(a, item.y / a.nslice, item.y % a.nslice, lds_raw); break;
             EBSD_GATHER_VARIANTS(EBSD_GV_CASE)
 #undef EBSD_GV_CASE
         default: break;
@@ -518,18 +547,6 @@ __device__ __forceinline__ double np_interp_unit(double x, const double *fp, int
     return slope * (x - (double)j) + fp[j];
 }
 
-// n / d for n, d < 2^16 by multiply-high (m = ceil(2^32 / d)).
-struct FDiv {
-    uint32_t d, m;
-};
-__device__ __forceinline__ FDiv fdiv_make(uint32_t d)
-{
-    FDiv f;
-    f.d = d;
-    f.m = d > 1 ? 0xFFFFFFFFu / d + 1u : 0u;
-    return f;
-}
-__device__ __forceinline__ uint32_t fdiv_q(uint32_t n, FDiv f) { return f.d > 1 ? __umulhi(n, f.m) : n; }
 
 // Top-nB of (running list U candidate list) by (value, position) descending -- a total
 // order, so ranks are unique: each entry counts the entries above it and, if its rank is
@@ -1508,14 +1525,22 @@ template <typename T> static int dev_upload(T **dptr, const T *src, size_t n)
     return 0;
 }
 
+static void free_layouts(ebsd_plan *p)
+{
+    for (ebsd_plan::Layout &l : p->layouts) {
+        (void)hipFree(l.gidx);
+        (void)hipFree(l.wofs);
+        (void)hipFree(l.wcnt);
+    }
+    p->layouts.clear();
+}
+
 static void plan_free(ebsd_plan *p)
 {
     if (!p) return;
-    (void)hipFree(p->d_gidx);
+    free_layouts(p);
     (void)hipFree(p->d_bhdr);
     (void)hipFree(p->d_lane);
-    (void)hipFree(p->d_wofs);
-    (void)hipFree(p->d_wcnt);
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     (void)hipFree(p->d_mask);
@@ -1583,20 +1608,21 @@ static int geo_tiles(ebsd_plan *p)
     return 0;
 }
 
-// Lay the plan groups out as one contiguous stream per wave slot: wave w of a work-group
-// with nw waves walks batches w, w + nw, w + 2nw, ... back to back, so its next groups can
-// be prefetched several iterations ahead without looking at batch boundaries.
-static int layout_streams(ebsd_plan *p, int nw)
+// Lay the plan groups out as one contiguous stream per wave slot: slot v of nv walks batches
+// v, v + nv, v + 2nv, ... back to back, so its next groups can be prefetched several
+// iterations ahead without looking at batch boundaries.  Layouts are cached per nv.
+static int get_layout(ebsd_plan *p, int nv, ebsd_plan::Layout *out)
 {
-    if (nw == p->stream_waves && p->d_gidx) return 0;
+    for (const ebsd_plan::Layout &l : p->layouts)
+        if (l.nv == nv) { *out = l; return 0; }
     std::vector<uint16_t> st;
-    std::vector<int> wofs(nw), wcnt(nw);
-    st.reserve(p->h_groups.size() + (size_t)(nw + 1) * 64 * 8);
+    std::vector<int> wofs(nv), wcnt(nv);
+    st.reserve(p->h_groups.size() + (size_t)(nv + 1) * 64 * 8);
     int ofs = 0;
-    for (int w = 0; w < nw; ++w) {
+    for (int w = 0; w < nv; ++w) {
         wofs[w] = ofs;
         int cnt = 0;
-        for (int b = w; b < p->nbatch; b += nw) {
+        for (int b = w; b < p->nbatch; b += nv) {
             const uint2 h = p->h_bhdr[b];
             const size_t from = (size_t)h.x * 64 * 8;
             st.insert(st.end(), p->h_groups.begin() + from, p->h_groups.begin() + from + (size_t)h.y * 64 * 8);
@@ -1606,17 +1632,99 @@ static int layout_streams(ebsd_plan *p, int nw)
         ofs += cnt;
     }
     // filler groups past the end keep the prefetch queue's reads in bounds
-    st.resize(st.size() + 8 * 64 * 8, (uint16_t)p->nPx);
-    (void)hipFree(p->d_gidx);
-    (void)hipFree(p->d_wofs);
-    (void)hipFree(p->d_wcnt);
-    p->d_gidx = nullptr;
-    p->d_wofs = p->d_wcnt = nullptr;
+    st.resize(st.size() + 8 * 64 * 8, (uint16_t)p->nwords);
+    ebsd_plan::Layout l{nv, nullptr, nullptr, nullptr};
+    if (dev_upload(reinterpret_cast<uint16_t **>(&l.gidx), st.data(), st.size()) ||
+        dev_upload(&l.wofs, wofs.data(), wofs.size()) || dev_upload(&l.wcnt, wcnt.data(), wcnt.size())) {
+        (void)hipFree(l.gidx);
+        (void)hipFree(l.wofs);
+        (void)hipFree(l.wcnt);
+        return fail(-3, "plan stream layout: device allocation failed");
+    }
+    p->layouts.push_back(l);
+    *out = l;
+    return 0;
+}
+
+// Slices per pack so that a launch of npacks packs has about two resident rounds of
+// work-groups, keeping at least two batches per slot.
+static int choose_slices(const ebsd_plan *p, int64_t npacks, int nw, int occ)
+{
+    const int64_t target = 2 * (int64_t)occ * p->ncu;
+    if (npacks >= target) return 1;
+    const int smax = std::max(1, p->nbatch / (2 * nw));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(smax, (target + npacks - 1) / npacks));
+}
+
+// Default staged row pitch: rows of W words put the pixels of a near-vertical line in a few
+// LDS banks when W is a multiple of 8 or 16 (measured: 160x160 u16 gathers 2x faster at
+// pitch 163 than at 160, 120x120 5% faster at 122 than at 120), so pad by 3 (W a multiple of
+// 16) or 2 words -- unless the padding would push a word size that fits at W out of LDS.
+static int default_pitch(const ebsd_plan *p)
+{
+    const int W = p->W, H = p->H;
+    const int want = (W % 16 == 0) ? 3 : (W % 8 == 0 ? 2 : 0);
+    for (int pad = want; pad > 0; --pad) {
+        bool ok = H * (W + pad) <= 65535;
+        for (int wb = 2; wb <= 16 && ok; wb *= 2) {
+            const bool fit0 = (size_t)(H * W + 1) * wb <= (size_t)p->lds_max;
+            const bool fit1 = (size_t)(H * (W + pad) + 1) * wb <= (size_t)p->lds_max;
+            ok = !fit0 || fit1;
+        }
+        if (ok) return W + pad;
+    }
+    return W;
+}
+
+// Lines sorted by pixel count, cut into 64-line batches of 8-entry groups (one lane per
+// line); entries are LDS word indices of the staged pixels at row pitch `spitch`.  The device
+// streams of the current work-group size are laid out from the result.
+static int compile_groups(ebsd_plan *p)
+{
+    const int nCells = p->nCells, W = p->W;
+    p->nwords = p->H * p->spitch;
+    if (p->nwords > 65535) return fail(-2, "ebsd_plan: staged pattern exceeds 65,535 LDS words");
+    std::vector<int> order(nCells);
+    for (int i = 0; i < nCells; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return p->h_cnt[x] > p->h_cnt[y]; });
+    p->nbatch = (nCells + 63) / 64;
+    std::vector<uint2> bhdr(p->nbatch);
+    std::vector<int2> lanes((size_t)p->nbatch * 64);
+    std::vector<uint16_t> gidx;
+    uint32_t blocks = 0;
+    for (int b = 0; b < p->nbatch; ++b) {
+        int mx = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int i = b * 64 + l;
+            if (i < nCells) mx = std::max(mx, p->h_cnt[order[i]]);
+        }
+        const int ngrp = std::max(1, (mx + 7) / 8); // an empty batch still writes its -1 cells
+        bhdr[b] = make_uint2(blocks, (uint32_t)ngrp);
+        gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)p->nwords);
+        for (int l = 0; l < 64; ++l) {
+            const int i = b * 64 + l;
+            if (i >= nCells) { lanes[(size_t)b * 64 + l] = make_int2(-1, 0); continue; }
+            const int cell = order[i];
+            lanes[(size_t)b * 64 + l] = make_int2(cell, p->h_cnt[cell]);
+            for (int k = 0; k < p->h_cnt[cell]; ++k) {
+                const size_t at = ((size_t)(blocks + k / 8) * 64 + l) * 8 + (k % 8);
+                const int e = p->h_ent[p->h_off[cell] + k];
+                gidx[at] = (uint16_t)((e / W) * p->spitch + e % W);
+            }
+        }
+        blocks += (uint32_t)ngrp;
+    }
+    p->h_groups.swap(gidx);
+    p->h_bhdr = bhdr;
+    (void)hipFree(p->d_bhdr);
+    (void)hipFree(p->d_lane);
+    p->d_bhdr = nullptr;
+    p->d_lane = nullptr;
+    free_layouts(p); // laid out again on first use
     int rc;
-    if ((rc = dev_upload(reinterpret_cast<uint16_t **>(&p->d_gidx), st.data(), st.size()))) return rc;
-    if ((rc = dev_upload(&p->d_wofs, wofs.data(), wofs.size()))) return rc;
-    if ((rc = dev_upload(&p->d_wcnt, wcnt.data(), wcnt.size()))) return rc;
-    p->stream_waves = nw;
+    if ((rc = dev_upload(&p->d_bhdr, bhdr.data(), bhdr.size()))) return rc;
+    if ((rc = dev_upload(&p->d_lane, lanes.data(), lanes.size()))) return rc;
     return 0;
 }
 
@@ -1673,41 +1781,8 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
         p->h_cnt[cell] = n;
     }
     p->h_off[nCells] = (int64_t)p->h_ent.size();
-    std::vector<int> order(nCells);
-    for (int i = 0; i < nCells; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return p->h_cnt[x] > p->h_cnt[y]; });
-    p->nbatch = (nCells + 63) / 64;
-    std::vector<uint2> bhdr(p->nbatch);
-    std::vector<int2> lanes((size_t)p->nbatch * 64);
-    std::vector<uint16_t> gidx;
-    uint32_t blocks = 0;
-    for (int b = 0; b < p->nbatch; ++b) {
-        int mx = 0;
-        for (int l = 0; l < 64; ++l) {
-            const int i = b * 64 + l;
-            if (i < nCells) mx = std::max(mx, p->h_cnt[order[i]]);
-        }
-        const int ngrp = std::max(1, (mx + 7) / 8); // an empty batch still writes its -1 cells
-        bhdr[b] = make_uint2(blocks, (uint32_t)ngrp);
-        gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)nPx);
-        for (int l = 0; l < 64; ++l) {
-            const int i = b * 64 + l;
-            if (i >= nCells) { lanes[(size_t)b * 64 + l] = make_int2(-1, 0); continue; }
-            const int cell = order[i];
-            lanes[(size_t)b * 64 + l] = make_int2(cell, p->h_cnt[cell]);
-            for (int k = 0; k < p->h_cnt[cell]; ++k) {
-                const size_t at = ((size_t)(blocks + k / 8) * 64 + l) * 8 + (k % 8);
-                gidx[at] = p->h_ent[p->h_off[cell] + k];
-            }
-        }
-        blocks += (uint32_t)ngrp;
-    }
-    p->h_groups.swap(gidx);
-    p->h_bhdr = bhdr;
-    if ((rc = layout_streams(p, p->gthreads / 64))) { plan_free(p); return rc; }
-    if ((rc = dev_upload(&p->d_bhdr, bhdr.data(), bhdr.size()))) { plan_free(p); return rc; }
-    if ((rc = dev_upload(&p->d_lane, lanes.data(), lanes.size()))) { plan_free(p); return rc; }
+    p->spitch = default_pitch(p);
+    if ((rc = compile_groups(p))) { plan_free(p); return rc; }
 
     // ---- peak geometry ----
     PeakGeo &g = p->g;
@@ -1795,6 +1870,22 @@ extern "C" int ebsd_plan_set_background(ebsd_plan *p, const double *bg, int exac
     return 0;
 }
 
+extern "C" int ebsd_plan_set_stage_pitch(ebsd_plan *p, int pitch)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_stage_pitch: null plan");
+    if (pitch <= 0) pitch = p->W;
+    if (pitch < p->W) return fail(-1, "ebsd_plan_set_stage_pitch: pitch below the pattern width");
+    if (pitch == p->spitch) return 0;
+    const int old = p->spitch;
+    p->spitch = pitch;
+    const int rc = compile_groups(p);
+    if (rc) {
+        p->spitch = old;
+        (void)compile_groups(p);
+    }
+    return rc;
+}
+
 extern "C" int ebsd_plan_set_peak_kernel(ebsd_plan *p, int kind)
 {
     if (!p) return fail(-1, "ebsd_plan_set_peak_kernel: null plan");
@@ -1817,8 +1908,6 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
     if (gather_threads > 0) {
         if (gather_threads % 64 || gather_threads > 1024) return fail(-1, "gather_threads must be a multiple of 64 <= 1024");
         p->gthreads = gather_threads;
-        int rc = layout_streams(p, gather_threads / 64);
-        if (rc) return rc;
     }
     if (peak_threads > 0) {
         if (peak_threads % 64 || peak_threads > 512) return fail(-1, "peak_threads must be a multiple of 64 <= 512");
@@ -1858,30 +1947,36 @@ static int launch_gather_t(ebsd_plan *p, const void *d_pats, int64_t N, float *d
                            hipStream_t s)
 {
     typedef GatherCfg<IN, FLT, P> C;
-    const size_t lds = (size_t)(p->nPx + 1) * C::WB;
+    const size_t lds = (size_t)(p->nwords + 1) * C::WB;
     if ((int)lds > p->lds_max) return 1; // caller falls back to a smaller pack
     auto kern = k_radon_gather<IN, FLT, P>;
     HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int occ = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->gthreads, lds));
+    if (occ < 1) return fail(-3, "radon gather kernel cannot be resident");
     GatherArgs a;
     a.pats = d_pats;
     a.N = N;
     a.nPx = p->nPx;
+    a.W = p->W;
+    a.pitch = p->spitch;
     a.nbatch = p->nbatch;
-    a.gidx = p->d_gidx;
     a.bhdr = p->d_bhdr;
     a.lane = p->d_lane;
-    a.wofs = p->d_wofs;
-    a.wcnt = p->d_wcnt;
-    if (p->stream_waves != p->gthreads / 64) return fail(-3, "plan streams laid out for another work-group size");
+    a.npacks = (N + P - 1) / P;
+    const int nw = p->gthreads / 64;
+    a.nslice = choose_slices(p, a.npacks, nw, occ);
+    ebsd_plan::Layout lay;
+    int rc;
+    if ((rc = get_layout(p, nw * a.nslice, &lay))) return rc;
+    a.gidx = lay.gidx;
+    a.wofs = lay.wofs;
+    a.wcnt = lay.wcnt;
     a.linebg = (use_bg && !FLT && p->bg_mode == 1) ? p->d_linebg : nullptr;
     a.bg = (use_bg && FLT) ? p->d_bg : nullptr;
     a.out = d_out;
     a.nCells = p->nCells;
-    a.npacks = (N + P - 1) / P;
-    int occ = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->gthreads, lds));
-    if (occ < 1) return fail(-3, "radon gather kernel cannot be resident");
-    const int64_t grid = std::min<int64_t>(a.npacks, (int64_t)occ * p->ncu);
+    const int64_t grid = std::min<int64_t>(a.npacks * a.nslice, (int64_t)occ * p->ncu);
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p->gthreads), lds, s, a);
     HIPCHK(hipGetLastError());
     return 0;
@@ -2564,7 +2659,7 @@ static int choose_variant(const ebsd_plan *p, int dtype, bool flt, VariantInfo *
     int best = -1;
     for (const VariantInfo &v : kVariants) {
         if (v.IN != dtype || v.FLT != flt || v.P > pmax) continue;
-        if ((size_t)(p->nPx + 1) * v.WB > (size_t)p->lds_max) continue;
+        if ((size_t)(p->nwords + 1) * v.WB > (size_t)p->lds_max) continue;
         if (best < 0 || v.P > kVariants[best].P) best = (int)(&v - kVariants);
     }
     if (best < 0) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
@@ -2666,24 +2761,20 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         VariantInfo v;
         if ((rc = choose_variant(p, dtype, flt, &v))) { batch_free(b); return rc; }
         var[i] = v.id;
-        const size_t lds = (size_t)(p->nPx + 1) * v.WB;
+        const size_t lds = (size_t)(p->nwords + 1) * v.WB;
         const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)p->lds_max / std::max<size_t>(lds, 1)));
         cls[i] = per_cu;
-        const int threads = per_cu >= 2 ? 512 : 1024;
-        if ((rc = layout_streams(p, threads / 64))) { batch_free(b); return rc; }
-        p->gthreads = threads;
         float *radon = (float *)work;
         work += (((size_t)N[i] * p->nCells * sizeof(float) + 255) / 256) * 256;
         GatherArgs &a = gs[i];
         a.pats = d_pats[i];
         a.N = N[i];
         a.nPx = p->nPx;
+        a.W = p->W;
+        a.pitch = p->spitch;
         a.nbatch = p->nbatch;
-        a.gidx = p->d_gidx;
         a.bhdr = p->d_bhdr;
         a.lane = p->d_lane;
-        a.wofs = p->d_wofs;
-        a.wcnt = p->d_wcnt;
         a.linebg = (use_bg && !flt && p->bg_mode == 1) ? p->d_linebg : nullptr;
         a.bg = (use_bg && flt) ? p->d_bg : nullptr;
         a.out = radon;
@@ -2704,17 +2795,44 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     std::vector<int> order(nsets);
     for (int i = 0; i < nsets; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return plans[x]->nPx > plans[y]->nPx; });
+    // a class of c work-groups per CU runs 512 threads (c >= 2) or 1,024; each set's packs are
+    // split into slices (work-groups sharing a pack's lines) so that the class has about two
+    // resident rounds of work-groups
     for (int c = 4; c >= 1; --c) {
         GatherClass gc;
+        gc.threads = c >= 2 ? 512 : 1024;
         gc.item0 = (int64_t)gitems.size();
+        int64_t packs = 0;
         for (int i : order) {
             if (cls[i] != c) continue;
-            gc.threads = plans[i]->gthreads;
-            gc.lds = std::max(gc.lds, (size_t)(plans[i]->nPx + 1) * kVariants[var[i]].WB);
-            for (int64_t k = 0; k < gs[i].npacks; ++k) gitems.push_back(make_int2(i, (int)k));
+            gc.lds = std::max(gc.lds, (size_t)(plans[i]->nwords + 1) * kVariants[var[i]].WB);
+            packs += gs[i].npacks;
+        }
+        if (packs == 0) continue;
+        int occ = 0;
+        if (hipFuncSetAttribute((const void *)k_radon_gather_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)gc.lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_radon_gather_multi, gc.threads, gc.lds) !=
+                hipSuccess || occ < 1) {
+            batch_free(b);
+            return fail(-3, "batched gather kernel cannot be resident");
+        }
+        const int nw = gc.threads / 64;
+        for (int i : order) {
+            if (cls[i] != c) continue;
+            GatherArgs &a = gs[i];
+            a.nslice = choose_slices(plans[i], packs, nw, occ);
+            ebsd_plan::Layout lay;
+            if ((rc = get_layout(plans[i], nw * a.nslice, &lay))) { batch_free(b); return rc; }
+            a.gidx = lay.gidx;
+            a.wofs = lay.wofs;
+            a.wcnt = lay.wcnt;
+            for (int64_t k = 0; k < a.npacks; ++k)
+                for (int sl = 0; sl < a.nslice; ++sl) gitems.push_back(make_int2(i, (int)(k * a.nslice + sl)));
         }
         gc.nitems = (int64_t)gitems.size() - gc.item0;
-        if (gc.nitems > 0) b->classes.push_back(gc);
+        gc.grid = (int)std::min<int64_t>(gc.nitems, (int64_t)occ * b->ncu);
+        b->classes.push_back(gc);
     }
     // peak items, one class per distinct geometry, in first-seen order (the streamed kernel
     // ignores the tiled kernel's tile shape, so sets that differ only there share a launch)
@@ -2759,19 +2877,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     if ((rc = dev_upload(&b->d_gitems, gitems.data(), gitems.size()))) { batch_free(b); return rc; }
     if ((rc = dev_upload(&b->d_psets, ps.data(), ps.size()))) { batch_free(b); return rc; }
     if ((rc = dev_upload(&b->d_pitems, pitems.data(), pitems.size()))) { batch_free(b); return rc; }
-    // occupancy-derived persistent grids
     int occ = 0;
-    for (GatherClass &gc : b->classes) {
-        occ = 0;
-        if (hipFuncSetAttribute((const void *)k_radon_gather_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)gc.lds) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_radon_gather_multi, gc.threads, gc.lds) !=
-                hipSuccess || occ < 1) {
-            batch_free(b);
-            return fail(-3, "batched gather kernel cannot be resident");
-        }
-        gc.grid = (int)std::min<int64_t>(gc.nitems, (int64_t)occ * b->ncu);
-    }
     // the dynamic-LDS limit of the gather kernel covers every class (set once, not per launch)
     size_t lds_all = 0;
     for (const GatherClass &gc : b->classes) lds_all = std::max(lds_all, gc.lds);
