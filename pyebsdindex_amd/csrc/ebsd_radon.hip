@@ -77,12 +77,12 @@ struct ebsd_plan {
     std::vector<int> h_cnt;      // per cell entry count
     std::vector<int64_t> h_off;  // per cell first entry in h_ent
     std::vector<uint16_t> h_ent; // per cell valid entries, plan order
-    std::vector<uint16_t> h_groups; // batch-major plan groups [block][lane][8]
+    std::vector<uint16_t> h_groups[2]; // batch-major groups [block][lane][8]: plan order, bank-scheduled
     std::vector<uint2> h_bhdr;   // per batch: first block in h_groups, number of blocks
     // device streams, one layout per count of (virtual) wave slots: a work-group of nw waves
     // that splits each pack's lines S ways walks them as nw * S slots
     struct Layout {
-        int nv;
+        int nv, kind; // kind 0: entries in plan order, 1: bank-scheduled order
         uint4 *gidx;
         int *wofs, *wcnt;
     };
@@ -366,24 +366,31 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                         group_accumulate<IN, FLT, P>(w, acc, accd);
                         if (gi + 1 == bend) {
                             if (li.x >= 0) {
+                                // All P values first, then the stores back to back: a store
+                                // between two loads' uses makes the wave wait for the store.
                                 const double cnt = (double)li.y;
+                                const double lb = (!FLT && a.linebg) ? a.linebg[li.x] : 0.0;
+                                // x / cnt correctly rounded from one reciprocal per line:
+                                // q = x * rc is within an ulp of x / cnt, the fma residual e is
+                                // exact, and q + e * rc rounds to x / cnt (Markstein's theorem,
+                                // rc the correctly rounded 1 / cnt).
+                                const double rc = 1.0 / cnt;
+                                float r[P];
 #pragma unroll
                                 for (int p = 0; p < P; ++p) {
-                                    const int64_t q = q0 + p;
-                                    if (q < a.N) {
-                                        float r;
-                                        if (li.y == 0) {
-                                            r = -1.0f;
-                                        } else if constexpr (FLT) {
-                                            r = (float)(accd[p] / cnt);
-                                        } else {
-                                            double sv = (double)acc[p];
-                                            if (a.linebg) sv -= a.linebg[li.x];
-                                            r = (float)(sv / cnt);
-                                        }
-                                        a.out[q * (int64_t)a.nCells + li.x] = r;
-                                    }
+                                    double x;
+                                    if constexpr (FLT)
+                                        x = accd[p];
+                                    else
+                                        x = (double)acc[p] - lb;
+                                    const double q = x * rc;
+                                    const double e = __builtin_fma(-q, cnt, x);
+                                    r[p] = (float)__builtin_fma(e, rc, q);
+                                    if (li.y == 0) r[p] = -1.0f;
                                 }
+#pragma unroll
+                                for (int p = 0; p < P; ++p)
+                                    if (q0 + p < a.N) a.out[(q0 + p) * (int64_t)a.nCells + li.x] = r[p];
                             }
 #pragma unroll
                             for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
@@ -1611,13 +1618,14 @@ static int geo_tiles(ebsd_plan *p)
 // Lay the plan groups out as one contiguous stream per wave slot: slot v of nv walks batches
 // v, v + nv, v + 2nv, ... back to back, so its next groups can be prefetched several
 // iterations ahead without looking at batch boundaries.  Layouts are cached per nv.
-static int get_layout(ebsd_plan *p, int nv, ebsd_plan::Layout *out)
+static int get_layout(ebsd_plan *p, int nv, int kind, ebsd_plan::Layout *out)
 {
     for (const ebsd_plan::Layout &l : p->layouts)
-        if (l.nv == nv) { *out = l; return 0; }
+        if (l.nv == nv && l.kind == kind) { *out = l; return 0; }
+    const std::vector<uint16_t> &hg = p->h_groups[kind];
     std::vector<uint16_t> st;
     std::vector<int> wofs(nv), wcnt(nv);
-    st.reserve(p->h_groups.size() + (size_t)(nv + 1) * 64 * 8);
+    st.reserve(hg.size() + (size_t)(nv + 1) * 64 * 8);
     int ofs = 0;
     for (int w = 0; w < nv; ++w) {
         wofs[w] = ofs;
@@ -1625,7 +1633,7 @@ static int get_layout(ebsd_plan *p, int nv, ebsd_plan::Layout *out)
         for (int b = w; b < p->nbatch; b += nv) {
             const uint2 h = p->h_bhdr[b];
             const size_t from = (size_t)h.x * 64 * 8;
-            st.insert(st.end(), p->h_groups.begin() + from, p->h_groups.begin() + from + (size_t)h.y * 64 * 8);
+            st.insert(st.end(), hg.begin() + from, hg.begin() + from + (size_t)h.y * 64 * 8);
             cnt += (int)h.y;
         }
         wcnt[w] = cnt;
@@ -1633,7 +1641,7 @@ static int get_layout(ebsd_plan *p, int nv, ebsd_plan::Layout *out)
     }
     // filler groups past the end keep the prefetch queue's reads in bounds
     st.resize(st.size() + 8 * 64 * 8, (uint16_t)p->nwords);
-    ebsd_plan::Layout l{nv, nullptr, nullptr, nullptr};
+    ebsd_plan::Layout l{nv, kind, nullptr, nullptr, nullptr};
     if (dev_upload(reinterpret_cast<uint16_t **>(&l.gidx), st.data(), st.size()) ||
         dev_upload(&l.wofs, wofs.data(), wofs.size()) || dev_upload(&l.wcnt, wcnt.data(), wcnt.size())) {
         (void)hipFree(l.gidx);
@@ -1676,6 +1684,73 @@ static int default_pitch(const ebsd_plan *p)
     return W;
 }
 
+// Integer line sums do not depend on the order of their entries, so each line's entries can
+// be reordered to spread the LDS words that a wave reads at one step over the banks.  Model
+// of an 8-byte-word read: 16 lanes per clock, word w in bank pair w mod 16, a clock per
+// distinct word in the busiest pair.  Greedy per step and 16-lane group: lanes with the
+// fewest distinct residues left choose first, each taking a residue that is least loaded at
+// this step (ties: the residue the lane has most entries left in).  Float sums keep plan order.
+static std::vector<uint16_t> bank_schedule(const std::vector<uint16_t> &g, const std::vector<uint2> &bhdr,
+                                           uint16_t filler)
+{
+    constexpr int NR = 16;
+    std::vector<uint16_t> out(g.size(), filler);
+    std::vector<uint16_t> pool[16][NR];
+    for (const uint2 &h : bhdr) {
+        const int steps = (int)h.y * 8;
+        for (int grp = 0; grp < 4; ++grp) {
+            for (int l = 0; l < 16; ++l)
+                for (int r = 0; r < NR; ++r) pool[l][r].clear();
+            for (int l = 0; l < 16; ++l) {
+                const int lane = grp * 16 + l;
+                for (int k = 0; k < steps; ++k) {
+                    const uint16_t w = g[((size_t)(h.x + k / 8) * 64 + lane) * 8 + (k % 8)];
+                    if (w != filler) pool[l][w % NR].push_back(w);
+                }
+            }
+            for (int k = 0; k < steps; ++k) {
+                int nres[16];
+                int lanes[16];
+                for (int l = 0; l < 16; ++l) {
+                    int n = 0;
+                    for (int r = 0; r < NR; ++r) n += !pool[l][r].empty();
+                    nres[l] = n;
+                    lanes[l] = l;
+                }
+                std::stable_sort(lanes, lanes + 16, [&](int x, int y) {
+                    const int ax = nres[x] ? nres[x] : 99, ay = nres[y] ? nres[y] : 99;
+                    return ax < ay;
+                });
+                uint16_t used[NR][16];
+                int nused[NR] = {0};
+                auto load = [&](int r, uint16_t w) {
+                    for (int i = 0; i < nused[r]; ++i)
+                        if (used[r][i] == w) return;
+                    used[r][nused[r]++] = w;
+                };
+                for (int li = 0; li < 16; ++li) {
+                    const int l = lanes[li];
+                    uint16_t w = filler;
+                    if (nres[l]) {
+                        int best = -1;
+                        for (int r = 0; r < NR; ++r) {
+                            if (pool[l][r].empty()) continue;
+                            if (best < 0 || nused[r] < nused[best] ||
+                                (nused[r] == nused[best] && pool[l][r].size() > pool[l][best].size()))
+                                best = r;
+                        }
+                        w = pool[l][best].back();
+                        pool[l][best].pop_back();
+                    }
+                    load(w % NR, w);
+                    out[((size_t)(h.x + k / 8) * 64 + grp * 16 + l) * 8 + (k % 8)] = w;
+                }
+            }
+        }
+    }
+    return out;
+}
+
 // Lines sorted by pixel count, cut into 64-line batches of 8-entry groups (one lane per
 // line); entries are LDS word indices of the staged pixels at row pitch `spitch`.  The device
 // streams of the current work-group size are laid out from the result.
@@ -1715,7 +1790,8 @@ static int compile_groups(ebsd_plan *p)
         }
         blocks += (uint32_t)ngrp;
     }
-    p->h_groups.swap(gidx);
+    p->h_groups[1] = bank_schedule(gidx, bhdr, (uint16_t)p->nwords);
+    p->h_groups[0].swap(gidx);
     p->h_bhdr = bhdr;
     (void)hipFree(p->d_bhdr);
     (void)hipFree(p->d_lane);
@@ -1968,7 +2044,7 @@ static int launch_gather_t(ebsd_plan *p, const void *d_pats, int64_t N, float *d
     a.nslice = choose_slices(p, a.npacks, nw, occ);
     ebsd_plan::Layout lay;
     int rc;
-    if ((rc = get_layout(p, nw * a.nslice, &lay))) return rc;
+    if ((rc = get_layout(p, nw * a.nslice, FLT ? 0 : 1, &lay))) return rc;
     a.gidx = lay.gidx;
     a.wofs = lay.wofs;
     a.wcnt = lay.wcnt;
@@ -2823,7 +2899,10 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
             GatherArgs &a = gs[i];
             a.nslice = choose_slices(plans[i], packs, nw, occ);
             ebsd_plan::Layout lay;
-            if ((rc = get_layout(plans[i], nw * a.nslice, &lay))) { batch_free(b); return rc; }
+            if ((rc = get_layout(plans[i], nw * a.nslice, kVariants[var[i]].FLT ? 0 : 1, &lay))) {
+                batch_free(b);
+                return rc;
+            }
             a.gidx = lay.gidx;
             a.wofs = lay.wofs;
             a.wcnt = lay.wcnt;
