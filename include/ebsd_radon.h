@@ -80,6 +80,13 @@ int ebsd_plan_set_tuning(ebsd_plan *plan, int gather_pack, int gather_threads, i
  * rows changes which LDS banks the pixels of a line fall in; results are identical. */
 int ebsd_plan_set_stage_pitch(ebsd_plan *plan, int pitch);
 
+/* Row segments of the integer (uint8/uint16, fast background) Radon gather: the pattern's
+ * rows are staged in `segments` consecutive ranges, one launch each, whose exact integer
+ * partial line sums add up in the output -- so that more patterns share an LDS word when a
+ * whole pattern would leave room for one or two.  0 = automatic (the fewest of 1, 2, 4, 8 at
+ * which the configured pack size fits), 1..8 = that many.  Results are identical. */
+int ebsd_plan_set_row_segments(ebsd_plan *plan, int segments);
+
 /* Which convolution/peak kernel runs the stages of band_detect.py:562-809 (same results):
  *   0 automatic (the streamed kernel where it applies), 1 theta-tiled work-group kernel,
  *   2 streamed kernel (one wavefront per block of theta columns; needs a 9x5 or 5x5 kernel

@@ -1,7 +1,7 @@
 """Autotuner for the launch shape of one plan (pattern size × dtype × background mode).
 
-Sweeps the gather's patterns per LDS word, work-group size and LDS row pitch, and the tiled
-peak kernel's theta tile width and work-group size, timing each on a sample of the caller's patterns
+Sweeps the gather's patterns per LDS word, work-group size, row segments (integer sums) and
+LDS row pitch, and the tiled peak kernel's theta tile width and work-group size, timing each on a sample of the caller's patterns
 with HIP events; the fastest shape is applied to the plan (``ebsd_plan_set_tuning``) and
 remembered per (device, H, W, nTheta, nRho, dtype, background mode).
 """
@@ -31,11 +31,12 @@ def _time(fn, reps):
 
 
 def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 8),
-             gather_threads=(512, 1024), peak_tiles=(24, 30, 36, 45, 60, 90), peak_threads=(256, 512),
-             verbose=False):
+             gather_threads=(512, 1024), row_segments=(1, 2, 3, 4), peak_tiles=(24, 30, 36, 45, 60, 90),
+             peak_threads=(256, 512), verbose=False):
     """Tune ``bd``'s device plan on (a sample of) the device tensor ``pats``; returns
-    ``{"gather_pack", "gather_threads", "stage_pitch", "peak_tile", "peak_threads", "gather_ms",
-    "peak_ms"}``."""
+    ``{"gather_pack", "gather_threads", "row_segments", "stage_pitch", "peak_tile",
+    "peak_threads", "gather_ms", "peak_ms"}`` (``row_segments`` 0: the plan's automatic
+    choice)."""
     import torch
     from . import _hip
     plan = bd.device_plan(gpu_id)
@@ -44,6 +45,7 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
     if key in _CACHE:
         best = _CACHE[key]
         plan.set_tuning(best["gather_pack"], best["gather_threads"], best["peak_tile"], best["peak_threads"])
+        plan.set_row_segments(best["row_segments"])
         plan.set_stage_pitch(best["stage_pitch"])
         return dict(best)
     xx, flags = bd._prepare(plan, x)
@@ -61,6 +63,22 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
         if best_g is None or ms < best_g[0]:
             best_g = (ms, gp, gt)
     plan.set_tuning(best_g[1], best_g[2], 0, 0)
+    # row segments of the integer gather (more patterns per LDS word in fewer rows), per pack
+    best_g = best_g + (0,)
+    if x.dtype != torch.float32 and not (bd.background_mode == 'exact' and bd.backgroundsub is not None):
+        for S, gp in itertools.product(row_segments, gather_packs):
+            try:
+                plan.set_row_segments(S)
+                plan.set_tuning(gp, 0, 0, 0)
+                ms = _time(lambda: plan.radon(xx, flags=flags, out=rad), reps)
+            except _hip.HipLibraryError:
+                continue
+            if verbose:
+                print(f"  gather segments={S} pack={gp}: {ms:.4f} ms")
+            if ms < best_g[0]:
+                best_g = (ms, gp, best_g[2], S)
+    plan.set_row_segments(best_g[3])
+    plan.set_tuning(best_g[1], 0, 0, 0)
     # LDS row pitch of the staged patterns (bank spread of the lines' pixels)
     best_s = None
     for pitch in sorted({plan.W, plan.W + 1, plan.W + 2, plan.W + 3, plan.W + 5}):
@@ -88,7 +106,8 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
         if best_p is None or ms < best_p[0]:
             best_p = (ms, pt, pth)
     plan.set_tuning(0, 0, best_p[1], best_p[2])
-    best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "stage_pitch": best_s[1], "peak_tile": best_p[1],
+    best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "row_segments": best_g[3],
+            "stage_pitch": best_s[1], "peak_tile": best_p[1],
             "peak_threads": best_p[2], "gather_ms": best_g[0], "peak_ms": best_p[0]}
     _CACHE[key] = best
     return dict(best)

@@ -100,6 +100,10 @@ class DevicePlan:
         """LDS words per staged pattern row in the Radon gather (0 = the pattern width)."""
         _hip.check(self.lib.ebsd_plan_set_stage_pitch(self.handle, int(pitch)))
 
+    def set_row_segments(self, segments):
+        """Row segments of the integer Radon gather (0 = automatic, 1..8); identical results."""
+        _hip.check(self.lib.ebsd_plan_set_row_segments(self.handle, int(segments)))
+
     def set_peak_kernel(self, kind):
         """0 automatic, 1 theta-tiled work-group kernel, 2 streamed wavefront kernel."""
         _hip.check(self.lib.ebsd_plan_set_peak_kernel(self.handle, int(kind)))

@@ -69,16 +69,10 @@ struct ebsd_plan {
     int device = 0, ncu = 0, lds_max = 0;
     int H = 0, W = 0, nPx = 0, nT = 0, nR = 0, nCells = 0;
     // compiled gather plan
-    int nbatch = 0;
     int spitch = 0;              // LDS words per staged pattern row (>= W); plan entries use it
-    int nwords = 0;              // H * spitch: the zero word's index, one less than the words staged
-    uint2 *d_bhdr = nullptr;     // per batch: first block, number of blocks
-    int2 *d_lane = nullptr;      // per batch lane: cell id (-1 = none), entry count
     std::vector<int> h_cnt;      // per cell entry count
     std::vector<int64_t> h_off;  // per cell first entry in h_ent
     std::vector<uint16_t> h_ent; // per cell valid entries, plan order
-    std::vector<uint16_t> h_groups[2]; // batch-major groups [block][lane][8]: plan order, bank-scheduled
-    std::vector<uint2> h_bhdr;   // per batch: first block in h_groups, number of blocks
     // device streams, one layout per count of (virtual) wave slots: a work-group of nw waves
     // that splits each pack's lines S ways walks them as nw * S slots
     struct Layout {
@@ -86,7 +80,23 @@ struct ebsd_plan {
         uint4 *gidx;
         int *wofs, *wcnt;
     };
-    std::vector<Layout> layouts;
+    // The lines' entries in staged rows [r0, r1), cut into 64-line batches: the whole pattern,
+    // or one of the row segments whose integer partial sums add up to the line sums.
+    struct GSeg {
+        int r0 = 0, r1 = 0;
+        int nwords = 0;              // (r1 - r0) * spitch: the zero word's index
+        int nbatch = 0;
+        uint2 *d_bhdr = nullptr;     // per batch: first block, number of blocks
+        int2 *d_lane = nullptr;      // per batch lane: cell id (-1 = none), entries of the whole line
+        std::vector<uint16_t> h_groups[2]; // batch-major groups [block][lane][8]: plan order, bank-scheduled
+        std::vector<uint2> h_bhdr;   // per batch: first block in h_groups, number of blocks
+        std::vector<Layout> layouts;
+    };
+    GSeg full;                   // all rows
+    std::vector<GSeg> rows;      // row segments, compiled on first use
+    int rseg = 0;                // row segments for integer sums: 0 automatic, else 1..8
+    uint32_t *d_part = nullptr;  // row segments' partial sums, part_cap elements (grown on demand;
+    size_t part_cap = 0;         // calls on one plan are ordered on the caller's stream)
     // background
     int bg_mode = 0;             // 0 none, 1 per-line sums, 2 exact per-pixel
     double *d_linebg = nullptr;  // nCells
@@ -128,6 +138,9 @@ struct GatherArgs {
     int64_t N;
     int nPx;
     int W, pitch;         // pattern row length; LDS words per staged row (>= W)
+    int px0, rows;        // staged rows [px0 / W, px0 / W + rows): all rows, or one row segment
+    int segmode;          // 0 whole lines; row segments: 1 first (store the integer partial
+                          // sums), 2 middle (add), 3 last (add and finish the line values)
     int nbatch;
     const uint4 *gidx;    // per-wave streams of plan groups
     const uint2 *bhdr;    // per batch: (unused, groups)
@@ -138,6 +151,8 @@ struct GatherArgs {
     const double *linebg; // fast background mode
     const double *bg;     // exact background mode
     float *out;           // (N, nCells)
+    uint32_t *part;       // row segments: integer partial sums (N, npos) by batch position
+    int npos;             //   (batch * 64 + lane; the segments share the line order)
     int nCells;
     int64_t npacks;
 };
@@ -284,9 +299,10 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
     const int wave = tid >> 6;
     const int nwaves = blockDim.x >> 6;
     const int nPx = a.nPx;
+    const int nst = a.rows * a.W; // staged pixels: global pixel a.px0 + j at local j
     const int nb = a.nbatch;
     const bool vec_ok = (a.W % VEC) == 0 && ((uintptr_t)a.pats % 8) == 0;
-    const int nvec = vec_ok ? nPx / VEC : 0;
+    const int nvec = vec_ok ? nst / VEC : 0;
     // pixel px is staged at word (px / W) * pitch + px % W (the plan's entries use the same map)
     const FDiv fW = fdiv_make(a.W);
     auto word_of = [&](int px) { const int y = (int)fdiv_q(px, fW); return y * a.pitch + (px - y * a.W); };
@@ -296,14 +312,15 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
 
     {
         const int64_t q0 = pack * P;
-        // ---- stage P patterns pixel-major (8-byte loads per pattern), zero word at nPx ----
+        // ---- stage P patterns pixel-major (8-byte loads per pattern), zero word at nst ----
+        const Tin *src = pats + a.px0;
         for (int v = tid; v < nvec; v += blockDim.x) {
             Tin x[P][VEC];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 uint2 raw = make_uint2(0u, 0u);
                 if (q0 + p < a.N)
-                    raw = *reinterpret_cast<const uint2 *>(pats + (q0 + p) * (int64_t)nPx + (int64_t)v * VEC);
+                    raw = *reinterpret_cast<const uint2 *>(src + (q0 + p) * (int64_t)nPx + (int64_t)v * VEC);
                 __builtin_memcpy(&x[p][0], &raw, 8);
             }
             const int w0 = word_of(v * VEC); // VEC divides W: a vector stays in one row
@@ -313,26 +330,26 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
 #pragma unroll
                 for (int p = 0; p < P; ++p) xp[p] = x[p][j];
                 uint32_t d[C::ND];
-                pack_word<IN, FLT, P>(d, xp, a.bg, v * VEC + j);
+                pack_word<IN, FLT, P>(d, xp, a.bg, a.px0 + v * VEC + j);
                 Word w;
                 __builtin_memcpy(&w, d, sizeof(Word));
                 lds[w0 + j] = w;
             }
         }
-        for (int px = nvec * VEC + tid; px <= nPx; px += blockDim.x) {
+        for (int px = nvec * VEC + tid; px <= nst; px += blockDim.x) {
             Tin xp[P];
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                xp[p] = (px < nPx && q0 + p < a.N) ? pats[(q0 + p) * (int64_t)nPx + px] : (Tin)0;
+                xp[p] = (px < nst && q0 + p < a.N) ? src[(q0 + p) * (int64_t)nPx + px] : (Tin)0;
             uint32_t d[C::ND];
-            pack_word<IN, FLT, P>(d, xp, px < nPx ? a.bg : nullptr, px);
-            if (px == nPx) {
+            pack_word<IN, FLT, P>(d, xp, px < nst ? a.bg : nullptr, a.px0 + px);
+            if (px == nst) {
 #pragma unroll
                 for (int k = 0; k < C::ND; ++k) d[k] = 0u;
             }
             Word w;
             __builtin_memcpy(&w, d, sizeof(Word));
-            lds[word_of(px)] = w; // px == nPx: the zero word H * pitch
+            lds[word_of(px)] = w; // px == nst: the zero word rows * pitch
         }
         __syncthreads();
 
@@ -355,6 +372,16 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             double accd[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+            // row segments after the first: the integer partial sums of the segments before,
+            // loaded when a batch starts so that the loads are done by its end
+            uint32_t prev[P];
+            auto load_prev = [&](int bb, int cell) {
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    prev[p] = (a.segmode >= 2 && cell >= 0 && q0 + p < a.N)
+                                  ? a.part[(q0 + p) * (int64_t)a.npos + bb * 64 + lane] : 0u;
+            };
+            load_prev(b, li.x);
             for (int g0 = 0; g0 < G; g0 += 4) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -365,7 +392,11 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                         eq[u] = S[(size_t)(gi + 4) * 64];
                         group_accumulate<IN, FLT, P>(w, acc, accd);
                         if (gi + 1 == bend) {
-                            if (li.x >= 0) {
+                            if (!FLT && (a.segmode == 1 || a.segmode == 2) && li.x >= 0) {
+#pragma unroll
+                                for (int p = 0; p < P; ++p)
+                                    if (q0 + p < a.N) a.part[(q0 + p) * (int64_t)a.npos + b * 64 + lane] = prev[p] + acc[p];
+                            } else if (li.x >= 0) {
                                 // All P values first, then the stores back to back: a store
                                 // between two loads' uses makes the wave wait for the store.
                                 const double cnt = (double)li.y;
@@ -382,7 +413,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                                     if constexpr (FLT)
                                         x = accd[p];
                                     else
-                                        x = (double)acc[p] - lb;
+                                        x = (double)(prev[p] + acc[p]) - lb;
                                     const double q = x * rc;
                                     const double e = __builtin_fma(-q, cnt, x);
                                     r[p] = (float)__builtin_fma(e, rc, q);
@@ -399,6 +430,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                             bend += ngn;
                             bn = b + nslot;
                             if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
+                            load_prev(b, b < nb ? li.x : -1);
                         }
                     }
                 }
@@ -1532,22 +1564,32 @@ template <typename T> static int dev_upload(T **dptr, const T *src, size_t n)
     return 0;
 }
 
-static void free_layouts(ebsd_plan *p)
+static void free_seg(ebsd_plan::GSeg &g)
 {
-    for (ebsd_plan::Layout &l : p->layouts) {
+    for (ebsd_plan::Layout &l : g.layouts) {
         (void)hipFree(l.gidx);
         (void)hipFree(l.wofs);
         (void)hipFree(l.wcnt);
     }
-    p->layouts.clear();
+    g.layouts.clear();
+    (void)hipFree(g.d_bhdr);
+    (void)hipFree(g.d_lane);
+    g.d_bhdr = nullptr;
+    g.d_lane = nullptr;
+}
+
+static void free_rows(ebsd_plan *p)
+{
+    for (ebsd_plan::GSeg &g : p->rows) free_seg(g);
+    p->rows.clear();
 }
 
 static void plan_free(ebsd_plan *p)
 {
     if (!p) return;
-    free_layouts(p);
-    (void)hipFree(p->d_bhdr);
-    (void)hipFree(p->d_lane);
+    free_seg(p->full);
+    free_rows(p);
+    (void)hipFree(p->d_part);
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     (void)hipFree(p->d_mask);
@@ -1618,11 +1660,11 @@ static int geo_tiles(ebsd_plan *p)
 // Lay the plan groups out as one contiguous stream per wave slot: slot v of nv walks batches
 // v, v + nv, v + 2nv, ... back to back, so its next groups can be prefetched several
 // iterations ahead without looking at batch boundaries.  Layouts are cached per nv.
-static int get_layout(ebsd_plan *p, int nv, int kind, ebsd_plan::Layout *out)
+static int get_layout(ebsd_plan::GSeg &g, int nv, int kind, ebsd_plan::Layout *out)
 {
-    for (const ebsd_plan::Layout &l : p->layouts)
+    for (const ebsd_plan::Layout &l : g.layouts)
         if (l.nv == nv && l.kind == kind) { *out = l; return 0; }
-    const std::vector<uint16_t> &hg = p->h_groups[kind];
+    const std::vector<uint16_t> &hg = g.h_groups[kind];
     std::vector<uint16_t> st;
     std::vector<int> wofs(nv), wcnt(nv);
     st.reserve(hg.size() + (size_t)(nv + 1) * 64 * 8);
@@ -1630,8 +1672,8 @@ static int get_layout(ebsd_plan *p, int nv, int kind, ebsd_plan::Layout *out)
     for (int w = 0; w < nv; ++w) {
         wofs[w] = ofs;
         int cnt = 0;
-        for (int b = w; b < p->nbatch; b += nv) {
-            const uint2 h = p->h_bhdr[b];
+        for (int b = w; b < g.nbatch; b += nv) {
+            const uint2 h = g.h_bhdr[b];
             const size_t from = (size_t)h.x * 64 * 8;
             st.insert(st.end(), hg.begin() + from, hg.begin() + from + (size_t)h.y * 64 * 8);
             cnt += (int)h.y;
@@ -1640,7 +1682,7 @@ static int get_layout(ebsd_plan *p, int nv, int kind, ebsd_plan::Layout *out)
         ofs += cnt;
     }
     // filler groups past the end keep the prefetch queue's reads in bounds
-    st.resize(st.size() + 8 * 64 * 8, (uint16_t)p->nwords);
+    st.resize(st.size() + 8 * 64 * 8, (uint16_t)g.nwords);
     ebsd_plan::Layout l{nv, kind, nullptr, nullptr, nullptr};
     if (dev_upload(reinterpret_cast<uint16_t **>(&l.gidx), st.data(), st.size()) ||
         dev_upload(&l.wofs, wofs.data(), wofs.size()) || dev_upload(&l.wcnt, wcnt.data(), wcnt.size())) {
@@ -1649,18 +1691,18 @@ static int get_layout(ebsd_plan *p, int nv, int kind, ebsd_plan::Layout *out)
         (void)hipFree(l.wcnt);
         return fail(-3, "plan stream layout: device allocation failed");
     }
-    p->layouts.push_back(l);
+    g.layouts.push_back(l);
     *out = l;
     return 0;
 }
 
 // Slices per pack so that a launch of npacks packs has about two resident rounds of
 // work-groups, keeping at least two batches per slot.
-static int choose_slices(const ebsd_plan *p, int64_t npacks, int nw, int occ)
+static int choose_slices(const ebsd_plan *p, const ebsd_plan::GSeg &g, int64_t npacks, int nw, int occ)
 {
     const int64_t target = 2 * (int64_t)occ * p->ncu;
     if (npacks >= target) return 1;
-    const int smax = std::max(1, p->nbatch / (2 * nw));
+    const int smax = std::max(1, g.nbatch / (2 * nw));
     return (int)std::max<int64_t>(1, std::min<int64_t>(smax, (target + npacks - 1) / npacks));
 }
 
@@ -1751,56 +1793,112 @@ static std::vector<uint16_t> bank_schedule(const std::vector<uint16_t> &g, const
     return out;
 }
 
-// Lines sorted by pixel count, cut into 64-line batches of 8-entry groups (one lane per
-// line); entries are LDS word indices of the staged pixels at row pitch `spitch`.  The device
-// streams of the current work-group size are laid out from the result.
-static int compile_groups(ebsd_plan *p)
+// Count of each line's entries in staged rows [r0, r1).
+static std::vector<int> row_counts(const ebsd_plan *p, int r0, int r1)
+{
+    const int e0 = r0 * p->W, e1 = r1 * p->W;
+    std::vector<int> cs(p->nCells, 0);
+    for (int cell = 0; cell < p->nCells; ++cell)
+        for (int64_t k = p->h_off[cell]; k < p->h_off[cell + 1]; ++k)
+            cs[cell] += (p->h_ent[k] >= e0 && p->h_ent[k] < e1);
+    return cs;
+}
+
+// The lines' entries in staged rows [r0, r1), lines in the given order (or sorted by their
+// count there) and cut into 64-line batches of 8-entry groups (one lane per line); entries
+// are LDS word indices of the staged pixels, (y - r0) * spitch + x.  Every line has a lane in
+// every row range (a line with no pixel there adds zero), and the lane keeps the count of the
+// whole line.  The device streams of a work-group size are laid out on first use.
+static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const std::vector<int> *given = nullptr)
 {
     const int nCells = p->nCells, W = p->W;
-    p->nwords = p->H * p->spitch;
-    if (p->nwords > 65535) return fail(-2, "ebsd_plan: staged pattern exceeds 65,535 LDS words");
+    free_seg(g);
+    g.r0 = r0;
+    g.r1 = r1;
+    g.nwords = (r1 - r0) * p->spitch;
+    if (g.nwords > 65535) return fail(-2, "ebsd_plan: staged pattern exceeds 65,535 LDS words");
+    const int e0 = r0 * W, e1 = r1 * W;
+    const std::vector<int> cs = row_counts(p, r0, r1);
     std::vector<int> order(nCells);
-    for (int i = 0; i < nCells; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return p->h_cnt[x] > p->h_cnt[y]; });
-    p->nbatch = (nCells + 63) / 64;
-    std::vector<uint2> bhdr(p->nbatch);
-    std::vector<int2> lanes((size_t)p->nbatch * 64);
+    if (given) {
+        order = *given;
+    } else {
+        for (int i = 0; i < nCells; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cs[x] > cs[y]; });
+    }
+    g.nbatch = (nCells + 63) / 64;
+    std::vector<uint2> bhdr(g.nbatch);
+    std::vector<int2> lanes((size_t)g.nbatch * 64);
     std::vector<uint16_t> gidx;
     uint32_t blocks = 0;
-    for (int b = 0; b < p->nbatch; ++b) {
+    for (int b = 0; b < g.nbatch; ++b) {
         int mx = 0;
         for (int l = 0; l < 64; ++l) {
             const int i = b * 64 + l;
-            if (i < nCells) mx = std::max(mx, p->h_cnt[order[i]]);
+            if (i < nCells) mx = std::max(mx, cs[order[i]]);
         }
-        const int ngrp = std::max(1, (mx + 7) / 8); // an empty batch still writes its -1 cells
+        const int ngrp = std::max(1, (mx + 7) / 8); // an empty batch still writes its cells
         bhdr[b] = make_uint2(blocks, (uint32_t)ngrp);
-        gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)p->nwords);
+        gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)g.nwords);
         for (int l = 0; l < 64; ++l) {
             const int i = b * 64 + l;
             if (i >= nCells) { lanes[(size_t)b * 64 + l] = make_int2(-1, 0); continue; }
             const int cell = order[i];
             lanes[(size_t)b * 64 + l] = make_int2(cell, p->h_cnt[cell]);
-            for (int k = 0; k < p->h_cnt[cell]; ++k) {
+            int k = 0;
+            for (int64_t j = p->h_off[cell]; j < p->h_off[cell + 1]; ++j) {
+                const int e = p->h_ent[j];
+                if (e < e0 || e >= e1) continue;
                 const size_t at = ((size_t)(blocks + k / 8) * 64 + l) * 8 + (k % 8);
-                const int e = p->h_ent[p->h_off[cell] + k];
-                gidx[at] = (uint16_t)((e / W) * p->spitch + e % W);
+                gidx[at] = (uint16_t)((e / W - r0) * p->spitch + e % W);
+                ++k;
             }
         }
         blocks += (uint32_t)ngrp;
     }
-    p->h_groups[1] = bank_schedule(gidx, bhdr, (uint16_t)p->nwords);
-    p->h_groups[0].swap(gidx);
-    p->h_bhdr = bhdr;
-    (void)hipFree(p->d_bhdr);
-    (void)hipFree(p->d_lane);
-    p->d_bhdr = nullptr;
-    p->d_lane = nullptr;
-    free_layouts(p); // laid out again on first use
+    g.h_groups[1] = bank_schedule(gidx, bhdr, (uint16_t)g.nwords);
+    g.h_groups[0].swap(gidx);
+    g.h_bhdr = bhdr;
     int rc;
-    if ((rc = dev_upload(&p->d_bhdr, bhdr.data(), bhdr.size()))) return rc;
-    if ((rc = dev_upload(&p->d_lane, lanes.data(), lanes.size()))) return rc;
+    if ((rc = dev_upload(&g.d_bhdr, bhdr.data(), bhdr.size()))) return rc;
+    if ((rc = dev_upload(&g.d_lane, lanes.data(), lanes.size()))) return rc;
+    return 0;
+}
+
+// The whole-pattern batches; row segments are compiled again on first use.
+static int compile_groups(ebsd_plan *p)
+{
+    free_rows(p);
+    return compile_seg(p, p->full, 0, p->H);
+}
+
+// S row segments of (nearly) equal height, segment s holding rows [s H / S, (s + 1) H / S).
+// All segments share one line order, so that a line keeps its batch and lane -- its partial
+// sum's position -- from segment to segment: lines sorted by their group counts per segment,
+// segment 0 first (descending), which keeps the lines of a batch of similar length in every
+// segment (about 96% of the entry slots used at 240x240 in two segments, 87% in four).
+static int compile_rows(ebsd_plan *p, int S)
+{
+    if ((int)p->rows.size() == S) return 0;
+    free_rows(p);
+    const int nCells = p->nCells;
+    std::vector<std::vector<int>> q(S);
+    for (int s = 0; s < S; ++s) {
+        q[s] = row_counts(p, s * p->H / S, (s + 1) * p->H / S);
+        for (int &c : q[s]) c = (c + 7) / 8;
+    }
+    std::vector<int> order(nCells);
+    for (int i = 0; i < nCells; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        for (int s = 0; s < S; ++s)
+            if (q[s][x] != q[s][y]) return q[s][x] > q[s][y];
+        return false;
+    });
+    p->rows.resize(S);
+    for (int s = 0; s < S; ++s) {
+        const int rc = compile_seg(p, p->rows[s], s * p->H / S, (s + 1) * p->H / S, &order);
+        if (rc) { free_rows(p); return rc; }
+    }
     return 0;
 }
 
@@ -1962,6 +2060,14 @@ extern "C" int ebsd_plan_set_stage_pitch(ebsd_plan *p, int pitch)
     return rc;
 }
 
+extern "C" int ebsd_plan_set_row_segments(ebsd_plan *p, int segments)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_row_segments: null plan");
+    if (segments < 0 || segments > 8) return fail(-1, "ebsd_plan_set_row_segments: segments must be in 0..8");
+    p->rseg = segments;
+    return 0;
+}
+
 extern "C" int ebsd_plan_set_peak_kernel(ebsd_plan *p, int kind)
 {
     if (!p) return fail(-1, "ebsd_plan_set_peak_kernel: null plan");
@@ -2003,7 +2109,7 @@ extern "C" int ebsd_plan_info(const ebsd_plan *p, int64_t *o)
     int64_t total = (int64_t)p->h_ent.size();
     o[0] = p->nCells;
     o[1] = total;                 // line-pixel pairs per pattern
-    o[2] = p->nbatch;
+    o[2] = p->full.nbatch;
     o[3] = p->g.tile;
     o[4] = p->g.ntiles;
     o[5] = (int64_t)peak_lds_layout(p->g).total * 4;
@@ -2018,33 +2124,40 @@ extern "C" size_t ebsd_workspace_bytes(const ebsd_plan *p, int64_t chunk)
     return (size_t)chunk * (size_t)p->nCells * sizeof(float);
 }
 
+// One launch over all packs for the row range of g (maxwords: the largest range's words, so
+// that every segment of a launch sequence fits the same pack size).
 template <int IN, bool FLT, int P>
-static int launch_gather_t(ebsd_plan *p, const void *d_pats, int64_t N, float *d_out, bool use_bg,
-                           hipStream_t s)
+static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int segmode, uint32_t *part,
+                           const void *d_pats, int64_t N, float *d_out, bool use_bg, hipStream_t s)
 {
     typedef GatherCfg<IN, FLT, P> C;
-    const size_t lds = (size_t)(p->nwords + 1) * C::WB;
+    const size_t lds = (size_t)(maxwords + 1) * C::WB;
     if ((int)lds > p->lds_max) return 1; // caller falls back to a smaller pack
     auto kern = k_radon_gather<IN, FLT, P>;
     HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int occ = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->gthreads, lds));
     if (occ < 1) return fail(-3, "radon gather kernel cannot be resident");
-    GatherArgs a;
+    GatherArgs a{};
     a.pats = d_pats;
     a.N = N;
     a.nPx = p->nPx;
     a.W = p->W;
     a.pitch = p->spitch;
-    a.nbatch = p->nbatch;
-    a.bhdr = p->d_bhdr;
-    a.lane = p->d_lane;
+    a.px0 = g.r0 * p->W;
+    a.rows = g.r1 - g.r0;
+    a.segmode = segmode;
+    a.part = part;
+    a.npos = g.nbatch * 64;
+    a.nbatch = g.nbatch;
+    a.bhdr = g.d_bhdr;
+    a.lane = g.d_lane;
     a.npacks = (N + P - 1) / P;
     const int nw = p->gthreads / 64;
-    a.nslice = choose_slices(p, a.npacks, nw, occ);
+    a.nslice = choose_slices(p, g, a.npacks, nw, occ);
     ebsd_plan::Layout lay;
     int rc;
-    if ((rc = get_layout(p, nw * a.nslice, FLT ? 0 : 1, &lay))) return rc;
+    if ((rc = get_layout(g, nw * a.nslice, FLT ? 0 : 1, &lay))) return rc;
     a.gidx = lay.gidx;
     a.wofs = lay.wofs;
     a.wcnt = lay.wcnt;
@@ -2058,6 +2171,73 @@ static int launch_gather_t(ebsd_plan *p, const void *d_pats, int64_t N, float *d
     return 0;
 }
 
+// Integer sums of P patterns per word over the whole pattern (S = 1) or over S row segments
+// in S launches whose partial sums add up in the output (exact: integers).
+static int ensure_part(ebsd_plan *p, size_t elems)
+{
+    if (p->part_cap >= elems) return 0;
+    if (p->d_part) {
+        HIPCHK(hipDeviceSynchronize()); // earlier launches may still read the old buffer
+        (void)hipFree(p->d_part);
+        p->d_part = nullptr;
+        p->part_cap = 0;
+    }
+    if (hipMalloc((void **)&p->d_part, elems * sizeof(uint32_t)) != hipSuccess) {
+        p->d_part = nullptr;
+        return fail(-3, "row-segment partial sums: device allocation failed");
+    }
+    p->part_cap = elems;
+    return 0;
+}
+
+// Partial sums per launch sequence: at most this many bytes (larger N runs in pieces).
+static const int64_t kPartBytes = 2ll << 30;
+
+template <int IN, int P>
+static int launch_gather_rows(ebsd_plan *p, int S, const void *d_pats, int64_t N, float *d_out, bool use_bg,
+                              hipStream_t s)
+{
+    if (S <= 1)
+        return launch_gather_t<IN, false, P>(p, p->full, p->full.nwords, 0, nullptr, d_pats, N, d_out, use_bg, s);
+    int maxwords = 0;
+    for (const ebsd_plan::GSeg &g : p->rows) maxwords = std::max(maxwords, g.nwords);
+    if ((size_t)(maxwords + 1) * GatherCfg<IN, false, P>::WB > (size_t)p->lds_max) return 1;
+    const int64_t npos = (int64_t)p->rows[0].nbatch * 64;
+    // equal pieces of whole packs, each piece's partial sums within kPartBytes
+    const int64_t cap = std::max<int64_t>(P, kPartBytes / 4 / npos / P * P);
+    const int64_t npieces = (N + cap - 1) / cap;
+    const int64_t piece = std::min(cap, ((N + npieces - 1) / npieces + P - 1) / P * P);
+    int rc;
+    if ((rc = ensure_part(p, (size_t)std::min(piece, N) * npos))) return rc;
+    const size_t esz = sizeof(typename InT<IN>::T);
+    for (int64_t q0 = 0; q0 < N; q0 += piece) {
+        const int64_t n = std::min(piece, N - q0);
+        const void *src = (const char *)d_pats + (size_t)q0 * p->nPx * esz;
+        float *dst = d_out + (size_t)q0 * p->nCells;
+        for (int k = 0; k < S; ++k) {
+            const int mode = k == 0 ? 1 : (k == S - 1 ? 3 : 2);
+            rc = launch_gather_t<IN, false, P>(p, p->rows[k], maxwords, mode, p->d_part, src, n, dst, use_bg, s);
+            if (rc) return rc < 0 ? rc : fail(-3, "row-segment gather launch failed");
+        }
+    }
+    return 0;
+}
+
+// Row segments for integer sums: the requested count, or the fewest (up to 8) at which the
+// configured pack size fits in LDS (measured, uint16 240x240 with a 360x180 Radon: 2.39 ms
+// per 1024 patterns whole, 1.09 ms in three segments of four patterns per word).
+static int row_segments(const ebsd_plan *p, int dtype)
+{
+    if (p->rseg > 0) return std::min(p->rseg, p->H);
+    const int P = dtype == EBSD_U8 ? p->gpack_u8 : p->gpack_u16;
+    const int WB = dtype == EBSD_U8 ? P : 2 * P;
+    for (int S = 1; S <= 8; ++S) {
+        const size_t words = (size_t)((p->H + S - 1) / S) * p->spitch + 1;
+        if (words * WB <= (size_t)p->lds_max) return std::min(S, p->H);
+    }
+    return std::min(8, p->H);
+}
+
 static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int64_t N, float *d_out,
                          hipStream_t s)
 {
@@ -2067,35 +2247,43 @@ static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int6
     // float32 input (or the exact background mode) stages float32(x - bg) per pixel
     const bool flt = (dtype == EBSD_F32) || (use_bg && p->bg_mode == 2);
     if (flt) {
+        ebsd_plan::GSeg &g = p->full;
+        const int mw = g.nwords;
         int P = p->gpack_f32;
         for (; rc == 1 && P >= 1; P /= 2) {
             if (dtype == EBSD_U8) {
-                if (P >= 4) rc = launch_gather_t<EBSD_U8, true, 4>(p, d_pats, N, d_out, use_bg, s);
-                else if (P >= 2) rc = launch_gather_t<EBSD_U8, true, 2>(p, d_pats, N, d_out, use_bg, s);
-                else rc = launch_gather_t<EBSD_U8, true, 1>(p, d_pats, N, d_out, use_bg, s);
+                if (P >= 4) rc = launch_gather_t<EBSD_U8, true, 4>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                else if (P >= 2) rc = launch_gather_t<EBSD_U8, true, 2>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                else rc = launch_gather_t<EBSD_U8, true, 1>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
             } else if (dtype == EBSD_U16) {
-                if (P >= 4) rc = launch_gather_t<EBSD_U16, true, 4>(p, d_pats, N, d_out, use_bg, s);
-                else if (P >= 2) rc = launch_gather_t<EBSD_U16, true, 2>(p, d_pats, N, d_out, use_bg, s);
-                else rc = launch_gather_t<EBSD_U16, true, 1>(p, d_pats, N, d_out, use_bg, s);
+                if (P >= 4) rc = launch_gather_t<EBSD_U16, true, 4>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                else if (P >= 2) rc = launch_gather_t<EBSD_U16, true, 2>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                else rc = launch_gather_t<EBSD_U16, true, 1>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
             } else {
-                if (P >= 4) rc = launch_gather_t<EBSD_F32, true, 4>(p, d_pats, N, d_out, use_bg, s);
-                else if (P >= 2) rc = launch_gather_t<EBSD_F32, true, 2>(p, d_pats, N, d_out, use_bg, s);
-                else rc = launch_gather_t<EBSD_F32, true, 1>(p, d_pats, N, d_out, use_bg, s);
+                if (P >= 4) rc = launch_gather_t<EBSD_F32, true, 4>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                else if (P >= 2) rc = launch_gather_t<EBSD_F32, true, 2>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                else rc = launch_gather_t<EBSD_F32, true, 1>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
             }
         }
-    } else if (dtype == EBSD_U8) {
+        if (rc == 1) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
+        return rc;
+    }
+    const int S = row_segments(p, dtype);
+    if (S > 1 && (rc = compile_rows(p, S))) return rc;
+    rc = 1;
+    if (dtype == EBSD_U8) {
         for (int P = p->gpack_u8; rc == 1 && P >= 2; P /= 2) {
-            if (P >= 16) rc = launch_gather_t<EBSD_U8, false, 16>(p, d_pats, N, d_out, use_bg, s);
-            else if (P >= 8) rc = launch_gather_t<EBSD_U8, false, 8>(p, d_pats, N, d_out, use_bg, s);
-            else if (P >= 4) rc = launch_gather_t<EBSD_U8, false, 4>(p, d_pats, N, d_out, use_bg, s);
-            else rc = launch_gather_t<EBSD_U8, false, 2>(p, d_pats, N, d_out, use_bg, s);
+            if (P >= 16) rc = launch_gather_rows<EBSD_U8, 16>(p, S, d_pats, N, d_out, use_bg, s);
+            else if (P >= 8) rc = launch_gather_rows<EBSD_U8, 8>(p, S, d_pats, N, d_out, use_bg, s);
+            else if (P >= 4) rc = launch_gather_rows<EBSD_U8, 4>(p, S, d_pats, N, d_out, use_bg, s);
+            else rc = launch_gather_rows<EBSD_U8, 2>(p, S, d_pats, N, d_out, use_bg, s);
         }
     } else {
         for (int P = p->gpack_u16; rc == 1 && P >= 1; P /= 2) {
-            if (P >= 8) rc = launch_gather_t<EBSD_U16, false, 8>(p, d_pats, N, d_out, use_bg, s);
-            else if (P >= 4) rc = launch_gather_t<EBSD_U16, false, 4>(p, d_pats, N, d_out, use_bg, s);
-            else if (P >= 2) rc = launch_gather_t<EBSD_U16, false, 2>(p, d_pats, N, d_out, use_bg, s);
-            else rc = launch_gather_t<EBSD_U16, false, 1>(p, d_pats, N, d_out, use_bg, s);
+            if (P >= 8) rc = launch_gather_rows<EBSD_U16, 8>(p, S, d_pats, N, d_out, use_bg, s);
+            else if (P >= 4) rc = launch_gather_rows<EBSD_U16, 4>(p, S, d_pats, N, d_out, use_bg, s);
+            else if (P >= 2) rc = launch_gather_rows<EBSD_U16, 2>(p, S, d_pats, N, d_out, use_bg, s);
+            else rc = launch_gather_rows<EBSD_U16, 1>(p, S, d_pats, N, d_out, use_bg, s);
         }
     }
     if (rc == 1) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
@@ -2735,7 +2923,7 @@ static int choose_variant(const ebsd_plan *p, int dtype, bool flt, VariantInfo *
     int best = -1;
     for (const VariantInfo &v : kVariants) {
         if (v.IN != dtype || v.FLT != flt || v.P > pmax) continue;
-        if ((size_t)(p->nwords + 1) * v.WB > (size_t)p->lds_max) continue;
+        if ((size_t)(p->full.nwords + 1) * v.WB > (size_t)p->lds_max) continue;
         if (best < 0 || v.P > kVariants[best].P) best = (int)(&v - kVariants);
     }
     if (best < 0) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
@@ -2837,7 +3025,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         VariantInfo v;
         if ((rc = choose_variant(p, dtype, flt, &v))) { batch_free(b); return rc; }
         var[i] = v.id;
-        const size_t lds = (size_t)(p->nwords + 1) * v.WB;
+        const size_t lds = (size_t)(p->full.nwords + 1) * v.WB;
         const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)p->lds_max / std::max<size_t>(lds, 1)));
         cls[i] = per_cu;
         float *radon = (float *)work;
@@ -2848,9 +3036,12 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         a.nPx = p->nPx;
         a.W = p->W;
         a.pitch = p->spitch;
-        a.nbatch = p->nbatch;
-        a.bhdr = p->d_bhdr;
-        a.lane = p->d_lane;
+        a.px0 = 0;
+        a.rows = p->H;
+        a.segmode = 0;
+        a.nbatch = p->full.nbatch;
+        a.bhdr = p->full.d_bhdr;
+        a.lane = p->full.d_lane;
         a.linebg = (use_bg && !flt && p->bg_mode == 1) ? p->d_linebg : nullptr;
         a.bg = (use_bg && flt) ? p->d_bg : nullptr;
         a.out = radon;
@@ -2881,7 +3072,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         int64_t packs = 0;
         for (int i : order) {
             if (cls[i] != c) continue;
-            gc.lds = std::max(gc.lds, (size_t)(plans[i]->nwords + 1) * kVariants[var[i]].WB);
+            gc.lds = std::max(gc.lds, (size_t)(plans[i]->full.nwords + 1) * kVariants[var[i]].WB);
             packs += gs[i].npacks;
         }
         if (packs == 0) continue;
@@ -2897,9 +3088,9 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         for (int i : order) {
             if (cls[i] != c) continue;
             GatherArgs &a = gs[i];
-            a.nslice = choose_slices(plans[i], packs, nw, occ);
+            a.nslice = choose_slices(plans[i], plans[i]->full, packs, nw, occ);
             ebsd_plan::Layout lay;
-            if ((rc = get_layout(plans[i], nw * a.nslice, kVariants[var[i]].FLT ? 0 : 1, &lay))) {
+            if ((rc = get_layout(plans[i]->full, nw * a.nslice, kVariants[var[i]].FLT ? 0 : 1, &lay))) {
                 batch_free(b);
                 return rc;
             }

@@ -153,6 +153,66 @@ def test_scan_scale_geometry_240(dtype):
     np.testing.assert_allclose(got['stats'][:, 2], st[2], rtol=1e-5)
 
 
+@pytest.mark.parametrize("shape,dtype,n", [((240, 240), "uint16", 7), ((100, 120), "uint16", 5),
+                                           ((61, 67), "uint8", 11), ((160, 160), "uint8", 3)])
+def test_row_segments_identical(shape, dtype, n):
+    """The integer gather over S row segments (partial sums added across S launches) gives
+    the same float32 Radon as one launch over the whole pattern, with and without the fast
+    background: uneven segment heights, a row length that defeats the 8-byte staging loads,
+    ragged packs (n not a multiple of the pack size) and lines with no pixels."""
+    import torch
+    print(f"row segments {shape} {dtype} n={n}", flush=True)
+    kw = dict(nTheta=360, nRho=180, nBands=12) if shape[0] == 240 else dict(nRho=130)
+    bd, ref = make_pair(shape, **kw)
+    pats = _pats(n, shape, dtype, seed=43)
+    plan = bd.device_plan()
+    x = _dev(pats)
+    bg = np.linspace(0.0, 9.5, shape[0] * shape[1]).reshape(shape)
+    for with_bg in (False, True):
+        plan.set_background(bg if with_bg else None, False)
+        outs = []
+        for S in (1, 2, 3, 4, 8):
+            plan.set_row_segments(S)
+            outs.append(plan.radon(x).cpu().numpy())
+        plan.set_row_segments(0)
+        outs.append(plan.radon(x).cpu().numpy())
+        for S, r in zip((2, 3, 4, 8, "auto"), outs[1:]):
+            assert np.array_equal(r, outs[0]), (with_bg, S)
+        if shape in ((100, 120), (160, 160)):
+            assert (outs[0] == -1.0).any()   # lines with no pixels are written too
+    plan.set_background(None, False)
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    plan.set_row_segments(3)
+    assert np.array_equal(plan.radon(x).cpu().numpy(), want)
+    plan.set_row_segments(0)
+
+
+def test_autotune_keeps_results():
+    """The autotuner (pack size, work-group size, row segments, row pitch, peak tile) only
+    changes launch shapes: the Radon and the bands after tuning equal those before."""
+    from pyebsdindex_amd.autotune import autotune
+    print("autotune 160x160 uint16", flush=True)
+    shape = (160, 160)
+    bd, ref = make_pair(shape)
+    x = _dev(_pats(48, shape, "uint16", seed=47))
+    plan = bd.device_plan()
+    rad0 = plan.radon(x).cpu().numpy()
+    b0 = {k: v.cpu().numpy() for k, v in bd.find_bands_device(x).items()}
+    best = autotune(bd, x, sample=48, reps=1, gather_packs=(2, 4), gather_threads=(1024,),
+                    row_segments=(1, 2, 3), peak_tiles=(45, 90), peak_threads=(512,))
+    assert best["row_segments"] in (0, 1, 2, 3)
+    assert np.array_equal(plan.radon(x).cpu().numpy(), rad0)
+    b1 = {k: v.cpu().numpy() for k, v in bd.find_bands_device(x).items()}
+    for k in b0:
+        if k == "stats":   # the tiled peak kernel's mean may round differently per tile width
+            np.testing.assert_allclose(b0[k], b1[k], rtol=1e-6)
+        else:
+            assert np.array_equal(b0[k], b1[k]), k
+    plan.set_row_segments(0)
+    plan.set_stage_pitch(0)
+
+
 @pytest.mark.parametrize("case", ["120-u16", "60-u8-plateau", "240-360x180", "uneven-theta", "kernel-5x5"])
 def test_peak_kernels_agree(case):
     """The streamed wavefront peak kernel and the theta-tiled work-group kernel compute the
