@@ -247,17 +247,74 @@ __device__ __forceinline__ void group_issue(uint32_t base, const uint4 &e,
     }
 }
 
+// Integer words of P >= 2 patterns: the accumulator of the first field of each dword j sums
+// whole dwords (two entries per v_add3_u32), the other fields' accumulators sum their fields
+// (one SDWA add each per entry).  The first field's sum is the dword sum minus the other
+// fields' sums shifted into place, mod 2^32 (gather_decode): for uint16 1.5 VALU ops per dword
+// and entry instead of 2, for uint8 3.5 instead of 4.
+template <int IN, bool FLT, int P> struct PairSum {
+    static constexpr bool value = !FLT && IN != EBSD_F32 && P >= 2;
+    static constexpr int F = IN == EBSD_U8 ? 4 : 2; // fields per dword
+    static constexpr int B = IN == EBSD_U8 ? 8 : 16; // bits per field
+};
+
+template <int IN, int P, int J, int I>
+__device__ __forceinline__ void add_fields(uint32_t *acc, uint32_t da, uint32_t db)
+{
+    typedef PairSum<IN, false, P> S;
+    if constexpr (I < S::F && J * S::F + I < P) {
+        constexpr int SEL = IN == EBSD_U8 ? I : 4 + I;
+        add_field<SEL>(acc[J * S::F + I], da);
+        add_field<SEL>(acc[J * S::F + I], db);
+        add_fields<IN, P, J, I + 1>(acc, da, db);
+    }
+}
+
+template <int IN, int P, int J = 0>
+__device__ __forceinline__ void add_pair(uint32_t *acc, const uint32_t *da, const uint32_t *db)
+{
+    typedef PairSum<IN, false, P> S;
+    if constexpr (J * S::F < P) {
+        acc[J * S::F] += da[J] + db[J];
+        add_fields<IN, P, J, 1>(acc, da[J], db[J]);
+        add_pair<IN, P, J + 1>(acc, da, db);
+    }
+}
+
 template <int IN, bool FLT, int P>
 __device__ __forceinline__ void group_accumulate(const typename WordT<GatherCfg<IN, FLT, P>::WB>::T *w,
                                                  uint32_t *acc, double *accd)
 {
     typedef GatherCfg<IN, FLT, P> C;
     typedef typename WordT<C::WB>::T Word;
+    if constexpr (PairSum<IN, FLT, P>::value) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        uint32_t d[C::ND] = {};
-        __builtin_memcpy(d, &w[k], sizeof(Word)); // no type punning through pointers
-        accumulate_word<IN, FLT, P>(d, acc, accd);
+        for (int k = 0; k < 8; k += 2) {
+            uint32_t da[C::ND] = {}, db[C::ND] = {};
+            __builtin_memcpy(da, &w[k], sizeof(Word)); // no type punning through pointers
+            __builtin_memcpy(db, &w[k + 1], sizeof(Word));
+            add_pair<IN, P>(acc, da, db);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t d[C::ND] = {};
+            __builtin_memcpy(d, &w[k], sizeof(Word)); // no type punning through pointers
+            accumulate_word<IN, FLT, P>(d, acc, accd);
+        }
+    }
+}
+
+// The P pattern sums from the accumulators (identity except for PairSum words).
+template <int IN, bool FLT, int P> __device__ __forceinline__ void gather_decode(uint32_t *acc)
+{
+    typedef PairSum<IN, FLT, P> S;
+    if constexpr (S::value) {
+#pragma unroll
+        for (int j = 0; j * S::F < P; ++j)
+#pragma unroll
+            for (int i = 1; i < S::F && j * S::F + i < P; ++i)
+                acc[j * S::F] -= acc[j * S::F + i] << (S::B * i);
     }
 }
 
@@ -392,6 +449,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                         eq[u] = S[(size_t)(gi + 4) * 64];
                         group_accumulate<IN, FLT, P>(w, acc, accd);
                         if (gi + 1 == bend) {
+                            gather_decode<IN, FLT, P>(acc);
                             if (!FLT && (a.segmode == 1 || a.segmode == 2) && li.x >= 0) {
 #pragma unroll
                                 for (int p = 0; p < P; ++p)
