@@ -69,7 +69,8 @@ int ebsd_plan_destroy(ebsd_plan *plan);
 int ebsd_plan_set_background(ebsd_plan *plan, const double *bg, int exact);
 
 /* Kernel launch shape.  Any argument <= 0 keeps the current value.
- *   gather_pack:    patterns interleaved per Radon work-group (u8: 4/8/16, u16: 2/4/8, f32: 1/2/4)
+ *   gather_pack:    patterns interleaved per Radon work-group (u8: 2 x gather_pack up to 8,
+ *                   u16: 1/2/4/8, f32: gather_pack / 2 up to 4)
  *   gather_threads: threads per Radon work-group (multiple of 64, <= 1024)
  *   peak_tile:      theta columns per tile of the convolution/peak kernel
  *   peak_threads:   threads per convolution/peak work-group (multiple of 64, <= 512) */

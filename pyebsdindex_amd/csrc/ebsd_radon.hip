@@ -439,15 +439,26 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                                   ? a.part[(q0 + p) * (int64_t)a.npos + bb * 64 + lane] : 0u;
             };
             load_prev(b, li.x);
+            // words up to 8 bytes: the LDS reads of group g+1 are issued before group g is
+            // accumulated, so a wave's reads overlap its own adds (two register sets)
+            constexpr bool PIPE = C::WB <= 8;
+            Word wq[PIPE ? 2 : 1][8];
+            if constexpr (PIPE) group_issue<IN, FLT, P>(lbase, eq[0], wq[0]);
             for (int g0 = 0; g0 < G; g0 += 4) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int gi = g0 + u;
                     if (gi < G) {
-                        Word w[8];
-                        group_issue<IN, FLT, P>(lbase, eq[u], w);
-                        eq[u] = S[(size_t)(gi + 4) * 64];
-                        group_accumulate<IN, FLT, P>(w, acc, accd);
+                        if constexpr (PIPE) {
+                            // group gi+1 is in eq[(u+1)&3] (past the end: a filler group)
+                            group_issue<IN, FLT, P>(lbase, eq[(u + 1) & 3], wq[(u + 1) & 1]);
+                            eq[u] = S[(size_t)(gi + 4) * 64];
+                            group_accumulate<IN, FLT, P>(wq[u & 1], acc, accd);
+                        } else {
+                            group_issue<IN, FLT, P>(lbase, eq[u], wq[0]);
+                            eq[u] = S[(size_t)(gi + 4) * 64];
+                            group_accumulate<IN, FLT, P>(wq[0], acc, accd);
+                        }
                         if (gi + 1 == bend) {
                             gather_decode<IN, FLT, P>(acc);
                             if (!FLT && (a.segmode == 1 || a.segmode == 2) && li.x >= 0) {
@@ -507,13 +518,15 @@ __global__ __launch_bounds__(1024) void k_radon_gather(GatherArgs a)
         gather_pack<IN, FLT, P>(a, it / a.nslice, (int)(it % a.nslice), lds_raw);
 }
 
-// Variants of the gather a batch may mix (element type, float staging, patterns per word).
+// Variants of the gather a batch may mix (element type, float staging, patterns per word);
+// ID is the variant's index in kVariants.  (Sixteen uint8 patterns per 16-byte word needed
+// more than 128 VGPRs and spilled, in its own kernel and in the mixed kernel; it is not built.)
 #define EBSD_GATHER_VARIANTS(X)                                                                    \
-    X(0, EBSD_U8, false, 16) X(1, EBSD_U8, false, 8) X(2, EBSD_U8, false, 4) X(3, EBSD_U8, false, 2)  \
-    X(4, EBSD_U16, false, 8) X(5, EBSD_U16, false, 4) X(6, EBSD_U16, false, 2) X(7, EBSD_U16, false, 1) \
-    X(8, EBSD_U8, true, 4) X(9, EBSD_U8, true, 2) X(10, EBSD_U8, true, 1)                          \
-    X(11, EBSD_U16, true, 4) X(12, EBSD_U16, true, 2) X(13, EBSD_U16, true, 1)                     \
-    X(14, EBSD_F32, true, 4) X(15, EBSD_F32, true, 2) X(16, EBSD_F32, true, 1)
+    X(0, EBSD_U8, false, 8) X(1, EBSD_U8, false, 4) X(2, EBSD_U8, false, 2)                         \
+    X(3, EBSD_U16, false, 8) X(4, EBSD_U16, false, 4) X(5, EBSD_U16, false, 2) X(6, EBSD_U16, false, 1) \
+    X(7, EBSD_U8, true, 4) X(8, EBSD_U8, true, 2) X(9, EBSD_U8, true, 1)                           \
+    X(10, EBSD_U16, true, 4) X(11, EBSD_U16, true, 2) X(12, EBSD_U16, true, 1)                     \
+    X(13, EBSD_F32, true, 4) X(14, EBSD_F32, true, 2) X(15, EBSD_F32, true, 1)
 
 template <int IN, bool FLT, int P> struct GatherVariant;
 #define EBSD_GV_ID(ID, IN, FLT, P) \
@@ -2145,7 +2158,7 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
     if (!p) return fail(-1, "ebsd_plan_set_tuning: null plan");
     if (gather_pack > 0) {
         // one value per element type, interpreted by the type-specific instantiation
-        p->gpack_u8 = std::min(16, std::max(2, gather_pack * 2));
+        p->gpack_u8 = std::min(8, std::max(2, gather_pack * 2));
         p->gpack_u16 = std::min(8, std::max(1, gather_pack));
         p->gpack_f32 = std::min(4, std::max(1, gather_pack / 2));
     }
@@ -2335,8 +2348,7 @@ static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int6
     rc = 1;
     if (dtype == EBSD_U8) {
         for (int P = p->gpack_u8; rc == 1 && P >= 2; P /= 2) {
-            if (P >= 16) rc = launch_gather_rows<EBSD_U8, 16>(p, S, d_pats, N, d_out, use_bg, s);
-            else if (P >= 8) rc = launch_gather_rows<EBSD_U8, 8>(p, S, d_pats, N, d_out, use_bg, s);
+            if (P >= 8) rc = launch_gather_rows<EBSD_U8, 8>(p, S, d_pats, N, d_out, use_bg, s);
             else if (P >= 4) rc = launch_gather_rows<EBSD_U8, 4>(p, S, d_pats, N, d_out, use_bg, s);
             else rc = launch_gather_rows<EBSD_U8, 2>(p, S, d_pats, N, d_out, use_bg, s);
         }
