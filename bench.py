@@ -208,18 +208,23 @@ def batched_measure(dev, steps, warmup, npats=32768, seed=7, autotune_sets=True)
     outs = [bd.device_plan().alloc_out(x.shape[0]) for bd, x in zip(dets, pats)]
     works = [bd.device_plan().workspace(x.shape[0], 2048) for bd, x in zip(dets, pats)]
 
-    def plain():
+    def plain(chunk=2048):
         for bd, x, o, w in zip(dets, pats, outs, works):
-            bd.find_bands_device(x, out=o, work=w, chunk=2048)
+            bd.find_bands_device(x, out=o, work=w, chunk=chunk)
 
-    for _ in range(warmup):
-        plain()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        plain()
-    torch.cuda.synchronize()
-    plain_s = (time.perf_counter() - t0) / steps
+    def time_plain(chunk):
+        for _ in range(warmup):
+            plain(chunk)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plain(chunk)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    plain_s = time_plain(2048)
+    # the same per-size launches at the reference's default chunk (band_detect.py:427-434)
+    plain512_s = time_plain(512)
     batch = BatchBandDetect(dets, pats)
     gs = torch.cuda.Stream(dev)
     for _ in range(warmup):
@@ -250,6 +255,8 @@ def batched_measure(dev, steps, warmup, npats=32768, seed=7, autotune_sets=True)
             "plain_patterns_per_s": round(n / plain_s, 1), "batched_patterns_per_s": round(n / run_s, 1),
             "graph_patterns_per_s": round(n / graph_s, 1), "graph_vs_plain": round(plain_s / graph_s, 3),
             "plain_ms": round(plain_s * 1e3, 4), "graph_ms": round(graph_s * 1e3, 4),
+            "plain_chunk512_patterns_per_s": round(n / plain512_s, 1),
+            "graph_vs_plain_chunk512": round(plain512_s / graph_s, 3),
             "peak_bins_identical": same, "max_subpixel_diff_bins": dmax, "autotuned": tuned}
 
 
@@ -366,11 +373,13 @@ def main():
         print("[bench] mixed-size batched workload", file=sys.stderr, flush=True)
         batched = batched_measure(dev, args.steps, args.warmup)
         print(f"[bench] batched {batched['graph_patterns_per_s']:.1f} patterns/s, plain per-size "
-              f"{batched['plain_patterns_per_s']:.1f} patterns/s", file=sys.stderr, flush=True)
+              f"{batched['plain_patterns_per_s']:.1f} patterns/s (chunks of 512: "
+              f"{batched['plain_chunk512_patterns_per_s']:.1f})", file=sys.stderr, flush=True)
         sweep = []
         for n in (512, 2048, 8192):
             small = batched_measure(dev, args.steps, args.warmup, npats=n, autotune_sets=False)
             sweep.append({k: small[k] for k in ("plain_patterns_per_s", "graph_patterns_per_s", "graph_vs_plain",
+                                                "plain_chunk512_patterns_per_s", "graph_vs_plain_chunk512",
                                                 "plain_ms", "graph_ms", "peak_bins_identical")})
             sweep[-1]["patterns"] = n
             print(f"[bench] {n} mixed patterns: graph {small['graph_patterns_per_s']:.1f} vs plain "

@@ -528,8 +528,9 @@ class BandDetect:
 
     def find_bands(self, patternsIn, verbose=0, chunksize=-1, clparams=None, gpu_id=None, **kwargs):
         """band_detect.py:411-526 on the GPU.  ``chunksize`` > 0 bounds the patterns per
-        device pass (default: as many as an 8192-pattern workspace holds); ``clparams`` may
-        be a ``HipParam`` whose ``gpu_id`` selects the device."""
+        device pass (default 2,048: the Radon of a pass stays in the MI355X's 256 MB
+        Infinity Cache at 120x120); ``clparams`` may be a ``HipParam`` whose ``gpu_id``
+        selects the device."""
         import torch
         tic0 = timer()
         if gpu_id is None and clparams is not None and hasattr(clparams, "gpu_id"):
