@@ -108,6 +108,8 @@ struct ebsd_plan {
     uint32_t *d_mrow = nullptr;  // rho-mask bits per rho row (PeakGeo::maskw words each)
     // adjoint structures, built on first use
     int *d_px_off = nullptr, *d_px_cell = nullptr; // pixel -> lines through it (CSR)
+    uint32_t *d_bp_tab = nullptr; // the same, padded to bp_m2 pairs of u16 cells per pixel,
+    int bp_m2 = 0;                // pair-major [bp_m2][nPx] (0: not built, nCells >= 65535)
     double *d_icnt = nullptr;                      // 1 / pixels per line
     int *d_pre_off = nullptr, *d_pre = nullptr;    // Radon cell -> padded cells copying it (CSR)
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
@@ -1668,6 +1670,7 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_mrow);
     (void)hipFree(p->d_px_off);
     (void)hipFree(p->d_px_cell);
+    (void)hipFree(p->d_bp_tab);
     (void)hipFree(p->d_icnt);
     (void)hipFree(p->d_pre_off);
     (void)hipFree(p->d_pre);
@@ -2711,28 +2714,33 @@ __global__ __launch_bounds__(1024) void k_conv_transpose(const float *dconv, int
 
 // Radon adjoint: d(pixel) = sum over the lines through the pixel of dR[line] / count[line]
 // (radon_fast.py:288-310 divides each line sum by its pixel count).  LDS: one work-group
-// per pattern stages dR[line] / count[line] in float64 once; each wavefront takes a pixel
-// at a time, its lanes read the pixel's lines with coalesced loads and the float64 sum
-// closes with a fixed butterfly (deterministic).  GLOBAL: one thread per pixel reads the
-// lines from memory (Radons too large for LDS).
+// per pattern stages dR[line] / count[line] in float64 once (and a zero at cell nCells);
+// each thread takes a pixel and sums its lines from the pair-major table (coalesced u16
+// cell pairs) into two float64 accumulators, even and odd list positions, added at the end:
+// a fixed order, deterministic.  GLOBAL: one thread per pixel reads the lines from memory
+// (Radons too large for LDS).
 template <bool LDS>
 __global__ __launch_bounds__(1024) void k_backproject(const float *dradon, int64_t N, int nPx, int nCells,
                                                       const int *px_off, const int *px_cell, const double *icnt,
-                                                      float *dpats)
+                                                      const uint32_t *tab, int m2, float *dpats)
 {
     if constexpr (LDS) {
         extern __shared__ __align__(16) double sdr[];
         const int64_t q = blockIdx.x;
         const float *dR = dradon + q * (int64_t)nCells;
         for (int c = threadIdx.x; c < nCells; c += blockDim.x) sdr[c] = (double)dR[c] * icnt[c];
+        if (threadIdx.x == 0) sdr[nCells] = 0.0;
         __syncthreads();
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        for (int px = wave; px < nPx; px += nw) {
-            const int k0 = px_off[px], k1 = px_off[px + 1];
-            double acc = 0.0;
-            for (int k = k0 + lane; k < k1; k += 64) acc += sdr[px_cell[k]];
-            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-            if (lane == 0) dpats[q * (int64_t)nPx + px] = (float)acc;
+        for (int px = threadIdx.x; px < nPx; px += blockDim.x) {
+            const uint32_t *t = tab + px;
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll 4
+            for (int k2 = 0; k2 < m2; ++k2) {
+                const uint32_t v = t[(size_t)k2 * nPx];
+                a0 += sdr[v & 0xFFFFu];
+                a1 += sdr[v >> 16];
+            }
+            dpats[q * (int64_t)nPx + px] = (float)(a0 + a1);
         }
     } else {
         const int64_t total = N * (int64_t)nPx;
@@ -2865,7 +2873,25 @@ static int ensure_adjoint(ebsd_plan *p)
     for (int i = 0; i < nCells; ++i) pre_off[i + 1] += pre_off[i];
     std::vector<int> pf(pre_off.begin(), pre_off.end() - 1);
     for (int k = 0; k < nC; ++k) pre[pf[src[k]]++] = k;
+    // the lines of pixel px as u16 cell pairs, pair-major so that a wavefront of consecutive
+    // pixels reads each pair step coalesced; short lists padded with cell nCells (a zero)
+    std::vector<uint32_t> tab;
+    int m2 = 0;
+    if (nCells < 65535) {
+        int m = 0;
+        for (int i = 0; i < nPx; ++i) m = std::max(m, px_off[i + 1] - px_off[i]);
+        m2 = (m + 1) / 2;
+        tab.assign((size_t)m2 * nPx, (uint32_t)nCells | ((uint32_t)nCells << 16));
+        for (int i = 0; i < nPx; ++i)
+            for (int k = px_off[i]; k < px_off[i + 1]; ++k) {
+                const int j = k - px_off[i];
+                uint32_t &w = tab[(size_t)(j / 2) * nPx + i];
+                w = (j & 1) ? ((w & 0xFFFFu) | ((uint32_t)px_cell[k] << 16)) : ((w & 0xFFFF0000u) | (uint32_t)px_cell[k]);
+            }
+    }
     int rc;
+    if (m2 > 0 && (rc = dev_upload(&p->d_bp_tab, tab.data(), tab.size()))) return rc;
+    p->bp_m2 = m2;
     if ((rc = dev_upload(&p->d_px_off, px_off.data(), px_off.size()))) return rc;
     if ((rc = dev_upload(&p->d_px_cell, px_cell.data(), px_cell.size()))) return rc;
     if ((rc = dev_upload(&p->d_icnt, icnt.data(), icnt.size()))) return rc;
@@ -2933,16 +2959,17 @@ extern "C" int ebsd_backproject(ebsd_plan *p, const float *d_grad_radon, int64_t
     if (!d_grad_radon || !d_grad_pats) return fail(-1, "ebsd_backproject: null device pointer");
     int rc;
     if ((rc = ensure_adjoint(p))) return rc;
-    const size_t lds = (size_t)p->nCells * 8;
-    if ((int)lds <= p->lds_max) {
+    const size_t lds = (size_t)(p->nCells + 1) * 8;
+    if (p->bp_m2 > 0 && (int)lds <= p->lds_max) {
         HIPCHK(hipFuncSetAttribute((const void *)k_backproject<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
         hipLaunchKernelGGL(k_backproject<true>, dim3((unsigned)N), dim3(1024), lds, (hipStream_t)stream, d_grad_radon,
-                           N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell, p->d_icnt, d_grad_pats);
+                           N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell, p->d_icnt, p->d_bp_tab, p->bp_m2,
+                           d_grad_pats);
     } else {
         hipLaunchKernelGGL(k_backproject<false>, dim3(grid_for(N * (int64_t)p->nPx)), dim3(256), 0,
                            (hipStream_t)stream, d_grad_radon, N, p->nPx, p->nCells, p->d_px_off, p->d_px_cell,
-                           p->d_icnt, d_grad_pats);
+                           p->d_icnt, (const uint32_t *)nullptr, 0, d_grad_pats);
     }
     HIPCHK(hipGetLastError());
     return 0;
