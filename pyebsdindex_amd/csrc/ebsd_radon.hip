@@ -112,6 +112,8 @@ struct ebsd_plan {
     int bp_m2 = 0;                // pair-major [bp_m2][nPx] (0: not built, nCells >= 65535)
     double *d_icnt = nullptr;                      // 1 / pixels per line
     int *d_pre_off = nullptr, *d_pre = nullptr;    // Radon cell -> padded cells copying it (CSR)
+    double *d_tscr = nullptr;    // transposed convolution: a padded array of float64 per work-group,
+    size_t tscr_cap = 0;         //   tscr_cap doubles (grown on demand)
     double *d_grid = nullptr;    // theta_deg[nT] then rho[nR], as float64
     // tuning
     int gpack_u8 = 8, gpack_u16 = 4, gpack_f32 = 2, gthreads = 1024;
@@ -1671,6 +1673,7 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_px_off);
     (void)hipFree(p->d_px_cell);
     (void)hipFree(p->d_bp_tab);
+    (void)hipFree(p->d_tscr);
     (void)hipFree(p->d_icnt);
     (void)hipFree(p->d_pre_off);
     (void)hipFree(p->d_pre);
@@ -2712,6 +2715,67 @@ __global__ __launch_bounds__(1024) void k_conv_transpose(const float *dconv, int
     }
 }
 
+// Transposed convolution, separable and streamed: thread per (padded column x, band of
+// CT_ROWS rows) walks down the band keeping the theta-tap sums E[yy][x] = sum_b v[b]
+// dC[yy][x - ckt + b] of rows y - ckr .. y + ckr in registers, so each E is computed once
+// instead of once per rho tap; dP[y][x] = sum_a u[a] E[y - ckr + a] goes to the work-group's
+// float64 scratch.  The same products and the same order as cell_value of k_conv_transpose
+// (rows and columns outside the padded array skipped), so dP is that kernel's value for a
+// single preimage; then each Radon cell sums the dP of the padded cells that copy it, in
+// `pre` order (cells with several copies round once per copy instead of once per tap).
+constexpr int CT_ROWS = 16;
+template <int KR, int KT>
+__global__ __launch_bounds__(1024) void k_conv_transpose_sep(const float *dconv, int64_t N, PeakGeo g,
+                                                             const int *pre_off, const int *pre, double *scratch,
+                                                             float *dradon)
+{
+    constexpr int CKR = KR / 2, CKT = KT / 2;
+    const int nCells = g.nR * g.nT, nTp = g.nTp, nRp = g.nRp, nC = nRp * nTp;
+    double *dP = scratch + (size_t)blockIdx.x * nC;
+    const int nbands = (nRp + CT_ROWS - 1) / CT_ROWS;
+    for (int64_t q = blockIdx.x; q < N; q += gridDim.x) {
+        const float *src = dconv + q * (int64_t)nC;
+        for (int it = threadIdx.x; it < nTp * nbands; it += blockDim.x) {
+            const int x = it % nTp, y0 = (it / nTp) * CT_ROWS, y1 = min(y0 + CT_ROWS, nRp);
+            auto erow = [&](int yy) -> double {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < KT; ++b) {
+                    const int xx = x - CKT + b;
+                    if (xx >= 0 && xx < nTp) t += (double)g.v[b] * (double)src[yy * nTp + xx];
+                }
+                return t;
+            };
+            double E[KR]; // E[a] holds row y - CKR + a
+#pragma unroll
+            for (int a = 0; a < KR - 1; ++a) {
+                const int yy = y0 - CKR + a;
+                E[a] = (yy >= 0 && yy < nRp) ? erow(yy) : 0.0;
+            }
+            for (int y = y0; y < y1; ++y) {
+                const int yn = y + CKR;
+                E[KR - 1] = (yn < nRp) ? erow(yn) : 0.0;
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < KR; ++a) {
+                    const int yy = y - CKR + a;
+                    if (yy >= 0 && yy < nRp) acc += (double)g.u[a] * E[a];
+                }
+                dP[y * nTp + x] = acc;
+#pragma unroll
+                for (int a = 0; a < KR - 1; ++a) E[a] = E[a + 1];
+            }
+        }
+        __syncthreads(); // the work-group's dP writes before its reads
+        for (int cell = threadIdx.x; cell < nCells; cell += blockDim.x) {
+            double acc = 0.0;
+            for (int k = pre_off[cell]; k < pre_off[cell + 1]; ++k) acc += dP[pre[k]];
+            dradon[q * (int64_t)nCells + cell] = (float)acc;
+        }
+        __syncthreads(); // dP reads before the next pattern's writes
+    }
+}
+
 // Radon adjoint: d(pixel) = sum over the lines through the pixel of dR[line] / count[line]
 // (radon_fast.py:288-310 divides each line sum by its pixel count).  LDS: one work-group
 // per pattern stages dR[line] / count[line] in float64 once (and a zero at cell nCells);
@@ -2937,6 +3001,29 @@ extern "C" int ebsd_conv_transpose(ebsd_plan *p, const float *d_grad_conv, int64
     if (!d_grad_conv || !d_grad_radon) return fail(-1, "ebsd_conv_transpose: null device pointer");
     int rc;
     if ((rc = ensure_adjoint(p))) return rc;
+    const PeakGeo &g = p->g;
+    if ((g.kr == 9 || g.kr == 5) && g.kt == 5) {
+        const int64_t grid = std::min<int64_t>(N, 2 * (int64_t)p->ncu);
+        const size_t need = (size_t)grid * g.nRp * g.nTp;
+        if (p->tscr_cap < need) {
+            if (p->d_tscr) {
+                HIPCHK(hipDeviceSynchronize()); // earlier launches may still use the old buffer
+                (void)hipFree(p->d_tscr);
+            }
+            p->d_tscr = nullptr;
+            p->tscr_cap = 0;
+            if (hipMalloc((void **)&p->d_tscr, need * sizeof(double)) != hipSuccess) {
+                p->d_tscr = nullptr;
+                return fail(-3, "ebsd_conv_transpose: device allocation failed");
+            }
+            p->tscr_cap = need;
+        }
+        auto kern = g.kr == 9 ? k_conv_transpose_sep<9, 5> : k_conv_transpose_sep<5, 5>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(1024), 0, (hipStream_t)stream, d_grad_conv, N, g,
+                           p->d_pre_off, p->d_pre, p->d_tscr, d_grad_radon);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     const size_t lds = (size_t)p->g.nRp * p->g.nTp * 4;
     if ((int)lds <= p->lds_max) {
         HIPCHK(hipFuncSetAttribute((const void *)k_conv_transpose<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
