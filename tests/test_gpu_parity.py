@@ -35,6 +35,28 @@ def test_radon_matches_oracle_bitwise(shape, dtype, n):
     assert np.array_equal(rad, want)
 
 
+@pytest.mark.parametrize("dtype", ["uint8", "uint16"])
+def test_radon_every_pack_size(dtype):
+    """Every patterns-per-word variant of the integer gather (u8: 2, 4, 8 per word; u16: 1, 2,
+    4, 8), each with its own accumulation layout, is bit-identical to the reference
+    restatement, with and without row segments; 13 patterns leave a ragged last pack."""
+    print(f"every pack size {dtype}", flush=True)
+    shape = (60, 60)
+    bd, ref = make_pair(shape)
+    pats = _pats(13, shape, dtype, seed=53)
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    plan = bd.device_plan()
+    x = _dev(pats)
+    for gp in (1, 2, 4, 8):
+        plan.set_tuning(gather_pack=gp)
+        for S in (1, 2):
+            plan.set_row_segments(S)
+            assert np.array_equal(plan.radon(x).cpu().numpy(), want), (gp, S)
+    plan.set_tuning(gather_pack=4)
+    plan.set_row_segments(0)
+
+
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_radon_background(mode):
     print(f"radon background {mode}", flush=True)
