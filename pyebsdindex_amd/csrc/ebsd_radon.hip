@@ -426,7 +426,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             int bend = (int)a.bhdr[b].y;
             int2 li = a.lane[(size_t)b * 64 + lane];
             int bn = b + nslot;
-            int2 lin = li;
+            int2 lin = make_int2(-1, 0);
             int ngn = 0;
             if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
             uint32_t acc[P];
@@ -451,8 +451,8 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             for (int g0 = 0; g0 < G; g0 += 4) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int gi = g0 + u;
-                    if (gi < G) {
+                    const int gi = g0 + u; // G is a multiple of 4 (the layout pads it)
+                    {
                         if constexpr (PIPE) {
                             // group gi+1 is in eq[(u+1)&3] (past the end: a filler group)
                             group_issue<IN, FLT, P>(lbase, eq[(u + 1) & 3], wq[(u + 1) & 1]);
@@ -502,7 +502,10 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                             li = lin;
                             bend += ngn;
                             bn = b + nslot;
+                            // past the slot's last batch: no further line end (the walk then
+                            // only runs the filler groups that round the stream up)
                             if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
+                            else { lin = make_int2(-1, 0); ngn = 0; }
                             load_prev(b, b < nb ? li.x : -1);
                         }
                     }
@@ -1755,6 +1758,10 @@ static int get_layout(ebsd_plan::GSeg &g, int nv, int kind, ebsd_plan::Layout *o
             st.insert(st.end(), hg.begin() + from, hg.begin() + from + (size_t)h.y * 64 * 8);
             cnt += (int)h.y;
         }
+        // a whole number of 4-group steps: the walk loop then has no per-group bound test
+        const int pad = (4 - cnt % 4) % 4;
+        st.resize(st.size() + (size_t)pad * 64 * 8, (uint16_t)g.nwords);
+        cnt += pad;
         wcnt[w] = cnt;
         ofs += cnt;
     }
