@@ -356,9 +356,8 @@ def main():
         if tj.get("patterns_per_launch") == chunk:
             traffic = tj.get("hbm_bytes_per_launch")
 
-    # the streamed measurement runs before the batched one: after the 32,768-pattern batched
-    # workload in the same process its uploads stopped overlapping band detection (about
-    # half the rate), which a fresh process does not show
+    # the streamed measurement runs before the batched one, so that it starts from an idle
+    # device (an earlier version lost its copy overlap after the batched workload)
     streaming = None
     if not args.no_streaming:
         print("[bench] scan-scale streaming workload", file=sys.stderr, flush=True)
