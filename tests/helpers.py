@@ -64,3 +64,34 @@ def compare_bands(got, res64, C64, padding, peakPad, rel_eps=2e-5, subpix_tol=1e
                         f"oracle {ol} valid {ov} C={oval}, eps={eps}")
             ties += 1
     return N * nB, ties
+
+
+def orientation_agreement(got, res, meta, pc, shape):
+    """Fit an orientation to each pattern's bands from ``got`` and from ``res`` (structured
+    band records with theta, rho, valid, max) with the same assignment to the generator's
+    planes and the same weighted Wahba fit (oracle/orientation.py), on the bands both found
+    at the same plane.  Returns (patterns fitted, worst got-vs-res misorientation in deg,
+    median got-vs-true misorientation in deg)."""
+    from oracle import gnomonic
+    from oracle import orientation as ori
+    from pyebsdindex_amd.synthetic import quat_to_matrix
+    n_got = gnomonic.radon2pole(got['theta'], got['rho'], pc, shape)
+    n_res = gnomonic.radon2pole(res['theta'], res['rho'], pc, shape)
+    R_true = quat_to_matrix(meta['quat'])
+    worst, fitted, vs_true = 0.0, 0, []
+    for q in range(n_got.shape[0]):
+        a = ori.assign_bands(n_got[q], meta['g_det'][q], meta['normals'], got['valid'][q])
+        b = ori.assign_bands(n_res[q], meta['g_det'][q], meta['normals'], res['valid'][q])
+        # a band the two paths pick differently (near-tie peaks) changes the data set
+        common = np.intersect1d(a[2], b[2])
+        ia, ib = np.searchsorted(a[2], common), np.searchsorted(b[2], common)
+        same = np.all(a[1][ia] == b[1][ib], axis=1)
+        ia, ib = ia[same], ib[same]
+        if ia.size < 3:
+            continue
+        Ra = ori.wahba_svd(a[0][ia], a[1][ia], got['max'][q][a[2][ia]].astype(np.float64))
+        Rb = ori.wahba_svd(b[0][ib], b[1][ib], res['max'][q][b[2][ib]].astype(np.float64))
+        worst = max(worst, ori.misorientation_deg(Ra, Rb))
+        vs_true.append(ori.misorientation_deg(Ra, R_true[q]))
+        fitted += 1
+    return fitted, worst, float(np.median(vs_true)) if vs_true else float('inf')
