@@ -17,6 +17,12 @@ in-order queue holds everything behind it: an upload enqueued after a batch's ba
 detection would then start only when that detection ends.  Enqueued ahead, an upload
 only ever waits behind the detection that last used its slot, which is the dependency it
 has anyway.  All streams keep the default priority.
+
+``run_file`` feeds the same pipeline from a UP1/UP2 pattern file: each batch is read with
+positioned reads straight into one buffer of a small pinned ring, just before its upload is
+enqueued, and a ring buffer is read into again only once the upload that used it has
+finished.  The reads of a batch run on the host while the device copies and detects the
+batches before it.
 """
 import numpy as np
 
@@ -92,37 +98,92 @@ class StreamingBandDetect:
         through band detection.  Returns, per batch, a dict of pinned host tensors (views of
         one result block per batch; ``blocks`` may pass the blocks in, e.g. from
         ``pinned_blocks``, to reuse them)."""
-        import torch
         batches = list(batches)
         for hb in batches:
-            if int(hb.shape[0]) > self.batch or tuple(hb.shape[1:]) != (self.plan.H, self.plan.W):
-                raise ValueError("batch larger than the pipeline's batch size or of another pattern size")
-            if hb.dtype != self.tdtype:
-                raise TypeError(f"batch dtype {hb.dtype}, pipeline dtype {self.tdtype}")
+            self._check_batch(hb)
+        return self._run(len(batches), lambda j: batches[j], [int(hb.shape[0]) for hb in batches], blocks)
+
+    def run_file(self, patfile, patStart=0, nPatterns=-1, blocks=None, ring=None, threads=8):
+        """Stream patterns [patStart, patStart+nPatterns) of a pattern file (``UPFile`` from
+        ``pyebsdindex_amd.ebsd_pattern``; -1 = to the end) through band detection.
+
+        Batches are read from the file straight into a ring of pinned host buffers
+        (``ring``, default ``depth + 1`` of them, from ``pinned_batch``) with positioned
+        reads on ``threads`` threads; a buffer is read again only after the upload that
+        used it has finished, so file reads of one batch overlap the copy and band
+        detection of the ones before.  Returns the per-batch results of ``run``."""
+        import torch
+        if np.dtype(patfile.filedatatype) != np.dtype(str(self.tdtype).replace("torch.", "")):
+            raise TypeError(f"file holds {np.dtype(patfile.filedatatype)}, pipeline dtype {self.tdtype}")
+        if (int(patfile.patternH), int(patfile.patternW)) != (self.plan.H, self.plan.W):
+            raise ValueError("file patterns are of another size than the pipeline's")
+        total = int(patfile.nPatterns) - int(patStart) if int(nPatterns) < 0 else int(nPatterns)
+        if patStart < 0 or total < 0 or patStart + total > int(patfile.nPatterns):
+            raise ValueError("pattern range outside the file")
+        counts = [min(self.batch, total - o) for o in range(0, total, self.batch)]
+        if ring is None:
+            ring = [self.pinned_batch() for _ in range(self.depth + 1)]
+        if len(ring) < 2:
+            raise ValueError("the host ring needs at least two buffers")
+        for hb in ring:
+            if int(hb.shape[0]) < self.batch or not hb.is_pinned():
+                raise ValueError("ring buffers must be pinned and hold a whole batch")
+        R = len(ring)
+        freed = [None] * R                    # event: last upload out of ring buffer r has finished
+
+        def get(j):
+            r, n = j % R, counts[j]
+            if freed[r] is not None:
+                freed[r].synchronize()
+            patfile.read_into(ring[r], int(patStart) + j * self.batch, n, threads=threads)
+            return ring[r][:n]
+
+        def uploaded(j):
+            ev = torch.cuda.Event()
+            ev.record(self.copy_stream)
+            freed[j % R] = ev
+
+        return self._run(len(counts), get, counts, blocks, uploaded)
+
+    def _check_batch(self, hb):
+        if int(hb.shape[0]) > self.batch or tuple(hb.shape[1:]) != (self.plan.H, self.plan.W):
+            raise ValueError("batch larger than the pipeline's batch size or of another pattern size")
+        if hb.dtype != self.tdtype:
+            raise TypeError(f"batch dtype {hb.dtype}, pipeline dtype {self.tdtype}")
+
+    def _run(self, nb, get_batch, counts, blocks, on_upload=None):
+        """The pipeline over ``nb`` batches: ``get_batch(j)`` returns batch j as a pinned host
+        tensor of ``counts[j]`` patterns, called in order just before its upload is
+        enqueued; ``on_upload(j)`` runs right after."""
+        import torch
         if blocks is None:
-            blocks = self.pinned_blocks(len(batches))
-        if len(blocks) < len(batches):
+            blocks = self.pinned_blocks(nb)
+        if len(blocks) < nb:
             raise ValueError("fewer result blocks than batches")
         cs, ks, rs = self.copy_stream, self.compute_stream, self.return_stream
         D = self.depth
 
         def upload(j):
-            s, hb = j % D, batches[j]
+            s = j % D
+            hb = get_batch(j)
+            self._check_batch(hb)
             if self._used[s]:
                 cs.wait_event(self.computed[s])              # slot s's last band detection has read it
             with torch.cuda.stream(cs):
                 self.slots[s][:hb.shape[0]].copy_(hb, non_blocking=True)
             self.copied[s].record(cs)
+            if on_upload is not None:
+                on_upload(j)
 
         results = []
         ahead = D - 1
         with torch.cuda.device(self.device):
-            for j in range(min(ahead, len(batches))):
+            for j in range(min(ahead, nb)):
                 upload(j)
-            for j, hb in enumerate(batches):
-                if j + ahead < len(batches):
+            for j in range(nb):
+                if j + ahead < nb:
                     upload(j + ahead)                        # its slot's last reader is batch j - 1
-                s, n = j % D, int(hb.shape[0])
+                s, n = j % D, int(counts[j])
                 ks.wait_event(self.copied[s])
                 if self._used[s]:
                     ks.wait_event(self.returned[s])          # slot s's last results are back on the host

@@ -37,3 +37,34 @@ def test_streaming_equals_resident(shape, dtype, kw, sizes, batch):
     got2 = sb.concat(sb.run(hb))
     for k in want:
         assert np.array_equal(got2[k].numpy(), want[k]), k
+
+
+@pytest.mark.parametrize("shape,dtype,ext,start,count,batch,ring", [
+    ((60, 60), "uint8", ".up1", 5, 50, 16, None),
+    ((120, 120), "uint16", ".up2", 0, -1, 8, 2),
+])
+def test_file_streaming_equals_resident(tmp_path, shape, dtype, ext, start, count, batch, ring):
+    """Patterns read from a UP1/UP2 file straight into the pinned ring and streamed give
+    exactly what find_bands gives on the same patterns resident on the device (ragged last
+    batch, an offset start, the smallest ring)."""
+    import torch
+    from pyebsdindex_amd.ebsd_pattern import UPFile, get_pattern_file_obj
+    from pyebsdindex_amd.streaming import StreamingBandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    print(f"file streaming {shape} {dtype} start {start} count {count} batch {batch}", flush=True)
+    n = 61
+    pats = kikuchi_patterns(n, shape, dtype, seed=57).numpy()
+    fn = tmp_path / f"scan{ext}"
+    w = UPFile(fn)
+    w.patternW, w.patternH, w.nCols, w.nRows, w.xStep, w.yStep = shape[1], shape[0], n, 1, 1.0, 1.0
+    w.write_header()
+    w.pat_writer(pats, 0, n, pats.dtype)
+    f = get_pattern_file_obj(str(fn))
+    bd, _ = make_pair(shape)
+    stop = n if count < 0 else start + count
+    want = {k: v.cpu().numpy() for k, v in bd.find_bands_device(torch.from_numpy(pats[start:stop]).cuda()).items()}
+    sb = StreamingBandDetect(bd, batch=batch, dtype=dtype, depth=2, chunk=batch)
+    rbufs = None if ring is None else [sb.pinned_batch() for _ in range(ring)]
+    got = sb.concat(sb.run_file(f, start, count, ring=rbufs, threads=3))
+    for k in want:
+        assert np.array_equal(got[k].numpy(), want[k]), k
