@@ -1,7 +1,7 @@
 """Build recipe for the oracle's C restatement (test infrastructure only).
 
-``python oracle/build.py`` compiles ``oracle/csrc/ref_loops.c`` with gcc/OpenMP into
-``oracle/libref_loops.so``.  The reference itself is Python + Numba (no C/C++ sources),
+``python oracle/build.py`` compiles ``oracle/csrc/ref_loops.c`` (band detection) and
+``oracle/csrc/ref_nlpar.c`` (NLPAR) with gcc/OpenMP into ``oracle/libref_loops.so``.  The reference itself is Python + Numba (no C/C++ sources),
 so there is no ``oracle/_ref`` build: the reference is read, not run.
 """
 import os
@@ -9,16 +9,16 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "ref_loops.c")
+SRCS = [os.path.join(HERE, "csrc", "ref_loops.c"), os.path.join(HERE, "csrc", "ref_nlpar.c")]
 LIB = os.path.join(HERE, "libref_loops.so")
 
 
 def build(force: bool = True) -> str:
-    if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= os.path.getmtime(SRC):
+    if not force and os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(s) for s in SRCS):
         return LIB
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O2", "-fno-fast-math", "-ffp-contract=off", "-fopenmp", "-fPIC",
-           "-shared", "-o", tmp, SRC, "-lm"]
+           "-shared", "-o", tmp] + SRCS + ["-lm"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

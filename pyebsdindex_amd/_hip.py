@@ -1,6 +1,7 @@
-"""ctypes binding of the C-ABI library ``libebsd_radon.so`` (declared in include/ebsd_radon.h).
+"""ctypes bindings of the C-ABI libraries ``libebsd_radon.so`` (include/ebsd_radon.h) and
+``libebsd_nlpar.so`` (include/ebsd_nlpar.h).
 
-There is deliberately no fallback: if the library is missing or fails to load, every
+There is deliberately no fallback: if a library is missing or fails to load, every
 entry point raises ``HipLibraryError``.
 """
 import ctypes
@@ -9,6 +10,7 @@ import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libebsd_radon.so")
+NLPAR_LIB_PATH = os.path.join(HERE, "libebsd_nlpar.so")
 
 EBSD_U8, EBSD_U16, EBSD_F32 = 0, 1, 2
 
@@ -20,6 +22,9 @@ SYMBOLS = (
     "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy", "ebsd_band_grad",
     "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands", "ebsd_radon2pole",
 )
+
+#: exported symbols of the NLPAR library, in declaration order of include/ebsd_nlpar.h
+NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_sigma", "ebsd_nlpar", "ebsd_nlpar_last_error")
 
 
 class HipLibraryError(RuntimeError):
@@ -34,6 +39,7 @@ class EbsdBands(ctypes.Structure):
 
 
 _lib = None
+_nlpar_lib = None
 _lock = threading.Lock()
 
 
@@ -95,3 +101,34 @@ def check(rc: int):
     if rc != 0:
         msg = load().ebsd_last_error().decode(errors="replace")
         raise HipLibraryError(f"ebsd library error {rc}: {msg}")
+
+
+def load_nlpar(path: str = NLPAR_LIB_PATH):
+    """Load the NLPAR library once; raise HipLibraryError if it is not built."""
+    global _nlpar_lib
+    with _lock:
+        if _nlpar_lib is not None:
+            return _nlpar_lib
+        if not os.path.exists(path):
+            raise HipLibraryError(
+                f"HIP library not found at {path}; build it with `python -m pyebsdindex_amd.build`")
+        try:
+            L = ctypes.CDLL(path)
+        except OSError as e:
+            raise HipLibraryError(f"failed to load {path}: {e}") from e
+        P, I, I64, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_float
+        L.ebsd_nlpar_max.argtypes = [P, I, I64, I64, P, P]
+        L.ebsd_nlpar_sigma.argtypes = [P, I, I, I, I64, P, I, I, F, P, P, P, P]
+        L.ebsd_nlpar.argtypes = [P, I, I, I, I64, P, I, P, I, F, F, F, F, I, I, I, I, P, I, P, P]
+        for name in ("ebsd_nlpar_max", "ebsd_nlpar_sigma", "ebsd_nlpar"):
+            getattr(L, name).restype = ctypes.c_int
+        L.ebsd_nlpar_last_error.argtypes = []
+        L.ebsd_nlpar_last_error.restype = ctypes.c_char_p
+        _nlpar_lib = L
+        return L
+
+
+def check_nlpar(rc: int):
+    if rc != 0:
+        msg = load_nlpar().ebsd_nlpar_last_error().decode(errors="replace")
+        raise HipLibraryError(f"ebsd NLPAR library error {rc}: {msg}")

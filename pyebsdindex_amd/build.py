@@ -1,8 +1,9 @@
-"""Build the gfx950 HIP library ``libebsd_radon.so`` in-tree.
+"""Build the gfx950 HIP libraries in-tree.
 
 ``python -m pyebsdindex_amd.build`` (or ``__graft_entry__.build()``) runs hipcc on
-``csrc/ebsd_radon.hip``.  The library is a plain C-ABI shared object (no torch
-symbols); ``pyebsdindex_amd._hip`` loads it with ctypes.
+``csrc/ebsd_radon.hip`` (band detection -> ``libebsd_radon.so``) and
+``csrc/ebsd_nlpar.hip`` (NLPAR -> ``libebsd_nlpar.so``).  Both are plain C-ABI shared
+objects (no torch symbols); ``pyebsdindex_amd._hip`` loads them with ctypes.
 """
 import os
 import subprocess
@@ -12,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "ebsd_radon.hip")
 LIB = os.path.join(HERE, "libebsd_radon.so")
+NLPAR_SRC = os.path.join(HERE, "csrc", "ebsd_nlpar.hip")
+NLPAR_LIB = os.path.join(HERE, "libebsd_nlpar.so")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC_FLAGS = [
@@ -25,15 +28,21 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def _hipcc(src: str, lib: str, verbose: bool) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", tmp, SRC]
+    tmp = lib + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", tmp, src]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
+
+
+def build(verbose: bool = False) -> str:
+    """Compile both libraries; returns the band-detection library's path."""
+    _hipcc(NLPAR_SRC, NLPAR_LIB, verbose)
+    return _hipcc(SRC, LIB, verbose)
 
 
 if __name__ == "__main__":

@@ -13,9 +13,9 @@ from helpers import STD
 from oracle import ref_cpu
 
 
-def _declared():
+def _declared(header="*.h"):
     names = []
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in glob.glob(os.path.join(ROOT, "include", header)):
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names += re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(ebsd_\w+)\s*\(", txt, flags=re.M)
@@ -25,11 +25,30 @@ def _declared():
 def test_library_exports_every_declared_symbol():
     from pyebsdindex_amd import _hip
     lib = _hip.load()
-    declared = _declared()
+    declared = _declared("ebsd_radon.h")
     assert len(declared) >= 11
     for name in declared:
         assert hasattr(lib, name), name
     assert set(_hip.SYMBOLS) <= set(declared)
+    nl = _hip.load_nlpar()
+    declared_nl = _declared("ebsd_nlpar.h")
+    assert set(declared_nl) == set(_hip.NLPAR_SYMBOLS)
+    for name in declared_nl:
+        assert hasattr(nl, name), name
+    # every header's declarations are covered by one of the two libraries
+    assert set(_declared()) == set(declared) | set(declared_nl)
+
+
+def test_nlpar_error_paths_without_device_compute():
+    from pyebsdindex_amd import _hip
+    nl = _hip.load_nlpar()
+    assert nl.ebsd_nlpar_sigma(None, 0, 3, 3, 16, None, 16, 1, 1.0, None, None, None, None) < 0
+    assert b"null" in nl.ebsd_nlpar_last_error()
+    # window larger than the chunk
+    assert nl.ebsd_nlpar(None, 0, 5, 5, 16, None, 16, None, 3, 0.7, 0.0, 0.0, 1.0, 0, 0, 5, 5, None, 0,
+                         None, None) < 0
+    assert b"2*searchradius+1" in nl.ebsd_nlpar_last_error()
+    assert nl.ebsd_nlpar_max(None, 0, 1, 16, None, None) < 0
 
 
 def test_error_paths_without_device_compute():

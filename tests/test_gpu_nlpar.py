@@ -1,0 +1,155 @@
+"""NLPAR on the device (libebsd_nlpar.so through its C ABI) against the oracle's C
+restatement of the reference's Numba kernels (nlpar_cpu.py:753-935) on the same seeded
+scans, plus the closed forms the oracle is pinned by, and the file-level NLPAR class.
+
+Tolerances: the device sums each squared distance in float32 partials added in float64,
+the reference sequentially in float32, so the distances differ by float32 rounding; the
+normalised distance subtracts two nearly equal terms (d^2 and n (s_a^2 + s_b^2)), which
+magnifies that rounding to about 1e-4 absolute.  Weights are compared to 2e-5 absolute,
+averaged patterns to 1e-4 relative (+1e-3 absolute), integer outputs to 1 count on at
+most 0.1% of the pixels (truncation of values that differ in the last bits)."""
+import numpy as np
+import pytest
+
+from oracle import nlpar_ref as nr
+
+pytestmark = pytest.mark.gpu
+
+
+def _grain_scan(nrows, ncols, shape, dtype, seed, grains=4):
+    """Scan of ``grains`` orientations in vertical stripes; every point is its grain's
+    noise-free pattern with its own Poisson noise."""
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    clean = kikuchi_patterns(grains, shape, "float32", seed=seed,
+                             background=2000.0 if dtype == "uint16" else 120.0,
+                             amplitude=(300.0, 900.0) if dtype == "uint16" else (20.0, 60.0)).numpy()
+    rng = np.random.default_rng(seed)
+    gid = (np.arange(ncols)[None, :] * grains // ncols).repeat(nrows, 0).reshape(-1)
+    pats = rng.poisson(np.maximum(clean[gid], 0)).astype(np.float64)
+    if dtype == "float32":
+        return pats.astype(np.float32)
+    top = 255 if dtype == "uint8" else 65535
+    return pats.clip(0, top).astype(dtype)
+
+
+def _dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.mark.parametrize("shape,dtype,nrows,ncols", [((60, 60), "uint8", 9, 11), ((40, 48), "uint16", 8, 7),
+                                                     ((30, 30), "float32", 7, 9)])
+def test_sigma_matches_oracle(shape, dtype, nrows, ncols):
+    from pyebsdindex_amd.nlpar import makeautomask, sigma_device
+    print(f"nlpar sigma {shape} {dtype} {nrows}x{ncols}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=21)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    s, d2, n2 = sigma_device(_dev(pats), nrows, ncols, idx, nn=1, return_d2=True)
+    rs, rd2, rn2 = nr.sigma(pats.reshape(nrows * ncols, -1).astype(np.float32), nrows, ncols, idx, nn=1)
+    assert np.allclose(s.cpu().numpy(), rs, rtol=1e-5)
+    assert np.array_equal(n2.cpu().numpy(), rn2)
+    assert np.allclose(d2.cpu().numpy(), rd2, rtol=1e-4, atol=2e-4)
+
+
+@pytest.mark.parametrize("shape,dtype,nrows,ncols,sr,dthresh", [
+    ((60, 60), "uint8", 9, 11, 3, 0.0), ((40, 48), "uint16", 8, 7, 2, 0.2), ((30, 30), "float32", 7, 9, 3, 0.0),
+    ((33, 35), "uint8", 7, 8, 3, 0.0)])   # odd pattern size: no 4-pixel vector loads
+def test_nlpar_matches_oracle(shape, dtype, nrows, ncols, sr, dthresh):
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device
+    print(f"nlpar {shape} {dtype} {nrows}x{ncols} sr={sr}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=22)
+    flat = pats.reshape(nrows * ncols, -1).astype(np.float32)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    rs = nr.sigma(flat, nrows, ncols, idx)[0]
+    want, wref = nr.nlpar(flat, nrows, ncols, idx, rs, lam=0.7, sr=sr, dthresh=dthresh, return_weights=True)
+    got, w = nlpar_device(_dev(pats), nrows, ncols, idx, _dev(rs), lam=0.7, sr=sr, dthresh=dthresh,
+                          return_weights=True)
+    assert np.allclose(w.cpu().numpy(), wref, atol=2e-5)
+    assert np.allclose(got.cpu().numpy().reshape(want.shape), want, rtol=1e-4, atol=1e-3)
+    # integer output as the reference writes it: clip, then truncate
+    if dtype != "float32":
+        import torch
+        odt = {"uint8": torch.uint8, "uint16": torch.uint16}[dtype]
+        gi = nlpar_device(_dev(pats), nrows, ncols, idx, _dev(rs), lam=0.7, sr=sr, dthresh=dthresh,
+                          out_dtype=odt).cpu().numpy().reshape(want.shape).astype(np.int64)
+        top = np.iinfo(dtype).max
+        wi = want.clip(0, top).astype(dtype).astype(np.int64)
+        diff = np.abs(gi - wi)
+        assert diff.max() <= 1 and (diff > 0).mean() <= 1e-3
+
+
+def test_closed_forms_on_device():
+    """lambda -> infinity: the shifted-window mean; lambda -> 0 with small sigmas: the
+    pattern itself."""
+    from pyebsdindex_amd.nlpar import nlpar_device
+    nrows, ncols, H, W, sr = 7, 9, 8, 8, 3
+    rng = np.random.default_rng(23)
+    data = (100 + 50 * rng.standard_normal((nrows * ncols, H, W))).astype(np.float32)
+    idx = np.arange(H * W)
+    sig = np.full((nrows, ncols), 0.05, np.float32)
+    same = nlpar_device(_dev(data), nrows, ncols, idx, _dev(sig), lam=1e-3, sr=sr).cpu().numpy()
+    assert np.array_equal(same, data)
+    mean = nlpar_device(_dev(data), nrows, ncols, idx, _dev(sig), lam=1e6, sr=sr).cpu().numpy()
+    cube = data.reshape(nrows, ncols, -1).astype(np.float64)
+    ws = 2 * sr + 1
+    for j in range(nrows):
+        for i in range(ncols):
+            y0, x0 = int(np.clip(j - sr, 0, nrows - ws)), int(np.clip(i - sr, 0, ncols - ws))
+            want = cube[y0:y0 + ws, x0:x0 + ws].reshape(-1, H * W).mean(axis=0)
+            assert np.allclose(mean[j * ncols + i].ravel(), want, rtol=2e-6)
+
+
+def test_calclim_region_is_bit_identical():
+    """A sub-region (the reference's calclim) gives bit for bit the points of the full run."""
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, sigma_device
+    nrows, ncols, shape = 10, 12, (40, 40)
+    pats = _grain_scan(nrows, ncols, shape, "uint8", seed=24)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    x = _dev(pats)
+    s = sigma_device(x, nrows, ncols, idx)
+    full = nlpar_device(x, nrows, ncols, idx, s).cpu().numpy().reshape(nrows, ncols, -1)
+    part = nlpar_device(x, nrows, ncols, idx, s, calclim=(3, 2, 5, 6)).cpu().numpy().reshape(6, 5, -1)
+    assert np.array_equal(part, full[2:8, 3:8])
+
+
+def test_nlpar_file_round_trip(tmp_path):
+    """NLPAR.calcnlpar on a UP1 file equals the oracle on the whole scan (clipped to uint8),
+    and a run in row bands equals the one-band run bit for bit; opt_lambda matches the same
+    optimisation on the oracle's distances."""
+    import scipy.optimize as opt
+    from pyebsdindex_amd.ebsd_pattern import UPFile, get_pattern_file_obj
+    from pyebsdindex_amd.nlpar import NLPAR, makeautomask
+    nrows, ncols, shape = 12, 10, (50, 50)
+    print("nlpar file round trip", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, "uint8", seed=25)
+    fn = tmp_path / "scan.up1"
+    w = UPFile(fn)
+    w.patternW, w.patternH, w.nCols, w.nRows, w.xStep, w.yStep = shape[1], shape[0], ncols, nrows, 1.0, 1.0
+    w.write_header()
+    w.pat_writer(pats, 0, nrows * ncols, np.uint8)
+
+    nl = NLPAR(str(fn), lam=0.8, searchradius=3)
+    out1 = nl.calcnlpar(fileout=str(tmp_path / "one.up1"))
+    nl2 = NLPAR(str(fn), lam=0.8, searchradius=3, band_bytes=4 * ncols * 2500)   # bands of a few rows
+    out2 = nl2.calcnlpar(fileout=str(tmp_path / "bands.up1"))
+    a = get_pattern_file_obj(out1).pat_reader(0, nrows * ncols)[0]
+    b = get_pattern_file_obj(out2).pat_reader(0, nrows * ncols)[0]
+    assert np.array_equal(a, b)
+    assert np.array_equal(nl.sigma, nl2.sigma)
+
+    flat = pats.reshape(nrows * ncols, -1).astype(np.float32)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    rs, rd2, _ = nr.sigma(flat, nrows, ncols, idx)
+    assert np.allclose(nl.sigma, rs, rtol=1e-5)
+    want = nr.nlpar(flat, nrows, ncols, idx, rs, lam=0.8, sr=3).clip(0, 255).astype(np.uint8)
+    diff = np.abs(a.reshape(want.shape).astype(int) - want.astype(int))
+    assert diff.max() <= 1 and (diff > 0).mean() <= 1e-3
+
+    lams = NLPAR(str(fn)).opt_lambda()
+
+    def loptfunc(lam, d2, tw, dthresh):
+        dw = np.exp(-np.maximum(d2, dthresh) / lam ** 2)
+        return np.mean(np.abs(tw - 1.0 / (np.sum(dw, axis=2) + 1e-12)))
+    ref = [opt.minimize(loptfunc, 1.0, args=(rd2, tw, np.float32(0.0)), method='Nelder-Mead',
+                        bounds=[[0.001, 10.0]], options={'fatol': 0.0001})['x'][0] for tw in (0.5, 0.34, 0.25)]
+    assert np.allclose(lams, ref, atol=2e-3)
