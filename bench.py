@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-size batched measurement")
     ap.add_argument("--no-streaming", action="store_true", help="skip the pinned-host streaming measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation measurement")
+    ap.add_argument("--no-nlpar", action="store_true", help="skip the NLPAR measurement")
     ap.add_argument("--stream-patterns", type=int, default=1048576,
                     help="patterns streamed in the scan-scale measurement")
     ap.add_argument("--roofline-reps", type=int, default=20)
@@ -172,6 +173,63 @@ def file_streaming_measure(bd, host, npats=16384, batch=2048, threads=16):
             "file_read_GBps": round(npats * bytes_per / t_read / 1e9, 2),
             "file_read_patterns_per_s": round(npats / t_read, 1),
             "valid_band_fraction": round(valid, 4)}
+
+
+VALU_F32_PEAK_TFLOPS = 157.3            # MI355X published fp32 vector peak
+
+
+def nlpar_measure(dev, nrows=256, ncols=256, shape=(60, 60), sr=3, reps=5, cpu_seconds=8.0, seed=17):
+    """NLPAR (non-local pattern averaging) of a 256x256 scan of 60x60 uint8 patterns (16
+    grains, Poisson noise), search radius 3, automask: the sigma and averaging kernels on
+    the device-resident scan, timed with HIP events; the oracle's C restatement of the
+    reference's Numba kernels timed on a 32x32 sub-scan on the host cores."""
+    import torch
+    from oracle import nlpar_ref
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, pattern_max, sigma_device
+    from pyebsdindex_amd.synthetic import grain_scan
+    pats = grain_scan(nrows, ncols, shape, "uint8", seed=seed, grains=16, device=dev)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    mx = pattern_max(pats)
+    sig = sigma_device(pats, nrows, ncols, idx, mxval=mx)
+    out = nlpar_device(pats, nrows, ncols, idx, sig, sr=sr, mxval=mx, out_dtype=torch.uint8)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_sig = t_avg = 0.0
+    for _ in range(reps):
+        ev[0].record()
+        sigma_device(pats, nrows, ncols, idx, mxval=mx)
+        ev[1].record()
+        nlpar_device(pats, nrows, ncols, idx, sig, sr=sr, mxval=mx, out_dtype=torch.uint8)
+        ev[2].record()
+        torch.cuda.synchronize()
+        t_sig += ev[0].elapsed_time(ev[1]) / reps
+        t_avg += ev[1].elapsed_time(ev[2]) / reps
+    n = nrows * ncols
+    M, P, win = idx.size, shape[0] * shape[1], (2 * sr + 1) ** 2
+    flops = (win - 1) * M * 3 + win * P * 2          # distances (sub, mul, add) + weighted sum (mul, add)
+    tflops = n * flops / (t_avg * 1e-3) / 1e12
+    # CPU: the restated Numba kernels (OpenMP over all host cores) on a 32x32 sub-scan
+    sub = pats.reshape(nrows, ncols, -1)[:32, :32].reshape(32 * 32, -1).float().cpu().numpy()
+    nlpar_ref.nlpar(sub, 32, 32, idx, nlpar_ref.sigma(sub, 32, 32, idx)[0], sr=sr)   # warm-up
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < cpu_seconds:
+        s = nlpar_ref.sigma(sub, 32, 32, idx)[0]
+        nlpar_ref.nlpar(sub, 32, 32, idx, s, sr=sr)
+        done += 32 * 32
+    cpu = None
+    if done:
+        cpu = {"value": round(done / (time.perf_counter() - t0), 1), "unit": "patterns/s", "cores": os.cpu_count(),
+               "kind": "port", "sample": f"{done} patterns: the 32x32 corner of the same scan, sigma + NLPAR, "
+                                         f"OpenMP over rows/columns as numba.prange"}
+    return {"workload": f"NLPAR of a {nrows}x{ncols} scan of {shape[0]}x{shape[1]} uint8 patterns (16 grains, "
+                        f"Poisson noise), search radius {sr}, lambda 0.7, automask ({M} pixels), uint8 output",
+            "patterns_per_s": round(n / ((t_sig + t_avg) * 1e-3), 1),
+            "kernels_ms": {"k_nlpar_sigma": round(t_sig, 4), "k_nlpar": round(t_avg, 4)},
+            "roofline": {"bound": "valu", "kernel": "k_nlpar", "achieved": round(tflops, 2),
+                         "peak": VALU_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / VALU_F32_PEAK_TFLOPS, 4),
+                         "algorithmic_flops_per_pattern": flops},
+            "cpu_baseline": cpu,
+            "output_checksum": int(out.to(torch.int64).sum().item())}
 
 
 def differentiation_measure(dev, n=256, reps=5, seed=13):
@@ -441,6 +499,14 @@ def main():
         print(f"[bench] forward+backward {diff['forward_backward_patterns_per_s']:.1f} patterns/s, PC error "
               f"{diff['pc_error_start']:.4f} -> {diff['pc_error_refined']:.5f}", file=sys.stderr, flush=True)
 
+    nlpar = None
+    if not args.no_nlpar:
+        print("[bench] NLPAR workload", file=sys.stderr, flush=True)
+        nlpar = nlpar_measure(dev, cpu_seconds=0.0 if args.no_cpu else 8.0)
+        print(f"[bench] NLPAR {nlpar['patterns_per_s']:.1f} patterns/s ({nlpar['kernels_ms']}), "
+              f"{nlpar['roofline']['achieved']} TFLOP/s; CPU {(nlpar['cpu_baseline'] or {}).get('value')} patterns/s",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle restatement of the reference CPU path)", file=sys.stderr, flush=True)
@@ -472,6 +538,7 @@ def main():
         "batched": batched,
         "streaming": streaming,
         "differentiation": diff,
+        "nlpar": nlpar,
     }
     print(json.dumps(line), flush=True)
 

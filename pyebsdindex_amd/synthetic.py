@@ -139,3 +139,25 @@ def kikuchi_patterns(n, shape=(60, 60), dtype="uint8", seed=0, family="FCC", pc=
         return out, {"quat": quats, "normals": normals, "g_det": g_det, "pc": pcs, "amp": amp,
                      "halfwidth": hw}
     return out
+
+
+def grain_scan(nrows, ncols, shape=(60, 60), dtype="uint8", seed=0, grains=4, family="FCC", device="cpu"):
+    """A scan (nrows*ncols, H, W) of ``grains`` orientations in vertical stripes: every
+    scan point is its grain's noise-free pattern with its own Poisson noise (for integer
+    types), the input NLPAR averages over."""
+    import torch
+    dt = {"uint8": torch.uint8, "uint16": torch.uint16, "float32": torch.float32}[str(dtype)]
+    clean = kikuchi_patterns(grains, shape, "float32", seed=seed, family=family, device=device,
+                             background=2000.0 if dt == torch.uint16 else 120.0,
+                             amplitude=(300.0, 900.0) if dt == torch.uint16 else (20.0, 60.0))
+    gid = (torch.arange(ncols, device=device) * grains // ncols).repeat(nrows)
+    gen = torch.Generator(device=torch.device(device))
+    gen.manual_seed(int(seed) + 104729)
+    out = torch.empty((nrows * ncols,) + tuple(shape), dtype=dt, device=device)
+    step = 4096
+    for s0 in range(0, nrows * ncols, step):
+        x = clean[gid[s0:s0 + step]].clamp(min=0).double()
+        if dt != torch.float32:
+            x = torch.poisson(x, generator=gen).clamp(0, 255 if dt == torch.uint8 else 65535)
+        out[s0:s0 + step] = x.to(dt)
+    return out
