@@ -33,6 +33,13 @@ enum { EBSD_NL_U8 = 0, EBSD_NL_U16 = 1, EBSD_NL_F32 = 2 };
  * Replaces the np.max(data) of each chunk (nlpar_cpu.py:764, :869). */
 int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t P, float *d_max, void *stream);
 
+/* Bytes of the optional workspace of ebsd_nlpar_sigma / ebsd_nlpar for a chunk of n
+ * patterns of P pixels.  With uint8 patterns (P a multiple of 4) and this workspace the
+ * kernels compute the distances of unsaturated pattern pairs as exact integer dot
+ * products; without it (NULL), or for other types, pixel by pixel.  No counterpart in the
+ * reference (its Numba kernels allocate nothing on a device). */
+size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P);
+
 /* Noise estimate (nlpar_cpu.py:753-819) for every point of an nr x nc chunk, neighbours
  * within +-nn rows/columns clipped to the chunk.  A pixel pair counts when both values
  * are below `sat` (the reference's mxval, compared in float32: pass the smallest float32
@@ -46,7 +53,7 @@ int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t P, float *d
  *                             entries beyond a clipped window are 0. */
 int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, int64_t P, const int32_t *d_idx,
                      int M, int nn, float sat, float *d_sigma, float *d_dout, float *d_nout,
-                     void *stream);
+                     void *d_work, size_t work_bytes, void *stream);
 
 /* The averaged patterns (nlpar_cpu.py:821-935) of the points in rows [r0, r0+nrc) and
  * columns [c0, c0+ncc) of an nr x nc chunk (the reference's calclim), over a
@@ -56,11 +63,12 @@ int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, int64_t P, c
  * on the centre, divided by their sum.  d_out receives (nrc*ncc, P) values of out_dtype:
  * float32, or uint8/uint16 clipped to the type's range and truncated as the reference's
  * pat_flt2int(method='clip') (ebsd_pattern.py:159-162).  d_weights, optional
- * (nrc*ncc*(2sr+1)^2 floats or NULL), receives the normalised weights. */
+ * (nrc*ncc*(2sr+1)^2 floats or NULL), receives the normalised weights.  d_work /
+ * work_bytes: the optional workspace of ebsd_nlpar_workspace_bytes(nr*nc, P). */
 int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t P, const int32_t *d_idx, int M,
                const float *d_sigma, int sr, float lam, float dthresh, float diff_offset, float sat,
                int r0, int c0, int nrc, int ncc, void *d_out, int out_dtype, float *d_weights,
-               void *stream);
+               void *d_work, size_t work_bytes, void *stream);
 
 const char *ebsd_nlpar_last_error(void);
 

@@ -24,7 +24,8 @@ SYMBOLS = (
 )
 
 #: exported symbols of the NLPAR library, in declaration order of include/ebsd_nlpar.h
-NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_sigma", "ebsd_nlpar", "ebsd_nlpar_last_error")
+NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_workspace_bytes", "ebsd_nlpar_sigma", "ebsd_nlpar",
+                 "ebsd_nlpar_last_error")
 
 
 class HipLibraryError(RuntimeError):
@@ -118,8 +119,10 @@ def load_nlpar(path: str = NLPAR_LIB_PATH):
             raise HipLibraryError(f"failed to load {path}: {e}") from e
         P, I, I64, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_float
         L.ebsd_nlpar_max.argtypes = [P, I, I64, I64, P, P]
-        L.ebsd_nlpar_sigma.argtypes = [P, I, I, I, I64, P, I, I, F, P, P, P, P]
-        L.ebsd_nlpar.argtypes = [P, I, I, I, I64, P, I, P, I, F, F, F, F, I, I, I, I, P, I, P, P]
+        L.ebsd_nlpar_workspace_bytes.argtypes = [I64, I64]
+        L.ebsd_nlpar_workspace_bytes.restype = ctypes.c_size_t
+        L.ebsd_nlpar_sigma.argtypes = [P, I, I, I, I64, P, I, I, F, P, P, P, P, ctypes.c_size_t, P]
+        L.ebsd_nlpar.argtypes = [P, I, I, I, I64, P, I, P, I, F, F, F, F, I, I, I, I, P, I, P, P, ctypes.c_size_t, P]
         for name in ("ebsd_nlpar_max", "ebsd_nlpar_sigma", "ebsd_nlpar"):
             getattr(L, name).restype = ctypes.c_int
         L.ebsd_nlpar_last_error.argtypes = []

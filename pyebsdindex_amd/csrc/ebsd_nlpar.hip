@@ -9,14 +9,21 @@
 //                  both sigmas, exponential weights, and the weighted average of the window's
 //                  patterns, in one work-group: the weights never leave LDS.
 //
-// One work-group of 256 threads handles one scan point.  Distances run over the mask
-// pixels `idx` in groups of NG = 8 window entries: each thread walks its share of the mask
-// pixels, reads the centre value once per pixel and the 8 neighbours' values, and keeps 8
-// float32 partial sums and 8 pair counts; the partials are then summed over the work-group
-// in float64 (the sums are order-independent up to rounding; the reference sums
-// sequentially in float32).  The average reads the window's patterns with one vector load
-// per thread per pattern (4 pixels) and accumulates in float32 in window order, product
-// then sum, as the reference does.
+// One work-group of 256 threads handles one scan point.  Its four wavefronts take turns over
+// the window's neighbours, a few at a time (4 in k_nlpar, 2 in k_nlpar_sigma), and each
+// sums its neighbours' squared distances over all mask pixels by itself, so only a
+// wavefront reduction follows.  Per-pixel path: the lanes walk the mask pixels `idx`, read
+// the centre value once per pixel and each neighbour's, and sum the squared float32
+// differences in float64 (the reference sums sequentially in float32).  uint8 path (with a
+// workspace): a prepass stores each pattern's masked sum of squares and whether it has a
+// masked pixel at or above the saturation threshold; for a pair of unsaturated patterns
+// d2 = sum a^2 + sum b^2 - 2 sum ab exactly, with sum ab from v_dot4_u32_u8 over 4 pixels of
+// the byte-masked centre and the neighbour per instruction.  Pairs that touch a saturated
+// pattern take the per-pixel path.  Weights are then computed one window entry per thread;
+// one thread adds them in window order (float32, as the reference).  The average reads the
+// window's patterns with 16-byte vector loads (16 pixels per thread) and accumulates
+// in float32 in window order with fused multiply-adds (the reference rounds product and
+// sum separately).
 //
 // Work-groups are numbered so that the 8 XCDs each take a contiguous run of scan points
 // (work-group b runs on XCD b mod 8): neighbouring points, whose windows share patterns,
@@ -51,7 +58,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int NG = 8;          // window entries per distance pass
 constexpr int kMaxWin = 441;   // (2*10+1)^2: search radius up to 10
 
 template <typename T> __device__ __forceinline__ float ldf(const T *p, int64_t i) { return (float)p[i]; }
@@ -63,77 +69,140 @@ __device__ __forceinline__ int64_t xcd_point(int64_t b, int64_t npts)
     return (b % 8) * per + b / 8;
 }
 
-// Sum 8 float partials and 8 int counts over the work-group; thread 0 of the work-group
-// gets the totals in d2[], n2[] (float64 / int).
-__device__ __forceinline__ void block_sum8(const float (&part)[NG], const int (&cnt)[NG], double *red_d,
-                                           int *red_n, double (&d2)[NG], int (&n2)[NG])
+template <typename V> __device__ __forceinline__ V wave_sum(V x)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double dv[NG];
-    int nv[NG];
 #pragma unroll
-    for (int k = 0; k < NG; ++k) {
-        dv[k] = (double)part[k];
-        nv[k] = cnt[k];
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            dv[k] += __shfl_xor(dv[k], off, 64);
-            nv[k] += __shfl_xor(nv[k], off, 64);
-        }
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            red_d[wave * NG + k] = dv[k];
-            red_n[wave * NG + k] = nv[k];
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            double s = 0.0;
-            int c = 0;
-            for (int w = 0; w < kWaves; ++w) {
-                s += red_d[w * NG + k];
-                c += red_n[w * NG + k];
-            }
-            d2[k] = s;
-            n2[k] = c;
-        }
-    }
-    __syncthreads();
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
 }
 
-// Partial sums of one group of NG neighbours q[k] against the centre point p over this
-// thread's mask pixels; a pair counts when both values are below sat.
-template <typename T>
-__device__ __forceinline__ void pair_partials(const T *__restrict__ pats, int64_t P, const int32_t *__restrict__ idx,
-                                              int M, int64_t p, const int64_t (&q)[NG], float sat, float (&part)[NG],
-                                              int (&cnt)[NG])
+// uint8 workspace: byte mask (0xff on mask pixels), per-pattern masked sum of squares and
+// saturation flag.  Null mask4 = no workspace (per-pixel path only).
+struct U8Work {
+    const uint32_t *mask4; // P/4 words of the byte mask
+    const double *sq;
+    const uint8_t *flag;
+};
+
+// Squared distances of pattern p to the NGW patterns q[k], summed over one wavefront's 64
+// lanes; every lane gets d2[k] (float64) and the pair count n2[k].  The dot-product path is
+// taken when T is uint8, the workspace is there and no pattern of the group is saturated
+// (the same choice in every lane: they read the same flags).
+template <int NGW, typename T>
+__device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64_t P, const int32_t *__restrict__ idx,
+                                               int M, int64_t p, const int64_t (&q)[NGW], float sat, U8Work wk,
+                                               double (&d2)[NGW], int (&n2)[NGW])
 {
+    const int lane = threadIdx.x & 63;
+    if constexpr (sizeof(T) == 1) {
+        bool fast = wk.mask4 != nullptr && !wk.flag[p];
 #pragma unroll
-    for (int k = 0; k < NG; ++k) {
-        part[k] = 0.f;
+        for (int k = 0; k < NGW; ++k) fast = fast && !wk.flag[q[k]];
+        if (fast) {
+            uint32_t acc[NGW];
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) acc[k] = 0u;
+            const uint32_t *a4 = reinterpret_cast<const uint32_t *>(pats + p * P);
+            const int64_t P4 = P / 4;
+            // unrolled so that several iterations' loads are in flight (latency, not
+            // bandwidth, bounds this loop)
+#pragma unroll 4
+            for (int64_t v = lane; v < P4; v += 64) {
+                const uint32_t am = a4[v] & wk.mask4[v];
+#pragma unroll
+                for (int k = 0; k < NGW; ++k) {
+                    const uint32_t b = reinterpret_cast<const uint32_t *>(pats + q[k] * P)[v];
+                    acc[k] = __builtin_amdgcn_udot4(am, b, acc[k], false);
+                }
+            }
+            // per-lane integer sums below 2^32 (patterns up to 2^22 pixels); their float64
+            // sum over the wave is exact, so d2 = sum a^2 + sum b^2 - 2 sum ab is exact
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) {
+                d2[k] = wk.sq[p] + wk.sq[q[k]] - 2.0 * wave_sum((double)acc[k]);
+                n2[k] = M;
+            }
+            return;
+        }
+    }
+    double part[NGW];
+    int cnt[NGW];
+#pragma unroll
+    for (int k = 0; k < NGW; ++k) {
+        part[k] = 0.0;
         cnt[k] = 0;
     }
     const T *c0 = pats + p * P;
-    for (int m = threadIdx.x; m < M; m += kThreads) {
+#pragma unroll 4
+    for (int m = lane; m < M; m += 64) {
         const int64_t x = idx[m];
         const float v0 = ldf(c0, x);
         const bool ok0 = v0 < sat;
 #pragma unroll
-        for (int k = 0; k < NG; ++k) {
+        for (int k = 0; k < NGW; ++k) {
             const float v1 = ldf(pats + q[k] * P, x);
             const bool ok = ok0 && (v1 < sat);
-            const float d = ok ? v0 - v1 : 0.f;
-            part[k] = fmaf(d, d, part[k]);
+            // the float32 difference, as the reference forms it; its square and the sum in float64
+            const double d = ok ? (double)(v0 - v1) : 0.0;
+            part[k] = fma(d, d, part[k]);
             cnt[k] += ok ? 1 : 0;
         }
     }
+#pragma unroll
+    for (int k = 0; k < NGW; ++k) {
+        d2[k] = wave_sum(part[k]);
+        n2[k] = wave_sum(cnt[k]);
+    }
+}
+
+// uint8 prepass: masked sum of squares (exact, float64) and saturation flag of every
+// pattern, one work-group per pattern
+__global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict__ pats, int64_t n, int64_t P,
+                                                     const int32_t *__restrict__ idx, int M, float sat,
+                                                     double *__restrict__ sq, uint8_t *__restrict__ flag)
+{
+    __shared__ double s_sq[kWaves];
+    __shared__ int s_sat[kWaves];
+    const int64_t p = blockIdx.x;
+    if (p >= n) return;
+    unsigned acc = 0; // at most ceil(M / 256) terms of 255^2 per thread
+    int st = 0;
+    for (int m = threadIdx.x; m < M; m += kThreads) {
+        const unsigned v = pats[p * P + idx[m]];
+        acc += v * v;
+        st |= ((float)v < sat) ? 0 : 1;
+    }
+    double dacc = (double)acc;
+    for (int off = 32; off >= 1; off >>= 1) {
+        dacc += __shfl_xor(dacc, off, 64);
+        st |= __shfl_xor(st, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_sq[threadIdx.x >> 6] = dacc;
+        s_sat[threadIdx.x >> 6] = st;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        int f = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            t += s_sq[w];
+            f |= s_sat[w];
+        }
+        sq[p] = t;
+        flag[p] = (uint8_t)f;
+    }
+}
+
+__global__ void k_mask_zero(int64_t P, uint8_t *__restrict__ mask)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < P; t += (int64_t)gridDim.x * blockDim.x)
+        mask[t] = 0;
+}
+
+__global__ void k_mask_scatter(const int32_t *__restrict__ idx, int M, uint8_t *__restrict__ mask)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < M; t += gridDim.x * blockDim.x) mask[idx[t]] = 0xff;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -142,59 +211,62 @@ __device__ __forceinline__ void pair_partials(const T *__restrict__ pats, int64_
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_nlpar_sigma(const T *__restrict__ pats, int nr, int nc, int64_t P,
                                                           const int32_t *__restrict__ idx, int M, int nn, float sat,
-                                                          float *__restrict__ sigma, float *__restrict__ dout,
-                                                          float *__restrict__ nout)
+                                                          U8Work wk, float *__restrict__ sigma,
+                                                          float *__restrict__ dout, float *__restrict__ nout)
 {
-    __shared__ double red_d[kWaves * NG];
-    __shared__ int red_n[kWaves * NG];
-    __shared__ float s_mind;
+    constexpr int NGW = 2; // 8 neighbours at nn = 1: one pair per wavefront
+    __shared__ float s_cand[kMaxWin];
     const int64_t npts = (int64_t)nr * nc;
     const int64_t p = xcd_point(blockIdx.x, npts);
     if (p >= npts) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = (int)(p / nc), i = (int)(p % nc);
     const int r0 = max(j - nn, 0), r1 = min(j + nn, nr - 1) + 1;
     const int cc0 = max(i - nn, 0), cc1 = min(i + nn, nc - 1) + 1;
     const int wn = cc1 - cc0;
     const int count = (r1 - r0) * wn;
     const int win = (2 * nn + 1) * (2 * nn + 1);
-    if (threadIdx.x == 0) s_mind = 1e24f;
-    for (int g0 = 0; g0 < count; g0 += NG) {
-        int64_t q[NG];
+    const int eself = (j - r0) * wn + (i - cc0);
+    const int ne = count - 1, ngroups = (ne + NGW - 1) / NGW;
+    for (int g = wave; g < ngroups; g += kWaves) {
+        int64_t q[NGW];
+        int es[NGW];
 #pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            const int e = min(g0 + k, count - 1);
+        for (int k = 0; k < NGW; ++k) {
+            const int t = g * NGW + k, tt = min(t, ne - 1);
+            const int e = tt < eself ? tt : tt + 1;
+            es[k] = t < ne ? e : -1;
             q[k] = (int64_t)(r0 + e / wn) * nc + (cc0 + e % wn);
         }
-        float part[NG];
-        int cnt[NG];
-        pair_partials(pats, P, idx, M, p, q, sat, part, cnt);
-        double d2[NG];
-        int n2[NG];
-        block_sum8(part, cnt, red_d, red_n, d2, n2);
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < NG && g0 + k < count; ++k) {
-                const int e = g0 + k;
-                float dd = 0.f, nnf = (float)P;
-                if (q[k] != p) {
-                    // n2 starts at 1e-12 in the reference; float32 d2 / (n2 * 2)
-                    const double n2d = 1.0e-12 + (double)n2[k];
-                    dd = (float)d2[k];
-                    nnf = (float)n2d;
-                    if (dd >= 1.e-3f) {
-                        const float s0 = (float)((double)dd / (double)(float)(n2d * 2.0));
-                        if (s0 < s_mind) s_mind = s0;
-                    }
-                }
+        double d2[NGW];
+        int n2[NGW];
+        wave_pair_sums<NGW>(pats, P, idx, M, p, q, sat, wk, d2, n2);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) {
+                if (es[k] < 0) continue;
+                // n2 starts at 1e-12 in the reference (float64); d2 is float32
+                const double n2d = 1.0e-12 + (double)n2[k];
+                const float dd = (float)d2[k];
+                float cand = 1e24f;
+                if (dd >= 1.e-3f) cand = (float)((double)dd / (double)(float)(n2d * 2.0));
+                s_cand[es[k]] = cand;
                 if (dout) {
-                    dout[p * win + e] = dd;
-                    nout[p * win + e] = nnf;
+                    dout[p * win + es[k]] = dd;
+                    nout[p * win + es[k]] = (float)n2d;
                 }
             }
         }
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        sigma[p] = sqrtf(s_mind);
+        float mind = 1e24f;
+        for (int e = 0; e < count; ++e)
+            if (e != eself && s_cand[e] < mind) mind = s_cand[e];
+        sigma[p] = sqrtf(mind);
         if (dout) {
+            dout[p * win + eself] = 0.f;
+            nout[p * win + eself] = (float)P;
             for (int e = count; e < win; ++e) {
                 dout[p * win + e] = 0.f;
                 nout[p * win + e] = 0.f;
@@ -231,37 +303,42 @@ __global__ void k_nlpar_dnorm(int nr, int nc, int nn, const float *__restrict__ 
 // ---------------------------------------------------------------------------------------
 // NLPAR (nlpar_cpu.py:821-935)
 // ---------------------------------------------------------------------------------------
-template <typename T> struct Vec4;
-template <> struct Vec4<uint8_t> {
-    typedef uint32_t raw;
-    static __device__ __forceinline__ void get(raw r, float (&v)[4])
-    {
-        v[0] = (float)(r & 0xff);
-        v[1] = (float)((r >> 8) & 0xff);
-        v[2] = (float)((r >> 16) & 0xff);
-        v[3] = (float)(r >> 24);
+// 16 consecutive pixels from a 16-byte-aligned address as float32
+template <typename T> __device__ __forceinline__ void load16(const T *__restrict__ src, float (&x)[16])
+{
+    const uint4 *p4 = reinterpret_cast<const uint4 *>(src);
+    if constexpr (sizeof(T) == 1) {
+        const uint4 r = p4[0];
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[4 * k + 0] = (float)(w[k] & 0xff);
+            x[4 * k + 1] = (float)((w[k] >> 8) & 0xff);
+            x[4 * k + 2] = (float)((w[k] >> 16) & 0xff);
+            x[4 * k + 3] = (float)(w[k] >> 24);
+        }
+    } else if constexpr (sizeof(T) == 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint4 r = p4[h];
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                x[8 * h + 2 * k] = (float)(w[k] & 0xffff);
+                x[8 * h + 2 * k + 1] = (float)(w[k] >> 16);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const uint4 r = p4[h];
+            x[4 * h + 0] = __uint_as_float(r.x);
+            x[4 * h + 1] = __uint_as_float(r.y);
+            x[4 * h + 2] = __uint_as_float(r.z);
+            x[4 * h + 3] = __uint_as_float(r.w);
+        }
     }
-};
-template <> struct Vec4<uint16_t> {
-    typedef uint2 raw;
-    static __device__ __forceinline__ void get(raw r, float (&v)[4])
-    {
-        v[0] = (float)(r.x & 0xffff);
-        v[1] = (float)(r.x >> 16);
-        v[2] = (float)(r.y & 0xffff);
-        v[3] = (float)(r.y >> 16);
-    }
-};
-template <> struct Vec4<float> {
-    typedef float4 raw;
-    static __device__ __forceinline__ void get(raw r, float (&v)[4])
-    {
-        v[0] = r.x;
-        v[1] = r.y;
-        v[2] = r.z;
-        v[3] = r.w;
-    }
-};
+}
 
 template <typename O> __device__ __forceinline__ O store_cvt(float x);
 template <> __device__ __forceinline__ float store_cvt<float>(float x) { return x; }
@@ -275,17 +352,45 @@ template <> __device__ __forceinline__ uint16_t store_cvt<uint16_t>(float x)
     return (uint16_t)(int)fminf(fmaxf(x, 0.f), 65535.f);
 }
 
+// 16 output values to a 16-byte-aligned address
+template <typename O> __device__ __forceinline__ void store16(O *__restrict__ dst, const float (&acc)[16])
+{
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    if constexpr (sizeof(O) == 1) {
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            w[k] = (uint32_t)store_cvt<O>(acc[4 * k]) | ((uint32_t)store_cvt<O>(acc[4 * k + 1]) << 8) |
+                   ((uint32_t)store_cvt<O>(acc[4 * k + 2]) << 16) | ((uint32_t)store_cvt<O>(acc[4 * k + 3]) << 24);
+        d4[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    } else if constexpr (sizeof(O) == 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                w[k] = (uint32_t)store_cvt<O>(acc[8 * h + 2 * k]) |
+                       ((uint32_t)store_cvt<O>(acc[8 * h + 2 * k + 1]) << 16);
+            d4[h] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+            d4[h] = make_uint4(__float_as_uint(acc[4 * h]), __float_as_uint(acc[4 * h + 1]),
+                               __float_as_uint(acc[4 * h + 2]), __float_as_uint(acc[4 * h + 3]));
+    }
+}
+
 template <typename T, typename O>
 __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, int nr, int nc, int64_t P,
                                                     const int32_t *__restrict__ idx, int M,
                                                     const float *__restrict__ sigma, int sr, double lam2, float dthresh,
                                                     float diff_offset, float sat, int r0, int c0, int nrc, int ncc,
-                                                    int vec, O *__restrict__ out, float *__restrict__ wout)
+                                                    int vec, U8Work wk, O *__restrict__ out, float *__restrict__ wout)
 {
-    __shared__ double red_d[kWaves * NG];
-    __shared__ int red_n[kWaves * NG];
     __shared__ float s_w[kMaxWin];
     __shared__ int64_t s_q[kMaxWin];
+    __shared__ float s_sum;
     const int64_t npts = (int64_t)nrc * ncc;
     const int64_t o = xcd_point(blockIdx.x, npts);
     if (o >= npts) return;
@@ -299,34 +404,29 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
     __syncthreads();
     const float sig0 = sigma[p];
     const float s0 = sig0 * sig0;
-    // distances of the non-centre entries, NG at a time (the centre is skipped: it gets -1e6)
-    int ecur = 0;
-    while (ecur < win) {
-        int64_t q[NG];
-        int es[NG];
-        int got = 0, e = ecur;
-        while (got < NG && e < win) {
-            if (s_q[e] != p) {
-                es[got] = e;
-                q[got] = s_q[e];
-                ++got;
-            }
-            ++e;
+    // distances of the non-centre entries, NGW per wavefront pass, waves taking turns (the
+    // centre gets d = -1e6 without a distance)
+    constexpr int NGW = 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int eself = (j - wy0) * ws + (i - wx0);
+    const int ne = win - 1, ngroups = (ne + NGW - 1) / NGW;
+    for (int g = wave; g < ngroups; g += kWaves) {
+        int64_t q[NGW];
+        int es[NGW];
+#pragma unroll
+        for (int k = 0; k < NGW; ++k) {
+            const int t = g * NGW + k, tt = min(t, ne - 1);
+            const int e = tt < eself ? tt : tt + 1;
+            es[k] = t < ne ? e : -1;
+            q[k] = s_q[e];
         }
-        ecur = e;
-        if (got == 0) break;
-        for (int k = got; k < NG; ++k) {
-            es[k] = -1;
-            q[k] = q[0];
-        }
-        float part[NG];
-        int cnt[NG];
-        pair_partials(pats, P, idx, M, p, q, sat, part, cnt);
-        double d2[NG];
-        int n2[NG];
-        block_sum8(part, cnt, red_d, red_n, d2, n2);
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < got; ++k) {
+        double d2[NGW];
+        int n2[NGW];
+        wave_pair_sums<NGW>(pats, P, idx, M, p, q, sat, wk, d2, n2);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) {
+                if (es[k] < 0) continue;
                 const float sg1 = sigma[q[k]];
                 const float s1 = sg1 * sg1;
                 float dd = (float)d2[k];
@@ -339,44 +439,49 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float sum = 0.f;
-        for (int e = 0; e < win; ++e) {
-            const bool self = (s_q[e] == p);
-            float w = self ? -1.0e6f : s_w[e];
-            w = fmaxf(w - dthresh, 0.f);
-            w = (float)exp(-1.0 * (double)w * lam2);
-            if (self) w += diff_offset;
-            s_w[e] = w;
-            sum += w;
-        }
-        for (int e = 0; e < win; ++e) {
-            const float w = s_w[e] / sum;
-            s_w[e] = w;
-            if (wout) wout[o * win + e] = w;
-        }
+    // weights, one entry per thread; their float32 sum in window order by one thread
+    for (int e = threadIdx.x; e < win; e += kThreads) {
+        const bool self = (e == eself);
+        float w = self ? -1.0e6f : s_w[e];
+        w = fmaxf(w - dthresh, 0.f);
+        w = (float)exp(-1.0 * (double)w * lam2);
+        if (self) w += diff_offset;
+        s_w[e] = w;
     }
     __syncthreads();
-    // weighted average over all P pixels, 4 per thread per step
-    typedef typename Vec4<T>::raw R;
+    if (threadIdx.x == 0) {
+        float sum = 0.f;
+        for (int e = 0; e < win; ++e) sum += s_w[e];
+        s_sum = sum;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < win; e += kThreads) {
+        const float w = s_w[e] / s_sum;
+        s_w[e] = w;
+        if (wout) wout[o * win + e] = w;
+    }
+    __syncthreads();
+    // weighted average over all P pixels, 16 per thread per step (16-byte loads), window
+    // entries unrolled so that several loads are in flight
     O *dst = out + o * P;
-    const int64_t P4 = vec ? P / 4 : 0;
-    for (int64_t v = threadIdx.x; v < P4; v += kThreads) {
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const int64_t P16 = vec ? P / 16 : 0;
+    for (int64_t v = threadIdx.x; v < P16; v += kThreads) {
+        float acc[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc[t] = 0.f;
+#pragma unroll 4
         for (int e = 0; e < win; ++e) {
-            const R r = reinterpret_cast<const R *>(pats + s_q[e] * P)[v];
-            float x[4];
-            Vec4<T>::get(r, x);
+            float x[16];
+            load16(pats + s_q[e] * P + 16 * v, x);
             const float w = s_w[e];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = __fadd_rn(acc[t], __fmul_rn(x[t], w));
+            for (int t = 0; t < 16; ++t) acc[t] = fmaf(x[t], w, acc[t]);
         }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) dst[4 * v + t] = store_cvt<O>(acc[t]);
+        store16(dst + 16 * v, acc);
     }
-    for (int64_t x = 4 * P4 + threadIdx.x; x < P; x += kThreads) {
+    for (int64_t x = 16 * P16 + threadIdx.x; x < P; x += kThreads) {
         float acc = 0.f;
-        for (int e = 0; e < win; ++e) acc = __fadd_rn(acc, __fmul_rn(ldf(pats + s_q[e] * P, x), s_w[e]));
+        for (int e = 0; e < win; ++e) acc = fmaf(ldf(pats + s_q[e] * P, x), s_w[e], acc);
         dst[x] = store_cvt<O>(acc);
     }
 }
@@ -396,8 +501,21 @@ template <typename T>
 __global__ void k_max(const T *__restrict__ pats, int64_t n, unsigned *__restrict__ key)
 {
     unsigned best = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        best = max(best, fkey(ldf(pats, t)));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t head = 0;
+    if (((uintptr_t)pats % 16) == 0) {
+        // 16 values per thread per step from 16-byte loads, the tail element by element
+        const int64_t n16 = n / 16;
+        for (int64_t t = t0; t < n16; t += stride) {
+            float x[16];
+            load16(pats + 16 * t, x);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) best = max(best, fkey(x[k]));
+        }
+        head = 16 * n16;
+    }
+    for (int64_t t = head + t0; t < n; t += stride) best = max(best, fkey(ldf(pats, t)));
     for (int off = 32; off >= 1; off >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, off, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(key, best);
 }
@@ -444,10 +562,43 @@ extern "C" int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t 
     return 0;
 }
 
-extern "C" int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, int64_t P, const int32_t *d_idx, int M,
-                                int nn, float sat, float *d_sigma, float *d_dout, float *d_nout, void *stream)
+static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+extern "C" size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P)
 {
-    if (nr <= 0 || nc <= 0 || P <= 0 || M < 0 || nn < 1) return fail(-1, "ebsd_nlpar_sigma: bad size");
+    if (n <= 0 || P <= 0) return 0;
+    return al256((size_t)P) + al256((size_t)n * sizeof(double)) + al256((size_t)n);
+}
+
+// Set up the uint8 workspace for n patterns (mask bytes, sums of squares, saturation flags)
+// when it applies: uint8 patterns, P a multiple of 4, 4-byte aligned patterns, a large
+// enough workspace.  Otherwise wk stays empty and the kernels take the general path.
+static int u8_setup(const void *d_pats, int dtype, int64_t n, int64_t P, const int32_t *d_idx, int M, float sat,
+                    void *d_work, size_t work_bytes, hipStream_t s, U8Work *wk)
+{
+    *wk = U8Work{nullptr, nullptr, nullptr};
+    if (dtype != EBSD_NL_U8 || !d_work || P % 4 != 0 || ((uintptr_t)d_pats % 4) != 0) return 0;
+    if (work_bytes < ebsd_nlpar_workspace_bytes(n, P) || ((uintptr_t)d_work % 256) != 0) return 0;
+    if (n > INT32_MAX) return 0;
+    uint8_t *base = (uint8_t *)d_work;
+    uint8_t *mask = base;
+    double *sq = (double *)(base + al256((size_t)P));
+    uint8_t *flag = base + al256((size_t)P) + al256((size_t)n * sizeof(double));
+    hipLaunchKernelGGL(k_mask_zero, dim3(grid_for(P)), dim3(256), 0, s, P, mask);
+    if (M > 0) hipLaunchKernelGGL(k_mask_scatter, dim3(grid_for(M)), dim3(256), 0, s, d_idx, M, mask);
+    hipLaunchKernelGGL(k_u8_prep, dim3((unsigned)n), dim3(kThreads), 0, s, (const uint8_t *)d_pats, n, P, d_idx, M,
+                       sat, sq, flag);
+    HIPCHK(hipGetLastError());
+    *wk = U8Work{(const uint32_t *)mask, sq, flag};
+    return 0;
+}
+
+extern "C" int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, int64_t P, const int32_t *d_idx, int M,
+                                int nn, float sat, float *d_sigma, float *d_dout, float *d_nout, void *d_work,
+                                size_t work_bytes, void *stream)
+{
+    if (nr <= 0 || nc <= 0 || P <= 0 || M < 0) return fail(-1, "ebsd_nlpar_sigma: bad size");
+    if (nn < 1 || (2 * nn + 1) * (2 * nn + 1) > kMaxWin) return fail(-1, "ebsd_nlpar_sigma: nn must be 1..10");
     if (M > P) return fail(-1, "ebsd_nlpar_sigma: more mask pixels than pattern pixels");
     if (!d_pats || !d_idx || !d_sigma) return fail(-1, "ebsd_nlpar_sigma: null pointer");
     if ((d_dout == nullptr) != (d_nout == nullptr)) return fail(-1, "ebsd_nlpar_sigma: d_dout and d_nout go together");
@@ -456,9 +607,11 @@ extern "C" int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, i
     const int64_t npts = (int64_t)nr * nc;
     const int64_t grid = ((npts + 7) / 8) * 8;
     if (grid > INT32_MAX) return fail(-1, "ebsd_nlpar_sigma: chunk too large");
+    U8Work wk;
+    if (int rc = u8_setup(d_pats, dtype, npts, P, d_idx, M, sat, d_work, work_bytes, s, &wk)) return rc;
 #define EBSD_NL_SIG(T)                                                                                     \
     hipLaunchKernelGGL(k_nlpar_sigma<T>, dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, nc, P, \
-                       d_idx, M, nn, sat, d_sigma, d_dout, d_nout)
+                       d_idx, M, nn, sat, wk, d_sigma, d_dout, d_nout)
     if (dtype == EBSD_NL_U8)
         EBSD_NL_SIG(uint8_t);
     else if (dtype == EBSD_NL_U16)
@@ -477,7 +630,8 @@ extern "C" int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, i
 
 extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t P, const int32_t *d_idx, int M,
                           const float *d_sigma, int sr, float lam, float dthresh, float diff_offset, float sat, int r0,
-                          int c0, int nrc, int ncc, void *d_out, int out_dtype, float *d_weights, void *stream)
+                          int c0, int nrc, int ncc, void *d_out, int out_dtype, float *d_weights, void *d_work,
+                          size_t work_bytes, void *stream)
 {
     if (nr <= 0 || nc <= 0 || P <= 0 || M < 0) return fail(-1, "ebsd_nlpar: bad size");
     if (M > P) return fail(-1, "ebsd_nlpar: more mask pixels than pattern pixels");
@@ -497,11 +651,13 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
     hipStream_t s = (hipStream_t)stream;
     // lam2 = 1.0 / lam**2 (nlpar_cpu.py:844): float32 square, float64 reciprocal
     const double lam2 = 1.0 / (double)(lam * lam);
-    // 4-pixel vector loads when every pattern starts on a 16-byte boundary
-    const int vec = (P % 4 == 0) && ((uintptr_t)d_pats % 16 == 0);
+    // 16-pixel vector loads and stores when every pattern starts on a 16-byte boundary
+    const int vec = (P % 16 == 0) && ((uintptr_t)d_pats % 16 == 0) && ((uintptr_t)d_out % 16 == 0);
+    U8Work wk;
+    if (int rc = u8_setup(d_pats, dtype, (int64_t)nr * nc, P, d_idx, M, sat, d_work, work_bytes, s, &wk)) return rc;
 #define EBSD_NL_RUN(T, O)                                                                                   \
     hipLaunchKernelGGL((k_nlpar<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, nc, P, \
-                       d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, vec, (O *)d_out, \
+                       d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, vec, wk, (O *)d_out, \
                        d_weights)
 #define EBSD_NL_OUT(T)                 \
     if (out_dtype == EBSD_NL_U8)       \

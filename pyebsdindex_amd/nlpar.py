@@ -57,6 +57,16 @@ def _indices_dev(indices, P, device):
     return torch.as_tensor(idx.astype(np.int32), device=device)
 
 
+def _workspace(x):
+    """(pointer, bytes) of the uint8 workspace for the chunk ``x`` (n, P), or (None, 0)."""
+    torch = _torch()
+    if x.dtype != torch.uint8:
+        return None, 0
+    nbytes = int(_hip.load_nlpar().ebsd_nlpar_workspace_bytes(x.shape[0], x.shape[1]))
+    buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    return buf, nbytes
+
+
 def _stream(stream):
     torch = _torch()
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -104,11 +114,13 @@ def pattern_max(pats, stream=None):
 
 
 def sigma_device(pats, nrows, ncols, indices, nn=1, saturation_protect=True, return_d2=False, mxval=None,
-                 stream=None):
+                 stream=None, fast=True):
     """Noise estimate of every point (nlpar_cpu.py:753-819) for device patterns
     (nrows*ncols, H, W) or (nrows, ncols, H, W).  Returns sigma (nrows, ncols) and, with
     ``return_d2``, the normalised distances and counts (nrows, ncols, (2nn+1)^2), all device
-    float32.  ``mxval``: the saturation reference value (default: the patterns' maximum)."""
+    float32.  ``mxval``: the saturation reference value (default: the patterns' maximum).
+    ``fast``: uint8 distances of unsaturated pairs as exact integer dot products (a
+    workspace of about one pattern plus 9 bytes per pattern); False: pixel by pixel."""
     torch = _torch()
     L = _hip.load_nlpar()
     x = pats.contiguous().reshape(nrows * ncols, -1)
@@ -121,20 +133,22 @@ def sigma_device(pats, nrows, ncols, indices, nn=1, saturation_protect=True, ret
     if return_d2:
         dout = torch.empty((nrows, ncols, win), dtype=torch.float32, device=x.device)
         nout = torch.empty_like(dout)
+    work, wbytes = _workspace(x) if fast else (None, 0)
     _hip.check_nlpar(L.ebsd_nlpar_sigma(_ptr(x), _code(x), nrows, ncols, P, _ptr(idx), idx.numel(), nn,
                                         _sat_sigma(mx, saturation_protect), _ptr(sig),
                                         None if dout is None else _ptr(dout), None if nout is None else _ptr(nout),
-                                        _stream(stream)))
+                                        None if work is None else _ptr(work), wbytes, _stream(stream)))
     return (sig, dout, nout) if return_d2 else sig
 
 
 def nlpar_device(pats, nrows, ncols, indices, sigma, lam=0.7, sr=3, dthresh=0.0, saturation_protect=True,
-                 diff_offset=0.0, calclim=None, out_dtype=None, return_weights=False, mxval=None, stream=None):
+                 diff_offset=0.0, calclim=None, out_dtype=None, return_weights=False, mxval=None, stream=None,
+                 fast=True):
     """Averaged patterns (nlpar_cpu.py:821-935) of the points in ``calclim`` = (cstart,
     rstart, ncolcalc, nrowcalc) (default: all) of device patterns (nrows*ncols, H, W) or
     (nrows, ncols, H, W), as (nrowcalc*ncolcalc, H, W) of ``out_dtype`` (a torch dtype:
     float32 (default), or uint8 / uint16 clipped and truncated as the reference writes
-    them).  ``sigma``: (nrows, ncols) device float32."""
+    them).  ``sigma``: (nrows, ncols) device float32.  ``fast``: as for ``sigma_device``."""
     torch = _torch()
     L = _hip.load_nlpar()
     x = pats.contiguous().reshape(nrows * ncols, -1)
@@ -150,10 +164,11 @@ def nlpar_device(pats, nrows, ncols, indices, sigma, lam=0.7, sr=3, dthresh=0.0,
     wts = torch.empty((nrc * ncc, (2 * sr + 1) ** 2), dtype=torch.float32, device=x.device) \
         if return_weights else None
     mx = pattern_max(x, stream) if mxval is None else float(mxval)
+    work, wbytes = _workspace(x) if fast else (None, 0)
     _hip.check_nlpar(L.ebsd_nlpar(_ptr(x), _code(x), nrows, ncols, P, _ptr(idx), idx.numel(), _ptr(sig), int(sr),
                                   float(lam), float(dthresh), float(diff_offset), _sat_nlpar(mx, saturation_protect),
                                   r0, c0, nrc, ncc, _ptr(out), _code(out), None if wts is None else _ptr(wts),
-                                  _stream(stream)))
+                                  None if work is None else _ptr(work), wbytes, _stream(stream)))
     return (out, wts) if return_weights else out
 
 

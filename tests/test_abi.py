@@ -42,13 +42,14 @@ def test_library_exports_every_declared_symbol():
 def test_nlpar_error_paths_without_device_compute():
     from pyebsdindex_amd import _hip
     nl = _hip.load_nlpar()
-    assert nl.ebsd_nlpar_sigma(None, 0, 3, 3, 16, None, 16, 1, 1.0, None, None, None, None) < 0
+    assert nl.ebsd_nlpar_sigma(None, 0, 3, 3, 16, None, 16, 1, 1.0, None, None, None, None, 0, None) < 0
     assert b"null" in nl.ebsd_nlpar_last_error()
     # window larger than the chunk
     assert nl.ebsd_nlpar(None, 0, 5, 5, 16, None, 16, None, 3, 0.7, 0.0, 0.0, 1.0, 0, 0, 5, 5, None, 0,
-                         None, None) < 0
+                         None, None, 0, None) < 0
     assert b"2*searchradius+1" in nl.ebsd_nlpar_last_error()
     assert nl.ebsd_nlpar_max(None, 0, 1, 16, None, None) < 0
+    assert nl.ebsd_nlpar_workspace_bytes(10, 3600) == 3840 + 256 + 256
 
 
 def test_error_paths_without_device_compute():

@@ -153,3 +153,44 @@ def test_nlpar_file_round_trip(tmp_path):
     ref = [opt.minimize(loptfunc, 1.0, args=(rd2, tw, np.float32(0.0)), method='Nelder-Mead',
                         bounds=[[0.001, 10.0]], options={'fatol': 0.0001})['x'][0] for tw in (0.5, 0.34, 0.25)]
     assert np.allclose(lams, ref, atol=2e-3)
+
+
+def test_saturated_patterns_and_both_paths():
+    """uint8 scans with saturated pixels in some patterns: pairs that touch them take the
+    per-pixel path, the others the exact dot-product path; both match the oracle, and the
+    dot-product path matches the per-pixel one."""
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, sigma_device
+    nrows, ncols, shape = 9, 10, (40, 40)
+    print("nlpar saturated uint8", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, "uint8", seed=26)
+    rng = np.random.default_rng(26)
+    for p in rng.choice(nrows * ncols, 12, replace=False):
+        pats[p].reshape(-1)[rng.choice(shape[0] * shape[1], 30, replace=False)] = 255
+    flat = pats.reshape(nrows * ncols, -1).astype(np.float32)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    rs = nr.sigma(flat, nrows, ncols, idx)[0]
+    want = nr.nlpar(flat, nrows, ncols, idx, rs, lam=0.7, sr=3)
+    x = _dev(pats)
+    for fast in (True, False):
+        s, d2, n2 = sigma_device(x, nrows, ncols, idx, return_d2=True, fast=fast)
+        assert np.allclose(s.cpu().numpy(), rs, rtol=1e-5), fast
+        got = nlpar_device(x, nrows, ncols, idx, _dev(rs), lam=0.7, sr=3, fast=fast).cpu().numpy()
+        assert np.allclose(got.reshape(want.shape), want, rtol=1e-4, atol=1e-3), fast
+
+
+@pytest.mark.parametrize("dtype,n,P", [("uint8", 37, 61), ("uint16", 5, 3600), ("float32", 9, 13)])
+def test_pattern_max(dtype, n, P):
+    """The device maximum equals numpy's, with ragged tails after the 16-value loads and
+    negative float values."""
+    import torch
+    from pyebsdindex_amd.nlpar import pattern_max
+    rng = np.random.default_rng(27)
+    if dtype == "float32":
+        x = (rng.standard_normal((n, P)) * 100 - 300).astype(np.float32)
+    else:
+        x = rng.integers(0, np.iinfo(dtype).max // 3, size=(n, P)).astype(dtype)
+    x.reshape(-1)[-1] = x.max() + (1 if dtype != "float32" else 0.5)   # the maximum in the tail
+    assert pattern_max(_dev(x)) == float(x.max())
+    # an unaligned start (offset by one element) takes the element-by-element path
+    flat = _dev(x.reshape(-1))
+    assert pattern_max(flat[1:].reshape(1, -1)) == float(x.reshape(-1)[1:].max())
