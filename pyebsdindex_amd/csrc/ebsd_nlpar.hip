@@ -15,19 +15,20 @@
 // wavefront reduction follows.  Per-pixel path: the lanes walk the mask pixels `idx`, read
 // the centre value once per pixel and each neighbour's, and sum the squared float32
 // differences in float64 (the reference sums sequentially in float32).  uint8 path (with a
-// workspace): a prepass stores each pattern's masked sum of squares and whether it has a
-// masked pixel at or above the saturation threshold; for a pair of unsaturated patterns
+// workspace): a prepass stores each pattern's masked sum of squares and the list of its
+// masked pixels at or above the saturation threshold (up to 32); then
 // d2 = sum a^2 + sum b^2 - 2 sum ab exactly, with sum ab from v_dot4_u32_u8 over 4 pixels of
-// the byte-masked centre and the neighbour per instruction.  Pairs that touch a saturated
-// pattern take the per-pixel path.  Weights are then computed one window entry per thread;
+// the byte-masked centre and the neighbour per instruction, minus the exact terms of the
+// pixels saturated in either pattern (which the reference leaves out).  Pairs that touch a
+// pattern with more saturated pixels take the per-pixel path.  Weights are then computed one window entry per thread;
 // one thread adds them in window order (float32, as the reference).  The average reads the
 // window's patterns with 16-byte vector loads (16 pixels per thread) and accumulates
 // in float32 in window order with fused multiply-adds (the reference rounds product and
 // sum separately).
 //
-// Work-groups are numbered so that the 8 XCDs each take a contiguous run of scan points
-// (work-group b runs on XCD b mod 8): neighbouring points, whose windows share patterns,
-// then run on the same XCD and find each other's patterns in that XCD's L2.
+// Work-groups are numbered so that the 8 XCDs each take a contiguous run of scan points in
+// an order of 8 x 32-point blocks (work-group b runs on XCD b mod 8): the points an XCD has
+// in flight form a compact patch whose windows share patterns and fit that XCD's L2.
 
 #include <hip/hip_runtime.h>
 
@@ -69,6 +70,29 @@ __device__ __forceinline__ int64_t xcd_point(int64_t b, int64_t npts)
     return (b % 8) * per + b / 8;
 }
 
+// Scan points in blocks of kBR x kBC: point ordinals run block by block (blocks row-major,
+// points row-major in a block), and each XCD takes a contiguous run of ordinals.  The
+// points one XCD has in flight then form a compact patch whose search windows fit its L2
+// (a 60x60 uint8 patch of 8 x 32 points with its 7 x 7 windows: 1.6 MB of the 4 MB).
+constexpr int kBR = 8, kBC = 32;
+
+__device__ __forceinline__ int64_t blocked_items(int nrows, int ncols)
+{
+    return (int64_t)((nrows + kBR - 1) / kBR) * ((ncols + kBC - 1) / kBC) * (kBR * kBC);
+}
+
+// (row, column) of work-group b over an nrows x ncols region; false for padding
+__device__ __forceinline__ bool blocked_point(int64_t b, int nrows, int ncols, int &row, int &col)
+{
+    const int64_t l = xcd_point(b, blocked_items(nrows, ncols));
+    const int nbc = (ncols + kBC - 1) / kBC;
+    const int64_t blk = l / (kBR * kBC);
+    const int w = (int)(l % (kBR * kBC));
+    row = (int)(blk / nbc) * kBR + w / kBC;
+    col = (int)(blk % nbc) * kBC + w % kBC;
+    return row < nrows && col < ncols;
+}
+
 template <typename V> __device__ __forceinline__ V wave_sum(V x)
 {
 #pragma unroll
@@ -76,18 +100,22 @@ template <typename V> __device__ __forceinline__ V wave_sum(V x)
     return x;
 }
 
-// uint8 workspace: byte mask (0xff on mask pixels), per-pattern masked sum of squares and
-// saturation flag.  Null mask4 = no workspace (per-pixel path only).
+// uint8 workspace: byte mask (0xff on mask pixels), per-pattern masked sum of squares, the
+// number of masked pixels at or above the saturation threshold (kNoList when more than
+// kSatList) and the list of those pixels.  Null mask4 = no workspace (per-pixel path only).
+constexpr int kSatList = 32;
+constexpr uint8_t kNoList = 255;
 struct U8Work {
     const uint32_t *mask4; // P/4 words of the byte mask
     const double *sq;
-    const uint8_t *flag;
+    const uint8_t *nsat;
+    const int32_t *list; // kSatList pixel indices per pattern
 };
 
 // Squared distances of pattern p to the NGW patterns q[k], summed over one wavefront's 64
 // lanes; every lane gets d2[k] (float64) and the pair count n2[k].  The dot-product path is
-// taken when T is uint8, the workspace is there and no pattern of the group is saturated
-// (the same choice in every lane: they read the same flags).
+// taken when T is uint8, the workspace is there and no pattern of the group has more than
+// kSatList saturated mask pixels (the same choice in every lane: they read the same counts).
 template <int NGW, typename T>
 __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64_t P, const int32_t *__restrict__ idx,
                                                int M, int64_t p, const int64_t (&q)[NGW], float sat, U8Work wk,
@@ -95,17 +123,16 @@ __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64
 {
     const int lane = threadIdx.x & 63;
     if constexpr (sizeof(T) == 1) {
-        bool fast = wk.mask4 != nullptr && !wk.flag[p];
+        bool fast = wk.mask4 != nullptr && wk.nsat[p] != kNoList;
 #pragma unroll
-        for (int k = 0; k < NGW; ++k) fast = fast && !wk.flag[q[k]];
+        for (int k = 0; k < NGW; ++k) fast = fast && wk.nsat[q[k]] != kNoList;
         if (fast) {
             uint32_t acc[NGW];
 #pragma unroll
             for (int k = 0; k < NGW; ++k) acc[k] = 0u;
             const uint32_t *a4 = reinterpret_cast<const uint32_t *>(pats + p * P);
             const int64_t P4 = P / 4;
-            // unrolled so that several iterations' loads are in flight (latency, not
-            // bandwidth, bounds this loop)
+            // unrolled so that several iterations' loads are in flight
 #pragma unroll 4
             for (int64_t v = lane; v < P4; v += 64) {
                 const uint32_t am = a4[v] & wk.mask4[v];
@@ -115,12 +142,36 @@ __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64
                     acc[k] = __builtin_amdgcn_udot4(am, b, acc[k], false);
                 }
             }
-            // per-lane integer sums below 2^32 (patterns up to 2^22 pixels); their float64
-            // sum over the wave is exact, so d2 = sum a^2 + sum b^2 - 2 sum ab is exact
+            // sum over all mask pixels: sum a^2 + sum b^2 - 2 sum ab; then take out the pixels
+            // saturated in either pattern (one list entry per lane), which the reference does
+            // not count.  Lane value (-2 ab - removed) * 256 + removed count: one exact float64
+            // wave sum carries both (the count is below 2 kSatList < 256).
+            const T *pp = pats + p * P;
+            const int nsp = wk.nsat[p];
 #pragma unroll
             for (int k = 0; k < NGW; ++k) {
-                d2[k] = wk.sq[p] + wk.sq[q[k]] - 2.0 * wave_sum((double)acc[k]);
-                n2[k] = M;
+                const T *pq = pats + q[k] * P;
+                int64_t corr = 0;
+                int cnt = 0;
+                if (lane < nsp) {
+                    const int32_t x = wk.list[p * kSatList + lane];
+                    const int d = (int)pp[x] - (int)pq[x];
+                    corr += d * d;
+                    ++cnt;
+                }
+                if (lane < (int)wk.nsat[q[k]]) {
+                    const int32_t x = wk.list[q[k] * kSatList + lane];
+                    if ((float)pp[x] < sat) { // else already taken out with p's list
+                        const int d = (int)pp[x] - (int)pq[x];
+                        corr += d * d;
+                        ++cnt;
+                    }
+                }
+                const int64_t lv = (-2 * (int64_t)acc[k] - corr) * 256 + cnt;
+                const int64_t X = (int64_t)wave_sum((double)lv);
+                const int64_t c = ((X % 256) + 256) % 256;
+                d2[k] = wk.sq[p] + wk.sq[q[k]] + (double)((X - c) / 256);
+                n2[k] = M - (int)c;
             }
             return;
         }
@@ -155,42 +206,38 @@ __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64
     }
 }
 
-// uint8 prepass: masked sum of squares (exact, float64) and saturation flag of every
-// pattern, one work-group per pattern
+// uint8 prepass: masked sum of squares (exact, float64), saturated mask pixels (count and
+// list) of every pattern, one work-group per pattern
 __global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict__ pats, int64_t n, int64_t P,
                                                      const int32_t *__restrict__ idx, int M, float sat,
-                                                     double *__restrict__ sq, uint8_t *__restrict__ flag)
+                                                     double *__restrict__ sq, uint8_t *__restrict__ nsat,
+                                                     int32_t *__restrict__ list)
 {
     __shared__ double s_sq[kWaves];
-    __shared__ int s_sat[kWaves];
+    __shared__ int s_ns;
     const int64_t p = blockIdx.x;
     if (p >= n) return;
+    if (threadIdx.x == 0) s_ns = 0;
+    __syncthreads();
     unsigned acc = 0; // at most ceil(M / 256) terms of 255^2 per thread
-    int st = 0;
     for (int m = threadIdx.x; m < M; m += kThreads) {
-        const unsigned v = pats[p * P + idx[m]];
+        const int32_t x = idx[m];
+        const unsigned v = pats[p * P + x];
         acc += v * v;
-        st |= ((float)v < sat) ? 0 : 1;
+        if (!((float)v < sat)) {
+            const int slot = atomicAdd(&s_ns, 1);
+            if (slot < kSatList) list[p * kSatList + slot] = x;
+        }
     }
     double dacc = (double)acc;
-    for (int off = 32; off >= 1; off >>= 1) {
-        dacc += __shfl_xor(dacc, off, 64);
-        st |= __shfl_xor(st, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        s_sq[threadIdx.x >> 6] = dacc;
-        s_sat[threadIdx.x >> 6] = st;
-    }
+    for (int off = 32; off >= 1; off >>= 1) dacc += __shfl_xor(dacc, off, 64);
+    if ((threadIdx.x & 63) == 0) s_sq[threadIdx.x >> 6] = dacc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
-        int f = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            t += s_sq[w];
-            f |= s_sat[w];
-        }
+        for (int w = 0; w < kWaves; ++w) t += s_sq[w];
         sq[p] = t;
-        flag[p] = (uint8_t)f;
+        nsat[p] = s_ns > kSatList ? kNoList : (uint8_t)s_ns;
     }
 }
 
@@ -216,11 +263,10 @@ __global__ __launch_bounds__(kThreads) void k_nlpar_sigma(const T *__restrict__ 
 {
     constexpr int NGW = 2; // 8 neighbours at nn = 1: one pair per wavefront
     __shared__ float s_cand[kMaxWin];
-    const int64_t npts = (int64_t)nr * nc;
-    const int64_t p = xcd_point(blockIdx.x, npts);
-    if (p >= npts) return;
+    int j, i;
+    if (!blocked_point(blockIdx.x, nr, nc, j, i)) return;
+    const int64_t p = (int64_t)j * nc + i;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int j = (int)(p / nc), i = (int)(p % nc);
     const int r0 = max(j - nn, 0), r1 = min(j + nn, nr - 1) + 1;
     const int cc0 = max(i - nn, 0), cc1 = min(i + nn, nc - 1) + 1;
     const int wn = cc1 - cc0;
@@ -391,10 +437,10 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
     __shared__ float s_w[kMaxWin];
     __shared__ int64_t s_q[kMaxWin];
     __shared__ float s_sum;
-    const int64_t npts = (int64_t)nrc * ncc;
-    const int64_t o = xcd_point(blockIdx.x, npts);
-    if (o >= npts) return;
-    const int j = r0 + (int)(o / ncc), i = c0 + (int)(o % ncc);
+    int jr, ic;
+    if (!blocked_point(blockIdx.x, nrc, ncc, jr, ic)) return;
+    const int64_t o = (int64_t)jr * ncc + ic;
+    const int j = r0 + jr, i = c0 + ic;
     const int64_t p = (int64_t)j * nc + i;
     const int ws = 2 * sr + 1, win = ws * ws;
     // window shifted inside the chunk (nlpar_cpu.py:872-876)
@@ -486,6 +532,286 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// NLPAR over column runs: one work-group computes TP vertically adjacent scan points of one
+// column.  Their windows share columns and overlap in all but TP-1 rows, so the work-group
+// walks the union of the windows once: each union pattern is read once per distance pass
+// against all TP centres, and once in the average for all TP outputs.  Every point keeps
+// its own window order for the weight sum and the average (entries outside its window are
+// skipped), so the result is bit for bit that of k_nlpar.
+// ---------------------------------------------------------------------------------------
+constexpr int TP = 4;
+constexpr int kMaxU = (21 + TP - 1) * 21; // union of TP windows at search radius 10
+
+// Distance sums of NGW union patterns q[u] against the TP centres c[k] over one wavefront
+// (every lane gets the totals), in the same per-lane order as wave_pair_sums, so each pair's
+// value is identical to it.
+template <int NGW, bool FAST, typename T>
+__device__ __forceinline__ void wave_pair_sums_cols(const T *__restrict__ pats, int64_t P,
+                                                    const int32_t *__restrict__ idx, int M, const int64_t (&c)[TP],
+                                                    const int64_t (&q)[NGW], float sat, U8Work wk,
+                                                    double (&d2)[TP][NGW], int (&n2)[TP][NGW])
+{
+    const int lane = threadIdx.x & 63;
+    if constexpr (FAST && sizeof(T) == 1) {
+        uint32_t acc[TP][NGW];
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) acc[k][u] = 0u;
+        const int64_t P4 = P / 4;
+#pragma unroll 2
+        for (int64_t v = lane; v < P4; v += 64) {
+            const uint32_t mk = wk.mask4[v];
+            uint32_t am[TP];
+#pragma unroll
+            for (int k = 0; k < TP; ++k) am[k] = reinterpret_cast<const uint32_t *>(pats + c[k] * P)[v] & mk;
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) {
+                const uint32_t b = reinterpret_cast<const uint32_t *>(pats + q[u] * P)[v];
+#pragma unroll
+                for (int k = 0; k < TP; ++k) acc[k][u] = __builtin_amdgcn_udot4(am[k], b, acc[k][u], false);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) {
+                d2[k][u] = wk.sq[c[k]] + wk.sq[q[u]] - 2.0 * wave_sum((double)acc[k][u]);
+                n2[k][u] = M;
+            }
+    } else {
+        double part[TP][NGW];
+        int cnt[TP][NGW];
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) {
+                part[k][u] = 0.0;
+                cnt[k][u] = 0;
+            }
+#pragma unroll 2
+        for (int m = lane; m < M; m += 64) {
+            const int64_t x = idx[m];
+            float v0[TP];
+            bool ok0[TP];
+#pragma unroll
+            for (int k = 0; k < TP; ++k) {
+                v0[k] = ldf(pats + c[k] * P, x);
+                ok0[k] = v0[k] < sat;
+            }
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) {
+                const float v1 = ldf(pats + q[u] * P, x);
+                const bool ok1 = v1 < sat;
+#pragma unroll
+                for (int k = 0; k < TP; ++k) {
+                    const bool ok = ok0[k] && ok1;
+                    const double d = ok ? (double)(v0[k] - v1) : 0.0;
+                    part[k][u] = fma(d, d, part[k][u]);
+                    cnt[k][u] += ok ? 1 : 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) {
+                d2[k][u] = wave_sum(part[k][u]);
+                n2[k][u] = wave_sum(cnt[k][u]);
+            }
+    }
+}
+
+struct ColTile {
+    int np;         // valid points of the run (1..TP)
+    int64_t c[TP];  // centre patterns (invalid ones repeat the last valid)
+    int wy0[TP];    // first window row of each point
+    int uy0, nu;    // first union row, union entries
+    float s0[TP];   // sigma^2 of each centre
+};
+
+// Normalised distances of every (point, union entry) pair inside that point's window,
+// into s_w[k][u]; the wavefronts take NGW union entries at a time.
+template <int NGW, bool FAST, typename T>
+__device__ __forceinline__ void cols_distances(const T *__restrict__ pats, int64_t P, const int32_t *__restrict__ idx,
+                                               int M, const float *__restrict__ sigma, float sat, U8Work wk, int ws,
+                                               const ColTile &tl, const int64_t *s_q, float (*s_w)[kMaxU])
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ngroups = (tl.nu + NGW - 1) / NGW;
+    for (int g = wave; g < ngroups; g += kWaves) {
+        int64_t q[NGW];
+        int us[NGW];
+#pragma unroll
+        for (int u = 0; u < NGW; ++u) {
+            const int t = g * NGW + u;
+            us[u] = t < tl.nu ? t : -1;
+            q[u] = s_q[min(t, tl.nu - 1)];
+        }
+        double d2[TP][NGW];
+        int n2[TP][NGW];
+        wave_pair_sums_cols<NGW, FAST>(pats, P, idx, M, tl.c, q, sat, wk, d2, n2);
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < NGW; ++u) {
+                if (us[u] < 0) continue;
+                const int row = tl.uy0 + us[u] / ws;
+                const float sg1 = sigma[q[u]];
+                const float s1 = sg1 * sg1;
+#pragma unroll
+                for (int k = 0; k < TP; ++k) {
+                    if (k >= tl.np || row < tl.wy0[k] || row >= tl.wy0[k] + ws || q[u] == tl.c[k]) continue;
+                    float dd = (float)d2[k][u];
+                    const float nf = (float)n2[k][u];
+                    dd -= nf * (tl.s0[k] + s1);
+                    const float dnorm = (s1 + tl.s0[k]) * sqrtf(2.0f * nf);
+                    dd = dnorm > 1.e-8f ? dd / dnorm : 1.0e6f * nf;
+                    s_w[k][us[u]] = dd;
+                }
+            }
+        }
+    }
+}
+
+template <typename T, typename O>
+__global__ __launch_bounds__(kThreads) void k_nlpar_cols(const T *__restrict__ pats, int nr, int nc, int64_t P,
+                                                         const int32_t *__restrict__ idx, int M,
+                                                         const float *__restrict__ sigma, int sr, double lam2,
+                                                         float dthresh, float diff_offset, float sat, int r0, int c0,
+                                                         int nrc, int ncc, int vec, U8Work wk, O *__restrict__ out,
+                                                         float *__restrict__ wout)
+{
+    __shared__ float s_w[TP][kMaxU];
+    __shared__ int64_t s_q[kMaxU];
+    __shared__ float s_sum[TP];
+    __shared__ int s_fast;
+    __shared__ int s_wy0[TP];       // per-point values indexed by a run-time point number
+    __shared__ int64_t s_c[TP];
+    const int nrb = (nrc + TP - 1) / TP;
+    const int64_t ntiles = (int64_t)nrb * ncc;
+    const int64_t t = xcd_point(blockIdx.x, ntiles);
+    if (t >= ntiles) return;
+    const int jb = (int)(t / ncc), i = c0 + (int)(t % ncc);
+    const int ws = 2 * sr + 1, win = ws * ws;
+    const int wx0 = max(i - sr, 0) - max(i + sr - (nc - 1), 0);
+    ColTile tl;
+    tl.np = min(TP, nrc - jb * TP);
+#pragma unroll
+    for (int k = 0; k < TP; ++k) {
+        const int j = r0 + jb * TP + min(k, tl.np - 1);
+        tl.c[k] = (int64_t)j * nc + i;
+        tl.wy0[k] = max(j - sr, 0) - max(j + sr - (nr - 1), 0);
+        const float sg = sigma[tl.c[k]];
+        tl.s0[k] = sg * sg;
+    }
+    tl.uy0 = tl.wy0[0];
+    tl.nu = (tl.wy0[TP - 1] + ws - tl.uy0) * ws;
+    for (int e = threadIdx.x; e < tl.nu; e += kThreads) s_q[e] = (int64_t)(tl.uy0 + e / ws) * nc + (wx0 + e % ws);
+    if (threadIdx.x < TP) {
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+            if (threadIdx.x == k) {
+                s_wy0[k] = tl.wy0[k];
+                s_c[k] = tl.c[k];
+            }
+    }
+    if (threadIdx.x == 0) {
+        // the column kernel's dot-product path needs no saturated pattern in the union
+        int fast = (sizeof(T) == 1 && wk.mask4 != nullptr) ? 1 : 0;
+        if (fast)
+            for (int e = 0; e < tl.nu; ++e)
+                if (wk.nsat[(int64_t)(tl.uy0 + e / ws) * nc + (wx0 + e % ws)] != 0) {
+                    fast = 0;
+                    break;
+                }
+        s_fast = fast;
+    }
+    __syncthreads();
+    if (s_fast)
+        cols_distances<4, true>(pats, P, idx, M, sigma, sat, wk, ws, tl, s_q, s_w);
+    else
+        cols_distances<2, false>(pats, P, idx, M, sigma, sat, wk, ws, tl, s_q, s_w);
+    __syncthreads();
+    // weights of every (point, union entry); 0 outside the point's window
+    for (int z = threadIdx.x; z < TP * tl.nu; z += kThreads) {
+        const int k = z / tl.nu, u = z % tl.nu;
+        const int row = tl.uy0 + u / ws;
+        if (k >= tl.np || row < s_wy0[k] || row >= s_wy0[k] + ws) {
+            s_w[k][u] = 0.f;
+            continue;
+        }
+        const bool self = (s_q[u] == s_c[k]);
+        float w = self ? -1.0e6f : s_w[k][u];
+        w = fmaxf(w - dthresh, 0.f);
+        w = (float)exp(-1.0 * (double)w * lam2);
+        if (self) w += diff_offset;
+        s_w[k][u] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < tl.np) {
+        const int k = threadIdx.x;
+        const int ub = (s_wy0[k] - tl.uy0) * ws; // the point's window is union entries [ub, ub + win)
+        float sum = 0.f;
+        for (int e = 0; e < win; ++e) sum += s_w[k][ub + e];
+        s_sum[k] = sum;
+    }
+    __syncthreads();
+    for (int z = threadIdx.x; z < TP * tl.nu; z += kThreads) {
+        const int k = z / tl.nu, u = z % tl.nu;
+        if (k >= tl.np) continue;
+        const int e = u - (s_wy0[k] - tl.uy0) * ws;
+        if (e < 0 || e >= win) continue;
+        const float w = s_w[k][u] / s_sum[k];
+        s_w[k][u] = w;
+        if (wout) wout[((int64_t)(jb * TP + k) * ncc + (i - c0)) * win + e] = w;
+    }
+    __syncthreads();
+    O *dst[TP];
+#pragma unroll
+    for (int k = 0; k < TP; ++k) dst[k] = out + ((int64_t)(jb * TP + min(k, tl.np - 1)) * ncc + (i - c0)) * P;
+    const int64_t P16 = vec ? P / 16 : 0;
+    for (int64_t v = threadIdx.x; v < P16; v += kThreads) {
+        float acc[TP][16];
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+#pragma unroll
+            for (int x = 0; x < 16; ++x) acc[k][x] = 0.f;
+#pragma unroll 1
+        for (int u = 0; u < tl.nu; ++u) {
+            float xv[16];
+            load16(pats + s_q[u] * P + 16 * v, xv);
+#pragma unroll
+            for (int k = 0; k < TP; ++k) {
+                const float w = s_w[k][u];
+                if (w != 0.f) {
+#pragma unroll
+                    for (int x = 0; x < 16; ++x) acc[k][x] = fmaf(xv[x], w, acc[k][x]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+            if (k < tl.np) store16(dst[k] + 16 * v, acc[k]);
+    }
+    for (int64_t x = 16 * P16 + threadIdx.x; x < P; x += kThreads) {
+        float acc[TP];
+#pragma unroll
+        for (int k = 0; k < TP; ++k) acc[k] = 0.f;
+        for (int u = 0; u < tl.nu; ++u) {
+            const float xv = ldf(pats + s_q[u] * P, x);
+#pragma unroll
+            for (int k = 0; k < TP; ++k) {
+                const float w = s_w[k][u];
+                if (w != 0.f) acc[k] = fmaf(xv, w, acc[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TP; ++k)
+            if (k < tl.np) dst[k][x] = store_cvt<O>(acc[k]);
+    }
+}
+
 // max of the pixel values as order-preserving integer keys
 __device__ __forceinline__ unsigned fkey(float x)
 {
@@ -564,32 +890,48 @@ extern "C" int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t 
 
 static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
+// scan points per NLPAR work-group: TP (column runs, k_nlpar_cols) or 1 (k_nlpar)
+static int g_points_per_group = 1;
+
+extern "C" int ebsd_nlpar_set_tiling(int points_per_group)
+{
+    if (points_per_group != 1 && points_per_group != TP)
+        return fail(-1, "ebsd_nlpar_set_tiling: 1 (one point per work-group) or 4 (column runs of 4)");
+    g_points_per_group = points_per_group;
+    return 0;
+}
+
 extern "C" size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P)
 {
     if (n <= 0 || P <= 0) return 0;
-    return al256((size_t)P) + al256((size_t)n * sizeof(double)) + al256((size_t)n);
+    return al256((size_t)P) + al256((size_t)n * sizeof(double)) + al256((size_t)n) +
+           al256((size_t)n * kSatList * sizeof(int32_t));
 }
 
-// Set up the uint8 workspace for n patterns (mask bytes, sums of squares, saturation flags)
+// Set up the uint8 workspace for n patterns (mask bytes, sums of squares, saturated pixels)
 // when it applies: uint8 patterns, P a multiple of 4, 4-byte aligned patterns, a large
-// enough workspace.  Otherwise wk stays empty and the kernels take the general path.
+// enough workspace.  Otherwise wk stays empty and the kernels take the per-pixel path.
 static int u8_setup(const void *d_pats, int dtype, int64_t n, int64_t P, const int32_t *d_idx, int M, float sat,
                     void *d_work, size_t work_bytes, hipStream_t s, U8Work *wk)
 {
-    *wk = U8Work{nullptr, nullptr, nullptr};
+    *wk = U8Work{nullptr, nullptr, nullptr, nullptr};
     if (dtype != EBSD_NL_U8 || !d_work || P % 4 != 0 || ((uintptr_t)d_pats % 4) != 0) return 0;
     if (work_bytes < ebsd_nlpar_workspace_bytes(n, P) || ((uintptr_t)d_work % 256) != 0) return 0;
     if (n > INT32_MAX) return 0;
     uint8_t *base = (uint8_t *)d_work;
     uint8_t *mask = base;
-    double *sq = (double *)(base + al256((size_t)P));
-    uint8_t *flag = base + al256((size_t)P) + al256((size_t)n * sizeof(double));
+    size_t off = al256((size_t)P);
+    double *sq = (double *)(base + off);
+    off += al256((size_t)n * sizeof(double));
+    uint8_t *nsat = base + off;
+    off += al256((size_t)n);
+    int32_t *list = (int32_t *)(base + off);
     hipLaunchKernelGGL(k_mask_zero, dim3(grid_for(P)), dim3(256), 0, s, P, mask);
     if (M > 0) hipLaunchKernelGGL(k_mask_scatter, dim3(grid_for(M)), dim3(256), 0, s, d_idx, M, mask);
     hipLaunchKernelGGL(k_u8_prep, dim3((unsigned)n), dim3(kThreads), 0, s, (const uint8_t *)d_pats, n, P, d_idx, M,
-                       sat, sq, flag);
+                       sat, sq, nsat, list);
     HIPCHK(hipGetLastError());
-    *wk = U8Work{(const uint32_t *)mask, sq, flag};
+    *wk = U8Work{(const uint32_t *)mask, sq, nsat, list};
     return 0;
 }
 
@@ -605,7 +947,7 @@ extern "C" int ebsd_nlpar_sigma(const void *d_pats, int dtype, int nr, int nc, i
     if (dtype < 0 || dtype > 2) return fail(-1, "ebsd_nlpar_sigma: dtype is 0 (u8), 1 (u16) or 2 (f32)");
     hipStream_t s = (hipStream_t)stream;
     const int64_t npts = (int64_t)nr * nc;
-    const int64_t grid = ((npts + 7) / 8) * 8;
+    const int64_t grid = (((int64_t)((nr + kBR - 1) / kBR) * ((nc + kBC - 1) / kBC) * (kBR * kBC) + 7) / 8) * 8;
     if (grid > INT32_MAX) return fail(-1, "ebsd_nlpar_sigma: chunk too large");
     U8Work wk;
     if (int rc = u8_setup(d_pats, dtype, npts, P, d_idx, M, sat, d_work, work_bytes, s, &wk)) return rc;
@@ -646,7 +988,10 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
     if (!(lam > 0.f)) return fail(-1, "ebsd_nlpar: lambda must be positive");
     const int64_t npts = (int64_t)nrc * ncc;
     if (npts == 0) return 0;
-    const int64_t grid = ((npts + 7) / 8) * 8;
+    const bool cols = g_points_per_group == TP;
+    const int64_t items = cols ? (int64_t)((nrc + TP - 1) / TP) * ncc
+                               : (int64_t)((nrc + kBR - 1) / kBR) * ((ncc + kBC - 1) / kBC) * (kBR * kBC);
+    const int64_t grid = ((items + 7) / 8) * 8;
     if (grid > INT32_MAX) return fail(-1, "ebsd_nlpar: region too large");
     hipStream_t s = (hipStream_t)stream;
     // lam2 = 1.0 / lam**2 (nlpar_cpu.py:844): float32 square, float64 reciprocal
@@ -655,10 +1000,17 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
     const int vec = (P % 16 == 0) && ((uintptr_t)d_pats % 16 == 0) && ((uintptr_t)d_out % 16 == 0);
     U8Work wk;
     if (int rc = u8_setup(d_pats, dtype, (int64_t)nr * nc, P, d_idx, M, sat, d_work, work_bytes, s, &wk)) return rc;
-#define EBSD_NL_RUN(T, O)                                                                                   \
-    hipLaunchKernelGGL((k_nlpar<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, nc, P, \
-                       d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, vec, wk, (O *)d_out, \
-                       d_weights)
+#define EBSD_NL_RUN(T, O)                                                                                         \
+    do {                                                                                                          \
+        if (cols)                                                                                                 \
+            hipLaunchKernelGGL((k_nlpar_cols<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, \
+                               nr, nc, P, d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc,  \
+                               vec, wk, (O *)d_out, d_weights);                                                   \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_nlpar<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr,  \
+                               nc, P, d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, vec, \
+                               wk, (O *)d_out, d_weights);                                                        \
+    } while (0)
 #define EBSD_NL_OUT(T)                 \
     if (out_dtype == EBSD_NL_U8)       \
         EBSD_NL_RUN(T, uint8_t);       \
