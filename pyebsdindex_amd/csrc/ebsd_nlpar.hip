@@ -207,9 +207,11 @@ __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64
 }
 
 // uint8 prepass: masked sum of squares (exact, float64), saturated mask pixels (count and
-// list) of every pattern, one work-group per pattern
+// list) of every pattern, one work-group per pattern; with the byte mask (mask4), word by
+// word with packed byte dot products, else pixel by pixel through idx
 __global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict__ pats, int64_t n, int64_t P,
-                                                     const int32_t *__restrict__ idx, int M, float sat,
+                                                     const int32_t *__restrict__ idx, int M,
+                                                     const uint32_t *__restrict__ mask4, float sat,
                                                      double *__restrict__ sq, uint8_t *__restrict__ nsat,
                                                      int32_t *__restrict__ list)
 {
@@ -219,14 +221,35 @@ __global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict_
     if (p >= n) return;
     if (threadIdx.x == 0) s_ns = 0;
     __syncthreads();
-    unsigned acc = 0; // at most ceil(M / 256) terms of 255^2 per thread
-    for (int m = threadIdx.x; m < M; m += kThreads) {
-        const int32_t x = idx[m];
-        const unsigned v = pats[p * P + x];
-        acc += v * v;
-        if (!((float)v < sat)) {
-            const int slot = atomicAdd(&s_ns, 1);
-            if (slot < kSatList) list[p * kSatList + slot] = x;
+    unsigned acc = 0; // at most ceil(P / 1024) words of 4 x 255^2 per thread
+    // the smallest saturated value: a byte b is saturated when !(b < sat)
+    const float tf = ceilf(sat);
+    if (mask4 != nullptr && tf >= 1.f) {
+        // word by word over the byte-masked pattern: masked-out bytes are 0, below tf
+        const unsigned tint = tf > 255.f ? 256u : (unsigned)tf;
+        const uint32_t *a4 = reinterpret_cast<const uint32_t *>(pats + p * P);
+        for (int64_t v = threadIdx.x; v < P / 4; v += kThreads) {
+            const uint32_t am = a4[v] & mask4[v];
+            acc = __builtin_amdgcn_udot4(am, am, acc, false);
+            const unsigned mx = max(max(am & 0xff, (am >> 8) & 0xff), max((am >> 16) & 0xff, am >> 24));
+            if (mx >= tint) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (((am >> (8 * k)) & 0xff) >= tint) {
+                        const int slot = atomicAdd(&s_ns, 1);
+                        if (slot < kSatList) list[p * kSatList + slot] = (int32_t)(4 * v + k);
+                    }
+            }
+        }
+    } else {
+        for (int m = threadIdx.x; m < M; m += kThreads) {
+            const int32_t x = idx[m];
+            const unsigned v = pats[p * P + x];
+            acc += v * v;
+            if (!((float)v < sat)) {
+                const int slot = atomicAdd(&s_ns, 1);
+                if (slot < kSatList) list[p * kSatList + slot] = x;
+            }
         }
     }
     double dacc = (double)acc;
@@ -928,8 +951,9 @@ static int u8_setup(const void *d_pats, int dtype, int64_t n, int64_t P, const i
     int32_t *list = (int32_t *)(base + off);
     hipLaunchKernelGGL(k_mask_zero, dim3(grid_for(P)), dim3(256), 0, s, P, mask);
     if (M > 0) hipLaunchKernelGGL(k_mask_scatter, dim3(grid_for(M)), dim3(256), 0, s, d_idx, M, mask);
+    // P is a multiple of 4 and the patterns 4-byte aligned here: the prepass reads words
     hipLaunchKernelGGL(k_u8_prep, dim3((unsigned)n), dim3(kThreads), 0, s, (const uint8_t *)d_pats, n, P, d_idx, M,
-                       sat, sq, nsat, list);
+                       (const uint32_t *)mask, sat, sq, nsat, list);
     HIPCHK(hipGetLastError());
     *wk = U8Work{(const uint32_t *)mask, sq, nsat, list};
     return 0;
