@@ -70,12 +70,6 @@ int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t P, const i
                int r0, int c0, int nrc, int ncc, void *d_out, int out_dtype, float *d_weights,
                void *d_work, size_t work_bytes, void *stream);
 
-/* Scan points per ebsd_nlpar work-group: 1 (default) or 4 (runs of 4 vertically adjacent
- * points share the union of their windows, each pattern of it read once for all four).
- * Both give the same values bit for bit.  Process-wide setting.
- * No counterpart in the reference (a launch-shape choice). */
-int ebsd_nlpar_set_tiling(int points_per_group);
-
 const char *ebsd_nlpar_last_error(void);
 
 #ifdef __cplusplus

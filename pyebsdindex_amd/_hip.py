@@ -25,7 +25,7 @@ SYMBOLS = (
 
 #: exported symbols of the NLPAR library, in declaration order of include/ebsd_nlpar.h
 NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_workspace_bytes", "ebsd_nlpar_sigma", "ebsd_nlpar",
-                 "ebsd_nlpar_set_tiling", "ebsd_nlpar_last_error")
+                 "ebsd_nlpar_last_error")
 
 
 class HipLibraryError(RuntimeError):
@@ -123,8 +123,7 @@ def load_nlpar(path: str = NLPAR_LIB_PATH):
         L.ebsd_nlpar_workspace_bytes.restype = ctypes.c_size_t
         L.ebsd_nlpar_sigma.argtypes = [P, I, I, I, I64, P, I, I, F, P, P, P, P, ctypes.c_size_t, P]
         L.ebsd_nlpar.argtypes = [P, I, I, I, I64, P, I, P, I, F, F, F, F, I, I, I, I, P, I, P, P, ctypes.c_size_t, P]
-        L.ebsd_nlpar_set_tiling.argtypes = [I]
-        for name in ("ebsd_nlpar_max", "ebsd_nlpar_sigma", "ebsd_nlpar", "ebsd_nlpar_set_tiling"):
+        for name in ("ebsd_nlpar_max", "ebsd_nlpar_sigma", "ebsd_nlpar"):
             getattr(L, name).restype = ctypes.c_int
         L.ebsd_nlpar_last_error.argtypes = []
         L.ebsd_nlpar_last_error.restype = ctypes.c_char_p

@@ -459,6 +459,7 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
 {
     __shared__ float s_w[kMaxWin];
     __shared__ int64_t s_q[kMaxWin];
+    __shared__ int64_t s_off[kMaxWin]; // element offset of each window pattern: q * P
     __shared__ float s_sum;
     int jr, ic;
     if (!blocked_point(blockIdx.x, nrc, ncc, jr, ic)) return;
@@ -469,7 +470,11 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
     // window shifted inside the chunk (nlpar_cpu.py:872-876)
     const int wx0 = max(i - sr, 0) - max(i + sr - (nc - 1), 0);
     const int wy0 = max(j - sr, 0) - max(j + sr - (nr - 1), 0);
-    for (int e = threadIdx.x; e < win; e += kThreads) s_q[e] = (int64_t)(wy0 + e / ws) * nc + (wx0 + e % ws);
+    for (int e = threadIdx.x; e < win; e += kThreads) {
+        const int64_t q = (int64_t)(wy0 + e / ws) * nc + (wx0 + e % ws);
+        s_q[e] = q;
+        s_off[e] = q * P;
+    }
     __syncthreads();
     const float sig0 = sigma[p];
     const float s0 = sig0 * sig0;
@@ -541,7 +546,7 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
 #pragma unroll 4
         for (int e = 0; e < win; ++e) {
             float x[16];
-            load16(pats + s_q[e] * P + 16 * v, x);
+            load16(pats + s_off[e] + 16 * v, x);
             const float w = s_w[e];
 #pragma unroll
             for (int t = 0; t < 16; ++t) acc[t] = fmaf(x[t], w, acc[t]);
@@ -550,288 +555,8 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
     }
     for (int64_t x = 16 * P16 + threadIdx.x; x < P; x += kThreads) {
         float acc = 0.f;
-        for (int e = 0; e < win; ++e) acc = fmaf(ldf(pats + s_q[e] * P, x), s_w[e], acc);
+        for (int e = 0; e < win; ++e) acc = fmaf(ldf(pats + s_off[e], x), s_w[e], acc);
         dst[x] = store_cvt<O>(acc);
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// NLPAR over column runs: one work-group computes TP vertically adjacent scan points of one
-// column.  Their windows share columns and overlap in all but TP-1 rows, so the work-group
-// walks the union of the windows once: each union pattern is read once per distance pass
-// against all TP centres, and once in the average for all TP outputs.  Every point keeps
-// its own window order for the weight sum and the average (entries outside its window are
-// skipped), so the result is bit for bit that of k_nlpar.
-// ---------------------------------------------------------------------------------------
-constexpr int TP = 4;
-constexpr int kMaxU = (21 + TP - 1) * 21; // union of TP windows at search radius 10
-
-// Distance sums of NGW union patterns q[u] against the TP centres c[k] over one wavefront
-// (every lane gets the totals), in the same per-lane order as wave_pair_sums, so each pair's
-// value is identical to it.
-template <int NGW, bool FAST, typename T>
-__device__ __forceinline__ void wave_pair_sums_cols(const T *__restrict__ pats, int64_t P,
-                                                    const int32_t *__restrict__ idx, int M, const int64_t (&c)[TP],
-                                                    const int64_t (&q)[NGW], float sat, U8Work wk,
-                                                    double (&d2)[TP][NGW], int (&n2)[TP][NGW])
-{
-    const int lane = threadIdx.x & 63;
-    if constexpr (FAST && sizeof(T) == 1) {
-        uint32_t acc[TP][NGW];
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) acc[k][u] = 0u;
-        const int64_t P4 = P / 4;
-#pragma unroll 2
-        for (int64_t v = lane; v < P4; v += 64) {
-            const uint32_t mk = wk.mask4[v];
-            uint32_t am[TP];
-#pragma unroll
-            for (int k = 0; k < TP; ++k) am[k] = reinterpret_cast<const uint32_t *>(pats + c[k] * P)[v] & mk;
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) {
-                const uint32_t b = reinterpret_cast<const uint32_t *>(pats + q[u] * P)[v];
-#pragma unroll
-                for (int k = 0; k < TP; ++k) acc[k][u] = __builtin_amdgcn_udot4(am[k], b, acc[k][u], false);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) {
-                d2[k][u] = wk.sq[c[k]] + wk.sq[q[u]] - 2.0 * wave_sum((double)acc[k][u]);
-                n2[k][u] = M;
-            }
-    } else {
-        double part[TP][NGW];
-        int cnt[TP][NGW];
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) {
-                part[k][u] = 0.0;
-                cnt[k][u] = 0;
-            }
-#pragma unroll 2
-        for (int m = lane; m < M; m += 64) {
-            const int64_t x = idx[m];
-            float v0[TP];
-            bool ok0[TP];
-#pragma unroll
-            for (int k = 0; k < TP; ++k) {
-                v0[k] = ldf(pats + c[k] * P, x);
-                ok0[k] = v0[k] < sat;
-            }
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) {
-                const float v1 = ldf(pats + q[u] * P, x);
-                const bool ok1 = v1 < sat;
-#pragma unroll
-                for (int k = 0; k < TP; ++k) {
-                    const bool ok = ok0[k] && ok1;
-                    const double d = ok ? (double)(v0[k] - v1) : 0.0;
-                    part[k][u] = fma(d, d, part[k][u]);
-                    cnt[k][u] += ok ? 1 : 0;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) {
-                d2[k][u] = wave_sum(part[k][u]);
-                n2[k][u] = wave_sum(cnt[k][u]);
-            }
-    }
-}
-
-struct ColTile {
-    int np;         // valid points of the run (1..TP)
-    int64_t c[TP];  // centre patterns (invalid ones repeat the last valid)
-    int wy0[TP];    // first window row of each point
-    int uy0, nu;    // first union row, union entries
-    float s0[TP];   // sigma^2 of each centre
-};
-
-// Normalised distances of every (point, union entry) pair inside that point's window,
-// into s_w[k][u]; the wavefronts take NGW union entries at a time.
-template <int NGW, bool FAST, typename T>
-__device__ __forceinline__ void cols_distances(const T *__restrict__ pats, int64_t P, const int32_t *__restrict__ idx,
-                                               int M, const float *__restrict__ sigma, float sat, U8Work wk, int ws,
-                                               const ColTile &tl, const int64_t *s_q, float (*s_w)[kMaxU])
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ngroups = (tl.nu + NGW - 1) / NGW;
-    for (int g = wave; g < ngroups; g += kWaves) {
-        int64_t q[NGW];
-        int us[NGW];
-#pragma unroll
-        for (int u = 0; u < NGW; ++u) {
-            const int t = g * NGW + u;
-            us[u] = t < tl.nu ? t : -1;
-            q[u] = s_q[min(t, tl.nu - 1)];
-        }
-        double d2[TP][NGW];
-        int n2[TP][NGW];
-        wave_pair_sums_cols<NGW, FAST>(pats, P, idx, M, tl.c, q, sat, wk, d2, n2);
-        if (lane == 0) {
-#pragma unroll
-            for (int u = 0; u < NGW; ++u) {
-                if (us[u] < 0) continue;
-                const int row = tl.uy0 + us[u] / ws;
-                const float sg1 = sigma[q[u]];
-                const float s1 = sg1 * sg1;
-#pragma unroll
-                for (int k = 0; k < TP; ++k) {
-                    if (k >= tl.np || row < tl.wy0[k] || row >= tl.wy0[k] + ws || q[u] == tl.c[k]) continue;
-                    float dd = (float)d2[k][u];
-                    const float nf = (float)n2[k][u];
-                    dd -= nf * (tl.s0[k] + s1);
-                    const float dnorm = (s1 + tl.s0[k]) * sqrtf(2.0f * nf);
-                    dd = dnorm > 1.e-8f ? dd / dnorm : 1.0e6f * nf;
-                    s_w[k][us[u]] = dd;
-                }
-            }
-        }
-    }
-}
-
-template <typename T, typename O>
-__global__ __launch_bounds__(kThreads) void k_nlpar_cols(const T *__restrict__ pats, int nr, int nc, int64_t P,
-                                                         const int32_t *__restrict__ idx, int M,
-                                                         const float *__restrict__ sigma, int sr, double lam2,
-                                                         float dthresh, float diff_offset, float sat, int r0, int c0,
-                                                         int nrc, int ncc, int vec, U8Work wk, O *__restrict__ out,
-                                                         float *__restrict__ wout)
-{
-    __shared__ float s_w[TP][kMaxU];
-    __shared__ int64_t s_q[kMaxU];
-    __shared__ float s_sum[TP];
-    __shared__ int s_fast;
-    __shared__ int s_wy0[TP];       // per-point values indexed by a run-time point number
-    __shared__ int64_t s_c[TP];
-    const int nrb = (nrc + TP - 1) / TP;
-    const int64_t ntiles = (int64_t)nrb * ncc;
-    const int64_t t = xcd_point(blockIdx.x, ntiles);
-    if (t >= ntiles) return;
-    const int jb = (int)(t / ncc), i = c0 + (int)(t % ncc);
-    const int ws = 2 * sr + 1, win = ws * ws;
-    const int wx0 = max(i - sr, 0) - max(i + sr - (nc - 1), 0);
-    ColTile tl;
-    tl.np = min(TP, nrc - jb * TP);
-#pragma unroll
-    for (int k = 0; k < TP; ++k) {
-        const int j = r0 + jb * TP + min(k, tl.np - 1);
-        tl.c[k] = (int64_t)j * nc + i;
-        tl.wy0[k] = max(j - sr, 0) - max(j + sr - (nr - 1), 0);
-        const float sg = sigma[tl.c[k]];
-        tl.s0[k] = sg * sg;
-    }
-    tl.uy0 = tl.wy0[0];
-    tl.nu = (tl.wy0[TP - 1] + ws - tl.uy0) * ws;
-    for (int e = threadIdx.x; e < tl.nu; e += kThreads) s_q[e] = (int64_t)(tl.uy0 + e / ws) * nc + (wx0 + e % ws);
-    if (threadIdx.x < TP) {
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-            if (threadIdx.x == k) {
-                s_wy0[k] = tl.wy0[k];
-                s_c[k] = tl.c[k];
-            }
-    }
-    if (threadIdx.x == 0) {
-        // the column kernel's dot-product path needs no saturated pattern in the union
-        int fast = (sizeof(T) == 1 && wk.mask4 != nullptr) ? 1 : 0;
-        if (fast)
-            for (int e = 0; e < tl.nu; ++e)
-                if (wk.nsat[(int64_t)(tl.uy0 + e / ws) * nc + (wx0 + e % ws)] != 0) {
-                    fast = 0;
-                    break;
-                }
-        s_fast = fast;
-    }
-    __syncthreads();
-    if (s_fast)
-        cols_distances<4, true>(pats, P, idx, M, sigma, sat, wk, ws, tl, s_q, s_w);
-    else
-        cols_distances<2, false>(pats, P, idx, M, sigma, sat, wk, ws, tl, s_q, s_w);
-    __syncthreads();
-    // weights of every (point, union entry); 0 outside the point's window
-    for (int z = threadIdx.x; z < TP * tl.nu; z += kThreads) {
-        const int k = z / tl.nu, u = z % tl.nu;
-        const int row = tl.uy0 + u / ws;
-        if (k >= tl.np || row < s_wy0[k] || row >= s_wy0[k] + ws) {
-            s_w[k][u] = 0.f;
-            continue;
-        }
-        const bool self = (s_q[u] == s_c[k]);
-        float w = self ? -1.0e6f : s_w[k][u];
-        w = fmaxf(w - dthresh, 0.f);
-        w = (float)exp(-1.0 * (double)w * lam2);
-        if (self) w += diff_offset;
-        s_w[k][u] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x < tl.np) {
-        const int k = threadIdx.x;
-        const int ub = (s_wy0[k] - tl.uy0) * ws; // the point's window is union entries [ub, ub + win)
-        float sum = 0.f;
-        for (int e = 0; e < win; ++e) sum += s_w[k][ub + e];
-        s_sum[k] = sum;
-    }
-    __syncthreads();
-    for (int z = threadIdx.x; z < TP * tl.nu; z += kThreads) {
-        const int k = z / tl.nu, u = z % tl.nu;
-        if (k >= tl.np) continue;
-        const int e = u - (s_wy0[k] - tl.uy0) * ws;
-        if (e < 0 || e >= win) continue;
-        const float w = s_w[k][u] / s_sum[k];
-        s_w[k][u] = w;
-        if (wout) wout[((int64_t)(jb * TP + k) * ncc + (i - c0)) * win + e] = w;
-    }
-    __syncthreads();
-    O *dst[TP];
-#pragma unroll
-    for (int k = 0; k < TP; ++k) dst[k] = out + ((int64_t)(jb * TP + min(k, tl.np - 1)) * ncc + (i - c0)) * P;
-    const int64_t P16 = vec ? P / 16 : 0;
-    for (int64_t v = threadIdx.x; v < P16; v += kThreads) {
-        float acc[TP][16];
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-#pragma unroll
-            for (int x = 0; x < 16; ++x) acc[k][x] = 0.f;
-#pragma unroll 1
-        for (int u = 0; u < tl.nu; ++u) {
-            float xv[16];
-            load16(pats + s_q[u] * P + 16 * v, xv);
-#pragma unroll
-            for (int k = 0; k < TP; ++k) {
-                const float w = s_w[k][u];
-                if (w != 0.f) {
-#pragma unroll
-                    for (int x = 0; x < 16; ++x) acc[k][x] = fmaf(xv[x], w, acc[k][x]);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-            if (k < tl.np) store16(dst[k] + 16 * v, acc[k]);
-    }
-    for (int64_t x = 16 * P16 + threadIdx.x; x < P; x += kThreads) {
-        float acc[TP];
-#pragma unroll
-        for (int k = 0; k < TP; ++k) acc[k] = 0.f;
-        for (int u = 0; u < tl.nu; ++u) {
-            const float xv = ldf(pats + s_q[u] * P, x);
-#pragma unroll
-            for (int k = 0; k < TP; ++k) {
-                const float w = s_w[k][u];
-                if (w != 0.f) acc[k] = fmaf(xv, w, acc[k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < TP; ++k)
-            if (k < tl.np) dst[k][x] = store_cvt<O>(acc[k]);
     }
 }
 
@@ -912,17 +637,6 @@ extern "C" int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t 
 }
 
 static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
-
-// scan points per NLPAR work-group: TP (column runs, k_nlpar_cols) or 1 (k_nlpar)
-static int g_points_per_group = 1;
-
-extern "C" int ebsd_nlpar_set_tiling(int points_per_group)
-{
-    if (points_per_group != 1 && points_per_group != TP)
-        return fail(-1, "ebsd_nlpar_set_tiling: 1 (one point per work-group) or 4 (column runs of 4)");
-    g_points_per_group = points_per_group;
-    return 0;
-}
 
 extern "C" size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P)
 {
@@ -1012,9 +726,7 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
     if (!(lam > 0.f)) return fail(-1, "ebsd_nlpar: lambda must be positive");
     const int64_t npts = (int64_t)nrc * ncc;
     if (npts == 0) return 0;
-    const bool cols = g_points_per_group == TP;
-    const int64_t items = cols ? (int64_t)((nrc + TP - 1) / TP) * ncc
-                               : (int64_t)((nrc + kBR - 1) / kBR) * ((ncc + kBC - 1) / kBC) * (kBR * kBC);
+    const int64_t items = (int64_t)((nrc + kBR - 1) / kBR) * ((ncc + kBC - 1) / kBC) * (kBR * kBC);
     const int64_t grid = ((items + 7) / 8) * 8;
     if (grid > INT32_MAX) return fail(-1, "ebsd_nlpar: region too large");
     hipStream_t s = (hipStream_t)stream;
@@ -1024,17 +736,10 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
     const int vec = (P % 16 == 0) && ((uintptr_t)d_pats % 16 == 0) && ((uintptr_t)d_out % 16 == 0);
     U8Work wk;
     if (int rc = u8_setup(d_pats, dtype, (int64_t)nr * nc, P, d_idx, M, sat, d_work, work_bytes, s, &wk)) return rc;
-#define EBSD_NL_RUN(T, O)                                                                                         \
-    do {                                                                                                          \
-        if (cols)                                                                                                 \
-            hipLaunchKernelGGL((k_nlpar_cols<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, \
-                               nr, nc, P, d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc,  \
-                               vec, wk, (O *)d_out, d_weights);                                                   \
-        else                                                                                                      \
-            hipLaunchKernelGGL((k_nlpar<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr,  \
-                               nc, P, d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, vec, \
-                               wk, (O *)d_out, d_weights);                                                        \
-    } while (0)
+#define EBSD_NL_RUN(T, O)                                                                                   \
+    hipLaunchKernelGGL((k_nlpar<T, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, nc, P, \
+                       d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, vec, wk, (O *)d_out, \
+                       d_weights)
 #define EBSD_NL_OUT(T)                 \
     if (out_dtype == EBSD_NL_U8)       \
         EBSD_NL_RUN(T, uint8_t);       \

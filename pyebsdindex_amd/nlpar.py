@@ -172,12 +172,6 @@ def nlpar_device(pats, nrows, ncols, indices, sigma, lam=0.7, sr=3, dthresh=0.0,
     return (out, wts) if return_weights else out
 
 
-def set_tiling(points_per_group):
-    """Scan points per NLPAR work-group: 1 (the default) or 4 (column runs that read the
-    union of their windows once).  Both give the same values bit for bit."""
-    _hip.check_nlpar(_hip.load_nlpar().ebsd_nlpar_set_tiling(int(points_per_group)))
-
-
 def _scalar(v):
     return v if np.isscalar(v) else v[0]
 

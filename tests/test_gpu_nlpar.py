@@ -197,32 +197,3 @@ def test_pattern_max(dtype, n, P):
     # an unaligned start (offset by one element) takes the element-by-element path
     flat = _dev(x.reshape(-1))
     assert pattern_max(flat[1:].reshape(1, -1)) == float(x.reshape(-1)[1:].max())
-
-
-@pytest.mark.parametrize("shape,dtype,nrows,ncols,sr,calclim,saturate", [
-    ((60, 60), "uint8", 11, 9, 3, None, False), ((60, 60), "uint8", 10, 9, 3, None, True),
-    ((40, 48), "uint16", 9, 8, 2, (1, 2, 6, 5), False), ((30, 30), "float32", 7, 9, 3, None, False),
-    ((33, 35), "uint8", 8, 7, 3, (0, 1, 7, 6), False)])
-def test_column_runs_bit_identical(shape, dtype, nrows, ncols, sr, calclim, saturate):
-    """Work-groups over runs of 4 vertically adjacent points give bit for bit the values and
-    weights of one point per work-group (ragged runs at the bottom, calclim, saturated
-    patterns, every type, odd pattern sizes)."""
-    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, set_tiling, sigma_device
-    print(f"nlpar column runs {shape} {dtype} {nrows}x{ncols} sr={sr}", flush=True)
-    pats = _grain_scan(nrows, ncols, shape, dtype, seed=28)
-    if saturate:
-        rng = np.random.default_rng(28)
-        for p in rng.choice(nrows * ncols, 10, replace=False):
-            pats[p].reshape(-1)[rng.choice(shape[0] * shape[1], 20, replace=False)] = 255
-    idx = np.nonzero(makeautomask(*shape).ravel())[0]
-    x = _dev(pats)
-    s = sigma_device(x, nrows, ncols, idx)
-    try:
-        set_tiling(1)
-        a, wa = nlpar_device(x, nrows, ncols, idx, s, sr=sr, calclim=calclim, return_weights=True)
-        set_tiling(4)
-        b, wb = nlpar_device(x, nrows, ncols, idx, s, sr=sr, calclim=calclim, return_weights=True)
-    finally:
-        set_tiling(1)
-    assert np.array_equal(wa.cpu().numpy(), wb.cpu().numpy())
-    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
