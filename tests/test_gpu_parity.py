@@ -273,3 +273,30 @@ def test_peak_kernels_agree(case):
         else:
             assert np.array_equal(a[k], b[k]), k
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("kind", ["mask", "maskindex"])
+def test_masked_plans(kind):
+    """Plans built with a pattern mask (a circular detector: masked pixels leave the lines,
+    radon_fast.py:167-193) or a mask index (pixels relabelled, here a flip of the pattern):
+    the device Radon is bit-identical to the reference restatement with the same plan, and
+    the bands agree with the FP64 oracle."""
+    print(f"masked plan {kind}", flush=True)
+    shape = (80, 80)
+    H, W = shape
+    if kind == "mask":
+        yy, xx = np.mgrid[0:H, 0:W]
+        kw = dict(patternmask=((yy - H / 2) ** 2 + (xx - W / 2) ** 2 < (0.45 * H) ** 2).astype(int))
+    else:
+        kw = dict(patternmaskindex=np.arange(H * W).reshape(H, W)[:, ::-1].copy())
+    bd, ref = make_pair(shape, **kw)
+    assert np.array_equal(bd.radonPlan.indexPlan, ref.radonPlan.indexPlan)
+    pats = _pats(12, shape, "uint8", seed=31)
+    rad = bd.device_plan().radon(_dev(pats)).cpu().numpy()
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    assert np.array_equal(rad, want)
+    got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(_dev(pats)).items()}
+    res, _, C64, _ = oracle64(ref, pats)
+    total, ties = compare_bands(got, res, C64, ref.padding, ref.peakPad)
+    assert ties <= max(1, total // 100)
