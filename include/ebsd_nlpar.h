@@ -34,10 +34,12 @@ enum { EBSD_NL_U8 = 0, EBSD_NL_U16 = 1, EBSD_NL_F32 = 2 };
 int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t P, float *d_max, void *stream);
 
 /* Bytes of the optional workspace of ebsd_nlpar_sigma / ebsd_nlpar for a chunk of n
- * patterns of P pixels.  With uint8 patterns (P a multiple of 4) and this workspace the
- * kernels compute the distances of unsaturated pattern pairs as exact integer dot
- * products; without it (NULL), or for other types, pixel by pixel.  No counterpart in the
- * reference (its Numba kernels allocate nothing on a device). */
+ * patterns of P pixels.  With uint8 or uint16 patterns (P a multiple of 4) and this
+ * workspace the kernels compute the distances as exact integer sums over packed words
+ * (uint8: byte dot products), taking out the pixels saturated in either pattern from a
+ * per-pattern list; patterns with more than 32 saturated pixels, float32 patterns, or no
+ * workspace (NULL): pixel by pixel.  No counterpart in the reference (its Numba kernels
+ * allocate nothing on a device). */
 size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P);
 
 /* Noise estimate (nlpar_cpu.py:753-819) for every point of an nr x nc chunk, neighbours

@@ -176,6 +176,64 @@ __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64
             return;
         }
     }
+    if constexpr (sizeof(T) == 2) {
+        bool fast = wk.mask4 != nullptr && wk.nsat[p] != kNoList;
+#pragma unroll
+        for (int k = 0; k < NGW; ++k) fast = fast && wk.nsat[q[k]] != kNoList;
+        if (fast) {
+            // two pixels per dword; masked-out pixels of both patterns set to 0 by a packed
+            // 0xffff-per-pixel mask, then |a - b| squared and summed exactly in 64 bits
+            uint64_t acc[NGW];
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) acc[k] = 0;
+            const uint32_t *a2 = reinterpret_cast<const uint32_t *>(pats + p * P);
+            const uint16_t *mb2 = reinterpret_cast<const uint16_t *>(wk.mask4); // 2 mask bytes per dword
+            const int64_t P2 = P / 2;
+#pragma unroll 2
+            for (int64_t v = lane; v < P2; v += 64) {
+                const uint32_t mb = mb2[v];
+                const uint32_t m = ((mb & 0xffu) ? 0xffffu : 0u) | ((mb & 0xff00u) ? 0xffff0000u : 0u);
+                const uint32_t aw = a2[v] & m;
+                const uint32_t alo = aw & 0xffffu, ahi = aw >> 16;
+#pragma unroll
+                for (int k = 0; k < NGW; ++k) {
+                    const uint32_t bw = reinterpret_cast<const uint32_t *>(pats + q[k] * P)[v] & m;
+                    const uint32_t blo = bw & 0xffffu, bhi = bw >> 16;
+                    const uint32_t dlo = alo > blo ? alo - blo : blo - alo;
+                    const uint32_t dhi = ahi > bhi ? ahi - bhi : bhi - ahi;
+                    acc[k] += (uint64_t)dlo * dlo;
+                    acc[k] += (uint64_t)dhi * dhi;
+                }
+            }
+            // take out the pixels saturated in either pattern, as for uint8; the sums stay
+            // below 2^53 (at most 2^22 pixels of 65535^2), so float64 carries them exactly
+            const T *pp = pats + p * P;
+            const int nsp = wk.nsat[p];
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) {
+                const T *pq = pats + q[k] * P;
+                int64_t corr = 0;
+                int cnt = 0;
+                if (lane < nsp) {
+                    const int32_t x = wk.list[p * kSatList + lane];
+                    const int64_t d = (int64_t)pp[x] - (int64_t)pq[x];
+                    corr += d * d;
+                    ++cnt;
+                }
+                if (lane < (int)wk.nsat[q[k]]) {
+                    const int32_t x = wk.list[q[k] * kSatList + lane];
+                    if ((float)pp[x] < sat) {
+                        const int64_t d = (int64_t)pp[x] - (int64_t)pq[x];
+                        corr += d * d;
+                        ++cnt;
+                    }
+                }
+                d2[k] = wave_sum((double)((int64_t)acc[k] - corr));
+                n2[k] = M - wave_sum(cnt);
+            }
+            return;
+        }
+    }
     double part[NGW];
     int cnt[NGW];
 #pragma unroll
@@ -209,7 +267,8 @@ __device__ __forceinline__ void wave_pair_sums(const T *__restrict__ pats, int64
 // uint8 prepass: masked sum of squares (exact, float64), saturated mask pixels (count and
 // list) of every pattern, one work-group per pattern; with the byte mask (mask4), word by
 // word with packed byte dot products, else pixel by pixel through idx
-__global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict__ pats, int64_t n, int64_t P,
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_int_prep(const T *__restrict__ pats, int64_t n, int64_t P,
                                                      const int32_t *__restrict__ idx, int M,
                                                      const uint32_t *__restrict__ mask4, float sat,
                                                      double *__restrict__ sq, uint8_t *__restrict__ nsat,
@@ -221,16 +280,18 @@ __global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict_
     if (p >= n) return;
     if (threadIdx.x == 0) s_ns = 0;
     __syncthreads();
-    unsigned acc = 0; // at most ceil(P / 1024) words of 4 x 255^2 per thread
-    // the smallest saturated value: a byte b is saturated when !(b < sat)
+    double acc = 0.0; // integer squares, exact in float64
+    // the smallest saturated value: a pixel value b is saturated when !(b < sat)
     const float tf = ceilf(sat);
-    if (mask4 != nullptr && tf >= 1.f) {
+    if (sizeof(T) == 1 && mask4 != nullptr && tf >= 1.f) {
+        // uint8: word by word over the byte-masked pattern
         // word by word over the byte-masked pattern: masked-out bytes are 0, below tf
         const unsigned tint = tf > 255.f ? 256u : (unsigned)tf;
         const uint32_t *a4 = reinterpret_cast<const uint32_t *>(pats + p * P);
+        unsigned wacc = 0; // at most ceil(P / 1024) words of 4 x 255^2 per thread
         for (int64_t v = threadIdx.x; v < P / 4; v += kThreads) {
             const uint32_t am = a4[v] & mask4[v];
-            acc = __builtin_amdgcn_udot4(am, am, acc, false);
+            wacc = __builtin_amdgcn_udot4(am, am, wacc, false);
             const unsigned mx = max(max(am & 0xff, (am >> 8) & 0xff), max((am >> 16) & 0xff, am >> 24));
             if (mx >= tint) {
 #pragma unroll
@@ -241,18 +302,19 @@ __global__ __launch_bounds__(kThreads) void k_u8_prep(const uint8_t *__restrict_
                     }
             }
         }
+        acc = (double)wacc;
     } else {
         for (int m = threadIdx.x; m < M; m += kThreads) {
             const int32_t x = idx[m];
             const unsigned v = pats[p * P + x];
-            acc += v * v;
+            acc += (double)v * (double)v;
             if (!((float)v < sat)) {
                 const int slot = atomicAdd(&s_ns, 1);
                 if (slot < kSatList) list[p * kSatList + slot] = x;
             }
         }
     }
-    double dacc = (double)acc;
+    double dacc = acc;
     for (int off = 32; off >= 1; off >>= 1) dacc += __shfl_xor(dacc, off, 64);
     if ((threadIdx.x & 63) == 0) s_sq[threadIdx.x >> 6] = dacc;
     __syncthreads();
@@ -652,7 +714,7 @@ static int u8_setup(const void *d_pats, int dtype, int64_t n, int64_t P, const i
                     void *d_work, size_t work_bytes, hipStream_t s, U8Work *wk)
 {
     *wk = U8Work{nullptr, nullptr, nullptr, nullptr};
-    if (dtype != EBSD_NL_U8 || !d_work || P % 4 != 0 || ((uintptr_t)d_pats % 4) != 0) return 0;
+    if (dtype == EBSD_NL_F32 || !d_work || P % 4 != 0 || ((uintptr_t)d_pats % 4) != 0) return 0;
     if (work_bytes < ebsd_nlpar_workspace_bytes(n, P) || ((uintptr_t)d_work % 256) != 0) return 0;
     if (n > INT32_MAX) return 0;
     uint8_t *base = (uint8_t *)d_work;
@@ -666,8 +728,12 @@ static int u8_setup(const void *d_pats, int dtype, int64_t n, int64_t P, const i
     hipLaunchKernelGGL(k_mask_zero, dim3(grid_for(P)), dim3(256), 0, s, P, mask);
     if (M > 0) hipLaunchKernelGGL(k_mask_scatter, dim3(grid_for(M)), dim3(256), 0, s, d_idx, M, mask);
     // P is a multiple of 4 and the patterns 4-byte aligned here: the prepass reads words
-    hipLaunchKernelGGL(k_u8_prep, dim3((unsigned)n), dim3(kThreads), 0, s, (const uint8_t *)d_pats, n, P, d_idx, M,
-                       (const uint32_t *)mask, sat, sq, nsat, list);
+    if (dtype == EBSD_NL_U8)
+        hipLaunchKernelGGL(k_int_prep<uint8_t>, dim3((unsigned)n), dim3(kThreads), 0, s, (const uint8_t *)d_pats, n, P,
+                           d_idx, M, (const uint32_t *)mask, sat, sq, nsat, list);
+    else
+        hipLaunchKernelGGL(k_int_prep<uint16_t>, dim3((unsigned)n), dim3(kThreads), 0, s, (const uint16_t *)d_pats, n,
+                           P, d_idx, M, (const uint32_t *)mask, sat, sq, nsat, list);
     HIPCHK(hipGetLastError());
     *wk = U8Work{(const uint32_t *)mask, sq, nsat, list};
     return 0;

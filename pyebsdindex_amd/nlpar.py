@@ -58,9 +58,10 @@ def _indices_dev(indices, P, device):
 
 
 def _workspace(x):
-    """(pointer, bytes) of the uint8 workspace for the chunk ``x`` (n, P), or (None, 0)."""
+    """(buffer, bytes) of the integer-pattern workspace for the chunk ``x`` (n, P), or
+    (None, 0) for float32 patterns."""
     torch = _torch()
-    if x.dtype != torch.uint8:
+    if x.dtype not in (torch.uint8, torch.uint16):
         return None, 0
     nbytes = int(_hip.load_nlpar().ebsd_nlpar_workspace_bytes(x.shape[0], x.shape[1]))
     buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
@@ -119,8 +120,8 @@ def sigma_device(pats, nrows, ncols, indices, nn=1, saturation_protect=True, ret
     (nrows*ncols, H, W) or (nrows, ncols, H, W).  Returns sigma (nrows, ncols) and, with
     ``return_d2``, the normalised distances and counts (nrows, ncols, (2nn+1)^2), all device
     float32.  ``mxval``: the saturation reference value (default: the patterns' maximum).
-    ``fast``: uint8 distances of unsaturated pairs as exact integer dot products (a
-    workspace of about one pattern plus 9 bytes per pattern); False: pixel by pixel."""
+    ``fast``: uint8/uint16 distances as exact integer sums over packed words (a workspace
+    of about one pattern plus 137 bytes per pattern); False: pixel by pixel."""
     torch = _torch()
     L = _hip.load_nlpar()
     x = pats.contiguous().reshape(nrows * ncols, -1)

@@ -155,20 +155,22 @@ def test_nlpar_file_round_trip(tmp_path):
     assert np.allclose(lams, ref, atol=2e-3)
 
 
-def test_saturated_patterns_and_both_paths():
-    """uint8 scans with saturated pixels in some patterns (1 to 60): the dot-product path
+@pytest.mark.parametrize("dtype", ["uint8", "uint16"])
+def test_saturated_patterns_and_both_paths(dtype):
+    """Integer scans with saturated pixels in some patterns (1 to 60): the packed-word path
     (with the saturated pixels' terms taken out, or the per-pixel path for patterns with
     more than 32) and the per-pixel path both match the oracle."""
     from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, sigma_device
     nrows, ncols, shape = 9, 10, (40, 40)
-    print("nlpar saturated uint8", flush=True)
-    pats = _grain_scan(nrows, ncols, shape, "uint8", seed=26)
+    print(f"nlpar saturated {dtype}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=26)
+    top = np.iinfo(dtype).max
     rng = np.random.default_rng(26)
     # 1-32 saturated pixels: exact dot products with the saturated terms taken out; more
     # than 32: per-pixel path
     for k, p in enumerate(rng.choice(nrows * ncols, 16, replace=False)):
         nsat = (1, 7, 32, 33, 60)[k % 5]
-        pats[p].reshape(-1)[rng.choice(shape[0] * shape[1], nsat, replace=False)] = 255
+        pats[p].reshape(-1)[rng.choice(shape[0] * shape[1], nsat, replace=False)] = top
     flat = pats.reshape(nrows * ncols, -1).astype(np.float32)
     idx = np.nonzero(makeautomask(*shape).ravel())[0]
     rs = nr.sigma(flat, nrows, ncols, idx)[0]
