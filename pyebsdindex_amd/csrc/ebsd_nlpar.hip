@@ -32,6 +32,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -653,7 +654,15 @@ __global__ void k_max(const T *__restrict__ pats, int64_t n, unsigned *__restric
     }
     for (int64_t t = head + t0; t < n; t += stride) best = max(best, fkey(ldf(pats, t)));
     for (int off = 32; off >= 1; off >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(key, best);
+    // one atomic per work-group (atomics on one address serialise)
+    __shared__ unsigned s_best[kWaves];
+    if ((threadIdx.x & 63) == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned b = s_best[0];
+        for (int w = 1; w < kWaves; ++w) b = max(b, s_best[w]);
+        atomicMax(key, b);
+    }
 }
 
 // in place: the key and the float share the 4 bytes
@@ -684,14 +693,13 @@ extern "C" int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t 
     hipLaunchKernelGGL(k_zero_key, dim3(1), dim3(1), 0, s, key);
     const int64_t total = n * P;
     if (total > 0) {
+        const unsigned g = (unsigned)std::min<int64_t>(1024, (total + 16 * kThreads - 1) / (16 * kThreads));
         if (dtype == EBSD_NL_U8)
-            hipLaunchKernelGGL(k_max<uint8_t>, dim3(grid_for(total)), dim3(256), 0, s, (const uint8_t *)d_pats, total,
-                               key);
+            hipLaunchKernelGGL(k_max<uint8_t>, dim3(g), dim3(kThreads), 0, s, (const uint8_t *)d_pats, total, key);
         else if (dtype == EBSD_NL_U16)
-            hipLaunchKernelGGL(k_max<uint16_t>, dim3(grid_for(total)), dim3(256), 0, s, (const uint16_t *)d_pats,
-                               total, key);
+            hipLaunchKernelGGL(k_max<uint16_t>, dim3(g), dim3(kThreads), 0, s, (const uint16_t *)d_pats, total, key);
         else
-            hipLaunchKernelGGL(k_max<float>, dim3(grid_for(total)), dim3(256), 0, s, (const float *)d_pats, total, key);
+            hipLaunchKernelGGL(k_max<float>, dim3(g), dim3(kThreads), 0, s, (const float *)d_pats, total, key);
     }
     hipLaunchKernelGGL(k_key_to_float, dim3(1), dim3(1), 0, s, key, d_max);
     HIPCHK(hipGetLastError());

@@ -302,7 +302,19 @@ class NLPAR:
         return [(r, min(nrows, r + rows)) for r in range(0, nrows, rows)]
 
     def _load_rows(self, f, a, b, dev):
-        """Rows [a, b) of the scan as device patterns (file type), through pinned memory."""
+        """Rows [a, b) of the scan as device patterns (file type), through pinned memory.
+        The last rows loaded are kept, so a scan that fits one band is read from the file
+        once for the maximum, the sigma and the averaging passes."""
+        key = (str(f.filepath), a, b, str(dev))
+        cached = getattr(self, "_rows_cache", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        self._rows_cache = None
+        x = self._read_rows(f, a, b, dev)
+        self._rows_cache = (key, x)
+        return x
+
+    def _read_rows(self, f, a, b, dev):
         torch = _torch()
         ncols = int(self.ncols)
         tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.uint16}[np.dtype(f.filedatatype)]
@@ -385,6 +397,7 @@ class NLPAR:
             self.lam = np.median(vals)
         if self.sigma is None:
             self.sigma = sigma
+        self._rows_cache = None
         return vals.flatten()
 
     # -- NLPAR (nlpar_cpu.py:283-540) ------------------------------------------------------
@@ -451,6 +464,7 @@ class NLPAR:
             if verbose >= 2:
                 print("rows complete: ", r1, "/", nrows, sep='', end='\r')
         fo.datamin, fo.datamax = newmin, newmax
+        self._rows_cache = None
         return str(fo.filepath)
 
     def auto_nlpar(self, filename=None, fileout=None, lindex=1, **kwargs):
