@@ -68,3 +68,33 @@ def test_file_streaming_equals_resident(tmp_path, shape, dtype, ext, start, coun
     got = sb.concat(sb.run_file(f, start, count, ring=rbufs, threads=3))
     for k in want:
         assert np.array_equal(got[k].numpy(), want[k]), k
+
+
+def test_file_streaming_rejects_mismatches(tmp_path):
+    """A file of another pattern type or size, a range past its end and a ring of one
+    buffer are refused before anything is copied."""
+    from pyebsdindex_amd.ebsd_pattern import UPFile, get_pattern_file_obj
+    from pyebsdindex_amd.streaming import StreamingBandDetect
+    bd, _ = make_pair((60, 60))
+    sb = StreamingBandDetect(bd, batch=8, dtype="uint8", depth=2, chunk=8)
+    fn = tmp_path / "s.up2"
+    w = UPFile(fn)
+    w.patternW, w.patternH, w.nCols, w.nRows, w.xStep, w.yStep = 60, 60, 4, 1, 1.0, 1.0
+    w.write_header(writeBlank=True)
+    with pytest.raises(TypeError):
+        sb.run_file(get_pattern_file_obj(str(fn)))
+    fn8 = tmp_path / "s.up1"
+    w8 = UPFile(fn8)
+    w8.patternW, w8.patternH, w8.nCols, w8.nRows, w8.xStep, w8.yStep = 60, 60, 4, 1, 1.0, 1.0
+    w8.write_header(writeBlank=True)
+    f8 = get_pattern_file_obj(str(fn8))
+    with pytest.raises(ValueError):
+        sb.run_file(f8, 2, 5)
+    with pytest.raises(ValueError):
+        sb.run_file(f8, ring=[sb.pinned_batch()])
+    fn9 = tmp_path / "t.up1"
+    w9 = UPFile(fn9)
+    w9.patternW, w9.patternH, w9.nCols, w9.nRows, w9.xStep, w9.yStep = 50, 60, 4, 1, 1.0, 1.0
+    w9.write_header(writeBlank=True)
+    with pytest.raises(ValueError):
+        sb.run_file(get_pattern_file_obj(str(fn9)))
