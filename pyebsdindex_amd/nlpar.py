@@ -332,7 +332,11 @@ class NLPAR:
     def calcsigma(self, chunksize=0, nn=1, saturation_protect=None, automask=None, stem_scale=None, verbose=0,
                   **kwargs):
         """Noise estimate of every scan point; returns (sigma, d2, n2) as numpy arrays
-        ((nrows, ncols), and (nrows, ncols, (2nn+1)^2) normalised distances and counts)."""
+        ((nrows, ncols), and (nrows, ncols, (2nn+1)^2) normalised distances and counts).
+        ``chunksize`` > 0 bounds the device bytes of patterns per band, as the reference's
+        chunk target bounds its host chunks (nlpar_cpu.py:218-223)."""
+        if chunksize and chunksize > 0:
+            self.band_bytes = int(chunksize)
         self.sigmann = nn
         if saturation_protect is not None:
             self.saturation_protect = saturation_protect
@@ -405,7 +409,10 @@ class NLPAR:
                   automask=None, stem_scale=None, filename=None, fileout=None, reset_sigma=False, backsub=False,
                   rescale=False, verbose=0, diff_offset=None, **kwargs):
         """Average every pattern of the file over its search window and write the result to
-        the output file (same type, clipped); returns the output path."""
+        the output file (same type, clipped); returns the output path.  ``chunksize`` > 0
+        bounds the device bytes of patterns per band (nlpar_cpu.py:352-358)."""
+        if chunksize and chunksize > 0:
+            self.band_bytes = int(chunksize)
         if searchradius is not None:
             self.searchradius = searchradius
         sr = int(_scalar(self.searchradius))
