@@ -199,3 +199,46 @@ def test_pattern_max(dtype, n, P):
     # an unaligned start (offset by one element) takes the element-by-element path
     flat = _dev(x.reshape(-1))
     assert pattern_max(flat[1:].reshape(1, -1)) == float(x.reshape(-1)[1:].max())
+
+
+@pytest.mark.parametrize("shape,dtype,nrows,ncols,nn,sr", [
+    ((60, 60), "uint8", 11, 12, 2, 1), ((48, 48), "uint8", 12, 11, 1, 5)])
+def test_other_radii(shape, dtype, nrows, ncols, nn, sr):
+    """Neighbourhoods of +-2 for sigma and search radii 1 and 5 (windows of 9 and 121)
+    against the oracle's restatement."""
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, sigma_device
+    print(f"nlpar radii {shape} {dtype} {nrows}x{ncols} nn={nn} sr={sr}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=29)
+    flat = pats.reshape(nrows * ncols, -1).astype(np.float32)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    x = _dev(pats)
+    s, d2, n2 = sigma_device(x, nrows, ncols, idx, nn=nn, return_d2=True)
+    rs, rd2, rn2 = nr.sigma(flat, nrows, ncols, idx, nn=nn)
+    assert np.allclose(s.cpu().numpy(), rs, rtol=1e-5)
+    assert np.array_equal(n2.cpu().numpy(), rn2)
+    assert np.allclose(d2.cpu().numpy(), rd2, rtol=1e-4, atol=2e-4)
+    want, wref = nr.nlpar(flat, nrows, ncols, idx, rs, lam=0.7, sr=sr, return_weights=True)
+    got, w = nlpar_device(x, nrows, ncols, idx, _dev(rs), lam=0.7, sr=sr, return_weights=True)
+    assert np.allclose(w.cpu().numpy(), wref, atol=2e-5)
+    assert np.allclose(got.cpu().numpy().reshape(want.shape), want, rtol=1e-4, atol=1e-3)
+
+
+def test_large_patterns_against_float64():
+    """240x240 uint16 (43,433 mask pixels): the reference's sequential float32 sums drift
+    from the exact distances (the restatement's sigma is 3.7e-5 off the float64 one here),
+    so the device, whose integer sums are exact, is held to the float64 oracle: sigma within
+    1e-6, the averaged patterns (with the float64 sigma) within 5e-5."""
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, sigma_device
+    nrows, ncols, shape = 7, 8, (240, 240)
+    print("nlpar 240x240 uint16 vs float64", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, "uint16", seed=29)
+    flat = pats.reshape(nrows * ncols, -1).astype(np.float64)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    x = _dev(pats)
+    s64 = nr.sigma64(flat, nrows, ncols, idx)
+    s = sigma_device(x, nrows, ncols, idx).cpu().numpy()
+    assert np.allclose(s, s64, rtol=1e-6)
+    sig = s64.astype(np.float32)
+    want = nr.nlpar64(flat, nrows, ncols, idx, sig, lam=0.7, sr=3)
+    got = nlpar_device(x, nrows, ncols, idx, _dev(sig), lam=0.7, sr=3).cpu().numpy()
+    assert np.allclose(got.reshape(want.shape), want, rtol=5e-5, atol=1e-3)
