@@ -2222,7 +2222,9 @@ static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int s
     const size_t lds = (size_t)(maxwords + 1) * C::WB;
     if ((int)lds > p->lds_max) return 1; // caller falls back to a smaller pack
     auto kern = k_radon_gather<IN, FLT, P>;
-    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // the cap is the device's LDS, not this launch's size: a graph that captured launches of
+    // one instantiation with different sizes then replays each within the cap
+    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, p->lds_max));
     int occ = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, p->gthreads, lds));
     if (occ < 1) return fail(-3, "radon gather kernel cannot be resident");
@@ -3148,6 +3150,10 @@ struct PeakClass {
     int grid = 0;
 };
 
+// patterns from which a set of a batch is gathered by the per-size launcher (as many as one
+// per-size chunk: the set then fills the GPU by itself)
+constexpr int64_t kSingleSetMin = 2048;
+
 struct ebsd_batch {
     int nsets = 0, device = 0, ncu = 0;
     GatherArgs *d_gsets = nullptr;
@@ -3159,6 +3165,15 @@ struct ebsd_batch {
     int2 *d_pitems = nullptr;
     int64_t npitems = 0;
     std::vector<PeakClass> pclasses;
+    // sets gathered by the per-size launcher (large sets whose gather needs no row segments)
+    struct Single {
+        ebsd_plan *plan;
+        const void *pats;
+        int dtype;
+        int64_t N;
+        float *radon;
+    };
+    std::vector<Single> singles;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -3208,6 +3223,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     // LDS occupancy class of each set: work-groups per CU its staged pack allows (1..4);
     // classes of 2+ work-groups per CU run 512-thread work-groups, class 1 runs 1024
     std::vector<int> cls(nsets);
+    std::vector<char> fltv(nsets, 0);
     for (int i = 0; i < nsets; ++i) {
         ebsd_plan *p = plans[i];
         if (!p || N[i] < 0) { batch_free(b); return fail(-1, "ebsd_batch_create: bad set"); }
@@ -3217,6 +3233,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         if (dtype > EBSD_F32) { batch_free(b); return fail(-1, "ebsd_batch_create: bad dtype"); }
         const bool use_bg = !(dtypes[i] & EBSD_FLAG_NOBG) && p->bg_mode != 0;
         const bool flt = dtype == EBSD_F32 || (use_bg && p->bg_mode == 2);
+        fltv[i] = flt;
         VariantInfo v;
         if ((rc = choose_variant(p, dtype, flt, &v))) { batch_free(b); return rc; }
         var[i] = v.id;
@@ -3250,6 +3267,15 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         S.mcol = p->d_mcol;
         S.mrow = p->d_mrow;
         S.out = outs[i];
+    }
+    // a set that fills the GPU by itself (kSingleSetMin patterns or more) and whose gather
+    // needs no row segments runs the per-size launcher, with the plan's tuned threads and
+    // pitch, inside the batch; the merged launches are for the others
+    for (int i = 0; i < nsets; ++i) {
+        if (N[i] < kSingleSetMin) continue;
+        if (!fltv[i] && row_segments(plans[i], dtypes[i] & 0xff) > 1) continue;
+        b->singles.push_back(ebsd_batch::Single{plans[i], d_pats[i], dtypes[i], N[i], gs[i].out});
+        cls[i] = 0;
     }
     // gather items, class by class (4 work-groups per CU first); inside a class the largest
     // patterns first, so the static round-robin over persistent work-groups ends cheap
@@ -3387,12 +3413,28 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
             batch_free(b);
             return fail(-3, "batched peak kernel LDS limit");
         }
+    // one launch of each per-size gather now: its plan structures and stream layouts exist
+    // afterwards, so a later launch (or its graph capture) allocates nothing
+    for (const ebsd_batch::Single &g : b->singles) {
+        if ((rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, nullptr))) {
+            batch_free(b);
+            return rc;
+        }
+    }
+    if (!b->singles.empty() && hipDeviceSynchronize() != hipSuccess) {
+        batch_free(b);
+        return fail(-3, "batched per-size gather warm-up failed");
+    }
     *out = b;
     return 0;
 }
 
 static int batch_launch(ebsd_batch *b, hipStream_t s)
 {
+    for (const ebsd_batch::Single &g : b->singles) {
+        const int rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, s);
+        if (rc) return rc;
+    }
     for (const GatherClass &gc : b->classes) {
         GatherMultiArgs gm;
         gm.sets = b->d_gsets;

@@ -65,3 +65,40 @@ def test_batch_against_fp64_oracle():
         got = {k: v.cpu().numpy() for k, v in o.items()}
         total, ties = compare_bands(got, res, C64, ref.padding, ref.peakPad)
         assert ties <= max(1, total // 50)
+
+
+def test_batch_with_large_sets():
+    """Sets of 2,048 patterns or more whose gather needs no row segments run the per-size
+    gather inside the batch (and its graph); a large set that needs row segments (160x160
+    uint16) and a small set stay in the merged launch.  Results equal each set's own
+    find_bands bit for bit, eagerly and on graph replays with eager work in between."""
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.batch import BatchBandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    print("batch with large sets", flush=True)
+    spec = [((60, 60), "uint8", 2300), ((120, 120), "uint16", 2100), ((160, 160), "uint16", 2048),
+            ((80, 80), "uint8", 40)]
+    dets, pats = [], []
+    for k, (shape, dtype, n) in enumerate(spec):
+        pats.append(kikuchi_patterns(n, shape, dtype, seed=120 + k, family="BCC", device="cuda"))
+        dets.append(BandDetect(patDim=shape, **STD))
+    want = [{k: v.cpu().numpy() for k, v in bd.find_bands_device(x).items()} for bd, x in zip(dets, pats)]
+    batch = BatchBandDetect(dets, pats)
+    batch.run()
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(batch.outs, want)):
+        for k in w:
+            assert np.array_equal(o[k].cpu().numpy(), w[k]), (i, k)
+    for rep in range(3):
+        for o in batch.outs:
+            for v in o.values():
+                v.zero_()
+        torch.cuda.synchronize()
+        batch.graph_launch()
+        batch.graph_stream.synchronize()
+        _ = dets[0].find_bands_device(pats[0][:64])
+        torch.cuda.synchronize()
+        for i, (o, w) in enumerate(zip(batch.outs, want)):
+            for k in w:
+                assert np.array_equal(o[k].cpu().numpy(), w[k]), (rep, i, k)
