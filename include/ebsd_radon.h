@@ -117,10 +117,14 @@ int ebsd_peaks(ebsd_plan *plan, const float *d_radon, int64_t N, const ebsd_band
  * Replaces: the rdnConv array of rdn_conv (band_detect.py:562-637). */
 int ebsd_conv(ebsd_plan *plan, const float *d_radon, int64_t N, float *d_conv, void *stream);
 
-/* Batches of mixed pattern sizes (each set with its own plan): one persistent Radon
- * launch and one persistent peak launch over heterogeneous work lists.  The batch binds the
- * given device pointers; ebsd_batch_graph_launch captures both launches into a hipGraph on
- * its first call (on a non-default stream) and replays it afterwards.
+/* Batches of mixed pattern sizes (each set with its own plan): persistent Radon launches
+ * (one per LDS-occupancy class) and peak launches (one per Radon geometry) over
+ * heterogeneous work lists; a set of 2,048 patterns or more whose gather needs no row
+ * segments is gathered by its own per-size launch in the same sequence (run once by
+ * ebsd_batch_create, so that later launches allocate nothing).  The batch binds the given
+ * device pointers; ebsd_batch_graph_launch captures the launch sequence into a hipGraph on
+ * its first call (on a non-default stream) and replays it afterwards.  The plans must not be
+ * retuned while the batch exists.
  * Replaces: calling find_bands once per pattern size (one BandDetect/plan per size, as
  * EBSDIndexer re-plans on a size change, pyebsdindex/_ebsd_index_single.py:696-700). */
 typedef struct ebsd_batch ebsd_batch;
