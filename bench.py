@@ -9,6 +9,7 @@ Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -40,6 +41,7 @@ def parse():
     ap.add_argument("--no-streaming", action="store_true", help="skip the pinned-host streaming measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation measurement")
     ap.add_argument("--no-nlpar", action="store_true", help="skip the NLPAR measurement")
+    ap.add_argument("--no-tripvote", action="store_true", help="skip the triplet-vote measurement")
     ap.add_argument("--stream-patterns", type=int, default=1048576,
                     help="patterns streamed in the scan-scale measurement")
     ap.add_argument("--roofline-reps", type=int, default=20)
@@ -235,6 +237,59 @@ def nlpar_measure(dev, nrows=256, ncols=256, shape=(60, 60), sr=3, reps=5, cpu_s
                          "algorithmic_flops_per_pattern": flops},
             "cpu_baseline": cpu,
             "output_checksum": int(out.to(torch.int64).sum().item())}
+
+
+def tripvote_measure(dev, npats=65536, nbands=9, reps=20, cpu_seconds=6.0, seed=23):
+    """Triplet vote of 65,536 patterns x 9 bands against the FCC library (pole families
+    {002} {111} {022} {113}, angTol 2): band normals of 4,096 random orientations with
+    0.3 degree noise, 3 % of the bands invalid; the device vote timed with HIP events on
+    the launching stream; the oracle's C restatement of the reference's Numba kernel timed
+    on a sample of the same normals on the host cores."""
+    import torch
+    from oracle import tripvote_ref
+    from pyebsdindex_amd import tripletvote as tv
+    from pyebsdindex_amd.synthetic import cubic_band_normals
+    ind = tv.addphase("FCC")
+    bn, fam_true = cubic_band_normals(ind.families, npats, nbands, seed=seed, noise_deg=0.3)
+    rng = np.random.default_rng(seed)
+    inten = rng.uniform(0.2, 1.0, (npats, nbands)).astype(np.float32)
+    inten[rng.random((npats, nbands)) < 0.03] = 0.0
+    bnd = torch.as_tensor(bn, device=dev)
+    intd = torch.as_tensor(inten, device=dev)
+    out = ind.tripvote(bnd, intd)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        out = ind.tripvote(bnd, intd)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    ntrip = int(ind.angtriplets["angles"].shape[0])
+    tests = math.comb(nbands, 3) * ntrip
+    agree = float((out[1].cpu().numpy() == fam_true).mean())
+    # CPU: the restated Numba kernel (OpenMP over patterns, all host cores) on a sample
+    sample = 2048
+    tripvote_ref.tripvote(bn[:64], inten[:64], ind.lut, 2.0, ind.angtriplets["angles"],
+                          ind.angtriplets["familyid"], ind.npolefamilies)
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < cpu_seconds:
+        s0 = done % (npats - sample)
+        tripvote_ref.tripvote(bn[s0:s0 + sample], inten[s0:s0 + sample], ind.lut, 2.0, ind.angtriplets["angles"],
+                              ind.angtriplets["familyid"], ind.npolefamilies)
+        done += sample
+    cpu = None
+    if done:
+        cpu = {"value": round(done / (time.perf_counter() - t0), 1), "unit": "patterns/s", "cores": os.cpu_count(),
+               "kind": "port", "sample": f"{done} patterns in blocks of {sample}, OpenMP over patterns as numba.prange"}
+    return {"workload": f"triplet vote of {npats} patterns x {nbands} bands, FCC library ({ntrip} triplets), "
+                        f"angTol 2, 0.3 degree noise, 3 % invalid bands",
+            "patterns_per_s": round(npats / (ms * 1e-3), 1), "kernel_ms": round(ms, 4),
+            "library_row_tests_per_pattern": tests,
+            "row_tests_per_s": round(npats * tests / (ms * 1e-3), 1),
+            "band_family_agreement": round(agree, 4),
+            "cpu_baseline": cpu}
 
 
 def nlpar_pmc_driver(dev, nrows=256, ncols=256, shape=(60, 60), sr=3, seed=17):
@@ -525,6 +580,15 @@ def main():
         print(f"[bench] forward+backward {diff['forward_backward_patterns_per_s']:.1f} patterns/s, PC error "
               f"{diff['pc_error_start']:.4f} -> {diff['pc_error_refined']:.5f}", file=sys.stderr, flush=True)
 
+    tripvote = None
+    if not args.no_tripvote:
+        print("[bench] triplet-vote workload", file=sys.stderr, flush=True)
+        tripvote = tripvote_measure(dev, cpu_seconds=0.0 if args.no_cpu else 6.0)
+        cpu_tv = tripvote["cpu_baseline"]["value"] if tripvote["cpu_baseline"] else float("nan")
+        print(f"[bench] triplet vote {tripvote['patterns_per_s']:.1f} patterns/s ({tripvote['kernel_ms']} ms per "
+              f"65,536), CPU {cpu_tv:.1f} patterns/s, family agreement "
+              f"{tripvote['band_family_agreement']}", file=sys.stderr, flush=True)
+
     nlpar = None
     if not args.no_nlpar:
         print("[bench] NLPAR workload", file=sys.stderr, flush=True)
@@ -565,6 +629,7 @@ def main():
         "streaming": streaming,
         "differentiation": diff,
         "nlpar": nlpar,
+        "tripvote": tripvote,
     }
     print(json.dumps(line), flush=True)
 

@@ -11,6 +11,7 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libebsd_radon.so")
 NLPAR_LIB_PATH = os.path.join(HERE, "libebsd_nlpar.so")
+TRIPVOTE_LIB_PATH = os.path.join(HERE, "libebsd_tripvote.so")
 
 EBSD_U8, EBSD_U16, EBSD_F32 = 0, 1, 2
 
@@ -26,6 +27,9 @@ SYMBOLS = (
 #: exported symbols of the NLPAR library, in declaration order of include/ebsd_nlpar.h
 NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_workspace_bytes", "ebsd_nlpar_sigma", "ebsd_nlpar",
                  "ebsd_nlpar_last_error")
+
+#: exported symbols of the triplet-vote library, in declaration order of include/ebsd_tripvote.h
+TRIPVOTE_SYMBOLS = ("ebsd_tripvote", "ebsd_tripvote_last_error")
 
 
 class HipLibraryError(RuntimeError):
@@ -135,3 +139,34 @@ def check_nlpar(rc: int):
     if rc != 0:
         msg = load_nlpar().ebsd_nlpar_last_error().decode(errors="replace")
         raise HipLibraryError(f"ebsd NLPAR library error {rc}: {msg}")
+
+
+_tripvote_lib = None
+
+
+def load_tripvote(path: str = TRIPVOTE_LIB_PATH):
+    """Load the triplet-vote library once; raise HipLibraryError if it is not built."""
+    global _tripvote_lib
+    with _lock:
+        if _tripvote_lib is not None:
+            return _tripvote_lib
+        if not os.path.exists(path):
+            raise HipLibraryError(
+                f"HIP library not found at {path}; build it with `python -m pyebsdindex_amd.build`")
+        try:
+            L = ctypes.CDLL(path)
+        except OSError as e:
+            raise HipLibraryError(f"failed to load {path}: {e}") from e
+        P, I = ctypes.c_void_p, ctypes.c_int
+        L.ebsd_tripvote.argtypes = [P, P, ctypes.c_int64, I, P, P, I, I, ctypes.c_double, P, P, P, P, P, P]
+        L.ebsd_tripvote.restype = ctypes.c_int
+        L.ebsd_tripvote_last_error.argtypes = []
+        L.ebsd_tripvote_last_error.restype = ctypes.c_char_p
+        _tripvote_lib = L
+        return L
+
+
+def check_tripvote(lib, rc: int):
+    if rc != 0:
+        msg = lib.ebsd_tripvote_last_error().decode(errors="replace")
+        raise HipLibraryError(f"ebsd triplet-vote library error {rc}: {msg}")

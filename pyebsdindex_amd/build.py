@@ -2,8 +2,9 @@
 
 ``python -m pyebsdindex_amd.build`` (or ``__graft_entry__.build()``) runs hipcc on
 ``csrc/ebsd_radon.hip`` (band detection -> ``libebsd_radon.so``) and
-``csrc/ebsd_nlpar.hip`` (NLPAR -> ``libebsd_nlpar.so``).  Both are plain C-ABI shared
-objects (no torch symbols); ``pyebsdindex_amd._hip`` loads them with ctypes.
+``csrc/ebsd_nlpar.hip`` (NLPAR -> ``libebsd_nlpar.so``) and ``csrc/ebsd_tripvote.hip``
+(triplet voting -> ``libebsd_tripvote.so``).  All are plain C-ABI shared objects (no torch
+symbols); ``pyebsdindex_amd._hip`` loads them with ctypes.
 """
 import os
 import subprocess
@@ -15,6 +16,8 @@ SRC = os.path.join(HERE, "csrc", "ebsd_radon.hip")
 LIB = os.path.join(HERE, "libebsd_radon.so")
 NLPAR_SRC = os.path.join(HERE, "csrc", "ebsd_nlpar.hip")
 NLPAR_LIB = os.path.join(HERE, "libebsd_nlpar.so")
+TRIPVOTE_SRC = os.path.join(HERE, "csrc", "ebsd_tripvote.hip")
+TRIPVOTE_LIB = os.path.join(HERE, "libebsd_tripvote.so")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC_FLAGS = [
@@ -40,7 +43,8 @@ def _hipcc(src: str, lib: str, verbose: bool) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile both libraries; returns the band-detection library's path."""
+    """Compile the three libraries; returns the band-detection library's path."""
+    _hipcc(TRIPVOTE_SRC, TRIPVOTE_LIB, verbose)
     _hipcc(NLPAR_SRC, NLPAR_LIB, verbose)
     return _hipcc(SRC, LIB, verbose)
 

@@ -1,0 +1,370 @@
+// Triplet voting of band normals against a phase's angle-triplet library, for gfx950.
+//
+// Replaces BandIndexer._tripvote_numba (pyebsdindex/tripletvote.py:1019-1207).  One
+// wavefront per pattern, up to four per work-group; the work-group copies the library into
+// LDS as columns when it fits in 64 KiB with the per-wave state.  A wave computes its
+// pattern's inter-band angles into LDS; its lanes then build the records of 64 band
+// triplets at a time (ascending angles, vertex order, near-equality flags) and the wave
+// walks them in the reference's (i < j < k) order, reading each with v_readlane.  For each
+// triplet the 64 lanes test 64 library rows at once; the rows that pass all three angle
+// tests are taken in ascending row order from the ballot mask and lane 0 adds their votes
+// to the LDS accumulators, so every accumulator receives the same float additions in the
+// same order as the reference's sequential loop.  The per-band metrics are computed one
+// band per lane.
+//
+// Arithmetic (the types Numba gives the reference):
+//   cosine   float32 ((x0 y0) + (x1 y1)) + (x2 y2), no contraction; |.|, clipped at 1
+//   angle    f32(f64(f32 arccos) * RADEG); the float32 arccos is the float64 arccos
+//            rounded to float32 (the correctly rounded value);  10000 for a band with
+//            intensity below 1e-6
+//   tests    float64 |library angle - triplet angle| <= angtol; kept as float32
+//   weights  angtol - 0.5 * f32(t_a + t_b) in float64, added to float32 accumulators
+//            through float64
+//   metrics  float32 running sum and maximum over families, truncated to int32;
+//            cm = f32(f64(largest - second largest) / votes); rank in float64
+//
+// Work per pattern: C(nb, 3) triplets x ceil(ntrip / 64) blocks of row tests (three LDS
+// reads and three float64 differences per lane), plus lane 0's LDS updates per match
+// (about 100 matches per pattern for 9 bands of an FCC scan).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "ebsd_tripvote.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kMaxBands = 64;
+constexpr size_t kLdsBudget = 64 * 1024;   // library + per-wave state of up to 4 waves
+constexpr double kRadeg = 180.0 / 3.14159265358979323846;
+constexpr float kInvalidAngle = 10000.0f;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+// index network of the reference (tripletvote.py:1035-1041): srt such that a[srt] is
+// ascending, the first place winning ties as in the reference
+template <typename T>
+__device__ __forceinline__ void sort3(const T a[3], int s[3])
+{
+    s[0] = 0; s[1] = 1; s[2] = 2;
+    int t;
+    if (a[s[0]] > a[s[2]]) { t = s[2]; s[2] = s[0]; s[0] = t; }
+    if (a[s[0]] > a[s[1]]) { t = s[1]; s[1] = s[0]; s[0] = t; }
+    if (a[s[1]] > a[s[2]]) { t = s[2]; s[2] = s[1]; s[1] = t; }
+}
+
+// LDS layout per work-group: the library as columns (a0, a1, a2 float64, then id0, id1, id2
+// int32) when it fits, then per wave the nb x nb band angles and the two nfam x nb
+// accumulators.
+//
+// Triplet records: lane t of a wave computes the record of triplet c*64 + t of the
+// lexicographic order (i < j < k): the ascending angles x0 <= x1 <= x2, and packed into
+// one word i, j, k (6 bits each), the vertex order u (2 bits each), the three
+// near-equality flags and a valid bit.  The wave then walks the 64 records in order,
+// reading each with v_readlane.
+struct TripletRecord {
+    float x0, x1, x2;
+    uint32_t meta;
+};
+
+__device__ __forceinline__ TripletRecord make_record(const float *angs, int nb, int i, int j, int k, double tol,
+                                                     bool skip_invalid)
+{
+    const float a3[3] = {angs[i * nb + j], angs[i * nb + k], angs[j * nb + k]};
+    TripletRecord r;
+    int s[3];
+    sort3(a3, s);
+    // the reference's LUT entry for s (the vertex order of the sorted triangle), then
+    // its argsort
+    const int lv[3] = {2 - s[2], 2 - s[1], 2 - s[0]};
+    int u[3];
+    sort3(lv, u);
+    r.x0 = a3[s[0]];
+    r.x1 = a3[s[1]];
+    r.x2 = a3[s[2]];
+    const bool e1 = fabsf(r.x0 - r.x1) < tol;
+    const bool e2 = fabsf(r.x1 - r.x2) < tol;
+    const bool e3 = fabsf(r.x2 - r.x0) < tol;
+    // a triplet with an invalid band (angle 10000) can match no library row while the
+    // tolerance is below 9000 degrees
+    const bool valid = !(skip_invalid && (a3[0] == kInvalidAngle || a3[1] == kInvalidAngle || a3[2] == kInvalidAngle));
+    r.meta = (uint32_t)i | ((uint32_t)j << 6) | ((uint32_t)k << 12) | ((uint32_t)u[0] << 18) |
+             ((uint32_t)u[1] << 20) | ((uint32_t)u[2] << 22) | ((uint32_t)e1 << 24) | ((uint32_t)e2 << 25) |
+             ((uint32_t)e3 << 26) | ((uint32_t)valid << 27);
+    return r;
+}
+
+__device__ __forceinline__ void add_vote(float *aw, float *ac, int idx, double w)
+{
+    aw[idx] = (float)((double)aw[idx] + w);
+    ac[idx] = ac[idx] + 1.0f;
+}
+
+template <bool kLdsLib>
+__global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, const float *__restrict__ inten,
+                                                  int64_t npats, int nb, const double *__restrict__ ta,
+                                                  const int32_t *__restrict__ ti, int ntrip, int nfam, double tol,
+                                                  float *__restrict__ accw, int32_t *__restrict__ fam,
+                                                  float *__restrict__ rank, float *__restrict__ cmo,
+                                                  float *__restrict__ acc)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int waves = blockDim.x / kWave;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    double *la0 = reinterpret_cast<double *>(lds_raw);
+    double *la1 = la0 + (kLdsLib ? ntrip : 0);
+    double *la2 = la1 + (kLdsLib ? ntrip : 0);
+    int32_t *li = reinterpret_cast<int32_t *>(la2 + (kLdsLib ? ntrip : 0));
+    float *wave_lds = reinterpret_cast<float *>(li + (kLdsLib ? 3 * ((ntrip + 3) & ~3) : 0));
+    const int per = nb * nb + 2 * nfam * nb;
+    float *angs = wave_lds + wave * per;
+    float *aw = angs + nb * nb;
+    float *ac = aw + nfam * nb;
+    const int64_t p = (int64_t)blockIdx.x * waves + wave;
+    const bool live = p < npats;             // uniform over the wave
+
+    if (kLdsLib) {
+        for (int t = threadIdx.x; t < ntrip; t += blockDim.x) {
+            la0[t] = ta[(int64_t)t * 3 + 0];
+            la1[t] = ta[(int64_t)t * 3 + 1];
+            la2[t] = ta[(int64_t)t * 3 + 2];
+            li[t] = ti[(int64_t)t * 3 + 0];
+            li[ntrip + t] = ti[(int64_t)t * 3 + 1];
+            li[2 * ntrip + t] = ti[(int64_t)t * 3 + 2];
+        }
+    }
+    if (live) {
+        for (int t = lane; t < nfam * nb; t += kWave) {
+            aw[t] = 0.0f;
+            ac[t] = 0.0f;
+        }
+        const float *b = bn + p * nb * 3;
+        const float *in = inten + p * nb;
+        for (int t = lane; t < nb * nb; t += kWave) {
+            const int r = t / nb, c = t - r * nb;
+            float d = __fadd_rn(__fadd_rn(__fmul_rn(b[r * 3], b[c * 3]), __fmul_rn(b[r * 3 + 1], b[c * 3 + 1])),
+                                __fmul_rn(b[r * 3 + 2], b[c * 3 + 2]));
+            d = fabsf(d);
+            if (d > 1.0f) d = 1.0f;
+            const float a = (float)acos((double)d);
+            float v = (float)((double)a * kRadeg);
+            if ((double)in[r] < 1e-6 || (double)in[c] < 1e-6) v = kInvalidAngle;
+            angs[t] = v;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+
+    const bool skip_invalid = tol < 9000.0;
+    const int ntri = nb * (nb - 1) * (nb - 2) / 6;
+    // lane's first triplet: the lexicographic combination of rank `lane`
+    int ci = 0, cj = 1, ck = 2;
+    {
+        int rem = lane;
+        while (ci < nb - 2) {
+            const int cnt = (nb - 1 - ci) * (nb - 2 - ci) / 2;   // triplets starting at ci
+            if (rem < cnt) break;
+            rem -= cnt;
+            ci++;
+        }
+        cj = ci + 1;
+        while (cj < nb - 1) {
+            const int cnt = nb - 1 - cj;                           // triplets starting at (ci, cj)
+            if (rem < cnt) break;
+            rem -= cnt;
+            cj++;
+        }
+        ck = cj + 1 + rem;
+    }
+    for (int c0 = 0; c0 < ntri; c0 += kWave) {
+        TripletRecord rec = {0.0f, 0.0f, 0.0f, 0u};
+        if (c0 + lane < ntri) rec = make_record(angs, nb, ci, cj, ck, tol, skip_invalid);
+        // advance the lane's combination by 64 ranks for the next chunk
+        for (int step = 0; step < kWave && c0 + kWave + lane < ntri; step++) {
+            if (++ck < nb) continue;
+            if (++cj < nb - 1) {
+                ck = cj + 1;
+                continue;
+            }
+            ci++;
+            cj = ci + 1;
+            ck = cj + 1;
+        }
+        const int nchunk = min(kWave, ntri - c0);
+        for (int t = 0; t < nchunk; t++) {
+            const uint32_t meta = (uint32_t)__builtin_amdgcn_readlane((int)rec.meta, t);
+            if (!(meta >> 27 & 1u)) continue;
+            const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.x0), t));
+            const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.x1), t));
+            const float x2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.x2), t));
+            const int i = meta & 63, j = meta >> 6 & 63, k = meta >> 12 & 63;
+            const int u0 = meta >> 18 & 3, u1 = meta >> 20 & 3, u2 = meta >> 22 & 3;
+            const bool e1 = meta >> 24 & 1u, e2 = meta >> 25 & 1u, e3 = meta >> 26 & 1u;
+            const double dx0 = x0, dx1 = x1, dx2 = x2;
+            for (int base = 0; base < ntrip; base += kWave) {
+                const int q = base + lane;
+                bool pass = false;
+                float t1f = 0.0f, t2f = 0.0f, t3f = 0.0f;
+                if (q < ntrip) {
+                    double r0, r1, r2;
+                    if (kLdsLib) {
+                        r0 = la0[q];
+                        r1 = la1[q];
+                        r2 = la2[q];
+                    } else {
+                        r0 = ta[(int64_t)q * 3 + 0];
+                        r1 = ta[(int64_t)q * 3 + 1];
+                        r2 = ta[(int64_t)q * 3 + 2];
+                    }
+                    const double t1 = fabs(r0 - dx0), t2 = fabs(r1 - dx1), t3 = fabs(r2 - dx2);
+                    // the reference stops at the first failing test; outcome and kept
+                    // values are the same
+                    pass = !(t1 > tol) && !(t2 > tol) && !(t3 > tol);
+                    t1f = (float)t1;
+                    t2f = (float)t2;
+                    t3f = (float)t3;
+                }
+                uint64_t m = __ballot(pass);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const float g0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t1f), l));
+                    const float g1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t2f), l));
+                    const float g2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t3f), l));
+                    if (lane == 0) {
+                        const int row = base + l;
+                        int id[3];
+                        if (kLdsLib) {
+                            id[0] = li[row];
+                            id[1] = li[ntrip + row];
+                            id[2] = li[2 * ntrip + row];
+                        } else {
+                            id[0] = ti[(int64_t)row * 3 + 0];
+                            id[1] = ti[(int64_t)row * 3 + 1];
+                            id[2] = ti[(int64_t)row * 3 + 2];
+                        }
+                        const int f0 = id[u0], f1 = id[u1], f2 = id[u2];
+                        if ((unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
+                            (unsigned)f2 < (unsigned)nfam) {
+                            const double w1 = tol - 0.5 * (double)(g0 + g1);
+                            const double w2 = tol - 0.5 * (double)(g0 + g2);
+                            const double w3 = tol - 0.5 * (double)(g1 + g2);
+                            add_vote(aw, ac, f0 * nb + i, w1);
+                            add_vote(aw, ac, f1 * nb + j, w2);
+                            add_vote(aw, ac, f2 * nb + k, w3);
+                            if (e1) {
+                                add_vote(aw, ac, f0 * nb + j, w1);
+                                add_vote(aw, ac, f1 * nb + i, w2);
+                                add_vote(aw, ac, f2 * nb + k, w3);
+                            }
+                            if (e2) {
+                                add_vote(aw, ac, f0 * nb + i, w1);
+                                add_vote(aw, ac, f1 * nb + k, w2);
+                                add_vote(aw, ac, f2 * nb + j, w3);
+                            }
+                            if (e3) {
+                                add_vote(aw, ac, f0 * nb + k, w1);
+                                add_vote(aw, ac, f1 * nb + j, w2);
+                                add_vote(aw, ac, f2 * nb + i, w3);
+                            }
+                            if (e1 && e2 && e3) {
+                                // the reference adds these six weights before the six
+                                // counts; the two arrays are separate, so per array the
+                                // order is the same
+                                add_vote(aw, ac, f0 * nb + k, w1);
+                                add_vote(aw, ac, f1 * nb + i, w2);
+                                add_vote(aw, ac, f2 * nb + j, w3);
+                                add_vote(aw, ac, f0 * nb + j, w1);
+                                add_vote(aw, ac, f1 * nb + k, w2);
+                                add_vote(aw, ac, f2 * nb + i, w3);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // lane 0's LDS writes become visible to the other lanes of the wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    if (lane < nb) {
+        const int b = lane;
+        float mx = -1.0e12f, sum = 0.0f;
+        float top = -INFINITY, second = -INFINITY;
+        int best = 0;
+        for (int f = 0; f < nfam; f++) {
+            const float v = aw[f * nb + b];
+            sum += v;
+            if (v > mx) {
+                mx = v;
+                best = f;
+            }
+            if (v > top) {
+                second = top;
+                top = v;
+            } else if (v > second) {
+                second = v;
+            }
+        }
+        const int mxvote = (int)mx, votes = (int)sum;
+        float c = 0.0f;
+        if (votes >= 1) c = (float)((double)(top - second) / (double)votes);
+        fam[p * nb + b] = best;
+        cmo[p * nb + b] = c;
+        rank[p * nb + b] = (float)((double)(nb - b) / (double)nb * (double)c * (double)mxvote);
+    }
+    float *ow = accw + p * nfam * nb, *oc = acc + p * nfam * nb;
+    for (int t = lane; t < nfam * nb; t += kWave) {
+        ow[t] = aw[t];
+        oc[t] = ac[t];
+    }
+}
+
+}  // namespace
+
+extern "C" const char *ebsd_tripvote_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity, int64_t npats, int nbands,
+                             const double *d_tripang, const int32_t *d_tripid, int ntrip, int nfam, double angtol,
+                             float *d_accw, int32_t *d_fam, float *d_rank, float *d_cm, float *d_acc, void *stream)
+{
+    if (npats < 0) return fail(-1, "npats must be >= 0");
+    if (nbands < 1 || nbands > kMaxBands) return fail(-1, "nbands must be in [1, 64]");
+    if (nfam < 2) return fail(-1, "nfam must be >= 2 (the confidence metric takes the second-best family)");
+    if (ntrip < 0) return fail(-1, "ntrip must be >= 0");
+    const size_t per_bytes = ((size_t)nbands * nbands + 2 * (size_t)nfam * nbands) * sizeof(float);
+    const size_t lib_bytes = (size_t)ntrip * 3 * sizeof(double) + (size_t)((ntrip + 3) & ~3) * 3 * sizeof(int32_t);
+    if (per_bytes > kLdsBudget) return fail(-1, "nbands^2 + 2 nfam nbands floats exceed the 64 KiB of LDS per pattern");
+    if (npats == 0) return 0;
+    if (!d_bandnorms || !d_intensity || !d_accw || !d_fam || !d_rank || !d_cm || !d_acc ||
+        (ntrip > 0 && (!d_tripang || !d_tripid)))
+        return fail(-1, "null device pointer");
+    int waves = (int)(kLdsBudget / per_bytes);
+    if (waves > 4) waves = 4;
+    const bool lds_lib = lib_bytes + waves * per_bytes <= kLdsBudget;
+    const size_t lds = (lds_lib ? lib_bytes : 0) + waves * per_bytes;
+    const int64_t groups = (npats + waves - 1) / waves;
+    if (groups > 0x7fffffff) return fail(-1, "too many patterns for one launch");
+    if (lds_lib)
+        hipLaunchKernelGGL(k_tripvote<true>, dim3((unsigned)groups), dim3(waves * kWave), lds, (hipStream_t)stream,
+                           d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid, ntrip, nfam, angtol, d_accw,
+                           d_fam, d_rank, d_cm, d_acc);
+    else
+        hipLaunchKernelGGL(k_tripvote<false>, dim3((unsigned)groups), dim3(waves * kWave), lds, (hipStream_t)stream,
+                           d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid, ntrip, nfam, angtol, d_accw,
+                           d_fam, d_rank, d_cm, d_acc);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(-2, std::string("k_tripvote launch: ") + hipGetErrorString(e));
+    return 0;
+}

@@ -161,3 +161,33 @@ def grain_scan(nrows, ncols, shape=(60, 60), dtype="uint8", seed=0, grains=4, fa
             x = torch.poisson(x, generator=gen).clamp(0, 255 if dt == torch.uint8 else 65535)
         out[s0:s0 + step] = x.to(dt)
     return out
+
+
+def cubic_band_normals(families, npats, nbands=9, seed=0, noise_deg=0.3, distinct=4096):
+    """Band normals of a cubic phase as radon2pole would give them for npats patterns:
+    ``distinct`` random orientations, each with nbands distinct poles (no two within 3
+    degrees, antiparallel counted as one) drawn from ``families`` (list of (n_i, 3)
+    integer pole arrays, e.g. BandIndexer.families), repeated to npats patterns, each
+    tilted by Gaussian noise of noise_deg degrees per component.  float32 (npats, nbands,
+    3) and the true family of each band (npats, nbands) int32."""
+    rng = np.random.default_rng(seed)
+    allp = np.concatenate([np.asarray(f, np.float64) / np.linalg.norm(f, axis=1, keepdims=True) for f in families])
+    famid = np.concatenate([np.full(len(f), i) for i, f in enumerate(families)])
+    cos3 = math.cos(math.radians(3.0))
+    m = min(distinct, npats)
+    base = np.zeros((m, nbands, 3))
+    bfam = np.zeros((m, nbands), np.int32)
+    rots = quat_to_matrix(random_quaternions(m, rng))
+    for p in range(m):
+        chosen = []
+        for idx in rng.permutation(allp.shape[0]):
+            if all(abs(allp[idx] @ allp[c]) < cos3 for c in chosen):
+                chosen.append(idx)
+                if len(chosen) == nbands:
+                    break
+        base[p, :len(chosen)] = allp[chosen] @ rots[p].T
+        bfam[p, :len(chosen)] = famid[chosen]
+    pick = np.arange(npats) % m
+    n = base[pick] + rng.normal(0.0, math.radians(noise_deg), (npats, nbands, 3))
+    n /= np.linalg.norm(n, axis=2, keepdims=True)
+    return n.astype(np.float32), bfam[pick]

@@ -95,3 +95,33 @@ def orientation_agreement(got, res, meta, pc, shape):
         vs_true.append(ori.misorientation_deg(Ra, R_true[q]))
         fitted += 1
     return fitted, worst, float(np.median(vs_true)) if vs_true else float('inf')
+
+
+def cubic_bands(indexer, npats, nbands, seed=0, noise_deg=0.0):
+    """Band normals of npats random orientations of a cubic phase: nbands distinct poles
+    (no two within 3 degrees, antiparallel counted as the same) drawn from the indexer's
+    pole families, rotated, optionally tilted by Gaussian noise of noise_deg degrees per
+    component; float32 (npats, nbands, 3) and the true family of each band."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(seed)
+    fams = indexer.families
+    allp = np.concatenate([f / np.linalg.norm(f, axis=1, keepdims=True) for f in fams])
+    famid = np.concatenate([np.full(f.shape[0], i) for i, f in enumerate(fams)])
+    out = np.zeros((npats, nbands, 3), np.float32)
+    fam = np.zeros((npats, nbands), np.int32)
+    cos3 = np.cos(np.radians(3.0))
+    for p in range(npats):
+        chosen = []
+        for idx in rng.permutation(allp.shape[0]):
+            if all(abs(allp[idx] @ allp[c]) < cos3 for c in chosen):
+                chosen.append(idx)
+            if len(chosen) == nbands:
+                break
+        R = Rotation.random(random_state=rng).as_matrix()
+        n = allp[chosen] @ R.T
+        if noise_deg:
+            n = n + rng.normal(0.0, np.radians(noise_deg), n.shape)
+            n /= np.linalg.norm(n, axis=1, keepdims=True)
+        out[p, :len(chosen)] = n
+        fam[p, :len(chosen)] = famid[chosen]
+    return out, fam

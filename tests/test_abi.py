@@ -35,8 +35,29 @@ def test_library_exports_every_declared_symbol():
     assert set(declared_nl) == set(_hip.NLPAR_SYMBOLS)
     for name in declared_nl:
         assert hasattr(nl, name), name
-    # every header's declarations are covered by one of the two libraries
-    assert set(_declared()) == set(declared) | set(declared_nl)
+    tv = _hip.load_tripvote()
+    declared_tv = _declared("ebsd_tripvote.h")
+    assert set(declared_tv) == set(_hip.TRIPVOTE_SYMBOLS)
+    for name in declared_tv:
+        assert hasattr(tv, name), name
+    # every header's declarations are covered by one of the three libraries
+    assert set(_declared()) == set(declared) | set(declared_nl) | set(declared_tv)
+
+
+def test_tripvote_error_paths_without_device_compute():
+    from pyebsdindex_amd import _hip
+    tv = _hip.load_tripvote()
+    args = lambda npats, nb, ntrip, nfam: (None, None, npats, nb, None, None, ntrip, nfam, 2.0,
+                                           None, None, None, None, None, None)
+    assert tv.ebsd_tripvote(*args(4, 65, 10, 4)) < 0
+    assert b"nbands" in tv.ebsd_tripvote_last_error()
+    assert tv.ebsd_tripvote(*args(4, 9, 10, 1)) < 0
+    assert b"nfam" in tv.ebsd_tripvote_last_error()
+    assert tv.ebsd_tripvote(*args(4, 9, 10, 4)) < 0
+    assert b"null" in tv.ebsd_tripvote_last_error()
+    assert tv.ebsd_tripvote(*args(4, 64, 10, 200)) < 0
+    assert b"LDS" in tv.ebsd_tripvote_last_error()
+    assert tv.ebsd_tripvote(*args(0, 9, 10, 4)) == 0          # no patterns: nothing to do
 
 
 def test_nlpar_error_paths_without_device_compute():

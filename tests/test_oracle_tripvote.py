@@ -1,0 +1,138 @@
+"""The triplet-vote oracle (oracle/tripvote_ref.py, csrc/ref_tripvote.c) and the cubic
+triplet library (pyebsdindex_amd/tripletvote.py), pinned on the CPU:
+
+* the library's pair-angle counts are the reference's own test values
+  (pyebsdindex/tests/test_tripletvote.py:32-47) and its angles are the closed-form
+  angles between cubic poles; the triplet table equals the reference's
+  (tripletvote.py:207-213);
+* a hand-solved scalene triangle: which band takes which family follows from the
+  geometry (the vertex between the 30 and 40 degree edges is family 0, ...), whatever
+  order the bands come in;
+* an invalid last band leaves the other bands' votes bit-identical to those of the
+  shorter band list (the reference's extra votes for near-equal angles swap bands by
+  their place in the triplet, so a permutation of the bands is not a symmetry of the
+  vote and is not tested);
+* on exact cubic poles in random orientations the vote names the true family of most
+  bands (a wrong vertex order or unsort would scramble them).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import tripvote_ref
+from pyebsdindex_amd import tripletvote as tv
+
+from helpers import cubic_bands
+
+
+def _vote(bn, ind, inten=None, tol=None):
+    return tripvote_ref.tripvote(bn, inten, ind.lut, ind.angTol if tol is None else tol,
+                                 ind.angtriplets["angles"], ind.angtriplets["familyid"], ind.npolefamilies)
+
+
+def test_lut_is_the_reference_table():
+    # tripletvote.py:207-213: LUT[:, luta[i]] = lutb[i]
+    luta = [[0, 1, 2], [0, 2, 1], [1, 0, 2], [1, 2, 0], [2, 0, 1], [2, 1, 0]]
+    lutb = [[0, 1, 2], [1, 0, 2], [0, 2, 1], [2, 0, 1], [1, 2, 0], [2, 1, 0]]
+    want = np.zeros((3, 3, 3, 3), dtype=np.int64)
+    for a, b in zip(luta, lutb):
+        want[:, a[0], a[1], a[2]] = b
+    assert np.array_equal(tv.triplet_lut(), want)
+
+
+@pytest.mark.parametrize("name,poles,nang,nuniq", [("FCC", tv.FCC_POLES, 21, 17), ("BCC", tv.BCC_POLES, 34, 28)])
+def test_library_golden_counts(name, poles, nang, nuniq):
+    ind = tv.addphase(name)
+    assert np.array_equal(ind.polefamilies, poles)
+    a = ind.angpairs["angles"]
+    assert a.size == nang and np.unique(a).size == nuniq
+
+
+def test_library_closed_form_angles():
+    ind = tv.addphase("FCC")
+    a, f = ind.angpairs["angles"], ind.angpairs["familyid"]
+    def angs(i, j):
+        return set(np.round(a[(f[:, 0] == i) & (f[:, 1] == j)], 2))
+    deg = lambda c: round(float(np.degrees(np.arccos(c))), 2)
+    # {002}: 90;  {002}-{111}: 54.74;  {111}-{111}: 70.53;  {002}-{022}: 45, 90
+    assert angs(0, 0) == {90.0}
+    assert angs(0, 1) == {deg(1 / np.sqrt(3))}
+    assert angs(1, 1) == {deg(1 / 3)}
+    assert angs(0, 2) == {45.0, 90.0}
+    # {002}-{113}: cos = 3/sqrt(11), 1/sqrt(11)
+    assert angs(0, 3) == {deg(3 / np.sqrt(11)), deg(1 / np.sqrt(11))}
+    fams = ind.families
+    assert [f_.shape[0] for f_ in fams] == [6, 8, 12, 24]
+    # every triplet is ascending, its ids are families, its angles are pair angles
+    t, ti = ind.angtriplets["angles"], ind.angtriplets["familyid"]
+    assert np.all(np.diff(t, axis=1) >= 0) and ti.min() >= 0 and ti.max() < ind.npolefamilies
+    assert set(np.round(t.ravel(), 2)) <= set(np.round(a, 2))
+
+
+def _triangle(ab, ac, bc):
+    """Unit vectors a, b, c with the given pairwise angles (degrees)."""
+    r = np.radians
+    a = np.array([0.0, 0.0, 1.0])
+    b = np.array([np.sin(r(ab)), 0.0, np.cos(r(ab))])
+    cz = np.cos(r(ac))
+    cx = (np.cos(r(bc)) - cz * np.cos(r(ab))) / np.sin(r(ab))
+    c = np.array([cx, np.sqrt(1 - cx * cx - cz * cz), cz])
+    return np.stack([a, b, c]).astype(np.float32)
+
+
+def test_hand_solved_triangle():
+    """Library row (30, 40, 50) with families (0, 1, 2) at vertices (A, B, C): A joins the
+    30 and 40 degree edges, B the 30 and 50, C the 40 and 50.  Bands with ab = 30,
+    ac = 50, bc = 40: a joins 30 and 50 -> family 1, b joins 30 and 40 -> family 0,
+    c joins 40 and 50 -> family 2; one vote each, weight angTol for exact angles."""
+    lut = tv.triplet_lut()
+    lib_a = np.array([[30.0, 40.0, 50.0]])
+    lib_i = np.array([[0, 1, 2]])
+    bands = _triangle(30, 50, 40)
+    want = {0: 1, 1: 0, 2: 2}
+    for order in itertools.permutations(range(3)):
+        bn = bands[list(order)]
+        accW, fam, rank, cm, acc = tripvote_ref.tripvote(bn, None, lut, 2.0, lib_a, lib_i, 3)
+        for pos, band in enumerate(order):
+            assert fam[0, pos] == want[band], (order, pos)
+            col = acc[0, :, pos]
+            assert col[want[band]] == 1 and col.sum() == 1
+            assert abs(accW[0, want[band], pos] - 2.0) < 1e-4
+            # one vote of weight ~2: trunc(sum) = 1, cm = 2 - 0, rank = (3 - pos)/3 * cm * 1
+            assert abs(cm[0, pos] - accW[0, want[band], pos]) < 1e-6
+            assert abs(rank[0, pos] - (3 - pos) / 3 * cm[0, pos]) < 1e-6
+
+
+def test_triangle_outside_tolerance_gets_no_vote():
+    lut = tv.triplet_lut()
+    accW, fam, rank, cm, acc = tripvote_ref.tripvote(_triangle(30, 50, 40), None, lut, 2.0,
+                                                     np.array([[30.0, 40.0, 52.5]]), np.array([[0, 1, 2]]), 3)
+    assert acc.sum() == 0 and np.all(cm == 0) and np.all(rank == 0) and np.all(fam == 0)
+
+
+@pytest.mark.parametrize("phase", ["FCC", "BCC"])
+def test_invalid_last_band_is_the_shorter_list(phase):
+    ind = tv.addphase(phase)
+    bn, _ = cubic_bands(ind, 40, 9, seed=3, noise_deg=0.2)
+    inten = np.ones((40, 9), np.float32)
+    inten[:, 8] = 0.0
+    full = _vote(bn, ind, inten)
+    short = _vote(np.ascontiguousarray(bn[:, :8]), ind)
+    assert np.array_equal(full[0][:, :, :8], short[0])      # weighted votes, bit for bit
+    assert np.array_equal(full[4][:, :, :8], short[4])      # counts
+    assert np.all(full[4][:, :, 8] == 0)
+    assert np.array_equal(full[1][:, :8], short[1])
+    assert np.array_equal(full[3][:, :8], short[3])
+
+
+@pytest.mark.parametrize("phase", ["FCC", "BCC"])
+def test_exact_poles_name_their_family(phase):
+    ind = tv.addphase(phase)
+    bn, fam_true = cubic_bands(ind, 200, 9, seed=11, noise_deg=0.0)
+    accW, fam, rank, cm, acc = _vote(bn, ind)
+    agree = float(np.mean(fam == fam_true))
+    assert agree > 0.85, agree
+    # the true family of a band has at least one vote in most of its 28 triplets
+    got = np.take_along_axis(acc, fam_true[:, None, :], axis=1)[:, 0, :]
+    assert np.median(got) >= 20
