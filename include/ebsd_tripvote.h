@@ -29,7 +29,14 @@ extern "C" {
  *   d_tripid     (ntrip, 3) int32 family of each vertex, in [0, nfam) (nfam >= 2, and
  *                nbands^2 + 2 nfam nbands floats within 64 KiB; a row with an id outside
  *                [0, nfam) votes for nothing);
- *   angtol       degrees.
+ *   angtol       degrees;
+ *   d_bin_start, d_bin_rows, nbins, bin_width: an optional row index (NULL = test every
+ *                row).  Bin b holds the rows d_bin_rows[d_bin_start[b] .. d_bin_start[b+1])
+ *                in ascending order; it must contain every row whose first angle lies
+ *                within angtol of some x in [b bin_width, (b+1) bin_width) -- the first bin
+ *                extended to -inf and the last to +inf.  A triplet whose smallest angle x0
+ *                falls in bin b tests only that list; the outputs are the same as without
+ *                the index.  No counterpart in the reference (its loop tests every row).
  * Outputs, the five arrays of the reference in its order:
  *   d_accw  (npats, nfam, nbands) float32 weighted votes (weight angtol minus the mean
  *           angle error of the two triangle edges at the band),
@@ -42,6 +49,7 @@ extern "C" {
  * built into the kernel. */
 int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity, int64_t npats, int nbands,
                   const double *d_tripang, const int32_t *d_tripid, int ntrip, int nfam, double angtol,
+                  const int32_t *d_bin_start, const int32_t *d_bin_rows, int nbins, double bin_width,
                   float *d_accw, int32_t *d_fam, float *d_rank, float *d_cm, float *d_acc, void *stream);
 
 const char *ebsd_tripvote_last_error(void);

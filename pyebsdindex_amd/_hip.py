@@ -158,7 +158,8 @@ def load_tripvote(path: str = TRIPVOTE_LIB_PATH):
         except OSError as e:
             raise HipLibraryError(f"failed to load {path}: {e}") from e
         P, I = ctypes.c_void_p, ctypes.c_int
-        L.ebsd_tripvote.argtypes = [P, P, ctypes.c_int64, I, P, P, I, I, ctypes.c_double, P, P, P, P, P, P]
+        L.ebsd_tripvote.argtypes = [P, P, ctypes.c_int64, I, P, P, I, I, ctypes.c_double, P, P, I,
+                                    ctypes.c_double, P, P, P, P, P, P]
         L.ebsd_tripvote.restype = ctypes.c_int
         L.ebsd_tripvote_last_error.argtypes = []
         L.ebsd_tripvote_last_error.restype = ctypes.c_char_p

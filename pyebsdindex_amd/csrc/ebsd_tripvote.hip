@@ -103,16 +103,22 @@ __device__ __forceinline__ TripletRecord make_record(const float *angs, int nb, 
     return r;
 }
 
-__device__ __forceinline__ void add_vote(float *aw, float *ac, int idx, double w)
+// a cell of the accumulators: the weighted vote and the count, side by side (one 8-byte
+// LDS read and write per vote)
+__device__ __forceinline__ void add_vote(float2 *cell, int idx, double w)
 {
-    aw[idx] = (float)((double)aw[idx] + w);
-    ac[idx] = ac[idx] + 1.0f;
+    float2 v = cell[idx];
+    v.x = (float)((double)v.x + w);
+    v.y = v.y + 1.0f;
+    cell[idx] = v;
 }
 
 template <bool kLdsLib>
 __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, const float *__restrict__ inten,
                                                   int64_t npats, int nb, const double *__restrict__ ta,
                                                   const int32_t *__restrict__ ti, int ntrip, int nfam, double tol,
+                                                  const int32_t *__restrict__ bin_start,
+                                                  const int32_t *__restrict__ bin_rows, int nbins, double bin_width,
                                                   float *__restrict__ accw, int32_t *__restrict__ fam,
                                                   float *__restrict__ rank, float *__restrict__ cmo,
                                                   float *__restrict__ acc)
@@ -125,10 +131,9 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
     double *la2 = la1 + (kLdsLib ? ntrip : 0);
     int32_t *li = reinterpret_cast<int32_t *>(la2 + (kLdsLib ? ntrip : 0));
     float *wave_lds = reinterpret_cast<float *>(li + (kLdsLib ? 3 * ((ntrip + 3) & ~3) : 0));
-    const int per = nb * nb + 2 * nfam * nb;
+    const int per = ((nb * nb + 1) & ~1) + 2 * nfam * nb;
     float *angs = wave_lds + wave * per;
-    float *aw = angs + nb * nb;
-    float *ac = aw + nfam * nb;
+    float2 *cell = reinterpret_cast<float2 *>(angs + ((nb * nb + 1) & ~1));
     const int64_t p = (int64_t)blockIdx.x * waves + wave;
     const bool live = p < npats;             // uniform over the wave
 
@@ -143,10 +148,7 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
         }
     }
     if (live) {
-        for (int t = lane; t < nfam * nb; t += kWave) {
-            aw[t] = 0.0f;
-            ac[t] = 0.0f;
-        }
+        for (int t = lane; t < nfam * nb; t += kWave) cell[t] = make_float2(0.0f, 0.0f);
         const float *b = bn + p * nb * 3;
         const float *in = inten + p * nb;
         for (int t = lane; t < nb * nb; t += kWave) {
@@ -210,11 +212,25 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
             const int u0 = meta >> 18 & 3, u1 = meta >> 20 & 3, u2 = meta >> 22 & 3;
             const bool e1 = meta >> 24 & 1u, e2 = meta >> 25 & 1u, e3 = meta >> 26 & 1u;
             const double dx0 = x0, dx1 = x1, dx2 = x2;
-            for (int base = 0; base < ntrip; base += kWave) {
-                const int q = base + lane;
+            // the rows to test: the whole library, or the index's list for the bin of x0
+            // (ascending rows, a superset of those within tol of x0); NaN angles pass every
+            // test of the reference, so they take the whole library
+            int first = 0, count = ntrip;
+            const bool indexed = bin_start != nullptr && x0 == x0 && x1 == x1 && x2 == x2;
+            if (indexed) {
+                int bin = (int)floor(dx0 / bin_width);
+                bin = bin < 0 ? 0 : (bin >= nbins ? nbins - 1 : bin);
+                first = bin_start[bin];
+                count = bin_start[bin + 1] - first;
+            }
+            for (int base = 0; base < count; base += kWave) {
+                const int slot = base + lane;
+                int q = slot;
+                if (indexed && slot < count) q = bin_rows[first + slot];
                 bool pass = false;
                 float t1f = 0.0f, t2f = 0.0f, t3f = 0.0f;
-                if (q < ntrip) {
+                int fid0 = 0, fid1 = 0, fid2 = 0;
+                if (slot < count) {
                     double r0, r1, r2;
                     if (kLdsLib) {
                         r0 = la0[q];
@@ -232,6 +248,22 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
                     t1f = (float)t1;
                     t2f = (float)t2;
                     t3f = (float)t3;
+                    if (pass) {
+                        // the family at each vertex of the sorted triangle
+                        int id[3];
+                        if (kLdsLib) {
+                            id[0] = li[q];
+                            id[1] = li[ntrip + q];
+                            id[2] = li[2 * ntrip + q];
+                        } else {
+                            id[0] = ti[(int64_t)q * 3 + 0];
+                            id[1] = ti[(int64_t)q * 3 + 1];
+                            id[2] = ti[(int64_t)q * 3 + 2];
+                        }
+                        fid0 = id[u0];
+                        fid1 = id[u1];
+                        fid2 = id[u2];
+                    }
                 }
                 uint64_t m = __ballot(pass);
                 while (m) {
@@ -240,52 +272,43 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
                     const float g0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t1f), l));
                     const float g1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t2f), l));
                     const float g2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t3f), l));
+                    const int f0 = __builtin_amdgcn_readlane(fid0, l);
+                    const int f1 = __builtin_amdgcn_readlane(fid1, l);
+                    const int f2 = __builtin_amdgcn_readlane(fid2, l);
                     if (lane == 0) {
-                        const int row = base + l;
-                        int id[3];
-                        if (kLdsLib) {
-                            id[0] = li[row];
-                            id[1] = li[ntrip + row];
-                            id[2] = li[2 * ntrip + row];
-                        } else {
-                            id[0] = ti[(int64_t)row * 3 + 0];
-                            id[1] = ti[(int64_t)row * 3 + 1];
-                            id[2] = ti[(int64_t)row * 3 + 2];
-                        }
-                        const int f0 = id[u0], f1 = id[u1], f2 = id[u2];
                         if ((unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
                             (unsigned)f2 < (unsigned)nfam) {
                             const double w1 = tol - 0.5 * (double)(g0 + g1);
                             const double w2 = tol - 0.5 * (double)(g0 + g2);
                             const double w3 = tol - 0.5 * (double)(g1 + g2);
-                            add_vote(aw, ac, f0 * nb + i, w1);
-                            add_vote(aw, ac, f1 * nb + j, w2);
-                            add_vote(aw, ac, f2 * nb + k, w3);
+                            add_vote(cell, f0 * nb + i, w1);
+                            add_vote(cell, f1 * nb + j, w2);
+                            add_vote(cell, f2 * nb + k, w3);
                             if (e1) {
-                                add_vote(aw, ac, f0 * nb + j, w1);
-                                add_vote(aw, ac, f1 * nb + i, w2);
-                                add_vote(aw, ac, f2 * nb + k, w3);
+                                add_vote(cell, f0 * nb + j, w1);
+                                add_vote(cell, f1 * nb + i, w2);
+                                add_vote(cell, f2 * nb + k, w3);
                             }
                             if (e2) {
-                                add_vote(aw, ac, f0 * nb + i, w1);
-                                add_vote(aw, ac, f1 * nb + k, w2);
-                                add_vote(aw, ac, f2 * nb + j, w3);
+                                add_vote(cell, f0 * nb + i, w1);
+                                add_vote(cell, f1 * nb + k, w2);
+                                add_vote(cell, f2 * nb + j, w3);
                             }
                             if (e3) {
-                                add_vote(aw, ac, f0 * nb + k, w1);
-                                add_vote(aw, ac, f1 * nb + j, w2);
-                                add_vote(aw, ac, f2 * nb + i, w3);
+                                add_vote(cell, f0 * nb + k, w1);
+                                add_vote(cell, f1 * nb + j, w2);
+                                add_vote(cell, f2 * nb + i, w3);
                             }
                             if (e1 && e2 && e3) {
                                 // the reference adds these six weights before the six
                                 // counts; the two arrays are separate, so per array the
                                 // order is the same
-                                add_vote(aw, ac, f0 * nb + k, w1);
-                                add_vote(aw, ac, f1 * nb + i, w2);
-                                add_vote(aw, ac, f2 * nb + j, w3);
-                                add_vote(aw, ac, f0 * nb + j, w1);
-                                add_vote(aw, ac, f1 * nb + k, w2);
-                                add_vote(aw, ac, f2 * nb + i, w3);
+                                add_vote(cell, f0 * nb + k, w1);
+                                add_vote(cell, f1 * nb + i, w2);
+                                add_vote(cell, f2 * nb + j, w3);
+                                add_vote(cell, f0 * nb + j, w1);
+                                add_vote(cell, f1 * nb + k, w2);
+                                add_vote(cell, f2 * nb + i, w3);
                             }
                         }
                     }
@@ -304,7 +327,7 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
         float top = -INFINITY, second = -INFINITY;
         int best = 0;
         for (int f = 0; f < nfam; f++) {
-            const float v = aw[f * nb + b];
+            const float v = cell[f * nb + b].x;
             sum += v;
             if (v > mx) {
                 mx = v;
@@ -326,8 +349,9 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
     }
     float *ow = accw + p * nfam * nb, *oc = acc + p * nfam * nb;
     for (int t = lane; t < nfam * nb; t += kWave) {
-        ow[t] = aw[t];
-        oc[t] = ac[t];
+        const float2 v = cell[t];
+        ow[t] = v.x;
+        oc[t] = v.y;
     }
 }
 
@@ -337,13 +361,16 @@ extern "C" const char *ebsd_tripvote_last_error(void) { return g_err.c_str(); }
 
 extern "C" int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity, int64_t npats, int nbands,
                              const double *d_tripang, const int32_t *d_tripid, int ntrip, int nfam, double angtol,
+                             const int32_t *d_bin_start, const int32_t *d_bin_rows, int nbins, double bin_width,
                              float *d_accw, int32_t *d_fam, float *d_rank, float *d_cm, float *d_acc, void *stream)
 {
     if (npats < 0) return fail(-1, "npats must be >= 0");
     if (nbands < 1 || nbands > kMaxBands) return fail(-1, "nbands must be in [1, 64]");
     if (nfam < 2) return fail(-1, "nfam must be >= 2 (the confidence metric takes the second-best family)");
     if (ntrip < 0) return fail(-1, "ntrip must be >= 0");
-    const size_t per_bytes = ((size_t)nbands * nbands + 2 * (size_t)nfam * nbands) * sizeof(float);
+    if (d_bin_start && (nbins < 1 || !(bin_width > 0.0) || !d_bin_rows))
+        return fail(-1, "a row index needs nbins >= 1, bin_width > 0 and its row list");
+    const size_t per_bytes = ((size_t)((nbands * nbands + 1) & ~1) + 2 * (size_t)nfam * nbands) * sizeof(float);
     const size_t lib_bytes = (size_t)ntrip * 3 * sizeof(double) + (size_t)((ntrip + 3) & ~3) * 3 * sizeof(int32_t);
     if (per_bytes > kLdsBudget) return fail(-1, "nbands^2 + 2 nfam nbands floats exceed the 64 KiB of LDS per pattern");
     if (npats == 0) return 0;
@@ -358,12 +385,12 @@ extern "C" int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity,
     if (groups > 0x7fffffff) return fail(-1, "too many patterns for one launch");
     if (lds_lib)
         hipLaunchKernelGGL(k_tripvote<true>, dim3((unsigned)groups), dim3(waves * kWave), lds, (hipStream_t)stream,
-                           d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid, ntrip, nfam, angtol, d_accw,
-                           d_fam, d_rank, d_cm, d_acc);
+                           d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid, ntrip, nfam, angtol,
+                           d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam, d_rank, d_cm, d_acc);
     else
         hipLaunchKernelGGL(k_tripvote<false>, dim3((unsigned)groups), dim3(waves * kWave), lds, (hipStream_t)stream,
-                           d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid, ntrip, nfam, angtol, d_accw,
-                           d_fam, d_rank, d_cm, d_acc);
+                           d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid, ntrip, nfam, angtol,
+                           d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam, d_rank, d_cm, d_acc);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(-2, std::string("k_tripvote launch: ") + hipGetErrorString(e));
     return 0;

@@ -21,6 +21,7 @@ vote in ``bandindex`` (tripletvote.py:548-633) are not part of this module.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 
@@ -167,6 +168,24 @@ def triplet_library(polefamilies) -> dict:
             "angtriplets": {"angles": lib_a[order], "familyid": lib_i[order]}}
 
 
+def row_index(tripAngles, angTol: float, bin_width: float = 0.5, max_angle: float = 90.0):
+    """Index of library rows by their first (smallest) angle, for the device vote: bin b
+    covers smallest triplet angles x0 in [b w, (b+1) w) (the first bin down to -inf, the
+    last up to +inf) and lists, ascending, every row whose first angle is within angTol
+    (plus 1e-6 of slack) of that range.  Returns (start (nbins+1,) int32, rows int32,
+    bin_width)."""
+    a0 = np.asarray(tripAngles, dtype=np.float64).reshape(-1, 3)[:, 0]
+    nbins = int(math.ceil(max_angle / bin_width)) + 1
+    lo = np.arange(nbins) * bin_width - angTol - 1e-6
+    hi = (np.arange(nbins) + 1) * bin_width + angTol + 1e-6
+    lo[0], hi[-1] = -np.inf, np.inf
+    lists = [np.nonzero((a0 >= lo[b]) & (a0 <= hi[b]))[0] for b in range(nbins)]
+    start = np.zeros(nbins + 1, dtype=np.int32)
+    start[1:] = np.cumsum([len(x) for x in lists])
+    rows = np.concatenate(lists).astype(np.int32) if lists else np.zeros(0, np.int32)
+    return start, rows, float(bin_width)
+
+
 def addphase(libtype: str = "FCC", angTol: float = 2.0, polefamilies=None):
     """A BandIndexer for an FCC (space group 225) or BCC (229) phase with the reference's
     default pole families (tripletvote.py:70-140), or cubic families of one's own."""
@@ -201,26 +220,30 @@ class BandIndexer:
 
     def _device_library(self, device):
         import torch
-        key = str(device)
+        key = (str(device), self.angTol)
         if key not in self._dev:
             a = torch.as_tensor(np.ascontiguousarray(self.angtriplets["angles"], dtype=np.float64), device=device)
             f = torch.as_tensor(np.ascontiguousarray(self.angtriplets["familyid"], dtype=np.int32), device=device)
-            self._dev[key] = (a, f)
+            start, rows, bw = row_index(self.angtriplets["angles"], self.angTol)
+            idx = (torch.as_tensor(start, device=device), torch.as_tensor(rows, device=device), bw) if rows.size else None
+            self._dev[key] = (a, f, idx)
         return self._dev[key]
 
-    def tripvote(self, bandnorms, band_intensity=None, stream=None):
+    def tripvote(self, bandnorms, band_intensity=None, stream=None, use_index=True):
         """Votes of (npats, nbands, 3) float32 band normals (device tensor); returns the
-        five outputs of _tripvote_numba as device tensors."""
-        a, f = self._device_library(bandnorms.device)
+        five outputs of _tripvote_numba as device tensors.  The library and its row index
+        stay on the device between calls."""
+        a, f, idx = self._device_library(bandnorms.device)
         return tripvote_device(bandnorms, band_intensity, self.lut, self.angTol, a, f,
-                               self.npolefamilies, stream=stream)
+                               self.npolefamilies, stream=stream, index=idx if use_index else None)
 
     def bandindex(self, *args, **kwargs):
         raise NotImplementedError("band assignment and the QUEST orientation fit that follow the vote "
                                   "(tripletvote.py:548-633) are not built on the device")
 
 
-def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, nfam, stream=None):
+def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, nfam, stream=None,
+                    index="auto"):
     """Device counterpart of BandIndexer._tripvote_numba (tripletvote.py:1019-1207).
 
     bandnorms: (npats, nbands, 3) or (nbands, 3) float32 unit band normals on the device.
@@ -231,7 +254,10 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
     Returns (accumulatorW, bandFam, bandRank, band_cm, accumulator) as device tensors of
     shapes (npats, nfam, nbands) float32, (npats, nbands) int32, (npats, nbands) float32
     x2, (npats, nfam, nbands) float32 -- the reference's output order (weighted votes
-    first, vote counts last)."""
+    first, vote counts last).
+    index: "auto" builds the row index of row_index() when tripAngles is a host array
+    (the outputs do not depend on it), None tests every row, or a (start, rows, width)
+    triple of device tensors from row_index()."""
     import torch
     lib = load_tripvote()
     if not isinstance(bandnorms, torch.Tensor) or bandnorms.device.type != "cuda":
@@ -257,6 +283,14 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
             raise ValueError(f"band_intensity must be ({npats}, {nb}), got {tuple(inten.shape)}")
     if isinstance(tripID, np.ndarray) and tripID.size and (tripID.min() < 0 or tripID.max() >= int(nfam)):
         raise ValueError(f"tripID holds families outside [0, {int(nfam)})")
+    if isinstance(index, str):
+        if index != "auto":
+            raise ValueError("index must be 'auto', None or a row_index() triple")
+        index = None
+        if isinstance(tripAngles, np.ndarray):
+            st, rw, bw = row_index(tripAngles, float(angTol))
+            if rw.size:
+                index = (torch.as_tensor(st, device=dev), torch.as_tensor(rw, device=dev), bw)
     ta = torch.as_tensor(tripAngles, device=dev).to(torch.float64).contiguous()
     ti = torch.as_tensor(tripID, device=dev).to(torch.int32).contiguous()
     if ta.dim() != 2 or ta.shape[1] != 3 or tuple(ti.shape) != tuple(ta.shape):
@@ -273,6 +307,10 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
     rc = lib.ebsd_tripvote(P(bn.data_ptr()), P(inten.data_ptr()), ctypes.c_int64(npats), ctypes.c_int(nb),
                            P(ta.data_ptr()), P(ti.data_ptr()), ctypes.c_int(int(ta.shape[0])),
                            ctypes.c_int(nfam), ctypes.c_double(float(angTol)),
+                           P(index[0].data_ptr() if index is not None else None),
+                           P(index[1].data_ptr() if index is not None else None),
+                           ctypes.c_int(int(index[0].numel()) - 1 if index is not None else 0),
+                           ctypes.c_double(index[2] if index is not None else 0.0),
                            P(accW.data_ptr()), P(fam.data_ptr()), P(rank.data_ptr()), P(cm.data_ptr()),
                            P(acc.data_ptr()), P(stream.cuda_stream))
     check_tripvote(lib, rc)

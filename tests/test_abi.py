@@ -48,7 +48,7 @@ def test_tripvote_error_paths_without_device_compute():
     from pyebsdindex_amd import _hip
     tv = _hip.load_tripvote()
     args = lambda npats, nb, ntrip, nfam: (None, None, npats, nb, None, None, ntrip, nfam, 2.0,
-                                           None, None, None, None, None, None)
+                                           None, None, 0, 0.0, None, None, None, None, None, None)
     assert tv.ebsd_tripvote(*args(4, 65, 10, 4)) < 0
     assert b"nbands" in tv.ebsd_tripvote_last_error()
     assert tv.ebsd_tripvote(*args(4, 9, 10, 1)) < 0

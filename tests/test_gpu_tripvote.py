@@ -156,3 +156,23 @@ def test_more_families(extra):
     print(f"tripvote families {ind.npolefamilies}, {ind.angtriplets['angles'].shape[0]} triplets", flush=True)
     bn, _ = cubic_bands(ind, 120, 9, seed=len(extra), noise_deg=0.2)
     _compare(ind, bn, None)
+
+
+@pytest.mark.parametrize("phase,tol", [("FCC", 2.0), ("BCC", 0.5), ("BCC", 3.0)])
+def test_row_index_gives_identical_outputs(phase, tol):
+    """The optional row index only skips rows that cannot pass the first test, and the
+    listed rows keep the library order: all five outputs are bit-identical with and
+    without it."""
+    import torch
+    from pyebsdindex_amd import tripletvote as tv
+    ind = tv.addphase(phase, angTol=tol)
+    bn, _ = cubic_bands(ind, 400, 9, seed=91, noise_deg=0.4)
+    rng = np.random.default_rng(3)
+    inten = rng.uniform(0.1, 1.0, (400, 9)).astype(np.float32)
+    inten[rng.random((400, 9)) < 0.05] = 0.0
+    dev = torch.device("cuda", 0)
+    bnd, intd = torch.as_tensor(bn, device=dev), torch.as_tensor(inten, device=dev)
+    a = ind.tripvote(bnd, intd, use_index=True)
+    b = ind.tripvote(bnd, intd, use_index=False)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)

@@ -136,3 +136,25 @@ def test_exact_poles_name_their_family(phase):
     # the true family of a band has at least one vote in most of its 28 triplets
     got = np.take_along_axis(acc, fam_true[:, None, :], axis=1)[:, 0, :]
     assert np.median(got) >= 20
+
+
+@pytest.mark.parametrize("phase,tol", [("FCC", 2.0), ("BCC", 0.5), ("BCC", 3.0)])
+def test_row_index_lists_every_candidate(phase, tol):
+    """The device's row index (tripletvote.row_index) lists, for each bin, every library
+    row whose first angle is within tol of an x in the bin -- checked at the bin edges and
+    inside -- in ascending order."""
+    ind = tv.addphase(phase)
+    a0 = ind.angtriplets["angles"][:, 0]
+    start, rows, bw = tv.row_index(ind.angtriplets["angles"], tol)
+    nb = start.size - 1
+    for b in range(nb):
+        lst = rows[start[b]:start[b + 1]]
+        assert np.all(np.diff(lst) > 0)
+        xs = [b * bw, b * bw + bw / 2, np.nextafter((b + 1) * bw, 0)]
+        if b == nb - 1:
+            xs += [95.0, 180.0]
+        if b == 0:
+            xs += [-1.0]
+        for x in np.float32(xs):
+            need = np.nonzero(np.abs(a0 - np.float64(x)) <= tol)[0]
+            assert set(need) <= set(lst), (b, x)
