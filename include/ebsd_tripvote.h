@@ -33,8 +33,9 @@ extern "C" {
  *   d_bin_start, d_bin_rows, nbins, bin_width: an optional row index (NULL = test every
  *                row).  Bin b holds the rows d_bin_rows[d_bin_start[b] .. d_bin_start[b+1])
  *                in ascending order; it must contain every row whose first angle lies
- *                within angtol of some x in [b bin_width, (b+1) bin_width) -- the first bin
- *                extended to -inf and the last to +inf.  A triplet whose smallest angle x0
+ *                within angtol + 1e-3 of some x in [b bin_width, (b+1) bin_width) -- the
+ *                first bin extended to -inf and the last to +inf (the kernel finds the bin
+ *                with a float32 product).  A triplet whose smallest angle x0
  *                falls in bin b tests only that list; the outputs are the same as without
  *                the index.  No counterpart in the reference (its loop tests every row).
  * Outputs, the five arrays of the reference in its order:

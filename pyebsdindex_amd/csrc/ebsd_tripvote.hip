@@ -167,6 +167,7 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
     if (!live) return;
 
     const bool skip_invalid = tol < 9000.0;
+    const float inv_bin_width = bin_start ? (float)(1.0 / bin_width) : 0.0f;
     const int ntri = nb * (nb - 1) * (nb - 2) / 6;
     // lane's first triplet: the lexicographic combination of rank `lane`
     int ci = 0, cj = 1, ck = 2;
@@ -218,7 +219,9 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
             int first = 0, count = ntrip;
             const bool indexed = bin_start != nullptr && x0 == x0 && x1 == x1 && x2 == x2;
             if (indexed) {
-                int bin = (int)floor(dx0 / bin_width);
+                // float32 product: within 2e-5 bins of the exact quotient, covered by the
+                // index's 1e-3 degree slack at the bin edges
+                int bin = (int)floorf(x0 * inv_bin_width);
                 bin = bin < 0 ? 0 : (bin >= nbins ? nbins - 1 : bin);
                 first = bin_start[bin];
                 count = bin_start[bin + 1] - first;

@@ -172,12 +172,13 @@ def row_index(tripAngles, angTol: float, bin_width: float = 0.5, max_angle: floa
     """Index of library rows by their first (smallest) angle, for the device vote: bin b
     covers smallest triplet angles x0 in [b w, (b+1) w) (the first bin down to -inf, the
     last up to +inf) and lists, ascending, every row whose first angle is within angTol
-    (plus 1e-6 of slack) of that range.  Returns (start (nbins+1,) int32, rows int32,
+    (plus 1e-3 degree of slack: the kernel finds the bin with a float32 product) of that
+    range.  Returns (start (nbins+1,) int32, rows int32,
     bin_width)."""
     a0 = np.asarray(tripAngles, dtype=np.float64).reshape(-1, 3)[:, 0]
     nbins = int(math.ceil(max_angle / bin_width)) + 1
-    lo = np.arange(nbins) * bin_width - angTol - 1e-6
-    hi = (np.arange(nbins) + 1) * bin_width + angTol + 1e-6
+    lo = np.arange(nbins) * bin_width - angTol - 1e-3
+    hi = (np.arange(nbins) + 1) * bin_width + angTol + 1e-3
     lo[0], hi[-1] = -np.inf, np.inf
     lists = [np.nonzero((a0 >= lo[b]) & (a0 <= hi[b]))[0] for b in range(nbins)]
     start = np.zeros(nbins + 1, dtype=np.int32)

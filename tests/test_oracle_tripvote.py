@@ -158,3 +158,8 @@ def test_row_index_lists_every_candidate(phase, tol):
         for x in np.float32(xs):
             need = np.nonzero(np.abs(a0 - np.float64(x)) <= tol)[0]
             assert set(need) <= set(lst), (b, x)
+            # the kernel's float32 bin of x may be off by one next to an edge: the
+            # neighbouring bin's list must hold the rows too
+            kb = int(np.floor(np.float32(x) * np.float32(1.0 / bw)))
+            kb = min(max(kb, 0), nb - 1)
+            assert set(need) <= set(rows[start[kb]:start[kb + 1]]), (b, x, kb)
