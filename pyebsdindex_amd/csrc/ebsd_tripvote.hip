@@ -358,6 +358,251 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
     }
 }
 
+// Two patterns per wavefront, one per 32-lane half (nbands <= 32): the same walk as
+// k_tripvote, but every instruction of the triplet walk, the row tests and the votes serves
+// two patterns.  Values that k_tripvote reads with v_readlane (uniform over the wave) are
+// per half here and move with ds_bpermute (__shfl).  Loops run to the larger of the two
+// halves' bounds, the other half masked; a half with no pattern has no valid triplet.
+constexpr int kHalf = 32;
+
+template <bool kLdsLib>
+__global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__ bn, const float *__restrict__ inten,
+                                                       int64_t npats, int nb, const double *__restrict__ ta,
+                                                       const int32_t *__restrict__ ti, int ntrip, int nfam,
+                                                       double tol, const int32_t *__restrict__ bin_start,
+                                                       const int32_t *__restrict__ bin_rows, int nbins,
+                                                       double bin_width, float *__restrict__ accw,
+                                                       int32_t *__restrict__ fam, float *__restrict__ rank,
+                                                       float *__restrict__ cmo, float *__restrict__ acc)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int halves = blockDim.x / kHalf;
+    const int half = threadIdx.x / kHalf, hl = threadIdx.x % kHalf;
+    const int lane = threadIdx.x % kWave;
+    const int hbase = lane & kHalf;              // first lane of this half in the wave (0 or 32)
+    double *la0 = reinterpret_cast<double *>(lds_raw);
+    double *la1 = la0 + (kLdsLib ? ntrip : 0);
+    double *la2 = la1 + (kLdsLib ? ntrip : 0);
+    int32_t *li = reinterpret_cast<int32_t *>(la2 + (kLdsLib ? ntrip : 0));
+    float *half_lds = reinterpret_cast<float *>(li + (kLdsLib ? 3 * ((ntrip + 3) & ~3) : 0));
+    const int per = ((nb * nb + 1) & ~1) + 2 * nfam * nb;
+    float *angs = half_lds + half * per;
+    float2 *cell = reinterpret_cast<float2 *>(angs + ((nb * nb + 1) & ~1));
+    const int64_t p = (int64_t)blockIdx.x * halves + half;
+    const bool live = p < npats;                 // uniform over the half
+
+    if (kLdsLib) {
+        for (int t = threadIdx.x; t < ntrip; t += blockDim.x) {
+            la0[t] = ta[(int64_t)t * 3 + 0];
+            la1[t] = ta[(int64_t)t * 3 + 1];
+            la2[t] = ta[(int64_t)t * 3 + 2];
+            li[t] = ti[(int64_t)t * 3 + 0];
+            li[ntrip + t] = ti[(int64_t)t * 3 + 1];
+            li[2 * ntrip + t] = ti[(int64_t)t * 3 + 2];
+        }
+    }
+    if (live) {
+        for (int t = hl; t < nfam * nb; t += kHalf) cell[t] = make_float2(0.0f, 0.0f);
+        const float *b = bn + p * nb * 3;
+        const float *in = inten + p * nb;
+        for (int t = hl; t < nb * nb; t += kHalf) {
+            const int r = t / nb, c = t - r * nb;
+            float d = __fadd_rn(__fadd_rn(__fmul_rn(b[r * 3], b[c * 3]), __fmul_rn(b[r * 3 + 1], b[c * 3 + 1])),
+                                __fmul_rn(b[r * 3 + 2], b[c * 3 + 2]));
+            d = fabsf(d);
+            if (d > 1.0f) d = 1.0f;
+            const float a = (float)acos((double)d);
+            float v = (float)((double)a * kRadeg);
+            if ((double)in[r] < 1e-6 || (double)in[c] < 1e-6) v = kInvalidAngle;
+            angs[t] = v;
+        }
+    }
+    __syncthreads();
+    // the two halves of a wave share every loop below (shuffles and ballots need both):
+    // only a wave with no pattern at all leaves
+    if (__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (int64_t)halves + (threadIdx.x / kWave) * 2) >= npats))
+        return;
+
+    const bool skip_invalid = tol < 9000.0;
+    const float inv_bin_width = bin_start ? (float)(1.0 / bin_width) : 0.0f;
+    const int ntri = nb * (nb - 1) * (nb - 2) / 6;
+    // the half-lane's first triplet: the lexicographic combination of rank hl
+    int ci = 0, cj = 1, ck = 2;
+    {
+        int rem = hl;
+        while (ci < nb - 2) {
+            const int cnt = (nb - 1 - ci) * (nb - 2 - ci) / 2;
+            if (rem < cnt) break;
+            rem -= cnt;
+            ci++;
+        }
+        cj = ci + 1;
+        while (cj < nb - 1) {
+            const int cnt = nb - 1 - cj;
+            if (rem < cnt) break;
+            rem -= cnt;
+            cj++;
+        }
+        ck = cj + 1 + rem;
+    }
+    for (int c0 = 0; c0 < ntri; c0 += kHalf) {
+        TripletRecord rec = {0.0f, 0.0f, 0.0f, 0u};
+        if (live && c0 + hl < ntri) rec = make_record(angs, nb, ci, cj, ck, tol, skip_invalid);
+        for (int step = 0; step < kHalf && c0 + kHalf + hl < ntri; step++) {
+            if (++ck < nb) continue;
+            if (++cj < nb - 1) {
+                ck = cj + 1;
+                continue;
+            }
+            ci++;
+            cj = ci + 1;
+            ck = cj + 1;
+        }
+        const int nchunk = min(kHalf, ntri - c0);
+        for (int t = 0; t < nchunk; t++) {
+            const int src = hbase + t;
+            const uint32_t meta = (uint32_t)__shfl((int)rec.meta, src);
+            const float x0 = __shfl(rec.x0, src), x1 = __shfl(rec.x1, src), x2 = __shfl(rec.x2, src);
+            const bool valid = meta >> 27 & 1u;
+            const int i = meta & 63, j = meta >> 6 & 63, k = meta >> 12 & 63;
+            const int u0 = meta >> 18 & 3, u1 = meta >> 20 & 3, u2 = meta >> 22 & 3;
+            const bool e1 = meta >> 24 & 1u, e2 = meta >> 25 & 1u, e3 = meta >> 26 & 1u;
+            const double dx0 = x0, dx1 = x1, dx2 = x2;
+            int first = 0, count = valid ? ntrip : 0;
+            const bool indexed = valid && bin_start != nullptr && x0 == x0 && x1 == x1 && x2 == x2;
+            if (indexed) {
+                // float32 product: within 2e-5 bins of the exact quotient, covered by the
+                // index's 1e-3 degree slack at the bin edges
+                int bin = (int)floorf(x0 * inv_bin_width);
+                bin = bin < 0 ? 0 : (bin >= nbins ? nbins - 1 : bin);
+                first = bin_start[bin];
+                count = bin_start[bin + 1] - first;
+            }
+            const int cmax = max(__builtin_amdgcn_readlane(count, 0), __builtin_amdgcn_readlane(count, kHalf));
+            for (int base = 0; base < cmax; base += kHalf) {
+                const int slot = base + hl;
+                int q = slot;
+                if (indexed && slot < count) q = bin_rows[first + slot];
+                bool pass = false;
+                float t1f = 0.0f, t2f = 0.0f, t3f = 0.0f;
+                int fid0 = 0, fid1 = 0, fid2 = 0;
+                if (slot < count) {
+                    double r0, r1, r2;
+                    if (kLdsLib) {
+                        r0 = la0[q];
+                        r1 = la1[q];
+                        r2 = la2[q];
+                    } else {
+                        r0 = ta[(int64_t)q * 3 + 0];
+                        r1 = ta[(int64_t)q * 3 + 1];
+                        r2 = ta[(int64_t)q * 3 + 2];
+                    }
+                    const double t1 = fabs(r0 - dx0), t2 = fabs(r1 - dx1), t3 = fabs(r2 - dx2);
+                    pass = !(t1 > tol) && !(t2 > tol) && !(t3 > tol);
+                    t1f = (float)t1;
+                    t2f = (float)t2;
+                    t3f = (float)t3;
+                    if (pass) {
+                        int id[3];
+                        if (kLdsLib) {
+                            id[0] = li[q];
+                            id[1] = li[ntrip + q];
+                            id[2] = li[2 * ntrip + q];
+                        } else {
+                            id[0] = ti[(int64_t)q * 3 + 0];
+                            id[1] = ti[(int64_t)q * 3 + 1];
+                            id[2] = ti[(int64_t)q * 3 + 2];
+                        }
+                        fid0 = id[u0];
+                        fid1 = id[u1];
+                        fid2 = id[u2];
+                    }
+                }
+                // each half takes its own passing rows in ascending order, both halves at once
+                uint64_t rem = __ballot(pass);
+                while (rem) {
+                    const uint32_t lo = (uint32_t)rem, hi = (uint32_t)(rem >> 32);
+                    const uint32_t mine = hbase ? hi : lo;
+                    const int from = hbase + (mine ? __builtin_ctz(mine) : 0);
+                    const float g0 = __shfl(t1f, from), g1 = __shfl(t2f, from), g2 = __shfl(t3f, from);
+                    const int f0 = __shfl(fid0, from), f1 = __shfl(fid1, from), f2 = __shfl(fid2, from);
+                    if (hl == 0 && mine != 0u && (unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
+                        (unsigned)f2 < (unsigned)nfam) {
+                        const double w1 = tol - 0.5 * (double)(g0 + g1);
+                        const double w2 = tol - 0.5 * (double)(g0 + g2);
+                        const double w3 = tol - 0.5 * (double)(g1 + g2);
+                        add_vote(cell, f0 * nb + i, w1);
+                        add_vote(cell, f1 * nb + j, w2);
+                        add_vote(cell, f2 * nb + k, w3);
+                        if (e1) {
+                            add_vote(cell, f0 * nb + j, w1);
+                            add_vote(cell, f1 * nb + i, w2);
+                            add_vote(cell, f2 * nb + k, w3);
+                        }
+                        if (e2) {
+                            add_vote(cell, f0 * nb + i, w1);
+                            add_vote(cell, f1 * nb + k, w2);
+                            add_vote(cell, f2 * nb + j, w3);
+                        }
+                        if (e3) {
+                            add_vote(cell, f0 * nb + k, w1);
+                            add_vote(cell, f1 * nb + j, w2);
+                            add_vote(cell, f2 * nb + i, w3);
+                        }
+                        if (e1 && e2 && e3) {
+                            add_vote(cell, f0 * nb + k, w1);
+                            add_vote(cell, f1 * nb + i, w2);
+                            add_vote(cell, f2 * nb + j, w3);
+                            add_vote(cell, f0 * nb + j, w1);
+                            add_vote(cell, f1 * nb + k, w2);
+                            add_vote(cell, f2 * nb + i, w3);
+                        }
+                    }
+                    rem = (uint64_t)(lo & (lo - 1u)) | ((uint64_t)(hi & (hi - 1u)) << 32);
+                }
+            }
+        }
+    }
+    // the half's lane 0 wrote the cells; the other lanes of the wave read them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!live) return;
+
+    if (hl < nb) {
+        const int b = hl;
+        float mx = -1.0e12f, sum = 0.0f;
+        float top = -INFINITY, second = -INFINITY;
+        int best = 0;
+        for (int f = 0; f < nfam; f++) {
+            const float v = cell[f * nb + b].x;
+            sum += v;
+            if (v > mx) {
+                mx = v;
+                best = f;
+            }
+            if (v > top) {
+                second = top;
+                top = v;
+            } else if (v > second) {
+                second = v;
+            }
+        }
+        const int mxvote = (int)mx, votes = (int)sum;
+        float c = 0.0f;
+        if (votes >= 1) c = (float)((double)(top - second) / (double)votes);
+        fam[p * nb + b] = best;
+        cmo[p * nb + b] = c;
+        rank[p * nb + b] = (float)((double)(nb - b) / (double)nb * (double)c * (double)mxvote);
+    }
+    float *ow = accw + p * nfam * nb, *oc = acc + p * nfam * nb;
+    for (int t = hl; t < nfam * nb; t += kHalf) {
+        const float2 v = cell[t];
+        ow[t] = v.x;
+        oc[t] = v.y;
+    }
+}
+
 }  // namespace
 
 extern "C" const char *ebsd_tripvote_last_error(void) { return g_err.c_str(); }
@@ -380,6 +625,30 @@ extern "C" int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity,
     if (!d_bandnorms || !d_intensity || !d_accw || !d_fam || !d_rank || !d_cm || !d_acc ||
         (ntrip > 0 && (!d_tripang || !d_tripid)))
         return fail(-1, "null device pointer");
+    if (nbands <= kHalf) {
+        // two patterns per wave; 8 per work-group when the LDS allows
+        int halves = (int)(kLdsBudget / per_bytes);
+        halves = halves > 8 ? 8 : (halves & ~1);
+        if (halves >= 2) {
+            const bool lib_in = lib_bytes + halves * per_bytes <= kLdsBudget;
+            const size_t lds = (lib_in ? lib_bytes : 0) + halves * per_bytes;
+            const int64_t groups = (npats + halves - 1) / halves;
+            if (groups > 0x7fffffff) return fail(-1, "too many patterns for one launch");
+            if (lib_in)
+                hipLaunchKernelGGL(k_tripvote_pair<true>, dim3((unsigned)groups), dim3(halves * kHalf), lds,
+                                   (hipStream_t)stream, d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid,
+                                   ntrip, nfam, angtol, d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam,
+                                   d_rank, d_cm, d_acc);
+            else
+                hipLaunchKernelGGL(k_tripvote_pair<false>, dim3((unsigned)groups), dim3(halves * kHalf), lds,
+                                   (hipStream_t)stream, d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid,
+                                   ntrip, nfam, angtol, d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam,
+                                   d_rank, d_cm, d_acc);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return fail(-2, std::string("k_tripvote_pair launch: ") + hipGetErrorString(e));
+            return 0;
+        }
+    }
     int waves = (int)(kLdsBudget / per_bytes);
     if (waves > 4) waves = 4;
     const bool lds_lib = lib_bytes + waves * per_bytes <= kLdsBudget;
