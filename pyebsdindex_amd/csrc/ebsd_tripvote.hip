@@ -358,15 +358,14 @@ __global__ __launch_bounds__(256) void k_tripvote(const float *__restrict__ bn, 
     }
 }
 
-// Two patterns per wavefront, one per 32-lane half (nbands <= 32): the same walk as
-// k_tripvote, but every instruction of the triplet walk, the row tests and the votes serves
-// two patterns.  Values that k_tripvote reads with v_readlane (uniform over the wave) are
-// per half here and move with ds_bpermute (__shfl).  Loops run to the larger of the two
-// halves' bounds, the other half masked; a half with no pattern has no valid triplet.
-constexpr int kHalf = 32;
-
-template <bool kLdsLib>
-__global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__ bn, const float *__restrict__ inten,
+// Several patterns per wavefront, one per group of G lanes (G = 32 or 16, nbands <= G): the
+// same walk as k_tripvote, but every instruction of the triplet walk, the row tests and the
+// votes serves 64 / G patterns.  Values that k_tripvote reads with v_readlane (uniform over
+// the wave) are per group here and move with ds_bpermute (__shfl).  Loops run to the
+// largest of the groups' bounds, the other groups masked; a group with no pattern has no
+// valid triplet.
+template <int G, bool kLdsLib>
+__global__ __launch_bounds__(256) void k_tripvote_group(const float *__restrict__ bn, const float *__restrict__ inten,
                                                        int64_t npats, int nb, const double *__restrict__ ta,
                                                        const int32_t *__restrict__ ti, int ntrip, int nfam,
                                                        double tol, const int32_t *__restrict__ bin_start,
@@ -376,10 +375,12 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
                                                        float *__restrict__ cmo, float *__restrict__ acc)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    const int halves = blockDim.x / kHalf;
-    const int half = threadIdx.x / kHalf, hl = threadIdx.x % kHalf;
+    constexpr int kGroups = kWave / G;           // patterns per wave
+    constexpr uint64_t kGroupMask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const int halves = blockDim.x / G;
+    const int half = threadIdx.x / G, hl = threadIdx.x % G;
     const int lane = threadIdx.x % kWave;
-    const int hbase = lane & kHalf;              // first lane of this half in the wave (0 or 32)
+    const int hbase = lane - hl;                 // first lane of this group in the wave
     double *la0 = reinterpret_cast<double *>(lds_raw);
     double *la1 = la0 + (kLdsLib ? ntrip : 0);
     double *la2 = la1 + (kLdsLib ? ntrip : 0);
@@ -402,10 +403,10 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
         }
     }
     if (live) {
-        for (int t = hl; t < nfam * nb; t += kHalf) cell[t] = make_float2(0.0f, 0.0f);
+        for (int t = hl; t < nfam * nb; t += G) cell[t] = make_float2(0.0f, 0.0f);
         const float *b = bn + p * nb * 3;
         const float *in = inten + p * nb;
-        for (int t = hl; t < nb * nb; t += kHalf) {
+        for (int t = hl; t < nb * nb; t += G) {
             const int r = t / nb, c = t - r * nb;
             float d = __fadd_rn(__fadd_rn(__fmul_rn(b[r * 3], b[c * 3]), __fmul_rn(b[r * 3 + 1], b[c * 3 + 1])),
                                 __fmul_rn(b[r * 3 + 2], b[c * 3 + 2]));
@@ -418,9 +419,9 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
         }
     }
     __syncthreads();
-    // the two halves of a wave share every loop below (shuffles and ballots need both):
+    // the groups of a wave share every loop below (shuffles and ballots need all of them):
     // only a wave with no pattern at all leaves
-    if (__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (int64_t)halves + (threadIdx.x / kWave) * 2) >= npats))
+    if (__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (int64_t)halves + (threadIdx.x / kWave) * kGroups >= npats)))
         return;
 
     const bool skip_invalid = tol < 9000.0;
@@ -445,10 +446,10 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
         }
         ck = cj + 1 + rem;
     }
-    for (int c0 = 0; c0 < ntri; c0 += kHalf) {
+    for (int c0 = 0; c0 < ntri; c0 += G) {
         TripletRecord rec = {0.0f, 0.0f, 0.0f, 0u};
         if (live && c0 + hl < ntri) rec = make_record(angs, nb, ci, cj, ck, tol, skip_invalid);
-        for (int step = 0; step < kHalf && c0 + kHalf + hl < ntri; step++) {
+        for (int step = 0; step < G && c0 + G + hl < ntri; step++) {
             if (++ck < nb) continue;
             if (++cj < nb - 1) {
                 ck = cj + 1;
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
             cj = ci + 1;
             ck = cj + 1;
         }
-        const int nchunk = min(kHalf, ntri - c0);
+        const int nchunk = min(G, ntri - c0);
         for (int t = 0; t < nchunk; t++) {
             const int src = hbase + t;
             const uint32_t meta = (uint32_t)__shfl((int)rec.meta, src);
@@ -478,8 +479,10 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
                 first = bin_start[bin];
                 count = bin_start[bin + 1] - first;
             }
-            const int cmax = max(__builtin_amdgcn_readlane(count, 0), __builtin_amdgcn_readlane(count, kHalf));
-            for (int base = 0; base < cmax; base += kHalf) {
+            int cmax = 0;
+#pragma unroll
+            for (int g = 0; g < kGroups; g++) cmax = max(cmax, __builtin_amdgcn_readlane(count, g * G));
+            for (int base = 0; base < cmax; base += G) {
                 const int slot = base + hl;
                 int q = slot;
                 if (indexed && slot < count) q = bin_rows[first + slot];
@@ -518,15 +521,14 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
                         fid2 = id[u2];
                     }
                 }
-                // each half takes its own passing rows in ascending order, both halves at once
+                // each group takes its own passing rows in ascending order, all groups at once
                 uint64_t rem = __ballot(pass);
                 while (rem) {
-                    const uint32_t lo = (uint32_t)rem, hi = (uint32_t)(rem >> 32);
-                    const uint32_t mine = hbase ? hi : lo;
-                    const int from = hbase + (mine ? __builtin_ctz(mine) : 0);
+                    const uint64_t mine = (rem >> hbase) & kGroupMask;
+                    const int from = hbase + (mine ? __builtin_ctzll(mine) : 0);
                     const float g0 = __shfl(t1f, from), g1 = __shfl(t2f, from), g2 = __shfl(t3f, from);
                     const int f0 = __shfl(fid0, from), f1 = __shfl(fid1, from), f2 = __shfl(fid2, from);
-                    if (hl == 0 && mine != 0u && (unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
+                    if (hl == 0 && mine != 0ull && (unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
                         (unsigned)f2 < (unsigned)nfam) {
                         const double w1 = tol - 0.5 * (double)(g0 + g1);
                         const double w2 = tol - 0.5 * (double)(g0 + g2);
@@ -558,12 +560,19 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
                             add_vote(cell, f2 * nb + i, w3);
                         }
                     }
-                    rem = (uint64_t)(lo & (lo - 1u)) | ((uint64_t)(hi & (hi - 1u)) << 32);
+                    // clear the lowest set bit of every group
+                    uint64_t next = 0;
+#pragma unroll
+                    for (int g = 0; g < kGroups; g++) {
+                        const uint64_t x = (rem >> (g * G)) & kGroupMask;
+                        next |= (x & (x - 1ull)) << (g * G);
+                    }
+                    rem = next;
                 }
             }
         }
     }
-    // the half's lane 0 wrote the cells; the other lanes of the wave read them
+    // the group's lane 0 wrote the cells; the other lanes of the wave read them
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -596,7 +605,7 @@ __global__ __launch_bounds__(256) void k_tripvote_pair(const float *__restrict__
         rank[p * nb + b] = (float)((double)(nb - b) / (double)nb * (double)c * (double)mxvote);
     }
     float *ow = accw + p * nfam * nb, *oc = acc + p * nfam * nb;
-    for (int t = hl; t < nfam * nb; t += kHalf) {
+    for (int t = hl; t < nfam * nb; t += G) {
         const float2 v = cell[t];
         ow[t] = v.x;
         oc[t] = v.y;
@@ -625,27 +634,34 @@ extern "C" int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity,
     if (!d_bandnorms || !d_intensity || !d_accw || !d_fam || !d_rank || !d_cm || !d_acc ||
         (ntrip > 0 && (!d_tripang || !d_tripid)))
         return fail(-1, "null device pointer");
-    if (nbands <= kHalf) {
-        // two patterns per wave; 8 per work-group when the LDS allows
-        int halves = (int)(kLdsBudget / per_bytes);
-        halves = halves > 8 ? 8 : (halves & ~1);
-        if (halves >= 2) {
-            const bool lib_in = lib_bytes + halves * per_bytes <= kLdsBudget;
-            const size_t lds = (lib_in ? lib_bytes : 0) + halves * per_bytes;
-            const int64_t groups = (npats + halves - 1) / halves;
-            if (groups > 0x7fffffff) return fail(-1, "too many patterns for one launch");
-            if (lib_in)
-                hipLaunchKernelGGL(k_tripvote_pair<true>, dim3((unsigned)groups), dim3(halves * kHalf), lds,
-                                   (hipStream_t)stream, d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid,
-                                   ntrip, nfam, angtol, d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam,
-                                   d_rank, d_cm, d_acc);
-            else
-                hipLaunchKernelGGL(k_tripvote_pair<false>, dim3((unsigned)groups), dim3(halves * kHalf), lds,
-                                   (hipStream_t)stream, d_bandnorms, d_intensity, npats, nbands, d_tripang, d_tripid,
-                                   ntrip, nfam, angtol, d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam,
-                                   d_rank, d_cm, d_acc);
+    if (nbands <= 32) {
+        // several patterns per wave: 16-lane groups up to 16 bands, else 32-lane halves;
+        // up to 256 lanes per work-group as the LDS allows
+        const int g = nbands <= 16 ? 16 : 32;
+        int groups_per_wg = (int)(kLdsBudget / per_bytes);
+        const int per_wave = kWave / g;
+        groups_per_wg = groups_per_wg > 256 / g ? 256 / g : groups_per_wg - groups_per_wg % per_wave;
+        if (groups_per_wg >= per_wave) {
+            const bool lib_in = lib_bytes + groups_per_wg * per_bytes <= kLdsBudget;
+            const size_t lds = (lib_in ? lib_bytes : 0) + groups_per_wg * per_bytes;
+            const int64_t grid = (npats + groups_per_wg - 1) / groups_per_wg;
+            if (grid > 0x7fffffff) return fail(-1, "too many patterns for one launch");
+            const dim3 gd((unsigned)grid), bd(groups_per_wg * g);
+            hipStream_t st = (hipStream_t)stream;
+#define EBSD_TV_GROUP(G, L)                                                                                  \
+    hipLaunchKernelGGL((k_tripvote_group<G, L>), gd, bd, lds, st, d_bandnorms, d_intensity, npats, nbands,   \
+                       d_tripang, d_tripid, ntrip, nfam, angtol, d_bin_start, d_bin_rows, nbins, bin_width, \
+                       d_accw, d_fam, d_rank, d_cm, d_acc)
+            if (g == 16) {
+                if (lib_in) EBSD_TV_GROUP(16, true);
+                else EBSD_TV_GROUP(16, false);
+            } else {
+                if (lib_in) EBSD_TV_GROUP(32, true);
+                else EBSD_TV_GROUP(32, false);
+            }
+#undef EBSD_TV_GROUP
             const hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return fail(-2, std::string("k_tripvote_pair launch: ") + hipGetErrorString(e));
+            if (e != hipSuccess) return fail(-2, std::string("k_tripvote_group launch: ") + hipGetErrorString(e));
             return 0;
         }
     }
