@@ -1,16 +1,19 @@
 // Triplet voting of band normals against a phase's angle-triplet library, for gfx950.
 //
-// Replaces BandIndexer._tripvote_numba (pyebsdindex/tripletvote.py:1019-1207).  One
-// wavefront per pattern, up to four per work-group; the work-group copies the library into
-// LDS as columns when it fits in 64 KiB with the per-wave state.  A wave computes its
-// pattern's inter-band angles into LDS; its lanes then build the records of 64 band
-// triplets at a time (ascending angles, vertex order, near-equality flags) and the wave
-// walks them in the reference's (i < j < k) order, reading each with v_readlane.  For each
-// triplet the 64 lanes test 64 library rows at once; the rows that pass all three angle
-// tests are taken in ascending row order from the ballot mask and lane 0 adds their votes
-// to the LDS accumulators, so every accumulator receives the same float additions in the
-// same order as the reference's sequential loop.  The per-band metrics are computed one
-// band per lane.
+// Replaces BandIndexer._tripvote_numba (pyebsdindex/tripletvote.py:1019-1207).
+//
+// k_tripvote_group<G> (up to 32 bands): 64 / G patterns per wavefront, one per group of G
+// lanes (G = 16 up to 16 bands, else 32); k_tripvote (more bands): one pattern per
+// wavefront.  The work-group copies the library into LDS as columns when it fits in 64 KiB
+// with the per-pattern state.  A group computes its pattern's inter-band angles into LDS;
+// its lanes then build the records of G band triplets at a time (ascending angles, vertex
+// order, near-equality flags) and the group walks them in the reference's (i < j < k)
+// order.  For each triplet the lanes test G library rows at once (the optional row index
+// narrows the rows to those whose first angle can pass); the rows that pass all three
+// tests are taken in ascending row order from the group's bits of the ballot mask and the
+// group's first lane adds their votes to the pattern's LDS cells, so every accumulator
+// receives the same float additions in the same order as the reference's sequential loop.
+// The per-band metrics are computed one band per lane.
 //
 // Arithmetic (the types Numba gives the reference):
 //   cosine   float32 ((x0 y0) + (x1 y1)) + (x2 y2), no contraction; |.|, clipped at 1
@@ -23,9 +26,9 @@
 //   metrics  float32 running sum and maximum over families, truncated to int32;
 //            cm = f32(f64(largest - second largest) / votes); rank in float64
 //
-// Work per pattern: C(nb, 3) triplets x ceil(ntrip / 64) blocks of row tests (three LDS
-// reads and three float64 differences per lane), plus lane 0's LDS updates per match
-// (about 100 matches per pattern for 9 bands of an FCC scan).
+// Work per pattern: C(nb, 3) triplets x the listed rows (three LDS reads and three float64
+// differences per lane and row), plus one lane's LDS updates per match (about 100 matches
+// per pattern for 9 bands of an FCC scan).
 
 #include <hip/hip_runtime.h>
 
