@@ -176,3 +176,12 @@ def test_row_index_gives_identical_outputs(phase, tol):
     b = ind.tripvote(bnd, intd, use_index=False)
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+def test_no_patterns():
+    """Zero patterns: empty outputs of the right shapes, no launch."""
+    import torch
+    from pyebsdindex_amd import tripletvote as tv
+    ind = tv.addphase("FCC")
+    out = ind.tripvote(torch.zeros((0, 9, 3), dtype=torch.float32, device="cuda"))
+    assert [tuple(t.shape) for t in out] == [(0, 4, 9), (0, 9), (0, 9), (0, 9), (0, 4, 9)]
