@@ -531,36 +531,39 @@ __global__ __launch_bounds__(256) void k_tripvote_group(const float *__restrict_
                     const int from = hbase + (mine ? __builtin_ctzll(mine) : 0);
                     const float g0 = __shfl(t1f, from), g1 = __shfl(t2f, from), g2 = __shfl(t3f, from);
                     const int f0 = __shfl(fid0, from), f1 = __shfl(fid1, from), f2 = __shfl(fid2, from);
-                    if (hl == 0 && mine != 0ull && (unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
+                    // each block of the reference's vote statements gives every band of the
+                    // triplet exactly one vote: lanes 0, 1, 2 of the group take bands i, j, k
+                    // and apply their band's votes in the block order, so each cell still
+                    // receives the reference's additions in the reference's order
+                    if (hl < 3 && mine != 0ull && (unsigned)f0 < (unsigned)nfam && (unsigned)f1 < (unsigned)nfam &&
                         (unsigned)f2 < (unsigned)nfam) {
                         const double w1 = tol - 0.5 * (double)(g0 + g1);
                         const double w2 = tol - 0.5 * (double)(g0 + g2);
                         const double w3 = tol - 0.5 * (double)(g1 + g2);
-                        add_vote(cell, f0 * nb + i, w1);
-                        add_vote(cell, f1 * nb + j, w2);
-                        add_vote(cell, f2 * nb + k, w3);
+                        const int band = hl == 0 ? i : (hl == 1 ? j : k);
+                        // block: votes (family index 0..2 into f, weight index 0..2 into w)
+                        // for bands (i, j, k); main (0,1,2), e1 (1,0,2), e2 (0,2,1),
+                        // e3 (2,1,0), all three (1,2,0) then (2,0,1)
+                        const int fv[3] = {f0, f1, f2};
+                        const double wv[3] = {w1, w2, w3};
+                        add_vote(cell, fv[hl] * nb + band, wv[hl]);
                         if (e1) {
-                            add_vote(cell, f0 * nb + j, w1);
-                            add_vote(cell, f1 * nb + i, w2);
-                            add_vote(cell, f2 * nb + k, w3);
+                            const int x = hl == 0 ? 1 : (hl == 1 ? 0 : 2);
+                            add_vote(cell, fv[x] * nb + band, wv[x]);
                         }
                         if (e2) {
-                            add_vote(cell, f0 * nb + i, w1);
-                            add_vote(cell, f1 * nb + k, w2);
-                            add_vote(cell, f2 * nb + j, w3);
+                            const int x = hl == 0 ? 0 : (hl == 1 ? 2 : 1);
+                            add_vote(cell, fv[x] * nb + band, wv[x]);
                         }
                         if (e3) {
-                            add_vote(cell, f0 * nb + k, w1);
-                            add_vote(cell, f1 * nb + j, w2);
-                            add_vote(cell, f2 * nb + i, w3);
+                            const int x = hl == 0 ? 2 : (hl == 1 ? 1 : 0);
+                            add_vote(cell, fv[x] * nb + band, wv[x]);
                         }
                         if (e1 && e2 && e3) {
-                            add_vote(cell, f0 * nb + k, w1);
-                            add_vote(cell, f1 * nb + i, w2);
-                            add_vote(cell, f2 * nb + j, w3);
-                            add_vote(cell, f0 * nb + j, w1);
-                            add_vote(cell, f1 * nb + k, w2);
-                            add_vote(cell, f2 * nb + i, w3);
+                            const int x = hl == 0 ? 1 : (hl == 1 ? 2 : 0);
+                            add_vote(cell, fv[x] * nb + band, wv[x]);
+                            const int y = hl == 0 ? 2 : (hl == 1 ? 0 : 1);
+                            add_vote(cell, fv[y] * nb + band, wv[y]);
                         }
                     }
                     // clear the lowest set bit of every group
