@@ -10,9 +10,10 @@
 // order, near-equality flags) and the group walks them in the reference's (i < j < k)
 // order.  For each triplet the lanes test G library rows at once (the optional row index
 // narrows the rows to those whose first angle can pass); the rows that pass all three
-// tests are taken in ascending row order from the group's bits of the ballot mask and the
-// group's first lane adds their votes to the pattern's LDS cells, so every accumulator
-// receives the same float additions in the same order as the reference's sequential loop.
+// tests are taken in ascending row order from the group's bits of the ballot mask; three
+// lanes of the group add a match's votes to the pattern's LDS cells, one lane per band of
+// the triplet, in the reference's statement order, so every accumulator receives the same
+// float additions in the same order as the reference's sequential loop.
 // The per-band metrics are computed one band per lane.
 //
 // Arithmetic (the types Numba gives the reference):
