@@ -445,7 +445,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
-              f"(patterns shard independently across GPUs, see DESIGN.md)", file=sys.stderr)
+              f"(patterns are independent: N GPUs would each run this bench on 1/N of the patterns)", file=sys.stderr)
         sys.exit(2)
     import torch
     from pyebsdindex_amd import BandDetect
