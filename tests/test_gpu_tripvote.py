@@ -185,3 +185,30 @@ def test_no_patterns():
     ind = tv.addphase("FCC")
     out = ind.tripvote(torch.zeros((0, 9, 3), dtype=torch.float32, device="cuda"))
     assert [tuple(t.shape) for t in out] == [(0, 4, 9), (0, 9), (0, 9), (0, 9), (0, 4, 9)]
+
+
+def test_vote_on_detected_bands():
+    """The chain on the device: synthetic FCC patterns (120x120 uint16, pattern centre
+    (0.5, 0.5, 0.6)) -> find_bands -> radon2pole -> vote, against the oracle's vote on the
+    same band normals and intensities (avemax of the valid bands, as the indexer passes)."""
+    import torch
+    from helpers import make_pair
+    from pyebsdindex_amd import tripletvote as tv
+    from pyebsdindex_amd.differentiable import radon2pole
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    pc = (0.5, 0.5, 0.6)
+    pats = kikuchi_patterns(64, (120, 120), "uint16", seed=5, family="FCC", pc=pc).numpy()
+    bd, _ = make_pair((120, 120))
+    got = bd.find_bands(pats)
+    dev = torch.device("cuda", 0)
+    theta = torch.as_tensor(np.ascontiguousarray(got["theta"]), device=dev)
+    rho = torch.as_tensor(np.ascontiguousarray(got["rho"]), device=dev)
+    n = radon2pole(theta, rho, pc, 120, 120)
+    inten = (got["avemax"] * got["valid"]).astype(np.float32)
+    ind = tv.addphase("FCC")
+    (accW, fam, rank, cm, acc), _ = _compare(ind, n.cpu().numpy(), inten)
+    valid = got["valid"] > 0
+    # a detected band of an FCC pattern is voted for: its best family holds votes from
+    # several triplets
+    best = acc.max(axis=1)
+    assert np.median(best[valid]) >= 5
