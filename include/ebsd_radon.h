@@ -11,8 +11,9 @@
  * Conventions: every pointer named d_* is device memory; every other pointer is host
  * memory.  Functions return 0 on success and a negative code on failure; the message
  * is available from ebsd_last_error() (per thread).  A plan is immutable after set-up
- * except through ebsd_plan_set_background/ebsd_plan_set_tuning; calls that use one
- * plan from several host threads at once must be serialised by the caller.
+ * except through the ebsd_plan_set_* calls, which fail while a batch (ebsd_batch_create)
+ * holds the plan; calls that use one plan from several host threads at once must be
+ * serialised by the caller.
  */
 #ifndef EBSD_RADON_H
 #define EBSD_RADON_H
@@ -123,8 +124,10 @@ int ebsd_conv(ebsd_plan *plan, const float *d_radon, int64_t N, float *d_conv, v
  * segments is gathered by its own per-size launch in the same sequence (run once by
  * ebsd_batch_create, so that later launches allocate nothing).  The batch binds the given
  * device pointers; ebsd_batch_graph_launch captures the launch sequence into a hipGraph on
- * its first call (on a non-default stream) and replays it afterwards.  The plans must not be
- * retuned while the batch exists.
+ * its first call (on a non-default stream) and replays it afterwards.  The batch's launches
+ * and graph hold device pointers into its plans, so while the batch exists every
+ * ebsd_plan_set_* call on those plans fails; ebsd_find_bands on them stays allowed (buffers
+ * it outgrows are kept until the plan is destroyed).
  * Replaces: calling find_bands once per pattern size (one BandDetect/plan per size, as
  * EBSDIndexer re-plans on a size change, pyebsdindex/_ebsd_index_single.py:696-700). */
 typedef struct ebsd_batch ebsd_batch;

@@ -5,8 +5,11 @@
  *   ora_tripvote  <- pyebsdindex/tripletvote.py:1019-1207  (BandIndexer._tripvote_numba)
  *
  * Arithmetic types follow Numba's typing of the reference source:
- *   - the inter-band cosines are float32 dot products of the float32 band normals
- *     (evaluated here as ((x0 y0) + (x1 y1)) + (x2 y2) without contraction), made
+ *   - the inter-band cosines are float32 dot products of the float32 band normals,
+ *     evaluated here as ((x0 y0) + (x1 y1)) + (x2 y2) without contraction.  PARITY
+ *     UNPINNED: the reference's 2-D .dot (tripletvote.py:1192) goes to a BLAS sgemm whose
+ *     summation order and use of fused multiply-adds are not restated; an angle at the
+ *     tolerance edge can vote differently (tests/test_oracle_tripvote.py); made
  *     absolute and clipped to [-1, 1]; the angle is the float32 arccos times RADEG in
  *     float64, rounded to float32 where the triplet is formed.  The float32 arccos is
  *     taken as the float64 arccos rounded to float32 (the correctly rounded value; the

@@ -10,6 +10,12 @@ How it is pinned: tests/test_oracle_tripvote.py -- a hand-solved three-band tria
 (the other bands' votes equal those of the shorter band list, bit for bit), band
 permutations (vote counts permute), and the library's golden pair-angle counts and
 closed-form angles.
+
+Parity unpinned: the form of the float32 inter-band cosine.  The reference takes it from a
+2-D ``.dot`` (tripletvote.py:1192) that Numba hands to BLAS, whose summation order and use
+of fused multiply-adds are not known here; the oracle (and the device) evaluate
+((x0 y0) + (x1 y1)) + (x2 y2) without contraction.  A test places a library angle at the
+tolerance edge to show that the two forms can vote differently there.
 """
 import ctypes
 

@@ -171,3 +171,27 @@ def check_tripvote(lib, rc: int):
     if rc != 0:
         msg = lib.ebsd_tripvote_last_error().decode(errors="replace")
         raise HipLibraryError(f"ebsd triplet-vote library error {rc}: {msg}")
+
+
+def launch_stream(stream=None, device=None):
+    """The stream a wrapper launches on: the caller's current stream, or ``stream``.  A
+    side stream first waits for the work already queued on the current stream (which may
+    have produced the inputs); the results are then ready in ``stream``'s order, and the
+    caller synchronises with it before reading them elsewhere."""
+    import torch
+    cur = torch.cuda.current_stream(device)
+    if stream is None or stream == cur:
+        return cur
+    stream.wait_stream(cur)
+    return stream
+
+
+def keep_for_stream(stream, *tensors):
+    """Tell the caching allocator that ``tensors`` (made on the current stream) are used on
+    ``stream``, so their memory is not reused before ``stream`` has passed this point."""
+    import torch
+    if stream is None or stream == torch.cuda.current_stream(stream.device):
+        return
+    for t in tensors:
+        if t is not None and hasattr(t, "record_stream") and t.is_cuda:
+            t.record_stream(stream)

@@ -158,11 +158,12 @@ class DevicePlan:
             out = self.alloc_out(N)
         if work is None:
             work = self.workspace(N, chunk)
-        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        s = _hip.launch_stream(stream, self.device)
         b = self.bands_struct(out)
         _hip.check(self.lib.ebsd_find_bands(self.handle, pats.data_ptr(), code | flags, N,
                                             _hip.ctypes.byref(b), work.data_ptr(),
                                             work.numel() * 4, s.cuda_stream))
+        _hip.keep_for_stream(s, pats, work, *out.values())
         return out
 
     def radon(self, pats, stream=None, flags=0, out=None):
@@ -173,9 +174,10 @@ class DevicePlan:
         N = pats.shape[0]
         rad = out if out is not None else torch.empty((N, self.nR, self.nT), dtype=torch.float32,
                                                       device=self.device)
-        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        s = _hip.launch_stream(stream, self.device)
         _hip.check(self.lib.ebsd_radon(self.handle, pats.data_ptr(), code | flags, N, rad.data_ptr(),
                                        s.cuda_stream))
+        _hip.keep_for_stream(s, pats, rad)
         return rad
 
     def peaks(self, radon, out=None, stream=None):
@@ -184,10 +186,11 @@ class DevicePlan:
         N = radon.shape[0]
         if out is None:
             out = self.alloc_out(N)
-        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        s = _hip.launch_stream(stream, self.device)
         b = self.bands_struct(out)
         _hip.check(self.lib.ebsd_peaks(self.handle, radon.data_ptr(), N, _hip.ctypes.byref(b),
                                        s.cuda_stream))
+        _hip.keep_for_stream(s, radon, *out.values())
         return out
 
     def conv(self, radon, stream=None):
@@ -196,8 +199,9 @@ class DevicePlan:
         N = radon.shape[0]
         p0, p1 = int(self.padding[0]), int(self.padding[1])
         c = torch.empty((N, self.nR + 2 * p0, self.nT + 2 * p1), dtype=torch.float32, device=self.device)
-        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        s = _hip.launch_stream(stream, self.device)
         _hip.check(self.lib.ebsd_conv(self.handle, radon.data_ptr(), N, c.data_ptr(), s.cuda_stream))
+        _hip.keep_for_stream(s, radon, c)
         return c
 
 
@@ -520,7 +524,8 @@ class BandDetect:
 
     def find_bands_device(self, pats, gpu_id=None, out=None, work=None, chunk=2048, stream=None):
         """find_bands on a device tensor (N,H,W); returns a dict of device tensors.  The
-        kernels run on ``stream`` (a torch.cuda.Stream; default: the current stream)."""
+        kernels run on ``stream`` (a torch.cuda.Stream; default: the current stream); a side
+        stream first waits for the current stream, and the outputs are ready in its order."""
         plan = self.device_plan(gpu_id)
         pats = pats.unsqueeze(0) if pats.dim() == 2 else pats
         x, flags = self._prepare(plan, pats)

@@ -70,19 +70,29 @@ class BatchBandDetect:
 
     def run(self, stream=None):
         """Launch the two persistent kernels on ``stream`` (default: current stream)."""
-        import torch
-        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        s = _hip.launch_stream(stream, self.device)
         _hip.check(self.lib.ebsd_batch_run(self.handle, s.cuda_stream))
         return self.outs
 
     def graph_launch(self, stream=None):
-        """Replay the captured graph (captured on the first call) on a non-default stream."""
+        """Replay the captured graph (captured on the first call) on a non-default stream.
+        Without ``stream`` the replay runs on a private stream ordered both ways with the
+        current stream: it starts after the work queued there, and work queued there
+        afterwards (reading the outputs, to_numpy) waits for it.  With ``stream`` it runs on
+        that (non-default) stream after the current stream's queued work; the outputs are
+        then ready in ``stream``'s order."""
         import torch
+        cur = torch.cuda.current_stream(self.device)
         if stream is None:
             if self.graph_stream is None:
                 self.graph_stream = torch.cuda.Stream(self.device)
-            stream = self.graph_stream
-        _hip.check(self.lib.ebsd_batch_graph_launch(self.handle, stream.cuda_stream))
+            s = self.graph_stream
+            s.wait_stream(cur)
+            _hip.check(self.lib.ebsd_batch_graph_launch(self.handle, s.cuda_stream))
+            cur.wait_stream(s)
+        else:
+            s = _hip.launch_stream(stream, self.device)
+            _hip.check(self.lib.ebsd_batch_graph_launch(self.handle, s.cuda_stream))
         return self.outs
 
     def to_numpy(self):

@@ -93,10 +93,14 @@ struct ebsd_plan {
         std::vector<Layout> layouts;
     };
     GSeg full;                   // all rows
-    std::vector<GSeg> rows;      // row segments, compiled on first use
+    std::vector<GSeg> rowsets[9]; // row segments by their count S (2..8), compiled on first use and
+                                  // kept: a batch's launches may hold pointers into any of them
     int rseg = 0;                // row segments for integer sums: 0 automatic, else 1..8
     uint32_t *d_part = nullptr;  // row segments' partial sums, part_cap elements (grown on demand;
     size_t part_cap = 0;         // calls on one plan are ordered on the caller's stream)
+    std::vector<uint32_t *> part_retired; // outgrown partial-sum buffers, freed with the plan:
+                                          // launches still queued on any stream may use them
+    int pins = 0;                // batches (ebsd_batch_create) holding pointers into this plan
     // background
     int bg_mode = 0;             // 0 none, 1 per-line sums, 2 exact per-pixel
     double *d_linebg = nullptr;  // nCells
@@ -143,8 +147,10 @@ struct GatherArgs {
     int nPx;
     int W, pitch;         // pattern row length; LDS words per staged row (>= W)
     int px0, rows;        // staged rows [px0 / W, px0 / W + rows): all rows, or one row segment
-    int segmode;          // 0 whole lines; row segments: 1 first (store the integer partial
-                          // sums), 2 middle (add), 3 last (add and finish the line values)
+    int segmode;          // 0 whole lines; row segments: 1 first (store the partial sums),
+                          // 2 middle (add), 3 last (add and finish the line values).  Integer
+                          // sums add the earlier segments' sum at the line end; float64 sums
+                          // start from it, so the sum runs in plan order across segments
     int nbatch;
     const uint4 *gidx;    // per-wave streams of plan groups
     const uint2 *bhdr;    // per batch: (unused, groups)
@@ -155,7 +161,8 @@ struct GatherArgs {
     const double *linebg; // fast background mode
     const double *bg;     // exact background mode
     float *out;           // (N, nCells)
-    uint32_t *part;       // row segments: integer partial sums (N, npos) by batch position
+    uint32_t *part;       // row segments: partial sums (N, npos) by batch position, uint32
+                          // (integer sums) or float64 (float sums, two words each)
     int npos;             //   (batch * 64 + lane; the segments share the line order)
     int nCells;
     int64_t npacks;
@@ -439,10 +446,25 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             auto load_prev = [&](int bb, int cell) {
 #pragma unroll
                 for (int p = 0; p < P; ++p)
-                    prev[p] = (a.segmode >= 2 && cell >= 0 && q0 + p < a.N)
+                    prev[p] = (!FLT && a.segmode >= 2 && cell >= 0 && q0 + p < a.N)
                                   ? a.part[(q0 + p) * (int64_t)a.npos + bb * 64 + lane] : 0u;
             };
             load_prev(b, li.x);
+            // float sums in row segments after the first start from the earlier segments'
+            // float64 sum (entries run in plan order, which is row-major): the next batch's
+            // starting sums are loaded one batch ahead
+            double *partd = reinterpret_cast<double *>(a.part);
+            double pdn[P];
+            auto load_pd = [&](double *d, int bb, int cell) {
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    d[p] = (FLT && a.segmode >= 2 && bb < nb && cell >= 0 && q0 + p < a.N)
+                               ? partd[(q0 + p) * (int64_t)a.npos + bb * 64 + lane] : 0.0;
+            };
+            if constexpr (FLT) {
+                load_pd(accd, b, li.x);
+                load_pd(pdn, bn, lin.x);
+            }
             // words up to 8 bytes: the LDS reads of group g+1 are issued before group g is
             // accumulated, so a wave's reads overlap its own adds (two register sets)
             constexpr bool PIPE = C::WB <= 8;
@@ -469,6 +491,10 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
 #pragma unroll
                                 for (int p = 0; p < P; ++p)
                                     if (q0 + p < a.N) a.part[(q0 + p) * (int64_t)a.npos + b * 64 + lane] = prev[p] + acc[p];
+                            } else if (FLT && (a.segmode == 1 || a.segmode == 2) && li.x >= 0) {
+#pragma unroll
+                                for (int p = 0; p < P; ++p)
+                                    if (q0 + p < a.N) partd[(q0 + p) * (int64_t)a.npos + b * 64 + lane] = accd[p];
                             } else if (li.x >= 0) {
                                 // All P values first, then the stores back to back: a store
                                 // between two loads' uses makes the wave wait for the store.
@@ -497,7 +523,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                                     if (q0 + p < a.N) a.out[(q0 + p) * (int64_t)a.nCells + li.x] = r[p];
                             }
 #pragma unroll
-                            for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+                            for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = FLT ? pdn[p] : 0.0; }
                             b = bn;
                             li = lin;
                             bend += ngn;
@@ -507,6 +533,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                             if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
                             else { lin = make_int2(-1, 0); ngn = 0; }
                             load_prev(b, b < nb ? li.x : -1);
+                            if constexpr (FLT) load_pd(pdn, bn, lin.x);
                         }
                     }
                 }
@@ -1658,8 +1685,10 @@ static void free_seg(ebsd_plan::GSeg &g)
 
 static void free_rows(ebsd_plan *p)
 {
-    for (ebsd_plan::GSeg &g : p->rows) free_seg(g);
-    p->rows.clear();
+    for (std::vector<ebsd_plan::GSeg> &rs : p->rowsets) {
+        for (ebsd_plan::GSeg &g : rs) free_seg(g);
+        rs.clear();
+    }
 }
 
 static void plan_free(ebsd_plan *p)
@@ -1668,6 +1697,7 @@ static void plan_free(ebsd_plan *p)
     free_seg(p->full);
     free_rows(p);
     (void)hipFree(p->d_part);
+    for (uint32_t *r : p->part_retired) (void)hipFree(r);
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     (void)hipFree(p->d_mask);
@@ -1740,10 +1770,13 @@ static int geo_tiles(ebsd_plan *p)
 // Lay the plan groups out as one contiguous stream per wave slot: slot v of nv walks batches
 // v, v + nv, v + 2nv, ... back to back, so its next groups can be prefetched several
 // iterations ahead without looking at batch boundaries.  Layouts are cached per nv.
-static int get_layout(ebsd_plan::GSeg &g, int nv, int kind, ebsd_plan::Layout *out)
+static int get_layout(ebsd_plan::GSeg &g, int nv, int kind, ebsd_plan::Layout *out, bool may_alloc = true)
 {
     for (const ebsd_plan::Layout &l : g.layouts)
         if (l.nv == nv && l.kind == kind) { *out = l; return 0; }
+    if (!may_alloc)
+        return fail(-2, "no-allocation path: the plan streams for " + std::to_string(nv) +
+                            " wave slots are not built (the gather's work-group size changed?)");
     const std::vector<uint16_t> &hg = g.h_groups[kind];
     std::vector<uint16_t> st;
     std::vector<int> wofs(nv), wcnt(nv);
@@ -1778,6 +1811,16 @@ static int get_layout(ebsd_plan::GSeg &g, int nv, int kind, ebsd_plan::Layout *o
     g.layouts.push_back(l);
     *out = l;
     return 0;
+}
+
+// The streams of one slice per pack for the configured work-group size, in both entry orders:
+// built with the segment, so that the no-allocation path finds them.
+static int prebuild_layouts(const ebsd_plan *p, ebsd_plan::GSeg &g)
+{
+    ebsd_plan::Layout l;
+    int rc;
+    if ((rc = get_layout(g, p->gthreads / 64, 0, &l))) return rc;
+    return get_layout(g, p->gthreads / 64, 1, &l);
 }
 
 // Slices per pack so that a launch of npacks packs has about two resident rounds of
@@ -1950,14 +1993,7 @@ static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const s
     int rc;
     if ((rc = dev_upload(&g.d_bhdr, bhdr.data(), bhdr.size()))) return rc;
     if ((rc = dev_upload(&g.d_lane, lanes.data(), lanes.size()))) return rc;
-    return 0;
-}
-
-// The whole-pattern batches; row segments are compiled again on first use.
-static int compile_groups(ebsd_plan *p)
-{
-    free_rows(p);
-    return compile_seg(p, p->full, 0, p->H);
+    return prebuild_layouts(p, g);
 }
 
 // S row segments of (nearly) equal height, segment s holding rows [s H / S, (s + 1) H / S).
@@ -1967,8 +2003,11 @@ static int compile_groups(ebsd_plan *p)
 // segment (about 96% of the entry slots used at 240x240 in two segments, 87% in four).
 static int compile_rows(ebsd_plan *p, int S)
 {
-    if ((int)p->rows.size() == S) return 0;
-    free_rows(p);
+    if (S < 2 || S > 8) return fail(-1, "row segments must be in 2..8");
+    std::vector<ebsd_plan::GSeg> &rows = p->rowsets[S];
+    if ((int)rows.size() == S) return 0;
+    for (ebsd_plan::GSeg &g : rows) free_seg(g);
+    rows.clear();
     const int nCells = p->nCells;
     std::vector<std::vector<int>> q(S);
     for (int s = 0; s < S; ++s) {
@@ -1982,12 +2021,29 @@ static int compile_rows(ebsd_plan *p, int S)
             if (q[s][x] != q[s][y]) return q[s][x] > q[s][y];
         return false;
     });
-    p->rows.resize(S);
+    rows.resize(S);
     for (int s = 0; s < S; ++s) {
-        const int rc = compile_seg(p, p->rows[s], s * p->H / S, (s + 1) * p->H / S, &order);
-        if (rc) { free_rows(p); return rc; }
+        const int rc = compile_seg(p, rows[s], s * p->H / S, (s + 1) * p->H / S, &order);
+        if (rc) {
+            for (ebsd_plan::GSeg &g : rows) free_seg(g);
+            rows.clear();
+            return rc;
+        }
     }
     return 0;
+}
+
+static int row_segments(const ebsd_plan *p, int dtype);
+
+// The whole-pattern batches, and the row segments that uint16 patterns (and float staging of
+// the same word size) use where a whole pattern does not fit; other segment counts are
+// compiled on first use.
+static int compile_groups(ebsd_plan *p)
+{
+    free_rows(p);
+    int rc = compile_seg(p, p->full, 0, p->H);
+    if (!rc && row_segments(p, EBSD_U16) > 1) rc = compile_rows(p, row_segments(p, EBSD_U16));
+    return rc;
 }
 
 extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_t *index_plan,
@@ -2105,9 +2161,20 @@ extern "C" int ebsd_plan_destroy(ebsd_plan *plan)
     return 0;
 }
 
+// A batch keeps device pointers into its plans (plan streams, background, masks) in its
+// kernel arguments and captured graph: while one exists, the plan's set-up is frozen.
+static int check_unpinned(const ebsd_plan *p, const char *fn)
+{
+    if (p->pins > 0)
+        return fail(-1, std::string(fn) + ": the plan is bound to a batch (ebsd_batch_create); "
+                                          "destroy the batch before changing the plan");
+    return 0;
+}
+
 extern "C" int ebsd_plan_set_background(ebsd_plan *p, const double *bg, int exact)
 {
     if (!p) return fail(-1, "ebsd_plan_set_background: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_background")) return -1;
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     p->d_linebg = nullptr;
@@ -2135,6 +2202,7 @@ extern "C" int ebsd_plan_set_background(ebsd_plan *p, const double *bg, int exac
 extern "C" int ebsd_plan_set_stage_pitch(ebsd_plan *p, int pitch)
 {
     if (!p) return fail(-1, "ebsd_plan_set_stage_pitch: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_stage_pitch")) return -1;
     if (pitch <= 0) pitch = p->W;
     if (pitch < p->W) return fail(-1, "ebsd_plan_set_stage_pitch: pitch below the pattern width");
     if (pitch == p->spitch) return 0;
@@ -2151,6 +2219,7 @@ extern "C" int ebsd_plan_set_stage_pitch(ebsd_plan *p, int pitch)
 extern "C" int ebsd_plan_set_row_segments(ebsd_plan *p, int segments)
 {
     if (!p) return fail(-1, "ebsd_plan_set_row_segments: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_row_segments")) return -1;
     if (segments < 0 || segments > 8) return fail(-1, "ebsd_plan_set_row_segments: segments must be in 0..8");
     p->rseg = segments;
     return 0;
@@ -2159,6 +2228,7 @@ extern "C" int ebsd_plan_set_row_segments(ebsd_plan *p, int segments)
 extern "C" int ebsd_plan_set_peak_kernel(ebsd_plan *p, int kind)
 {
     if (!p) return fail(-1, "ebsd_plan_set_peak_kernel: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_peak_kernel")) return -1;
     if (kind < 0 || kind > 2) return fail(-1, "ebsd_plan_set_peak_kernel: kind is 0, 1 or 2");
     if (kind == 2 && p->g.sbw <= 0) return fail(-1, "ebsd_plan_set_peak_kernel: the streamed kernel does not apply to this plan");
     p->pkind = kind;
@@ -2169,6 +2239,7 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
                                     int peak_threads)
 {
     if (!p) return fail(-1, "ebsd_plan_set_tuning: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_tuning")) return -1;
     if (gather_pack > 0) {
         // one value per element type, interpreted by the type-specific instantiation
         p->gpack_u8 = std::min(8, std::max(2, gather_pack * 2));
@@ -2178,6 +2249,11 @@ extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_th
     if (gather_threads > 0) {
         if (gather_threads % 64 || gather_threads > 1024) return fail(-1, "gather_threads must be a multiple of 64 <= 1024");
         p->gthreads = gather_threads;
+        int rc = prebuild_layouts(p, p->full);
+        for (std::vector<ebsd_plan::GSeg> &rs : p->rowsets)
+            for (ebsd_plan::GSeg &g : rs)
+                if (!rc) rc = prebuild_layouts(p, g);
+        if (rc) return rc;
     }
     if (peak_threads > 0) {
         if (peak_threads % 64 || peak_threads > 512) return fail(-1, "peak_threads must be a multiple of 64 <= 512");
@@ -2216,7 +2292,8 @@ extern "C" size_t ebsd_workspace_bytes(const ebsd_plan *p, int64_t chunk)
 // that every segment of a launch sequence fits the same pack size).
 template <int IN, bool FLT, int P>
 static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int segmode, uint32_t *part,
-                           const void *d_pats, int64_t N, float *d_out, bool use_bg, hipStream_t s)
+                           const void *d_pats, int64_t N, float *d_out, bool use_bg, bool noalloc,
+                           hipStream_t s)
 {
     typedef GatherCfg<IN, FLT, P> C;
     const size_t lds = (size_t)(maxwords + 1) * C::WB;
@@ -2244,10 +2321,11 @@ static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int s
     a.lane = g.d_lane;
     a.npacks = (N + P - 1) / P;
     const int nw = p->gthreads / 64;
-    a.nslice = choose_slices(p, g, a.npacks, nw, occ);
+    // the no-allocation path walks the plan streams prebuilt for one slice per pack
+    a.nslice = noalloc ? 1 : choose_slices(p, g, a.npacks, nw, occ);
     ebsd_plan::Layout lay;
     int rc;
-    if ((rc = get_layout(g, nw * a.nslice, FLT ? 0 : 1, &lay))) return rc;
+    if ((rc = get_layout(g, nw * a.nslice, FLT ? 0 : 1, &lay, !noalloc))) return rc;
     a.gidx = lay.gidx;
     a.wofs = lay.wofs;
     a.wcnt = lay.wcnt;
@@ -2266,61 +2344,85 @@ static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int s
 static int ensure_part(ebsd_plan *p, size_t elems)
 {
     if (p->part_cap >= elems) return 0;
-    if (p->d_part) {
-        HIPCHK(hipDeviceSynchronize()); // earlier launches may still read the old buffer
-        (void)hipFree(p->d_part);
-        p->d_part = nullptr;
-        p->part_cap = 0;
-    }
-    if (hipMalloc((void **)&p->d_part, elems * sizeof(uint32_t)) != hipSuccess) {
-        p->d_part = nullptr;
+    // grow geometrically; the outgrown buffer is kept until the plan is destroyed, so a launch
+    // still queued on another stream never writes into freed memory (and no device-wide
+    // synchronisation is needed here)
+    const size_t want = std::max(elems, p->part_cap + p->part_cap / 2);
+    uint32_t *fresh = nullptr;
+    if (hipMalloc((void **)&fresh, want * sizeof(uint32_t)) != hipSuccess)
         return fail(-3, "row-segment partial sums: device allocation failed");
-    }
-    p->part_cap = elems;
+    if (p->d_part) p->part_retired.push_back(p->d_part);
+    p->d_part = fresh;
+    p->part_cap = want;
     return 0;
 }
 
 // Partial sums per launch sequence: at most this many bytes (larger N runs in pieces).
 static const int64_t kPartBytes = 2ll << 30;
 
-template <int IN, int P>
+// Where a gather's row-segment partial sums live, and whether it may allocate.
+struct GatherOpts {
+    uint32_t *part = nullptr; // caller's partial-sum buffer (nullptr: the plan's, grown on demand)
+    size_t part_bytes = 0;
+    bool noalloc = false;     // no device allocation: prebuilt plan streams, one slice per pack
+};
+
+// Patterns of one launch sequence over S row segments whose partial sums (pbytes per line
+// and pattern) fit in cap_bytes: whole packs, in equal pieces.
+static int64_t seg_piece(int64_t N, int P, int64_t npos, int pbytes, size_t cap_bytes)
+{
+    const int64_t cap = std::max<int64_t>(P, (int64_t)(cap_bytes / (size_t)pbytes / (size_t)npos) / P * P);
+    const int64_t npieces = (N + cap - 1) / cap;
+    return std::min(cap, ((N + npieces - 1) / npieces + P - 1) / P * P);
+}
+
+template <int IN, bool FLT, int P>
 static int launch_gather_rows(ebsd_plan *p, int S, const void *d_pats, int64_t N, float *d_out, bool use_bg,
-                              hipStream_t s)
+                              const GatherOpts &o, hipStream_t s)
 {
     if (S <= 1)
-        return launch_gather_t<IN, false, P>(p, p->full, p->full.nwords, 0, nullptr, d_pats, N, d_out, use_bg, s);
+        return launch_gather_t<IN, FLT, P>(p, p->full, p->full.nwords, 0, nullptr, d_pats, N, d_out, use_bg,
+                                           o.noalloc, s);
+    std::vector<ebsd_plan::GSeg> &rows = p->rowsets[S];
     int maxwords = 0;
-    for (const ebsd_plan::GSeg &g : p->rows) maxwords = std::max(maxwords, g.nwords);
-    if ((size_t)(maxwords + 1) * GatherCfg<IN, false, P>::WB > (size_t)p->lds_max) return 1;
-    const int64_t npos = (int64_t)p->rows[0].nbatch * 64;
-    // equal pieces of whole packs, each piece's partial sums within kPartBytes
-    const int64_t cap = std::max<int64_t>(P, kPartBytes / 4 / npos / P * P);
-    const int64_t npieces = (N + cap - 1) / cap;
-    const int64_t piece = std::min(cap, ((N + npieces - 1) / npieces + P - 1) / P * P);
-    int rc;
-    if ((rc = ensure_part(p, (size_t)std::min(piece, N) * npos))) return rc;
+    for (const ebsd_plan::GSeg &g : rows) maxwords = std::max(maxwords, g.nwords);
+    if ((size_t)(maxwords + 1) * GatherCfg<IN, FLT, P>::WB > (size_t)p->lds_max) return 1;
+    const int64_t npos = (int64_t)rows[0].nbatch * 64;
+    const int pbytes = FLT ? 8 : 4;
+    uint32_t *part = o.part;
+    int64_t piece;
+    if (part) {
+        if (o.part_bytes < (size_t)P * npos * pbytes)
+            return fail(-2, "row-segment partial sums: buffer smaller than one pack's");
+        piece = seg_piece(N, P, npos, pbytes, std::min<size_t>(o.part_bytes, (size_t)kPartBytes));
+    } else {
+        if (o.noalloc) return fail(-2, "row-segment partial sums: no buffer on the no-allocation path");
+        piece = seg_piece(N, P, npos, pbytes, (size_t)kPartBytes);
+        int rc;
+        if ((rc = ensure_part(p, (size_t)std::min(piece, N) * npos * (pbytes / 4)))) return rc;
+        part = p->d_part;
+    }
     const size_t esz = sizeof(typename InT<IN>::T);
+    int rc;
     for (int64_t q0 = 0; q0 < N; q0 += piece) {
         const int64_t n = std::min(piece, N - q0);
         const void *src = (const char *)d_pats + (size_t)q0 * p->nPx * esz;
         float *dst = d_out + (size_t)q0 * p->nCells;
         for (int k = 0; k < S; ++k) {
             const int mode = k == 0 ? 1 : (k == S - 1 ? 3 : 2);
-            rc = launch_gather_t<IN, false, P>(p, p->rows[k], maxwords, mode, p->d_part, src, n, dst, use_bg, s);
+            rc = launch_gather_t<IN, FLT, P>(p, rows[k], maxwords, mode, part, src, n, dst, use_bg, o.noalloc, s);
             if (rc) return rc < 0 ? rc : fail(-3, "row-segment gather launch failed");
         }
     }
     return 0;
 }
 
-// Row segments for integer sums: the requested count, or the fewest (up to 8) at which the
-// configured pack size fits in LDS (measured, uint16 240x240 with a 360x180 Radon: 2.39 ms
-// per 1024 patterns whole, 1.09 ms in three segments of four patterns per word).
-static int row_segments(const ebsd_plan *p, int dtype)
+// Row segments: the requested count, or the fewest (up to 8) at which words of `WB` bytes
+// fit in LDS (measured, uint16 240x240 with a 360x180 Radon: 2.39 ms per 1024 patterns
+// whole, 1.09 ms in three segments of four patterns per word).
+static int segments_for(const ebsd_plan *p, int WB)
 {
     if (p->rseg > 0) return std::min(p->rseg, p->H);
-    const int P = dtype == EBSD_U8 ? p->gpack_u8 : p->gpack_u16;
-    const int WB = dtype == EBSD_U8 ? P : 2 * P;
     for (int S = 1; S <= 8; ++S) {
         const size_t words = (size_t)((p->H + S - 1) / S) * p->spitch + 1;
         if (words * WB <= (size_t)p->lds_max) return std::min(S, p->H);
@@ -2328,8 +2430,30 @@ static int row_segments(const ebsd_plan *p, int dtype)
     return std::min(8, p->H);
 }
 
+// Integer sums: words of the configured pack size.
+static int row_segments(const ebsd_plan *p, int dtype)
+{
+    const int P = dtype == EBSD_U8 ? p->gpack_u8 : p->gpack_u16;
+    return segments_for(p, dtype == EBSD_U8 ? P : 2 * P);
+}
+
+// Float sums (float32 patterns, exact background): float32 words of the configured pack size.
+static int row_segments_flt(const ebsd_plan *p) { return segments_for(p, 4 * p->gpack_f32); }
+
+// The segments a gather of this dtype uses: S > 1 needs the plan's row segments compiled for
+// S (compiled here, or on the no-allocation path already compiled -- else one whole-pattern
+// pass with a smaller pack, when one fits).
+static int gather_segments(ebsd_plan *p, int S, const GatherOpts &o)
+{
+    if (S <= 1) return 1;
+    if ((int)p->rowsets[S].size() == S) return S;
+    if (o.noalloc) return 1;
+    const int rc = compile_rows(p, S);
+    return rc ? rc : S;
+}
+
 static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int64_t N, float *d_out,
-                         hipStream_t s)
+                         hipStream_t s, const GatherOpts &o = GatherOpts())
 {
     int rc = 1;
     const int dtype = dtype_flags & 0xff;
@@ -2337,42 +2461,40 @@ static int launch_gather(ebsd_plan *p, const void *d_pats, int dtype_flags, int6
     // float32 input (or the exact background mode) stages float32(x - bg) per pixel
     const bool flt = (dtype == EBSD_F32) || (use_bg && p->bg_mode == 2);
     if (flt) {
-        ebsd_plan::GSeg &g = p->full;
-        const int mw = g.nwords;
-        int P = p->gpack_f32;
-        for (; rc == 1 && P >= 1; P /= 2) {
+        const int S = gather_segments(p, row_segments_flt(p), o);
+        if (S < 0) return S;
+        for (int P = p->gpack_f32; rc == 1 && P >= 1; P /= 2) {
             if (dtype == EBSD_U8) {
-                if (P >= 4) rc = launch_gather_t<EBSD_U8, true, 4>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
-                else if (P >= 2) rc = launch_gather_t<EBSD_U8, true, 2>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
-                else rc = launch_gather_t<EBSD_U8, true, 1>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                if (P >= 4) rc = launch_gather_rows<EBSD_U8, true, 4>(p, S, d_pats, N, d_out, use_bg, o, s);
+                else if (P >= 2) rc = launch_gather_rows<EBSD_U8, true, 2>(p, S, d_pats, N, d_out, use_bg, o, s);
+                else rc = launch_gather_rows<EBSD_U8, true, 1>(p, S, d_pats, N, d_out, use_bg, o, s);
             } else if (dtype == EBSD_U16) {
-                if (P >= 4) rc = launch_gather_t<EBSD_U16, true, 4>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
-                else if (P >= 2) rc = launch_gather_t<EBSD_U16, true, 2>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
-                else rc = launch_gather_t<EBSD_U16, true, 1>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                if (P >= 4) rc = launch_gather_rows<EBSD_U16, true, 4>(p, S, d_pats, N, d_out, use_bg, o, s);
+                else if (P >= 2) rc = launch_gather_rows<EBSD_U16, true, 2>(p, S, d_pats, N, d_out, use_bg, o, s);
+                else rc = launch_gather_rows<EBSD_U16, true, 1>(p, S, d_pats, N, d_out, use_bg, o, s);
             } else {
-                if (P >= 4) rc = launch_gather_t<EBSD_F32, true, 4>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
-                else if (P >= 2) rc = launch_gather_t<EBSD_F32, true, 2>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
-                else rc = launch_gather_t<EBSD_F32, true, 1>(p, g, mw, 0, nullptr, d_pats, N, d_out, use_bg, s);
+                if (P >= 4) rc = launch_gather_rows<EBSD_F32, true, 4>(p, S, d_pats, N, d_out, use_bg, o, s);
+                else if (P >= 2) rc = launch_gather_rows<EBSD_F32, true, 2>(p, S, d_pats, N, d_out, use_bg, o, s);
+                else rc = launch_gather_rows<EBSD_F32, true, 1>(p, S, d_pats, N, d_out, use_bg, o, s);
             }
         }
         if (rc == 1) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
         return rc;
     }
-    const int S = row_segments(p, dtype);
-    if (S > 1 && (rc = compile_rows(p, S))) return rc;
-    rc = 1;
+    const int S = gather_segments(p, row_segments(p, dtype), o);
+    if (S < 0) return S;
     if (dtype == EBSD_U8) {
         for (int P = p->gpack_u8; rc == 1 && P >= 2; P /= 2) {
-            if (P >= 8) rc = launch_gather_rows<EBSD_U8, 8>(p, S, d_pats, N, d_out, use_bg, s);
-            else if (P >= 4) rc = launch_gather_rows<EBSD_U8, 4>(p, S, d_pats, N, d_out, use_bg, s);
-            else rc = launch_gather_rows<EBSD_U8, 2>(p, S, d_pats, N, d_out, use_bg, s);
+            if (P >= 8) rc = launch_gather_rows<EBSD_U8, false, 8>(p, S, d_pats, N, d_out, use_bg, o, s);
+            else if (P >= 4) rc = launch_gather_rows<EBSD_U8, false, 4>(p, S, d_pats, N, d_out, use_bg, o, s);
+            else rc = launch_gather_rows<EBSD_U8, false, 2>(p, S, d_pats, N, d_out, use_bg, o, s);
         }
     } else {
         for (int P = p->gpack_u16; rc == 1 && P >= 1; P /= 2) {
-            if (P >= 8) rc = launch_gather_rows<EBSD_U16, 8>(p, S, d_pats, N, d_out, use_bg, s);
-            else if (P >= 4) rc = launch_gather_rows<EBSD_U16, 4>(p, S, d_pats, N, d_out, use_bg, s);
-            else if (P >= 2) rc = launch_gather_rows<EBSD_U16, 2>(p, S, d_pats, N, d_out, use_bg, s);
-            else rc = launch_gather_rows<EBSD_U16, 1>(p, S, d_pats, N, d_out, use_bg, s);
+            if (P >= 8) rc = launch_gather_rows<EBSD_U16, false, 8>(p, S, d_pats, N, d_out, use_bg, o, s);
+            else if (P >= 4) rc = launch_gather_rows<EBSD_U16, false, 4>(p, S, d_pats, N, d_out, use_bg, o, s);
+            else if (P >= 2) rc = launch_gather_rows<EBSD_U16, false, 2>(p, S, d_pats, N, d_out, use_bg, o, s);
+            else rc = launch_gather_rows<EBSD_U16, false, 1>(p, S, d_pats, N, d_out, use_bg, o, s);
         }
     }
     if (rc == 1) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
@@ -3174,6 +3296,9 @@ struct ebsd_batch {
         float *radon;
     };
     std::vector<Single> singles;
+    uint32_t *part = nullptr;        // the singles' row-segment partial sums (in the workspace;
+    size_t part_bytes = 0;           // the singles run one after another, so they share it)
+    std::vector<ebsd_plan *> pinned; // distinct plans of the batch, pinned while it lives
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -3181,6 +3306,7 @@ struct ebsd_batch {
 static void batch_free(ebsd_batch *b)
 {
     if (!b) return;
+    for (ebsd_plan *p : b->pinned) --p->pins;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     (void)hipFree(b->d_gsets);
@@ -3191,15 +3317,25 @@ static void batch_free(ebsd_batch *b)
     delete b;
 }
 
+// Partial-sum bytes a large set may need when its per-size gather runs in row segments
+// (float64 sums, the larger kind; at most one launch sequence's worth).
+static size_t single_part_bytes(const ebsd_plan *p, int64_t N)
+{
+    if (N < kSingleSetMin) return 0;
+    const size_t npos = (size_t)((p->nCells + 63) / 64) * 64;
+    return std::min<size_t>((size_t)N * npos * 8, (size_t)kPartBytes);
+}
+
 extern "C" size_t ebsd_batch_workspace_bytes(int nsets, ebsd_plan *const *plans, const int64_t *N)
 {
     if (nsets <= 0 || !plans || !N) return 0;
-    size_t total = 0;
+    size_t total = 0, part = 0;
     for (int i = 0; i < nsets; ++i) {
         if (!plans[i] || N[i] < 0) return 0;
         total += (((size_t)N[i] * plans[i]->nCells * sizeof(float) + 255) / 256) * 256;
+        part = std::max(part, single_part_bytes(plans[i], N[i]));
     }
-    return total;
+    return total + part;
 }
 
 extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void *const *d_pats,
@@ -3268,12 +3404,16 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         S.mrow = p->d_mrow;
         S.out = outs[i];
     }
-    // a set that fills the GPU by itself (kSingleSetMin patterns or more) and whose gather
-    // needs no row segments runs the per-size launcher, with the plan's tuned threads and
-    // pitch, inside the batch; the merged launches are for the others
+    // a set that fills the GPU by itself (kSingleSetMin patterns or more) runs the per-size
+    // launcher, with the plan's tuned shape and row segments, inside the batch; the partial
+    // sums of its row segments go to the batch's own buffer (after the Radons in the
+    // workspace; the singles run one after another and share it).  The warm-up below
+    // compiles every row segmentation they use, and a plan keeps its segmentations (one per
+    // segment count) until it is destroyed; the merged launches are for the others
+    b->part = (uint32_t *)work;
+    b->part_bytes = (size_t)((char *)d_work + ws_bytes - work);
     for (int i = 0; i < nsets; ++i) {
         if (N[i] < kSingleSetMin) continue;
-        if (!fltv[i] && row_segments(plans[i], dtypes[i] & 0xff) > 1) continue;
         b->singles.push_back(ebsd_batch::Single{plans[i], d_pats[i], dtypes[i], N[i], gs[i].out});
         cls[i] = 0;
     }
@@ -3415,8 +3555,11 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         }
     // one launch of each per-size gather now: its plan structures and stream layouts exist
     // afterwards, so a later launch (or its graph capture) allocates nothing
+    GatherOpts go;
+    go.part = b->part;
+    go.part_bytes = b->part_bytes;
     for (const ebsd_batch::Single &g : b->singles) {
-        if ((rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, nullptr))) {
+        if ((rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, nullptr, go))) {
             batch_free(b);
             return rc;
         }
@@ -3425,14 +3568,22 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         batch_free(b);
         return fail(-3, "batched per-size gather warm-up failed");
     }
+    for (int i = 0; i < nsets; ++i)
+        if (std::find(b->pinned.begin(), b->pinned.end(), plans[i]) == b->pinned.end()) {
+            b->pinned.push_back(plans[i]);
+            ++plans[i]->pins;
+        }
     *out = b;
     return 0;
 }
 
 static int batch_launch(ebsd_batch *b, hipStream_t s)
 {
+    GatherOpts go;
+    go.part = b->part;
+    go.part_bytes = b->part_bytes;
     for (const ebsd_batch::Single &g : b->singles) {
-        const int rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, s);
+        const int rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, s, go);
         if (rc) return rc;
     }
     for (const GatherClass &gc : b->classes) {

@@ -68,9 +68,10 @@ def _workspace(x):
     return buf, nbytes
 
 
-def _stream(stream):
-    torch = _torch()
-    s = stream if stream is not None else torch.cuda.current_stream()
+def _stream(stream, device=None):
+    """The launch stream as a handle: the current stream, or ``stream`` after it has waited
+    for the current stream's queued work (see _hip.launch_stream)."""
+    s = _hip.launch_stream(stream, device)
     return ctypes.c_void_p(s.cuda_stream)
 
 
@@ -110,7 +111,9 @@ def pattern_max(pats, stream=None):
     out = torch.empty(1, dtype=torch.float32, device=pats.device)
     n = pats.shape[0] if pats.dim() > 2 else 1
     P = pats.numel() // max(n, 1)
-    _hip.check_nlpar(L.ebsd_nlpar_max(_ptr(pats), _code(pats), n, P, _ptr(out), _stream(stream)))
+    _hip.check_nlpar(L.ebsd_nlpar_max(_ptr(pats), _code(pats), n, P, _ptr(out), _stream(stream, pats.device)))
+    if stream is not None:
+        stream.synchronize()   # the reduction ran on ``stream``: done before the read
     return float(out.item())
 
 
@@ -138,7 +141,9 @@ def sigma_device(pats, nrows, ncols, indices, nn=1, saturation_protect=True, ret
     _hip.check_nlpar(L.ebsd_nlpar_sigma(_ptr(x), _code(x), nrows, ncols, P, _ptr(idx), idx.numel(), nn,
                                         _sat_sigma(mx, saturation_protect), _ptr(sig),
                                         None if dout is None else _ptr(dout), None if nout is None else _ptr(nout),
-                                        None if work is None else _ptr(work), wbytes, _stream(stream)))
+                                        None if work is None else _ptr(work), wbytes, _stream(stream, x.device)))
+    if stream is not None:
+        _hip.keep_for_stream(stream, x, idx, sig, dout, nout, work)
     return (sig, dout, nout) if return_d2 else sig
 
 
@@ -169,7 +174,9 @@ def nlpar_device(pats, nrows, ncols, indices, sigma, lam=0.7, sr=3, dthresh=0.0,
     _hip.check_nlpar(L.ebsd_nlpar(_ptr(x), _code(x), nrows, ncols, P, _ptr(idx), idx.numel(), _ptr(sig), int(sr),
                                   float(lam), float(dthresh), float(diff_offset), _sat_nlpar(mx, saturation_protect),
                                   r0, c0, nrc, ncc, _ptr(out), _code(out), None if wts is None else _ptr(wts),
-                                  None if work is None else _ptr(work), wbytes, _stream(stream)))
+                                  None if work is None else _ptr(work), wbytes, _stream(stream, x.device)))
+    if stream is not None:
+        _hip.keep_for_stream(stream, x, idx, sig, out, wts, work)
     return (out, wts) if return_weights else out
 
 

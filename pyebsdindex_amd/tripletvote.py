@@ -173,8 +173,8 @@ def row_index(tripAngles, angTol: float, bin_width: float = 0.5, max_angle: floa
     covers smallest triplet angles x0 in [b w, (b+1) w) (the first bin down to -inf, the
     last up to +inf) and lists, ascending, every row whose first angle is within angTol
     (plus 1e-3 degree of slack: the kernel finds the bin with a float32 product) of that
-    range.  Returns (start (nbins+1,) int32, rows int32,
-    bin_width)."""
+    range.  Returns (start (nbins+1,) int32, rows int32, bin_width, angTol): the index
+    only serves votes whose tolerance is at most the one it was built for."""
     a0 = np.asarray(tripAngles, dtype=np.float64).reshape(-1, 3)[:, 0]
     nbins = int(math.ceil(max_angle / bin_width)) + 1
     lo = np.arange(nbins) * bin_width - angTol - 1e-3
@@ -184,7 +184,7 @@ def row_index(tripAngles, angTol: float, bin_width: float = 0.5, max_angle: floa
     start = np.zeros(nbins + 1, dtype=np.int32)
     start[1:] = np.cumsum([len(x) for x in lists])
     rows = np.concatenate(lists).astype(np.int32) if lists else np.zeros(0, np.int32)
-    return start, rows, float(bin_width)
+    return start, rows, float(bin_width), float(angTol)
 
 
 def addphase(libtype: str = "FCC", angTol: float = 2.0, polefamilies=None):
@@ -225,8 +225,9 @@ class BandIndexer:
         if key not in self._dev:
             a = torch.as_tensor(np.ascontiguousarray(self.angtriplets["angles"], dtype=np.float64), device=device)
             f = torch.as_tensor(np.ascontiguousarray(self.angtriplets["familyid"], dtype=np.int32), device=device)
-            start, rows, bw = row_index(self.angtriplets["angles"], self.angTol)
-            idx = (torch.as_tensor(start, device=device), torch.as_tensor(rows, device=device), bw) if rows.size else None
+            start, rows, bw, tol = row_index(self.angtriplets["angles"], self.angTol)
+            idx = (torch.as_tensor(start, device=device), torch.as_tensor(rows, device=device), bw, tol) \
+                if rows.size else None
             self._dev[key] = (a, f, idx)
         return self._dev[key]
 
@@ -257,8 +258,9 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
     x2, (npats, nfam, nbands) float32 -- the reference's output order (weighted votes
     first, vote counts last).
     index: "auto" builds the row index of row_index() when tripAngles is a host array
-    (the outputs do not depend on it), None tests every row, or a (start, rows, width)
-    triple of device tensors from row_index()."""
+    (the outputs do not depend on it), None tests every row, or the (start, rows, width,
+    built_tol) of row_index() with start and rows as device tensors; an index built for a
+    tolerance below angTol would drop matching rows and is refused."""
     import torch
     lib = load_tripvote()
     if not isinstance(bandnorms, torch.Tensor) or bandnorms.device.type != "cuda":
@@ -286,12 +288,17 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
         raise ValueError(f"tripID holds families outside [0, {int(nfam)})")
     if isinstance(index, str):
         if index != "auto":
-            raise ValueError("index must be 'auto', None or a row_index() triple")
+            raise ValueError("index must be 'auto', None or the 4-tuple of row_index()")
         index = None
         if isinstance(tripAngles, np.ndarray):
-            st, rw, bw = row_index(tripAngles, float(angTol))
+            st, rw, bw, tol = row_index(tripAngles, float(angTol))
             if rw.size:
-                index = (torch.as_tensor(st, device=dev), torch.as_tensor(rw, device=dev), bw)
+                index = (torch.as_tensor(st, device=dev), torch.as_tensor(rw, device=dev), bw, tol)
+    elif index is not None:
+        if len(index) != 4:
+            raise ValueError("index must be the (start, rows, width, built_tol) 4-tuple of row_index()")
+        if float(index[3]) < float(angTol):
+            raise ValueError(f"row index built for angTol {float(index[3])} cannot serve angTol {float(angTol)}")
     ta = torch.as_tensor(tripAngles, device=dev).to(torch.float64).contiguous()
     ti = torch.as_tensor(tripID, device=dev).to(torch.int32).contiguous()
     if ta.dim() != 2 or ta.shape[1] != 3 or tuple(ti.shape) != tuple(ta.shape):
@@ -302,8 +309,8 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
     fam = torch.empty((npats, nb), dtype=torch.int32, device=dev)
     rank = torch.empty((npats, nb), dtype=torch.float32, device=dev)
     cm = torch.empty_like(rank)
-    if stream is None:
-        stream = torch.cuda.current_stream(dev)
+    from ._hip import keep_for_stream, launch_stream
+    stream = launch_stream(stream, dev)
     P = ctypes.c_void_p
     rc = lib.ebsd_tripvote(P(bn.data_ptr()), P(inten.data_ptr()), ctypes.c_int64(npats), ctypes.c_int(nb),
                            P(ta.data_ptr()), P(ti.data_ptr()), ctypes.c_int(int(ta.shape[0])),
@@ -315,6 +322,8 @@ def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, 
                            P(accW.data_ptr()), P(fam.data_ptr()), P(rank.data_ptr()), P(cm.data_ptr()),
                            P(acc.data_ptr()), P(stream.cuda_stream))
     check_tripvote(lib, rc)
+    keep_for_stream(stream, bn, inten, ta, ti, accW, fam, rank, cm, acc,
+                    *(index[:2] if index is not None else ()))
     return accW, fam, rank, cm, acc
 
 

@@ -102,3 +102,52 @@ def test_batch_with_large_sets():
         for i, (o, w) in enumerate(zip(batch.outs, want)):
             for k in w:
                 assert np.array_equal(o[k].cpu().numpy(), w[k]), (rep, i, k)
+
+
+def test_batch_pins_its_plans():
+    """A batch holds device pointers into its plans: while it exists the plans refuse every
+    set-up change, and a find_bands on one of them with a larger N (its row-segment partial
+    sums grow; the outgrown buffer is kept until the plan goes) leaves the batch's eager run
+    and graph replays bit-identical to per-set launches.  After the batch is destroyed the
+    plans accept changes again."""
+    import gc
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd._hip import HipLibraryError
+    from pyebsdindex_amd.batch import BatchBandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    print("batch pins its plans", flush=True)
+    spec = [((240, 240), "uint16", 24), ((120, 120), "uint16", 2100), ((60, 60), "uint8", 30)]
+    dets, pats = [], []
+    for k, (shape, dtype, n) in enumerate(spec):
+        kw = dict(STD, nTheta=360, nRho=180, nBands=12) if shape[0] == 240 else STD
+        pats.append(kikuchi_patterns(n, shape, dtype, seed=150 + k, family="BCC", device="cuda"))
+        dets.append(BandDetect(patDim=shape, **kw))
+    want = [{k: v.cpu().numpy() for k, v in bd.find_bands_device(x).items()} for bd, x in zip(dets, pats)]
+    batch = BatchBandDetect(dets, pats)
+    plan = dets[0].device_plan()
+    for call in (lambda: plan.set_row_segments(2), lambda: plan.set_tuning(gather_pack=2),
+                 lambda: plan.set_stage_pitch(241), lambda: plan.set_peak_kernel(1),
+                 lambda: plan.set_background(np.zeros((240, 240)), False)):
+        with pytest.raises(HipLibraryError, match="bound to a batch"):
+            call()
+    # a larger find_bands on the 240x240 plan (row segments: its partial sums grow)
+    big = kikuchi_patterns(600, (240, 240), "uint16", seed=160, family="BCC", device="cuda")
+    ref_big = {k: v.cpu().numpy() for k, v in dets[0].find_bands_device(big).items()}
+    for rep in range(3):
+        for o in batch.outs:
+            for v in o.values():
+                v.zero_()
+        # no explicit synchronisation: the replay's private stream is ordered both ways
+        # with the current stream (the zeroing before it, the reads after it)
+        batch.graph_launch() if rep else batch.run()
+        for i, (o, w) in enumerate(zip(batch.outs, want)):
+            for k in w:
+                assert np.array_equal(o[k].cpu().numpy(), w[k]), (rep, i, k)
+    again = {k: v.cpu().numpy() for k, v in dets[0].find_bands_device(big).items()}
+    for k in ref_big:
+        assert np.array_equal(again[k], ref_big[k]), k
+    del batch
+    gc.collect()
+    plan.set_row_segments(2)
+    plan.set_row_segments(0)

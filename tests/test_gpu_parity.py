@@ -114,6 +114,55 @@ def test_find_bands_vs_fp64(shape, dtype, n, seed):
     np.testing.assert_allclose(got['width'][same], res['width'][same], rtol=1e-3, atol=1e-4)
 
 
+@pytest.mark.parametrize("case,mode", [("120-u16", "fast"), ("120-u16", "exact"),
+                                       ("240-360x180", "fast"), ("240-360x180", "exact")])
+def test_find_bands_background_vs_fp64(case, mode):
+    """Whole find_bands with background subtraction -- the benchmarked configurations: 120x120
+    uint16 FCC with a 180x90 Radon, and 240x240 uint16 HCP (c/a 1.587) with a 360x180 Radon and
+    12 bands -- against the FP64 oracle given the same background (x - b per pixel in float64,
+    radon_fast.py:255-259): integer peak bins identical except FP64 near-ties, sub-pixel
+    positions within 1e-3 bin, stats within 1e-5.  The exact mode is also held to the
+    reference restatement, which rounds every x - b to float32 as the reference does: the
+    Radon bit for bit, and its bands."""
+    print(f"find_bands background {case} {mode}", flush=True)
+    if case == "120-u16":
+        shape, kw, n, fam = (120, 120), {}, 24, "FCC"
+    else:
+        shape, kw, n, fam = (240, 240), dict(nTheta=360, nRho=180, nBands=12), 5, "HCP"
+    bd, ref = make_pair(shape, background_mode=mode, **kw)
+    pats = _pats(n, shape, "uint16", seed=61, family=fam)
+    bd.collect_background(patsIn=pats.astype(np.float64))
+    bg = bd.backgroundsub
+    assert bg is not None and bg.shape == shape
+    got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(_dev(pats)).items()}
+    res, st, C64, R64 = oracle64(ref, pats, background=bg)
+    total, ties = compare_bands(got, res, C64, ref.padding, ref.peakPad)
+    assert ties <= max(1, total // 100)
+    assert (got['valid'] == 1).mean() >= 0.9
+    np.testing.assert_allclose(got['stats'][:, 0], st[0], rtol=0, atol=1e-5 * np.abs(st[0]).max())
+    np.testing.assert_allclose(got['stats'][:, 1], st[1], rtol=0, atol=1e-5 * np.abs(st[2]).max())
+    np.testing.assert_allclose(got['stats'][:, 2], st[2], rtol=1e-5)
+    same = (got['valid'] == 1) & (res['valid'] == 1) & np.all(got['maxloc'] == res['maxloc'], -1)
+    np.testing.assert_allclose(got['max'][same], res['max'][same], rtol=2e-5, atol=1e-5 * np.abs(res['max']).max())
+    np.testing.assert_allclose(got['theta'][same], res['theta'][same], atol=1e-3 * np.pi / 180)
+    np.testing.assert_allclose(got['rho'][same], res['rho'][same], atol=1e-3 * 2 * bd.rhoMax / bd.nRho)
+    # the device Radon itself: fast within 1e-6 of FP64, exact bit-identical to the restatement
+    plan = bd.device_plan()
+    x, flags = bd._prepare(plan, _dev(pats))
+    rad = plan.radon(x, flags=flags).cpu().numpy()
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding, background=bg)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    if mode == "exact":
+        assert np.array_equal(rad, want)
+        ref.backgroundsub = bg
+        rb = ref.find_bands(pats)
+        agree = np.all(got['maxloc'] == rb['maxloc'], -1) & (got['valid'] == rb['valid'])
+        assert agree.mean() >= 0.98
+        np.testing.assert_allclose(got['aveloc'][agree], rb['aveloc'][agree], atol=1e-3)
+    else:
+        assert np.abs(rad - R64).max() <= 1e-6 * np.abs(R64).max()
+
+
 def test_find_bands_api_matches_reference_restatement(al_sim_pattern):
     """BandDetect.find_bands (numpy in, structured array out) vs the reference-faithful CPU
     restatement on the reference's own test pattern, plus synthetic 8-bit patterns."""
@@ -208,6 +257,32 @@ def test_row_segments_identical(shape, dtype, n):
     plan.set_row_segments(3)
     assert np.array_equal(plan.radon(x).cpu().numpy(), want)
     plan.set_row_segments(0)
+
+
+@pytest.mark.parametrize("shape,dtype,mode", [((240, 240), "float32", None), ((160, 160), "float32", None),
+                                              ((100, 120), "uint16", "exact"), ((240, 240), "uint8", "exact")])
+def test_float_row_segments_bit_identical(shape, dtype, mode):
+    """Float sums (float32 patterns, the exact background mode) in row segments: each segment
+    continues the float64 sum of the ones before it (a line's entries run in plan order,
+    which is row-major), so the Radon equals the reference restatement's bit for bit for
+    every segment count -- also where a whole 240x240 float pattern does not fit in LDS."""
+    print(f"float row segments {shape} {dtype} {mode}", flush=True)
+    kw = dict(nTheta=360, nRho=180, nBands=12) if shape[0] == 240 else {}
+    bd, ref = make_pair(shape, background_mode=mode or "fast", **kw)
+    pats = _pats(5, shape, dtype, seed=71)
+    bg = None
+    if mode:
+        bg = np.linspace(-40.0, 55.5, shape[0] * shape[1]).reshape(shape) + 0.1234567
+        bd.backgroundsub = bg
+    plan = bd.device_plan()
+    x, flags = bd._prepare(plan, _dev(pats))
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(pats, ref.padding, background=bg)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    for S in ((2, 3, 4) if shape[0] == 240 else (1, 2, 3)):
+        plan.set_row_segments(S)
+        assert np.array_equal(plan.radon(x, flags=flags).cpu().numpy(), want), S
+    plan.set_row_segments(0)
+    assert np.array_equal(plan.radon(x, flags=flags).cpu().numpy(), want)
 
 
 def test_autotune_keeps_results():

@@ -176,6 +176,12 @@ def test_row_index_gives_identical_outputs(phase, tol):
     b = ind.tripvote(bnd, intd, use_index=False)
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+    # an index built for a smaller tolerance would drop matching rows: refused
+    st, rw, bw, built = tv.row_index(ind.angtriplets["angles"], tol - 0.25)
+    small = (torch.as_tensor(st, device=dev), torch.as_tensor(rw, device=dev), bw, built)
+    with pytest.raises(ValueError, match="cannot serve"):
+        tv.tripvote_device(bnd, intd, ind.lut, tol, ind.angtriplets["angles"], ind.angtriplets["familyid"],
+                           ind.npolefamilies, index=small)
 
 
 def test_no_patterns():

@@ -145,7 +145,8 @@ def test_row_index_lists_every_candidate(phase, tol):
     inside -- in ascending order."""
     ind = tv.addphase(phase)
     a0 = ind.angtriplets["angles"][:, 0]
-    start, rows, bw = tv.row_index(ind.angtriplets["angles"], tol)
+    start, rows, bw, built = tv.row_index(ind.angtriplets["angles"], tol)
+    assert built == tol
     nb = start.size - 1
     for b in range(nb):
         lst = rows[start[b]:start[b + 1]]
@@ -163,3 +164,66 @@ def test_row_index_lists_every_candidate(phase, tol):
             kb = int(np.floor(np.float32(x) * np.float32(1.0 / bw)))
             kb = min(max(kb, 0), nb - 1)
             assert set(need) <= set(rows[start[kb]:start[kb + 1]]), (b, x, kb)
+
+
+def _angle_deg(x, y, fused):
+    """The float32 inter-band angle of the oracle (((x0 y0) + (x1 y1)) + (x2 y2), no
+    contraction) or of a fused form (fma(x2, y2, fma(x1, y1, x0 y0)), each fma rounded once
+    to float32); arccos correctly rounded to float32, then times RADEG in float64."""
+    from fractions import Fraction
+    x, y = np.asarray(x, np.float32), np.asarray(y, np.float32)
+    if fused:
+        acc = np.float32(x[0] * y[0])
+        for k in (1, 2):
+            acc = np.float32(float(Fraction(float(x[k])) * Fraction(float(y[k])) + Fraction(float(acc))))
+        c = acc
+    else:
+        c = np.float32(np.float32(x[0] * y[0]) + np.float32(x[1] * y[1])) + np.float32(x[2] * y[2])
+    c = min(abs(np.float32(c)), np.float32(1.0))
+    return np.float32(float(np.float32(np.arccos(np.float64(c)))) * (180.0 / np.pi))
+
+
+def test_cosine_form_at_the_tolerance_edge():
+    """Parity unpinned, made visible: the reference's inter-band cosines come from a float32
+    2-D ``.dot`` (tripletvote.py:1192), which Numba hands to BLAS; whether that sgemm fuses
+    the multiply-adds is not known here.  The oracle and the device evaluate
+    ((x0 y0) + (x1 y1)) + (x2 y2) without contraction.  For a band pair whose angle differs
+    by an ulp between the two forms, a library row placed exactly angTol away from the
+    oracle's angle gets the oracle's vote, and would not get it from the fused form."""
+    rng = np.random.default_rng(17)
+    a = b = None
+    for _ in range(200000):
+        x = rng.normal(size=3)
+        y = x + rng.normal(scale=0.9, size=3)
+        x = (x / np.linalg.norm(x)).astype(np.float32)
+        y = (y / np.linalg.norm(y)).astype(np.float32)
+        ga, gf = _angle_deg(x, y, False), _angle_deg(x, y, True)
+        if ga != gf and 15.0 < ga < 60.0:
+            a, b = x, y
+            break
+    assert a is not None, "no pair with differing cosine forms found"
+    # a third band far (>= 10 degrees) from the pair's angle on both edges
+    for _ in range(10000):
+        z = rng.normal(size=3)
+        z = (z / np.linalg.norm(z)).astype(np.float32)
+        e1, e2 = _angle_deg(a, z, False), _angle_deg(b, z, False)
+        if min(abs(e1 - ga), abs(e2 - ga), abs(e1 - e2)) > 10.0:
+            break
+    bn = np.stack([a, b, z])[None]
+    angs = np.array([ga, e1, e2], dtype=np.float32)  # (AB, AC, BC)
+    srt = np.argsort(angs)
+    # the library row: the pattern's own angles, the pair's angle moved by exactly
+    # angTol = 1 away from the oracle's value, on the side the fused value lies away from
+    step = 1.0 if gf < ga else -1.0
+    row = angs.astype(np.float64)
+    row[0] = float(ga) + step
+    lib = row[srt][None]
+    fam = np.array([[0, 0, 0]], dtype=np.int64)
+    accW, bandFam, rank, cm, acc = tripvote_ref.tripvote(bn, None, tv.triplet_lut(), 1.0, lib, fam, 1)
+    assert acc.sum() >= 3            # the oracle's angle is angTol away: the row votes
+    assert abs(row[0] - float(gf)) > 1.0   # the fused form's angle is just outside angTol
+    # one ulp further the oracle too leaves the row out
+    row2 = row.copy()
+    row2[0] = np.nextafter(row[0], row[0] + step)
+    accW, bandFam, rank, cm, acc = tripvote_ref.tripvote(bn, None, tv.triplet_lut(), 1.0, row2[srt][None], fam, 1)
+    assert acc.sum() == 0
