@@ -105,6 +105,22 @@ size_t ebsd_workspace_bytes(const ebsd_plan *plan, int64_t chunk);
 int ebsd_find_bands(ebsd_plan *plan, const void *d_pats, int dtype, int64_t N,
                     const ebsd_bands *out, void *d_work, size_t ws_bytes, void *stream);
 
+/* ebsd_find_bands without a workspace and without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Each chunk's Radon (and,
+ * where the gather runs in row segments, their partial sums) goes to a file-scope device
+ * array of 64 MiB that every GPU the library runs on holds until the process ends; patterns
+ * are processed in chunks that fit it (1,035 per chunk at 120x120 with a 180x90 Radon).  A
+ * pattern whose Radon alone exceeds 64 MiB is refused with an error.  The call uses only
+ * structures built by the plan's set-up calls: row segments already compiled for the
+ * pattern type (the uint16 ones are compiled by ebsd_plan_create), otherwise one
+ * whole-pattern pass with a smaller pack.  Every call on one GPU shares that GPU's array
+ * and leaves its contents behind (each call overwrites what it reads, nothing needs
+ * clearing): calls that overlap on one GPU -- on different streams or host threads -- race.
+ * The plan's device must be current.  Results equal ebsd_find_bands's bit for bit. */
+int ebsd_find_bands_noalloc(ebsd_plan *plan, const void *d_pats, int dtype, int64_t N,
+                            const ebsd_bands *out, void *stream);
+
 /* Stage-level entry points (the reference's per-stage methods, for tests and adjoints). */
 /* Radon of N patterns into d_radon (N, nRho, nTheta) float32, -1 where a line is empty.
  * Replaces: Radon.radon_faster (radon_fast.py:242-279) without padding. */

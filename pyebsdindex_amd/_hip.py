@@ -18,7 +18,7 @@ EBSD_U8, EBSD_U16, EBSD_F32 = 0, 1, 2
 #: exported symbols, in declaration order of include/ebsd_radon.h
 SYMBOLS = (
     "ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background", "ebsd_plan_set_tuning",
-    "ebsd_plan_set_stage_pitch", "ebsd_plan_set_row_segments", "ebsd_plan_set_peak_kernel", "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
+    "ebsd_plan_set_stage_pitch", "ebsd_plan_set_row_segments", "ebsd_plan_set_peak_kernel", "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_find_bands_noalloc", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
     "ebsd_plan_info", "ebsd_last_error", "ebsd_batch_workspace_bytes", "ebsd_batch_create",
     "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy", "ebsd_band_grad",
     "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands", "ebsd_radon2pole",
@@ -73,6 +73,7 @@ def load(path: str = LIB_PATH):
         L.ebsd_workspace_bytes.argtypes = [P, I64]
         L.ebsd_workspace_bytes.restype = ctypes.c_size_t
         L.ebsd_find_bands.argtypes = [P, P, I, I64, ctypes.POINTER(EbsdBands), P, ctypes.c_size_t, P]
+        L.ebsd_find_bands_noalloc.argtypes = [P, P, I, I64, ctypes.POINTER(EbsdBands), P]
         L.ebsd_radon.argtypes = [P, P, I, I64, P, P]
         L.ebsd_peaks.argtypes = [P, P, I64, ctypes.POINTER(EbsdBands), P]
         L.ebsd_conv.argtypes = [P, P, I64, P, P]
@@ -92,7 +93,8 @@ def load(path: str = LIB_PATH):
         L.ebsd_last_error.restype = ctypes.c_char_p
         for name in ("ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background",
                      "ebsd_plan_set_tuning", "ebsd_plan_set_peak_kernel", "ebsd_plan_set_stage_pitch",
-                     "ebsd_plan_set_row_segments", "ebsd_find_bands", "ebsd_radon", "ebsd_peaks",
+                     "ebsd_plan_set_row_segments", "ebsd_find_bands", "ebsd_find_bands_noalloc",
+                     "ebsd_radon", "ebsd_peaks",
                      "ebsd_conv", "ebsd_plan_info", "ebsd_batch_create", "ebsd_batch_run",
                      "ebsd_batch_graph_launch", "ebsd_batch_destroy",
                      "ebsd_band_grad", "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands",

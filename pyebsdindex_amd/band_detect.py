@@ -166,6 +166,23 @@ class DevicePlan:
         _hip.keep_for_stream(s, pats, work, *out.values())
         return out
 
+    def find_bands_noalloc(self, pats, out=None, stream=None, flags=0):
+        """``ebsd_find_bands_noalloc`` on device patterns: the library allocates no device
+        memory (the chunks' Radons live in its per-GPU scratch array); ``out`` is allocated
+        here unless given.  Calls that overlap on one GPU race on that array."""
+        pats, code = self._check_pats(pats)
+        if code is None:
+            raise TypeError(f"unsupported device pattern dtype {pats.dtype}")
+        N = pats.shape[0]
+        if out is None:
+            out = self.alloc_out(N)
+        s = _hip.launch_stream(stream, self.device)
+        b = self.bands_struct(out)
+        _hip.check(self.lib.ebsd_find_bands_noalloc(self.handle, pats.data_ptr(), code | flags, N,
+                                                    _hip.ctypes.byref(b), s.cuda_stream))
+        _hip.keep_for_stream(s, pats, *out.values())
+        return out
+
     def radon(self, pats, stream=None, flags=0, out=None):
         import torch
         pats, code = self._check_pats(pats)

@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "ebsd_radon.h"
@@ -101,6 +102,7 @@ struct ebsd_plan {
     std::vector<uint32_t *> part_retired; // outgrown partial-sum buffers, freed with the plan:
                                           // launches still queued on any stream may use them
     int pins = 0;                // batches (ebsd_batch_create) holding pointers into this plan
+    void *scratch = nullptr;     // device address of this GPU's no-allocation scratch array
     // background
     int bg_mode = 0;             // 0 none, 1 per-line sums, 2 exact per-pixel
     double *d_linebg = nullptr;  // nCells
@@ -2033,6 +2035,36 @@ static int compile_rows(ebsd_plan *p, int S)
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// no-allocation scratch (ebsd_find_bands_noalloc): a file-scope device array, one copy per
+// GPU, held until the process ends; it takes each chunk's Radon and, where the gather runs
+// in row segments, their partial sums
+// ---------------------------------------------------------------------------------------
+static constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices]; // per device ordinal: the array's device address
+
+// The device address of the current device's scratch array, resolved once per device with
+// that device current.  In host code the array's name is only a host-side stand-in: kernels
+// and copies take this address.
+static int scratch_address(int device, void **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return fail(-3, "no-allocation scratch: device ordinal out of range");
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(-3, std::string("no-allocation scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
 static int row_segments(const ebsd_plan *p, int dtype);
 
 // The whole-pattern batches, and the row segments that uint16 patterns (and float staging of
@@ -2074,6 +2106,7 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
         return fail(-3, "hipGetDeviceProperties failed");
     }
     p->ncu = prop.multiProcessorCount;
+    if ((rc = scratch_address(p->device, &p->scratch))) { delete p; return rc; }
     p->lds_max = (int)prop.sharedMemPerBlock;
     if (prop.maxSharedMemoryPerMultiProcessor > 0)
         p->lds_max = std::max(p->lds_max, (int)std::min<size_t>(prop.maxSharedMemoryPerMultiProcessor, 160 * 1024));
@@ -2601,6 +2634,58 @@ extern "C" int ebsd_find_bands(ebsd_plan *p, const void *d_pats, int dtype, int6
         if (rc) return rc;
         ebsd_bands o = offset_bands(*out, q0, p->g.nB);
         rc = launch_peaks(p, (const float *)d_work, n, &o, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int ebsd_find_bands_noalloc(ebsd_plan *p, const void *d_pats, int dtype, int64_t N,
+                                       const ebsd_bands *out, void *stream)
+{
+    if (!p) return fail(-1, "ebsd_find_bands_noalloc: null plan");
+    if (N < 0) return fail(-1, "ebsd_find_bands_noalloc: negative N");
+    if (N == 0) return 0;
+    if (!d_pats) return fail(-1, "ebsd_find_bands_noalloc: null device pointer");
+    if (!bands_ok(out)) return fail(-1, "ebsd_find_bands_noalloc: null output pointer");
+    if ((dtype & 0xff) > EBSD_F32 || (dtype & ~(0xff | EBSD_FLAG_NOBG)))
+        return fail(-1, "ebsd_find_bands_noalloc: bad dtype");
+    if (!p->scratch) return fail(-3, "ebsd_find_bands_noalloc: no scratch address for the plan's device");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+        return fail(-1, "ebsd_find_bands_noalloc: the plan's device is not the current device");
+    // the gather's row segments, if it runs in them with segmentations already compiled
+    const int dt = dtype & 0xff;
+    const bool use_bg = !(dtype & EBSD_FLAG_NOBG) && p->bg_mode != 0;
+    const bool flt = dt == EBSD_F32 || (use_bg && p->bg_mode == 2);
+    const int S = flt ? row_segments_flt(p) : row_segments(p, dt);
+    const bool seg = S > 1 && (int)p->rowsets[S].size() == S;
+    // patterns per chunk: their Radons, then (row segments) their partial sums, in the scratch
+    const size_t per = (size_t)p->nCells * sizeof(float);
+    const size_t npos = (size_t)((p->nCells + 63) / 64) * 64;
+    const size_t pbytes = seg ? npos * (flt ? 8 : 4) : 0;
+    int64_t chunk = (int64_t)(kScratchBytes / (per + pbytes));
+    if (seg) chunk = chunk / 8 * 8; // whole packs of up to 8 patterns share the partial sums
+    if (chunk < 1)
+        return fail(-2, "ebsd_find_bands_noalloc: one pattern's Radon (" + std::to_string(per + pbytes) +
+                            " bytes) exceeds the no-allocation scratch of " + std::to_string(kScratchBytes) +
+                            " bytes");
+    const size_t rbytes = (((size_t)chunk * per + 255) / 256) * 256;
+    GatherOpts o;
+    o.noalloc = true;
+    if (seg) {
+        o.part = (uint32_t *)((char *)p->scratch + rbytes);
+        o.part_bytes = kScratchBytes - rbytes;
+    }
+    float *radon = (float *)p->scratch;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t esz = dt == EBSD_U8 ? 1 : (dt == EBSD_U16 ? 2 : 4);
+    for (int64_t q0 = 0; q0 < N; q0 += chunk) {
+        const int64_t n = std::min(chunk, N - q0);
+        const void *src = (const char *)d_pats + (size_t)q0 * p->nPx * esz;
+        int rc = launch_gather(p, src, dtype, n, radon, s, o);
+        if (rc) return rc;
+        ebsd_bands ob = offset_bands(*out, q0, p->g.nB);
+        rc = launch_peaks(p, radon, n, &ob, s);
         if (rc) return rc;
     }
     return 0;
