@@ -20,8 +20,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "patterns/s (Radon+band detect), batched vs plain, at ≤1e-3-bin peak error"
-LDS_PEAK_GBS = 256 * 128 * 2.4          # 256 CU x 128 B/clk x 2.4 GHz (nominal LDS read bandwidth)
+LDS_NOMINAL_GBS = 256 * 128 * 2.4       # 256 CU x 128 B/clk x 2.4 GHz (the nominal figure once quoted)
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions/s: 256 CU x 4 SIMD, 4 clocks each
+PUBLISHED_C1 = 6020.0                   # find_bands, 1,000 60x60 patterns, OpenCL GPU (2019 Mac Pro),
+                                        # doc/tutorials/ebsd_index_demo.ipynb:206-210
 
 
 def parse():
@@ -42,6 +45,7 @@ def parse():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation measurement")
     ap.add_argument("--no-nlpar", action="store_true", help="skip the NLPAR measurement")
     ap.add_argument("--no-tripvote", action="store_true", help="skip the triplet-vote measurement")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 1,000 x 60x60 uint8 measurement")
     ap.add_argument("--stream-patterns", type=int, default=1048576,
                     help="patterns streamed in the scan-scale measurement")
     ap.add_argument("--roofline-reps", type=int, default=20)
@@ -55,6 +59,92 @@ def parse():
     ap.add_argument("--chunk", type=int, default=2048,
                     help="patterns per Radon+peak pass (the Radon of a chunk stays in the MALL)")
     return ap.parse_args()
+
+
+def lds_ceiling(reps=2):
+    """Conflict-free LDS read bandwidth of this GPU (ds_read_b32/b64/b128, bytes/s), from
+    the measurement-only probe library, the best of ``reps`` runs per width."""
+    import ctypes
+    import torch
+    from pyebsdindex_amd.build import PROBE_LIB
+    L = ctypes.CDLL(PROBE_LIB)
+    L.ebsd_probe_lds_read.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    s = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for w in (4, 8, 16):
+        best = 0.0
+        for _ in range(reps):
+            v = ctypes.c_double()
+            if L.ebsd_probe_lds_read(w, 1024, 2048, ctypes.byref(v), s) != 0:
+                raise RuntimeError("LDS probe failed")
+            best = max(best, v.value)
+        out[f"ds_read_b{8 * w}"] = round(best / 1e9, 1)
+    return out
+
+
+def gather_slots(bd):
+    """LDS reads per pattern of the whole-pattern integer gather: every 64-line batch (lines
+    sorted by their 8-entry group count, descending) is padded to its longest line's groups,
+    the filler entries reading a zero word -- the entry slots the kernel actually reads."""
+    plan = np.asarray(bd.radonPlan.indexPlan)
+    nPx = int(np.prod(bd.patDim))
+    cnt = (plan < nPx).sum(axis=2).ravel()
+    g = np.sort((cnt + 7) // 8, kind="stable")[::-1]
+    mx = np.maximum(g[::64], 1)  # the first line of each batch is its longest
+    return int(mx.sum()) * 64 * 8
+
+
+def valu_counts(H, W, dtype, chunk):
+    """VALU instructions per launch of the gather and the peak kernel from the committed
+    rocprof counter summary of this workload, or None."""
+    f = os.path.join(ROOT, "profiles", f"valu_counts_{H}x{W}_{dtype}.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        j = json.load(fh)
+    return j if j.get("patterns_per_launch") == chunk else None
+
+
+def c1_measure(dev, steps=50, warmup=10, cpu_seconds=8.0, seed=31):
+    """The reference's own CPU-runnable case: 1,000 synthetic 60x60 uint8 FCC patterns, Radon
+    180x90, 9 bands, no background -- on the device (resident), through the drop-in API
+    (numpy in, structured array out, as EBSDIndexer calls it), and the oracle's
+    restatement of the reference CPU path on the host cores."""
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    params = dict(nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9)
+    bd = BandDetect(patDim=(60, 60), **params)
+    x = kikuchi_patterns(1000, (60, 60), "uint8", seed=seed, device=dev)
+    host = x.cpu().numpy()
+    out = bd.device_plan().alloc_out(1000)
+    work = bd.device_plan().workspace(1000, 1000)
+    for _ in range(warmup):
+        bd.find_bands_device(x, out=out, work=work, chunk=1000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bd.find_bands_device(x, out=out, work=work, chunk=1000)
+    torch.cuda.synchronize()
+    dev_rate = 1000 * steps / (time.perf_counter() - t0)
+    bd.find_bands(host)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = bd.find_bands(host)
+    api_rate = 1000 * reps / (time.perf_counter() - t0)
+    cpu = None
+    if cpu_seconds > 0:
+        pps, done, dt = cpu_baseline(host, None, params, cpu_seconds)
+        cpu = {"value": round(pps, 1), "unit": "patterns/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": f"{done} patterns (the 1,000, cycled) in {dt:.1f} s, chunks of 512"}
+    return {"workload": "1000 x 60x60 uint8 FCC patterns, Radon 180x90, kernel 9x5, 9 bands, no background",
+            "device_patterns_per_s": round(dev_rate, 1), "api_patterns_per_s": round(api_rate, 1),
+            "published_opencl_patterns_per_s": PUBLISHED_C1,
+            "published_source": "doc/tutorials/ebsd_index_demo.ipynb:206-210 (OpenCL GPU of a 2019 Mac Pro)",
+            "api_vs_published": round(api_rate / PUBLISHED_C1, 1),
+            "valid_band_fraction": round(float((res["valid"] == 1).mean()), 4), "cpu_baseline": cpu}
 
 
 def cpu_baseline(pats_host, bg, params, target_s):
@@ -529,6 +619,55 @@ def main():
     pairs = int(info["pairs"])
     lds_bytes = 4.0 * pairs * chunk                   # algorithmic LDS bytes per gather launch
     lds_gbs = lds_bytes / (gather_ms * 1e-3) / 1e9
+    print("[bench] LDS read ceiling (conflict-free ds_read, measured now)", file=sys.stderr, flush=True)
+    ceiling = lds_ceiling()
+    lds_peak = ceiling["ds_read_b128"]
+    # the bytes the gather's ds_read_b64 actually move: every padded entry slot, one 8-byte
+    # word per 4 uint16 patterns (8 uint8 patterns)
+    pack = {"uint16": 4, "uint8": 8}.get(args.dtype)
+    slots = gather_slots(bd)
+    actual = {"bytes_per_launch": slots * 8.0 * chunk / pack if pack else None}
+    if pack:
+        actual["achieved"] = round(actual["bytes_per_launch"] / (gather_ms * 1e-3) / 1e9, 1)
+        actual["peak"] = ceiling["ds_read_b64"]
+        actual["frac"] = round(actual["achieved"] / actual["peak"], 4)
+        actual["entry_slots_per_pattern"] = slots
+    vc = valu_counts(H, W, args.dtype, chunk)
+
+    # throughput variants of the headline workload: the exact background mode (the
+    # reference's per-pixel float32 rounding, float64 sums), and the drop-in API from host
+    # memory (numpy in, structured array out)
+    exact = None
+    if bg is not None:
+        bdx = BandDetect(patDim=(H, W), background_mode="exact", **params)
+        bdx.backgroundsub = bg
+        outx = bdx.device_plan().alloc_out(N)
+        workx = bdx.device_plan().workspace(N, chunk)
+        for _ in range(3):
+            bdx.find_bands_device(pats, out=outx, work=workx, chunk=chunk)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k = max(5, args.steps // 4)
+        for _ in range(k):
+            bdx.find_bands_device(pats, out=outx, work=workx, chunk=chunk)
+        torch.cuda.synchronize()
+        exact = {"patterns_per_s": round(N * k / (time.perf_counter() - t0), 1),
+                 "note": "background_mode='exact': float32(x - b) per pixel, float64 line sums in plan "
+                         "order -- the reference's arithmetic, Radon bit-identical to it"}
+        del bdx, outx, workx
+    print("[bench] drop-in API from host memory", file=sys.stderr, flush=True)
+    host_pats = pats.cpu().numpy()
+    bd.find_bands(host_pats)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        api_res = bd.find_bands(host_pats)
+    api_rate = 3 * N / (time.perf_counter() - t0)
+    api = {"patterns_per_s": round(api_rate, 1),
+           "workload": f"BandDetect.find_bands on a numpy ({N}, {H}, {W}) {args.dtype} array -> structured "
+                       f"array (the call EBSDIndexer makes): pinned-ring pipeline, host copies on "
+                       f"{8} threads overlapped with uploads and band detection",
+           "valid_band_fraction": round(float((api_res["valid"] == 1).mean()), 4)}
+    del host_pats, api_res
     esz = {"uint8": 1, "uint16": 2, "float32": 4}[args.dtype]
     hbm_bytes = N * (H * W * esz + 9 * 28 + 12)       # input + band output per launch
     hbm_gbs = hbm_bytes / (ms * 1e-3) / 1e9
@@ -597,6 +736,13 @@ def main():
               f"{nlpar['roofline']['achieved']} TFLOP/s; CPU {(nlpar['cpu_baseline'] or {}).get('value')} patterns/s",
               file=sys.stderr, flush=True)
 
+    c1 = None
+    if not args.no_c1:
+        print("[bench] C1: 1,000 60x60 uint8 patterns", file=sys.stderr, flush=True)
+        c1 = c1_measure(dev, cpu_seconds=0.0 if args.no_cpu else 6.0)
+        print(f"[bench] C1 device {c1['device_patterns_per_s']:.1f}, API {c1['api_patterns_per_s']:.1f} patterns/s "
+              f"(published OpenCL {PUBLISHED_C1})", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle restatement of the reference CPU path)", file=sys.stderr, flush=True)
@@ -613,13 +759,32 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16" if esz == 2 else
         ("u8" if esz == 1 else "f32"), "data": "synthetic",
-        "config": {"workload": f"plain find_bands, {N} x {H}x{W} {args.dtype} FCC patterns, "
-                               f"{'background subtraction, ' if bg is not None else ''}Radon 180x90, "
-                               f"kernel 9x5, 9 bands", "global_batch": N, "chunk": chunk, "parallelism": "none"},
+        "config": {"workload": f"plain (per-size, device-resident) find_bands, {N} x {H}x{W} {args.dtype} FCC "
+                               f"patterns, {'background subtraction (fast mode), ' if bg is not None else ''}"
+                               f"Radon 180x90, kernel 9x5, 9 bands; the metric's batched-vs-plain comparison "
+                               f"is in 'batched'", "global_batch": N, "chunk": chunk, "parallelism": "none"},
         "roofline": {"bound": "lds", "kernel": "k_radon_gather", "achieved": round(lds_gbs, 1),
-                     "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4),
+                     "peak": lds_peak, "unit": "GB/s", "frac": round(lds_gbs / lds_peak, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": lds_bytes,
-                     "avg_launch_ms": round(gather_ms, 4), "pairs_per_pattern": pairs},
+                     "avg_launch_ms": round(gather_ms, 4), "pairs_per_pattern": pairs,
+                     "peak_kind": "conflict-free ds_read_b128 measured in this run (the LDS read ceiling)",
+                     "lds_ceiling_measured_GBps": ceiling, "peak_nominal": LDS_NOMINAL_GBS,
+                     "lds_actual": actual,
+                     "valu": None if vc is None else {
+                         "instructions_per_launch": vc["k_radon_gather"],
+                         "achieved": round(vc["k_radon_gather"] / (gather_ms * 1e-3) / 1e9, 1),
+                         "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instructions/s",
+                         "frac": round(vc["k_radon_gather"] / (gather_ms * 1e-3) / VALU_ISSUE_PEAK, 4)}},
+        "roofline_peaks": None if vc is None else {
+            "bound": "valu", "kernel": "k_peaks_stream", "unit": "G wave-instructions/s",
+            "achieved": round(vc["k_peaks_stream"] / (peaks_ms * 1e-3) / 1e9, 1),
+            "peak": VALU_ISSUE_PEAK / 1e9,
+            "frac": round(vc["k_peaks_stream"] / (peaks_ms * 1e-3) / VALU_ISSUE_PEAK, 4),
+            "instructions_per_launch": vc["k_peaks_stream"], "avg_launch_ms": round(peaks_ms, 4),
+            "counts_from": f"profiles/valu_counts_{H}x{W}_{args.dtype}.json (rocprofv3 SQ_INSTS_VALU)"},
+        "exact_mode": exact,
+        "api": api,
+        "c1": c1,
         "roofline_hbm": {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_step": hbm_bytes},
         "kernels_ms_per_chunk": {"k_radon_gather": round(gather_ms, 4), "k_peaks": round(peaks_ms, 4),

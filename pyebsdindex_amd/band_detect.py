@@ -548,6 +548,56 @@ class BandDetect:
         x, flags = self._prepare(plan, pats)
         return plan.find_bands(x, out=out, work=work, chunk=chunk, flags=flags, stream=stream)
 
+    #: host inputs of at least this many patterns take the pipelined path of find_bands
+    STREAM_MIN = 512
+
+    def _find_bands_streamed(self, patternsIn, plan, chunksize):
+        """find_bands of host patterns (numpy or CPU tensor, uint8/uint16/float32, at least
+        STREAM_MIN of them) through a pinned-host pipeline: batches are copied into a pinned
+        ring by a thread pool, uploaded on a copy stream and detected on a compute stream
+        while the next batch is copied, and the results come back in pinned blocks.  Returns
+        the structured array, or None when the input does not take this path."""
+        import torch
+        from .streaming import StreamingBandDetect
+        if isinstance(patternsIn, torch.Tensor):
+            if patternsIn.is_cuda:
+                return None
+            arr = patternsIn.numpy()
+        else:
+            arr = np.asarray(patternsIn)
+        if arr.ndim != 3 or arr.shape[0] < self.STREAM_MIN or _dtype_code(arr.dtype) is None:
+            return None
+        if tuple(arr.shape[1:]) != (plan.H, plan.W):
+            raise ValueError(f"patterns must be (N, {plan.H}, {plan.W})")
+        arr = np.ascontiguousarray(arr)
+        N = int(arr.shape[0])
+        batch = int(chunksize) if chunksize and chunksize > 0 else 2048
+        batch = max(1, min(batch, N))
+        key = (id(plan), str(arr.dtype), batch)
+        pipes = self.__dict__.setdefault("_pipelines", {})
+        sb = pipes.get(key)
+        if sb is None:
+            pipes.clear()   # one pipeline at a time: its device slots and pinned ring are large
+            sb = StreamingBandDetect(self, batch=batch, dtype=str(arr.dtype), gpu_id=plan.device.index,
+                                     depth=2, chunk=batch)
+            pipes[key] = sb
+        blocks = sb.__dict__.get("_blocks")
+        nb = -(-N // batch)
+        if blocks is None or len(blocks) < nb:
+            blocks = sb._blocks = sb.pinned_blocks(nb)
+        res = sb.run_array(arr, blocks=blocks[:nb])
+        bandData = np.zeros((N, self.nBands), dtype=self.dataType)
+        stats = np.empty((N, 3), dtype=np.float32)
+        off = 0
+        for r in res:
+            n = int(r["max"].shape[0])
+            for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid"):
+                bandData[k][off:off + n] = r[k].numpy()
+            stats[off:off + n] = r["stats"].numpy()
+            off += n
+        self.last_stats = stats
+        return bandData
+
     def find_bands(self, patternsIn, verbose=0, chunksize=-1, clparams=None, gpu_id=None, **kwargs):
         """band_detect.py:411-526 on the GPU.  ``chunksize`` > 0 bounds the patterns per
         device pass (default 2,048: the Radon of a pass stays in the MI355X's 256 MB
@@ -558,6 +608,11 @@ class BandDetect:
         if gpu_id is None and clparams is not None and hasattr(clparams, "gpu_id"):
             gpu_id = clparams.gpu_id
         plan = self.device_plan(gpu_id)
+        fast = self._find_bands_streamed(patternsIn, plan, chunksize)
+        if fast is not None:
+            if verbose > 0:
+                print('Total Band Find Time:', timer() - tic0)
+            return fast
         if isinstance(patternsIn, torch.Tensor):
             pats = patternsIn.to(plan.device)
         else:

@@ -3,7 +3,8 @@
 ``python -m pyebsdindex_amd.build`` (or ``__graft_entry__.build()``) runs hipcc on
 ``csrc/ebsd_radon.hip`` (band detection -> ``libebsd_radon.so``) and
 ``csrc/ebsd_nlpar.hip`` (NLPAR -> ``libebsd_nlpar.so``) and ``csrc/ebsd_tripvote.hip``
-(triplet voting -> ``libebsd_tripvote.so``).  All are plain C-ABI shared objects (no torch
+(triplet voting -> ``libebsd_tripvote.so``), plus the measurement-only
+``csrc/ebsd_ldsprobe.hip`` (the LDS read ceiling -> ``libebsd_ldsprobe.so``, used by bench.py).  All are plain C-ABI shared objects (no torch
 symbols); ``pyebsdindex_amd._hip`` loads them with ctypes.
 """
 import os
@@ -18,6 +19,8 @@ NLPAR_SRC = os.path.join(HERE, "csrc", "ebsd_nlpar.hip")
 NLPAR_LIB = os.path.join(HERE, "libebsd_nlpar.so")
 TRIPVOTE_SRC = os.path.join(HERE, "csrc", "ebsd_tripvote.hip")
 TRIPVOTE_LIB = os.path.join(HERE, "libebsd_tripvote.so")
+PROBE_SRC = os.path.join(HERE, "csrc", "ebsd_ldsprobe.hip")
+PROBE_LIB = os.path.join(HERE, "libebsd_ldsprobe.so")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC_FLAGS = [
@@ -43,7 +46,8 @@ def _hipcc(src: str, lib: str, verbose: bool) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile the three libraries; returns the band-detection library's path."""
+    """Compile the libraries; returns the band-detection library's path."""
+    _hipcc(PROBE_SRC, PROBE_LIB, verbose)
     _hipcc(TRIPVOTE_SRC, TRIPVOTE_LIB, verbose)
     _hipcc(NLPAR_SRC, NLPAR_LIB, verbose)
     return _hipcc(SRC, LIB, verbose)

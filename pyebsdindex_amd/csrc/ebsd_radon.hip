@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <mutex>
 #include <vector>
 
@@ -77,7 +78,8 @@ struct ebsd_plan {
     // device streams, one layout per count of (virtual) wave slots: a work-group of nw waves
     // that splits each pack's lines S ways walks them as nw * S slots
     struct Layout {
-        int nv, kind; // kind 0: entries in plan order, 1: bank-scheduled order
+        int nv, kind; // kind 0: entries in plan order, 1..4: bank-scheduled for LDS words of
+                      // 2, 4, 8 or 16 bytes (sched_kind)
         uint4 *gidx;
         int *wofs, *wcnt;
     };
@@ -89,7 +91,9 @@ struct ebsd_plan {
         int nbatch = 0;
         uint2 *d_bhdr = nullptr;     // per batch: first block, number of blocks
         int2 *d_lane = nullptr;      // per batch lane: cell id (-1 = none), entries of the whole line
-        std::vector<uint16_t> h_groups[2]; // batch-major groups [block][lane][8]: plan order, bank-scheduled
+        double *d_lrc = nullptr;     // per batch lane: 1 / entries (IEEE, 0 for an empty line)
+        std::vector<uint16_t> h_groups[5]; // batch-major groups [block][lane][8]: plan order, then
+                                           // bank-scheduled per word size (built on first use)
         std::vector<uint2> h_bhdr;   // per batch: first block in h_groups, number of blocks
         std::vector<Layout> layouts;
     };
@@ -157,6 +161,7 @@ struct GatherArgs {
     const uint4 *gidx;    // per-wave streams of plan groups
     const uint2 *bhdr;    // per batch: (unused, groups)
     const int2 *lane;     // per batch lane: (cell, count)
+    const double *lrc;    // per batch lane: 1 / count, correctly rounded
     const int *wofs;      // per wave slot: first group of its stream
     const int *wcnt;      // per wave slot: groups in its stream
     int nslice;           // work-groups sharing a pack's lines (slot = slice * waves + wave)
@@ -331,6 +336,57 @@ template <int IN, bool FLT, int P> __device__ __forceinline__ void gather_decode
     }
 }
 
+// Narrow sums of uint16 words (P >= 2): when a pack's largest value V is at most 8,191, the
+// two 16-bit fields of a dword can be summed over F >= 1 whole plan groups (8 F V <= 65,535)
+// without a carry from the low field into the high one.  A per-dword accumulator then adds
+// two entries per v_add3_u32, and a flush every F groups (and at each line end) moves its two
+// fields into the per-pattern sums with two SDWA adds: 2 + 2/F VALU ops per dword and entry
+// pair instead of 6.
+template <int IN, bool FLT, int P> struct Narrow {
+    static constexpr bool ok = !FLT && IN == EBSD_U16 && P >= 2;
+    static constexpr int ND = P / 2; // dwords per word
+};
+
+// FRESH: the group's sums start from zero (a flush after every group), else they add to dacc.
+template <int P, bool FRESH>
+__device__ __forceinline__ void group_accumulate_narrow(const typename WordT<2 * P>::T *w, uint32_t *dacc)
+{
+    typedef typename WordT<2 * P>::T Word;
+    constexpr int ND = P / 2;
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+        uint32_t da[ND], db[ND];
+        __builtin_memcpy(da, &w[k], sizeof(Word));
+        __builtin_memcpy(db, &w[k + 1], sizeof(Word));
+#pragma unroll
+        for (int j = 0; j < ND; ++j) dacc[j] = (FRESH && k == 0) ? da[j] + db[j] : dacc[j] + da[j] + db[j];
+    }
+}
+
+// Moves the fields of the dword sums into the per-pattern sums; RESET: zero the dword sums
+// for groups that add to them.
+template <int P, bool RESET> __device__ __forceinline__ void narrow_flush(uint32_t *dacc, uint32_t *acc)
+{
+#pragma unroll
+    for (int j = 0; j < P / 2; ++j) {
+        add_field<4>(acc[2 * j], dacc[j]);
+        add_field<5>(acc[2 * j + 1], dacc[j]);
+        if constexpr (RESET) dacc[j] = 0u;
+    }
+}
+
+// Groups per flush of the narrow sums for a pack whose largest value is vmax (0: the narrow
+// sums do not apply).
+__device__ __forceinline__ int narrow_groups(uint32_t vmax)
+{
+    if (vmax > 8191u) return 0;
+    return vmax == 0u ? (1 << 20) : (int)(65535u / (8u * vmax));
+}
+
+// Extra LDS bytes of a gather work-group past its staged words: the per-wave maxima of the
+// staged values (16 waves of 4 bytes, 16-byte aligned).
+static constexpr int kGatherLdsExtra = 128;
+
 // Build the LDS word of pixel px from element values x[p] (p < P), as packed integers or
 // float32(x - bg).
 template <int IN, bool FLT, int P>
@@ -366,7 +422,9 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
     const Tin *pats = reinterpret_cast<const Tin *>(a.pats);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    // the wave index is the same on every lane: said so, the batch walk's indices, bounds
+    // and line-end tests stay scalar (otherwise each test is an exec-masked vector branch)
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nwaves = blockDim.x >> 6;
     const int nPx = a.nPx;
     const int nst = a.rows * a.W; // staged pixels: global pixel a.px0 + j at local j
@@ -382,6 +440,8 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
 
     {
         const int64_t q0 = pack * P;
+        constexpr bool NOK = Narrow<IN, FLT, P>::ok;
+        uint32_t vmax = 0u; // largest staged value (narrow sums)
         // ---- stage P patterns pixel-major (8-byte loads per pattern), zero word at nst ----
         const Tin *src = pats + a.px0;
         for (int v = tid; v < nvec; v += blockDim.x) {
@@ -398,7 +458,10 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             for (int j = 0; j < VEC; ++j) {
                 Tin xp[P];
 #pragma unroll
-                for (int p = 0; p < P; ++p) xp[p] = x[p][j];
+                for (int p = 0; p < P; ++p) {
+                    xp[p] = x[p][j];
+                    if constexpr (NOK) vmax = max(vmax, (uint32_t)xp[p]);
+                }
                 uint32_t d[C::ND];
                 pack_word<IN, FLT, P>(d, xp, a.bg, a.px0 + v * VEC + j);
                 Word w;
@@ -409,8 +472,10 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
         for (int px = nvec * VEC + tid; px <= nst; px += blockDim.x) {
             Tin xp[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p)
+            for (int p = 0; p < P; ++p) {
                 xp[p] = (px < nst && q0 + p < a.N) ? src[(q0 + p) * (int64_t)nPx + px] : (Tin)0;
+                if constexpr (NOK) vmax = max(vmax, (uint32_t)xp[p]);
+            }
             uint32_t d[C::ND];
             pack_word<IN, FLT, P>(d, xp, px < nst ? a.bg : nullptr, a.px0 + px);
             if (px == nst) {
@@ -421,27 +486,56 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             __builtin_memcpy(&w, d, sizeof(Word));
             lds[word_of(px)] = w; // px == nst: the zero word rows * pitch
         }
+        // the pack's largest value: per-wave maxima past the staged words, then every wave
+        // takes the maximum of those (a uniform choice of the accumulation per pack)
+        uint32_t *wmax = reinterpret_cast<uint32_t *>(
+            lds_raw + ((((size_t)nst / a.W * a.pitch + 1) * C::WB + 15) & ~(size_t)15));
+        if constexpr (NOK) {
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, o));
+            if (lane == 0) wmax[wave] = vmax;
+        }
         __syncthreads();
+        int F = 0; // narrow sums: groups per flush (0: the wide sums)
+        if constexpr (NOK) {
+            uint32_t m = 0u;
+            for (int w2 = 0; w2 < nwaves; ++w2) m = max(m, wmax[w2]);
+            F = __builtin_amdgcn_readfirstlane(narrow_groups(m));
+        }
 
         // ---- gather: each lane owns one line of the wave's current 64-line batch; the
         //      wave walks its stream of plan groups with three groups in flight ----
         const int nslot = nwaves * a.nslice, slot = slice * nwaves + wave;
-        int b = slot;
-        if (b < nb) {
+        // one walk, instantiated for the wide and the narrow sums (chosen per pack)
+        // mode 0: wide sums; 1: narrow, flushed every group; 2: narrow, flushed every F groups
+        auto walk = [&](auto mode_tag) {
+          constexpr int MODE = decltype(mode_tag)::value;
+          constexpr bool NARROW = MODE > 0;
+          int b = slot;
+          if (b < nb) {
             const uint4 *S = a.gidx + (size_t)a.wofs[slot] * 64 + lane;
             const int G = a.wcnt[slot];
             // four plan groups in flight; each register is refilled right after use
             uint4 eq[4] = {S[0], S[64], S[128], S[192]}; // filler past the end keeps these in bounds
             int bend = (int)a.bhdr[b].y;
             int2 li = a.lane[(size_t)b * 64 + lane];
+            double rcl = a.lrc[(size_t)b * 64 + lane], rcn = 0.0;
             int bn = b + nslot;
             int2 lin = make_int2(-1, 0);
             int ngn = 0;
-            if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
+            if (bn < nb) {
+                lin = a.lane[(size_t)bn * 64 + lane];
+                rcn = a.lrc[(size_t)bn * 64 + lane];
+                ngn = (int)a.bhdr[bn].y;
+            }
             uint32_t acc[P];
             double accd[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) { acc[p] = 0u; accd[p] = 0.0; }
+            uint32_t dacc[NARROW ? P / 2 : 1];  // narrow sums: two fields per dword
+#pragma unroll
+            for (int j = 0; j < (NARROW ? P / 2 : 1); ++j) dacc[j] = 0u;
+            int fc = 0;                         // groups since the last flush
             // row segments after the first: the integer partial sums of the segments before,
             // loaded when a batch starts so that the loads are done by its end
             uint32_t prev[P];
@@ -468,7 +562,9 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                 load_pd(pdn, bn, lin.x);
             }
             // words up to 8 bytes: the LDS reads of group g+1 are issued before group g is
-            // accumulated, so a wave's reads overlap its own adds (two register sets)
+            // accumulated, so a wave's reads overlap its own adds (two register sets).  Two
+            // groups ahead (three sets, the walk unrolled 12 times) measured slower: 0.159
+            // against 0.145 ms per 2,048 120x120 uint16 patterns.
             constexpr bool PIPE = C::WB <= 8;
             Word wq[PIPE ? 2 : 1][8];
             if constexpr (PIPE) group_issue<IN, FLT, P>(lbase, eq[0], wq[0]);
@@ -477,18 +573,31 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                 for (int u = 0; u < 4; ++u) {
                     const int gi = g0 + u; // G is a multiple of 4 (the layout pads it)
                     {
+                        const Word *wg;
                         if constexpr (PIPE) {
                             // group gi+1 is in eq[(u+1)&3] (past the end: a filler group)
                             group_issue<IN, FLT, P>(lbase, eq[(u + 1) & 3], wq[(u + 1) & 1]);
-                            eq[u] = S[(size_t)(gi + 4) * 64];
-                            group_accumulate<IN, FLT, P>(wq[u & 1], acc, accd);
+                            wg = wq[u & 1];
                         } else {
                             group_issue<IN, FLT, P>(lbase, eq[u], wq[0]);
-                            eq[u] = S[(size_t)(gi + 4) * 64];
-                            group_accumulate<IN, FLT, P>(wq[0], acc, accd);
+                            wg = wq[0];
                         }
-                        if (gi + 1 == bend) {
-                            gather_decode<IN, FLT, P>(acc);
+                        eq[u] = S[(size_t)(gi + 4) * 64];
+                        const bool line_end = gi + 1 == bend;
+                        if constexpr (MODE == 1) {
+                            group_accumulate_narrow<P, true>(wg, dacc);
+                            narrow_flush<P, false>(dacc, acc);
+                        } else if constexpr (MODE == 2) {
+                            group_accumulate_narrow<P, false>(wg, dacc);
+                            if (++fc == F || line_end) {
+                                narrow_flush<P, true>(dacc, acc);
+                                fc = 0;
+                            }
+                        } else {
+                            group_accumulate<IN, FLT, P>(wg, acc, accd);
+                        }
+                        if (line_end) {
+                            if constexpr (!NARROW) gather_decode<IN, FLT, P>(acc);
                             if (!FLT && (a.segmode == 1 || a.segmode == 2) && li.x >= 0) {
 #pragma unroll
                                 for (int p = 0; p < P; ++p)
@@ -505,8 +614,8 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                                 // x / cnt correctly rounded from one reciprocal per line:
                                 // q = x * rc is within an ulp of x / cnt, the fma residual e is
                                 // exact, and q + e * rc rounds to x / cnt (Markstein's theorem,
-                                // rc the correctly rounded 1 / cnt).
-                                const double rc = 1.0 / cnt;
+                                // rc the correctly rounded 1 / cnt, computed on the host).
+                                const double rc = rcl;
                                 float r[P];
 #pragma unroll
                                 for (int p = 0; p < P; ++p) {
@@ -532,14 +641,29 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
                             bn = b + nslot;
                             // past the slot's last batch: no further line end (the walk then
                             // only runs the filler groups that round the stream up)
-                            if (bn < nb) { lin = a.lane[(size_t)bn * 64 + lane]; ngn = (int)a.bhdr[bn].y; }
-                            else { lin = make_int2(-1, 0); ngn = 0; }
+                            rcl = rcn;
+                            if (bn < nb) {
+                                lin = a.lane[(size_t)bn * 64 + lane];
+                                rcn = a.lrc[(size_t)bn * 64 + lane];
+                                ngn = (int)a.bhdr[bn].y;
+                            } else {
+                                lin = make_int2(-1, 0);
+                                ngn = 0;
+                            }
                             load_prev(b, b < nb ? li.x : -1);
                             if constexpr (FLT) load_pd(pdn, bn, lin.x);
                         }
                     }
                 }
             }
+          }
+        };
+        if constexpr (Narrow<IN, FLT, P>::ok) {
+            if (F == 1) walk(std::integral_constant<int, 1>{});
+            else if (F > 1) walk(std::integral_constant<int, 2>{});
+            else walk(std::integral_constant<int, 0>{});
+        } else {
+            walk(std::integral_constant<int, 0>{});
         }
         __syncthreads();
     }
@@ -1681,6 +1805,8 @@ static void free_seg(ebsd_plan::GSeg &g)
     g.layouts.clear();
     (void)hipFree(g.d_bhdr);
     (void)hipFree(g.d_lane);
+    (void)hipFree(g.d_lrc);
+    g.d_lrc = nullptr;
     g.d_bhdr = nullptr;
     g.d_lane = nullptr;
 }
@@ -1772,13 +1898,27 @@ static int geo_tiles(ebsd_plan *p)
 // Lay the plan groups out as one contiguous stream per wave slot: slot v of nv walks batches
 // v, v + nv, v + 2nv, ... back to back, so its next groups can be prefetched several
 // iterations ahead without looking at batch boundaries.  Layouts are cached per nv.
+// The schedule kind of LDS words of WB bytes (layout kinds 1..4).
+static int sched_kind(int WB) { return WB <= 2 ? 1 : WB == 4 ? 2 : WB == 8 ? 3 : 4; }
+
+static std::vector<uint16_t> bank_schedule(const std::vector<uint16_t> &g, const std::vector<uint2> &bhdr,
+                                           uint16_t filler, int WB);
+
 static int get_layout(ebsd_plan::GSeg &g, int nv, int kind, ebsd_plan::Layout *out, bool may_alloc = true)
 {
     for (const ebsd_plan::Layout &l : g.layouts)
         if (l.nv == nv && l.kind == kind) { *out = l; return 0; }
-    if (!may_alloc)
+    if (!may_alloc) {
+        // any bank schedule gives the same integer sums: take one that is built
+        for (const ebsd_plan::Layout &l : g.layouts)
+            if (l.nv == nv && kind > 0 && l.kind > 0) { *out = l; return 0; }
         return fail(-2, "no-allocation path: the plan streams for " + std::to_string(nv) +
                             " wave slots are not built (the gather's work-group size changed?)");
+    }
+    if (kind > 0 && g.h_groups[kind].empty()) {
+        static const int kWB[5] = {0, 2, 4, 8, 16};
+        g.h_groups[kind] = bank_schedule(g.h_groups[0], g.h_bhdr, (uint16_t)g.nwords, kWB[kind]);
+    }
     const std::vector<uint16_t> &hg = g.h_groups[kind];
     std::vector<uint16_t> st;
     std::vector<int> wofs(nv), wcnt(nv);
@@ -1822,7 +1962,7 @@ static int prebuild_layouts(const ebsd_plan *p, ebsd_plan::GSeg &g)
     ebsd_plan::Layout l;
     int rc;
     if ((rc = get_layout(g, p->gthreads / 64, 0, &l))) return rc;
-    return get_layout(g, p->gthreads / 64, 1, &l);
+    return get_layout(g, p->gthreads / 64, sched_kind(8), &l); // the default packs' 8-byte words
 }
 
 // Slices per pack so that a launch of npacks packs has about two resident rounds of
@@ -1846,8 +1986,8 @@ static int default_pitch(const ebsd_plan *p)
     for (int pad = want; pad > 0; --pad) {
         bool ok = H * (W + pad) <= 65535;
         for (int wb = 2; wb <= 16 && ok; wb *= 2) {
-            const bool fit0 = (size_t)(H * W + 1) * wb <= (size_t)p->lds_max;
-            const bool fit1 = (size_t)(H * (W + pad) + 1) * wb <= (size_t)p->lds_max;
+            const bool fit0 = (size_t)(H * W + 1) * wb + kGatherLdsExtra <= (size_t)p->lds_max;
+            const bool fit1 = (size_t)(H * (W + pad) + 1) * wb + kGatherLdsExtra <= (size_t)p->lds_max;
             ok = !fit0 || fit1;
         }
         if (ok) return W + pad;
@@ -1856,50 +1996,62 @@ static int default_pitch(const ebsd_plan *p)
 }
 
 // Integer line sums do not depend on the order of their entries, so each line's entries can
-// be reordered to spread the LDS words that a wave reads at one step over the banks.  Model
-// of an 8-byte-word read: 16 lanes per clock, word w in bank pair w mod 16, a clock per
-// distinct word in the busiest pair.  Greedy per step and 16-lane group: lanes with the
-// fewest distinct residues left choose first, each taking a residue that is least loaded at
-// this step (ties: the residue the lane has most entries left in).  Float sums keep plan order.
+// be reordered to spread the LDS words that a wave reads at one step over the banks.  The LDS
+// serves 256 bytes a clock over 64 four-byte banks: a read of W-byte words is served G = 256 /
+// max(W, 4) lanes at a time (64 for 2- and 4-byte words, 32 for 8, 16 for 16), and lane
+// group's words collide when they differ but share a bank set -- word w's residue w mod G
+// (a 2-byte word shares its dword, index w / 2, with its neighbour: same dword, no collision).
+// Measured on the MI355X: conflict-free ds_read_b64 135 TB/s, ds_read_b128 150 TB/s, and the
+// gather's bank-conflict cycles were 54% of its LDS cycles under the 16-lane model this
+// replaces.  Greedy per step and lane group: lanes with the fewest distinct residues left
+// choose first, each taking a residue that is least loaded at this step (ties: the residue
+// the lane has most entries left in).  Float sums keep plan order.
+
 static std::vector<uint16_t> bank_schedule(const std::vector<uint16_t> &g, const std::vector<uint2> &bhdr,
-                                           uint16_t filler)
+                                           uint16_t filler, int WB)
 {
-    constexpr int NR = 16;
+    const int G = 256 / std::max(WB, 4); // lanes served together = bank sets
+    const int NR = G;
+    const int shift = WB == 2 ? 1 : 0;   // 2-byte words: the dword is the bank item
     std::vector<uint16_t> out(g.size(), filler);
-    std::vector<uint16_t> pool[16][NR];
+    std::vector<std::vector<std::vector<uint16_t>>> pool(G, std::vector<std::vector<uint16_t>>(NR));
+    std::vector<int> nres(G), lanes(G), nused(NR);
+    std::vector<std::vector<int>> used(NR);
     for (const uint2 &h : bhdr) {
         const int steps = (int)h.y * 8;
-        for (int grp = 0; grp < 4; ++grp) {
-            for (int l = 0; l < 16; ++l)
+        for (int grp = 0; grp < 64 / G; ++grp) {
+            for (int l = 0; l < G; ++l)
                 for (int r = 0; r < NR; ++r) pool[l][r].clear();
-            for (int l = 0; l < 16; ++l) {
-                const int lane = grp * 16 + l;
+            for (int l = 0; l < G; ++l) {
+                const int lane = grp * G + l;
                 for (int k = 0; k < steps; ++k) {
                     const uint16_t w = g[((size_t)(h.x + k / 8) * 64 + lane) * 8 + (k % 8)];
-                    if (w != filler) pool[l][w % NR].push_back(w);
+                    if (w != filler) pool[l][(w >> shift) % NR].push_back(w);
                 }
             }
+            for (int l = 0; l < G; ++l) {
+                int n = 0;
+                for (int r = 0; r < NR; ++r) n += !pool[l][r].empty();
+                nres[l] = n; // distinct residues the lane has left (kept up to date below)
+            }
             for (int k = 0; k < steps; ++k) {
-                int nres[16];
-                int lanes[16];
-                for (int l = 0; l < 16; ++l) {
-                    int n = 0;
-                    for (int r = 0; r < NR; ++r) n += !pool[l][r].empty();
-                    nres[l] = n;
-                    lanes[l] = l;
-                }
-                std::stable_sort(lanes, lanes + 16, [&](int x, int y) {
-                    const int ax = nres[x] ? nres[x] : 99, ay = nres[y] ? nres[y] : 99;
+                for (int l = 0; l < G; ++l) lanes[l] = l;
+                std::stable_sort(lanes.begin(), lanes.end(), [&](int x, int y) {
+                    const int ax = nres[x] ? nres[x] : 1 << 20, ay = nres[y] ? nres[y] : 1 << 20;
                     return ax < ay;
                 });
-                uint16_t used[NR][16];
-                int nused[NR] = {0};
-                auto load = [&](int r, uint16_t w) {
-                    for (int i = 0; i < nused[r]; ++i)
-                        if (used[r][i] == w) return;
-                    used[r][nused[r]++] = w;
+                for (int r = 0; r < NR; ++r) {
+                    nused[r] = 0;
+                    used[r].clear();
+                }
+                auto load = [&](uint16_t w) {
+                    const int item = w >> shift, r = item % NR;
+                    for (int u : used[r])
+                        if (u == item) return;
+                    used[r].push_back(item);
+                    ++nused[r];
                 };
-                for (int li = 0; li < 16; ++li) {
+                for (int li = 0; li < G; ++li) {
                     const int l = lanes[li];
                     uint16_t w = filler;
                     if (nres[l]) {
@@ -1912,9 +2064,10 @@ static std::vector<uint16_t> bank_schedule(const std::vector<uint16_t> &g, const
                         }
                         w = pool[l][best].back();
                         pool[l][best].pop_back();
+                        if (pool[l][best].empty()) --nres[l];
                     }
-                    load(w % NR, w);
-                    out[((size_t)(h.x + k / 8) * 64 + grp * 16 + l) * 8 + (k % 8)] = w;
+                    load(w);
+                    out[((size_t)(h.x + k / 8) * 64 + grp * G + l) * 8 + (k % 8)] = w;
                 }
             }
         }
@@ -1989,12 +2142,16 @@ static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const s
         }
         blocks += (uint32_t)ngrp;
     }
-    g.h_groups[1] = bank_schedule(gidx, bhdr, (uint16_t)g.nwords);
+    for (int k = 1; k < 5; ++k) g.h_groups[k].clear();
     g.h_groups[0].swap(gidx);
     g.h_bhdr = bhdr;
     int rc;
     if ((rc = dev_upload(&g.d_bhdr, bhdr.data(), bhdr.size()))) return rc;
     if ((rc = dev_upload(&g.d_lane, lanes.data(), lanes.size()))) return rc;
+    std::vector<double> lrc(lanes.size(), 0.0);
+    for (size_t i = 0; i < lanes.size(); ++i)
+        if (lanes[i].y > 0) lrc[i] = 1.0 / (double)lanes[i].y;
+    if ((rc = dev_upload(&g.d_lrc, lrc.data(), lrc.size()))) return rc;
     return prebuild_layouts(p, g);
 }
 
@@ -2329,7 +2486,7 @@ static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int s
                            hipStream_t s)
 {
     typedef GatherCfg<IN, FLT, P> C;
-    const size_t lds = (size_t)(maxwords + 1) * C::WB;
+    const size_t lds = (size_t)(maxwords + 1) * C::WB + kGatherLdsExtra;
     if ((int)lds > p->lds_max) return 1; // caller falls back to a smaller pack
     auto kern = k_radon_gather<IN, FLT, P>;
     // the cap is the device's LDS, not this launch's size: a graph that captured launches of
@@ -2352,13 +2509,14 @@ static int launch_gather_t(ebsd_plan *p, ebsd_plan::GSeg &g, int maxwords, int s
     a.nbatch = g.nbatch;
     a.bhdr = g.d_bhdr;
     a.lane = g.d_lane;
+    a.lrc = g.d_lrc;
     a.npacks = (N + P - 1) / P;
     const int nw = p->gthreads / 64;
     // the no-allocation path walks the plan streams prebuilt for one slice per pack
     a.nslice = noalloc ? 1 : choose_slices(p, g, a.npacks, nw, occ);
     ebsd_plan::Layout lay;
     int rc;
-    if ((rc = get_layout(g, nw * a.nslice, FLT ? 0 : 1, &lay, !noalloc))) return rc;
+    if ((rc = get_layout(g, nw * a.nslice, FLT ? 0 : sched_kind(C::WB), &lay, !noalloc))) return rc;
     a.gidx = lay.gidx;
     a.wofs = lay.wofs;
     a.wcnt = lay.wcnt;
@@ -2419,7 +2577,7 @@ static int launch_gather_rows(ebsd_plan *p, int S, const void *d_pats, int64_t N
     std::vector<ebsd_plan::GSeg> &rows = p->rowsets[S];
     int maxwords = 0;
     for (const ebsd_plan::GSeg &g : rows) maxwords = std::max(maxwords, g.nwords);
-    if ((size_t)(maxwords + 1) * GatherCfg<IN, FLT, P>::WB > (size_t)p->lds_max) return 1;
+    if ((size_t)(maxwords + 1) * GatherCfg<IN, FLT, P>::WB + kGatherLdsExtra > (size_t)p->lds_max) return 1;
     const int64_t npos = (int64_t)rows[0].nbatch * 64;
     const int pbytes = FLT ? 8 : 4;
     uint32_t *part = o.part;
@@ -2458,7 +2616,7 @@ static int segments_for(const ebsd_plan *p, int WB)
     if (p->rseg > 0) return std::min(p->rseg, p->H);
     for (int S = 1; S <= 8; ++S) {
         const size_t words = (size_t)((p->H + S - 1) / S) * p->spitch + 1;
-        if (words * WB <= (size_t)p->lds_max) return std::min(S, p->H);
+        if (words * WB + kGatherLdsExtra <= (size_t)p->lds_max) return std::min(S, p->H);
     }
     return std::min(8, p->H);
 }
@@ -3327,7 +3485,7 @@ static int choose_variant(const ebsd_plan *p, int dtype, bool flt, VariantInfo *
     int best = -1;
     for (const VariantInfo &v : kVariants) {
         if (v.IN != dtype || v.FLT != flt || v.P > pmax) continue;
-        if ((size_t)(p->full.nwords + 1) * v.WB > (size_t)p->lds_max) continue;
+        if ((size_t)(p->full.nwords + 1) * v.WB + kGatherLdsExtra > (size_t)p->lds_max) continue;
         if (best < 0 || v.P > kVariants[best].P) best = (int)(&v - kVariants);
     }
     if (best < 0) return fail(-2, "pattern too large for the Radon kernel's LDS staging");
@@ -3458,7 +3616,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         VariantInfo v;
         if ((rc = choose_variant(p, dtype, flt, &v))) { batch_free(b); return rc; }
         var[i] = v.id;
-        const size_t lds = (size_t)(p->full.nwords + 1) * v.WB;
+        const size_t lds = (size_t)(p->full.nwords + 1) * v.WB + kGatherLdsExtra;
         const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)p->lds_max / std::max<size_t>(lds, 1)));
         cls[i] = per_cu;
         float *radon = (float *)work;
@@ -3475,6 +3633,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         a.nbatch = p->full.nbatch;
         a.bhdr = p->full.d_bhdr;
         a.lane = p->full.d_lane;
+        a.lrc = p->full.d_lrc;
         a.linebg = (use_bg && !flt && p->bg_mode == 1) ? p->d_linebg : nullptr;
         a.bg = (use_bg && flt) ? p->d_bg : nullptr;
         a.out = radon;
@@ -3518,7 +3677,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         int64_t packs = 0;
         for (int i : order) {
             if (cls[i] != c) continue;
-            gc.lds = std::max(gc.lds, (size_t)(plans[i]->full.nwords + 1) * kVariants[var[i]].WB);
+            gc.lds = std::max(gc.lds, (size_t)(plans[i]->full.nwords + 1) * kVariants[var[i]].WB + kGatherLdsExtra);
             packs += gs[i].npacks;
         }
         if (packs == 0) continue;
@@ -3536,7 +3695,8 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
             GatherArgs &a = gs[i];
             a.nslice = choose_slices(plans[i], plans[i]->full, packs, nw, occ);
             ebsd_plan::Layout lay;
-            if ((rc = get_layout(plans[i]->full, nw * a.nslice, kVariants[var[i]].FLT ? 0 : 1, &lay))) {
+            if ((rc = get_layout(plans[i]->full, nw * a.nslice,
+                                 kVariants[var[i]].FLT ? 0 : sched_kind(kVariants[var[i]].WB), &lay))) {
                 batch_free(b);
                 return rc;
             }

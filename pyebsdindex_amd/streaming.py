@@ -24,7 +24,23 @@ enqueued, and a ring buffer is read into again only once the upload that used it
 finished.  The reads of a batch run on the host while the device copies and detects the
 batches before it.
 """
+import threading
+
 import numpy as np
+
+_pools = {}
+_pools_lock = threading.Lock()
+
+
+def _copy_pool(threads):
+    """A kept pool of host threads for the copies into pinned memory (numpy releases the
+    interpreter lock while it copies)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with _pools_lock:
+        pool = _pools.get(int(threads))
+        if pool is None:
+            pool = _pools[int(threads)] = ThreadPoolExecutor(max_workers=int(threads))
+        return pool
 
 
 class StreamingBandDetect:
@@ -144,6 +160,57 @@ class StreamingBandDetect:
             freed[j % R] = ev
 
         return self._run(len(counts), get, counts, blocks, uploaded)
+
+    def run_array(self, arr, blocks=None, ring=None, threads=8):
+        """Stream a host array (N, H, W) of the pipeline's dtype (numpy, or a CPU tensor)
+        through band detection: each batch is copied by ``threads`` threads into one buffer
+        of a pinned host ring (``ring``, default ``depth + 1`` buffers from
+        ``pinned_batch``) just before its upload, and a ring buffer is written again only
+        after the upload that read it has finished, so the host copy of one batch overlaps
+        the upload and band detection of the ones before.  Returns the results of ``run``."""
+        import torch
+        if isinstance(arr, torch.Tensor):
+            arr = arr.numpy()
+        arr = np.asarray(arr)
+        if arr.ndim != 3 or tuple(arr.shape[1:]) != (self.plan.H, self.plan.W):
+            raise ValueError(f"patterns must be (N, {self.plan.H}, {self.plan.W})")
+        if arr.dtype != np.dtype(str(self.tdtype).replace("torch.", "")):
+            raise TypeError(f"array holds {arr.dtype}, pipeline dtype {self.tdtype}")
+        total = int(arr.shape[0])
+        counts = [min(self.batch, total - o) for o in range(0, total, self.batch)]
+        if ring is None:
+            ring = self.host_ring()
+        R = len(ring)
+        if R < 2:
+            raise ValueError("the host ring needs at least two buffers")
+        freed = [None] * R
+        pool = _copy_pool(threads)
+
+        def get(j):
+            r, n = j % R, counts[j]
+            if freed[r] is not None:
+                freed[r].synchronize()
+            dst = ring[r][:n].numpy()
+            src = arr[j * self.batch:j * self.batch + n]
+            step = max(1, -(-n // max(1, threads)))
+            jobs = [pool.submit(np.copyto, dst[o:o + step], src[o:o + step]) for o in range(0, n, step)]
+            for f in jobs:
+                f.result()
+            return ring[r][:n]
+
+        def uploaded(j):
+            ev = torch.cuda.Event()
+            ev.record(self.copy_stream)
+            freed[j % R] = ev
+
+        return self._run(len(counts), get, counts, blocks, uploaded)
+
+    def host_ring(self):
+        """The pipeline's own pinned host ring (``depth + 1`` batches), made on first use."""
+        ring = getattr(self, "_ring", None)
+        if ring is None:
+            ring = self._ring = [self.pinned_batch() for _ in range(self.depth + 1)]
+        return ring
 
     def _check_batch(self, hb):
         if int(hb.shape[0]) > self.batch or tuple(hb.shape[1:]) != (self.plan.H, self.plan.W):

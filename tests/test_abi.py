@@ -40,8 +40,16 @@ def test_library_exports_every_declared_symbol():
     assert set(declared_tv) == set(_hip.TRIPVOTE_SYMBOLS)
     for name in declared_tv:
         assert hasattr(tv, name), name
-    # every header's declarations are covered by one of the three libraries
-    assert set(_declared()) == set(declared) | set(declared_nl) | set(declared_tv)
+    # the measurement-only probe (LDS read ceiling for the bench's roofline)
+    from pyebsdindex_amd.build import PROBE_LIB
+    pr = ctypes.CDLL(PROBE_LIB)
+    declared_pr = _declared("ebsd_probe.h")
+    assert declared_pr == ["ebsd_probe_lds_read"]
+    for name in declared_pr:
+        assert hasattr(pr, name), name
+    assert pr.ebsd_probe_lds_read(3, 1, 1, None, None) == -1   # bad arguments, no launch
+    # every header's declarations are covered by one of the libraries
+    assert set(_declared()) == set(declared) | set(declared_nl) | set(declared_tv) | set(declared_pr)
 
 
 def test_tripvote_error_paths_without_device_compute():

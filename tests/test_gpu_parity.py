@@ -57,6 +57,32 @@ def test_radon_every_pack_size(dtype):
     plan.set_row_segments(0)
 
 
+@pytest.mark.parametrize("vmax", [65535, 8192, 8191, 4095, 1023, 255, 1, 0])
+def test_radon_u16_value_ranges(vmax):
+    """uint16 sums per pack: the wide form (a pack holding a value above 8,191) and the narrow
+    form with every flush interval (8,191: every group, 4,095: every two, ..., 0: only at
+    line ends), for each patterns-per-word variant and row segmentation, with one pack of
+    large values among small ones -- bit-identical to the reference restatement."""
+    print(f"u16 value range {vmax}", flush=True)
+    shape = (80, 80)
+    bd, ref = make_pair(shape)
+    x = _pats(11, shape, "uint16", seed=57).astype(np.float64)
+    x = np.rint(x / max(x.max(), 1.0) * vmax).astype(np.uint16)
+    x[3, 40, 41] = vmax          # the largest value exactly, in one pattern
+    x[9] = np.minimum(x[9].astype(np.int64) * 8, 65535).astype(np.uint16)  # a pack of larger values
+    p0, p1 = ref.padding
+    want = ref.radonPlan.radon_faster(x, ref.padding)[p0:-p0, p1:-p1].transpose(2, 0, 1)
+    plan = bd.device_plan()
+    xd = _dev(x)
+    for gp in (2, 4, 8):
+        plan.set_tuning(gather_pack=gp)
+        for S in (1, 2):
+            plan.set_row_segments(S)
+            assert np.array_equal(plan.radon(xd).cpu().numpy(), want), (gp, S)
+    plan.set_tuning(gather_pack=4)
+    plan.set_row_segments(0)
+
+
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_radon_background(mode):
     print(f"radon background {mode}", flush=True)

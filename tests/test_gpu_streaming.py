@@ -98,3 +98,36 @@ def test_file_streaming_rejects_mismatches(tmp_path):
     w9.write_header(writeBlank=True)
     with pytest.raises(ValueError):
         sb.run_file(get_pattern_file_obj(str(fn9)))
+
+
+@pytest.mark.parametrize("shape,dtype,n,chunk,mode", [((120, 120), "uint16", 3000, -1, "fast"),
+                                                      ((60, 60), "uint8", 1700, 600, None),
+                                                      ((80, 80), "float32", 520, 256, "exact")])
+def test_find_bands_api_pipeline_equals_device(shape, dtype, n, chunk, mode):
+    """BandDetect.find_bands on host patterns (the call EBSDIndexer makes,
+    _ebsd_index_single.py:703-707) runs them through the pinned-host pipeline; the
+    structured array equals the device path's fields bit for bit (ragged last batch, a
+    chunksize of the caller's, a background), and equals the direct path's (one upload)."""
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    print(f"api pipeline {shape} {dtype} n={n} chunk={chunk}", flush=True)
+    bd = BandDetect(patDim=shape, nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9,
+                    background_mode=mode or "fast")
+    pats = kikuchi_patterns(n, shape, dtype, seed=81).numpy()
+    if mode:
+        bd.collect_background(patsIn=pats[:200].astype(np.float64))
+    got = bd.find_bands(pats, chunksize=chunk)
+    assert getattr(bd, "_pipelines", None), "the host input did not take the pipelined path"
+    dev = {k: v.cpu().numpy() for k, v in bd.find_bands_device(torch.from_numpy(pats).cuda()).items()}
+    for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid"):
+        assert np.array_equal(got[k], dev[k]), k
+    assert np.array_equal(bd.last_stats, dev["stats"])
+    # the same call through the direct path (one upload of the whole array)
+    bd.STREAM_MIN = n + 1
+    direct = bd.find_bands(pats, chunksize=chunk)
+    assert np.array_equal(direct, got)
+    # a second pipelined call reuses the pipeline and its pinned buffers
+    bd.STREAM_MIN = 512
+    again = bd.find_bands(pats, chunksize=chunk)
+    assert np.array_equal(again, got)
