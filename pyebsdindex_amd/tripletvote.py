@@ -51,46 +51,155 @@ def triplet_lut() -> np.ndarray:
     return lut
 
 
-def _cubic_rotations() -> np.ndarray:
-    """The 24 proper rotations of the cube as integer matrices (signed permutations with
-    determinant +1), the proper operations of Laue class m-3m."""
-    out = []
-    for perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
-        for signs in np.ndindex(2, 2, 2):
-            m = np.zeros((3, 3), dtype=np.int64)
-            for r in range(3):
-                m[r, perm[r]] = -1 if signs[r] else 1
-            if round(np.linalg.det(m)) == 1:
-                out.append(m)
-    return np.array(out)
+# --------------------------------------------------------------------------------------
+# crystal geometry (pyebsdindex/crystallometry.py:63-165, crystal_sym.py:35-482): the
+# Cartesian frame has a along x and c* along z; pole families are the orbits of integer
+# plane normals under the proper rotations of the phase's Laue class
+# --------------------------------------------------------------------------------------
+HEX_POLES = np.array([[1, 0, -1, 0], [0, 0, 0, 2], [1, 0, -1, 1], [1, 0, -1, 2], [1, 1, -2, 0],
+                      [1, 0, -1, 3], [1, 1, -2, 2], [2, 0, -2, 1]], dtype=np.int32)
+
+# space group -> Laue class, as the first space group of each class (EDAX numbering of the
+# classes: 1 = -1, 2 = 2/m, 22 = mmm, 4 = 4/m, 42 = 4/mmm, 3 = -3, 32 = -3m, 6 = 6/m,
+# 62 = 6/mmm, 23 = m-3, 43 = m-3m; crystal_sym.py:379-406)
+_LAUE_FIRST_SG = ((1, 1), (3, 2), (16, 22), (75, 4), (89, 42), (143, 3), (149, 32), (168, 6), (177, 62),
+                  (195, 23), (207, 43))
+
+
+def laue_class(spacegroup: int) -> int:
+    """The Laue class (EDAX code) of a space group 1..230."""
+    sg = int(spacegroup)
+    if not 1 <= sg <= 230:
+        raise ValueError(f"space group {spacegroup} outside 1..230")
+    code = 1
+    for first, c in _LAUE_FIRST_SG:
+        if sg >= first:
+            code = c
+    return code
+
+
+def _axis_rotation(axis, degrees) -> np.ndarray:
+    """Rotation matrix about a Cartesian axis (float64)."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    t = np.radians(degrees)
+    K = np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+    return np.eye(3) + np.sin(t) * K + (1.0 - np.cos(t)) * (K @ K)
+
+
+def laue_rotations(code: int) -> np.ndarray:
+    """The proper rotations of a Laue class as Cartesian matrices (n, 3, 3), with the axis
+    conventions of crystal_sym.py:35-377: n-fold axes along z (c), the hexagonal two-folds
+    every 30 degrees in the basal plane from x (a), the trigonal ones at 0, 120 and 240
+    degrees, the tetragonal ones along x, y and the two diagonals, the monoclinic two-fold
+    along y (b)."""
+    z = (0.0, 0.0, 1.0)
+    R = []
+    if code in (43, 23):
+        for perm in ((0, 1, 2), (1, 2, 0), (2, 0, 1), (0, 2, 1), (2, 1, 0), (1, 0, 2)):
+            for signs in np.ndindex(2, 2, 2):
+                m = np.zeros((3, 3))
+                for r in range(3):
+                    m[r, perm[r]] = -1.0 if signs[r] else 1.0
+                if round(np.linalg.det(m)) != 1:
+                    continue
+                cyclic = perm in ((0, 1, 2), (1, 2, 0), (2, 0, 1))
+                if code == 43 or cyclic:
+                    R.append(m)
+    elif code in (62, 6):
+        R = [_axis_rotation(z, 60.0 * k) for k in range(6)]
+        if code == 62:
+            R += [_axis_rotation((np.cos(np.radians(30.0 * k)), np.sin(np.radians(30.0 * k)), 0.0), 180.0)
+                  for k in range(6)]
+    elif code in (32, 3):
+        R = [_axis_rotation(z, 120.0 * k) for k in range(3)]
+        if code == 32:
+            R += [_axis_rotation((np.cos(np.radians(a)), np.sin(np.radians(a)), 0.0), 180.0) for a in (0, 120, 240)]
+    elif code in (42, 4):
+        R = [_axis_rotation(z, 90.0 * k) for k in range(4)]
+        if code == 42:
+            R += [_axis_rotation(ax, 180.0) for ax in ((1, 1, 0), (-1, 1, 0), (1, 0, 0), (0, 1, 0))]
+    elif code == 22:
+        R = [np.eye(3)] + [_axis_rotation(ax, 180.0) for ax in ((1, 0, 0), (0, 1, 0), (0, 0, 1))]
+    elif code == 2:
+        R = [np.eye(3), _axis_rotation((0, 1, 0), 180.0)]
+    elif code == 1:
+        R = [np.eye(3)]
+    else:
+        raise ValueError(f"unknown Laue class {code}")
+    return np.array(R)
+
+
+class Crystal:
+    """Metric tensors and structure matrices of a unit cell (a, b, c in any length unit,
+    angles in degrees): the standard formulas, with the reference's frame (a along x, c* along
+    z) and its zeroing of entries below 1e-6 (crystallometry.py:63-165)."""
+
+    def __init__(self, a, b, c, alpha, beta, gamma):
+        ca, cb, cg = (np.cos(np.radians(x)) for x in (alpha, beta, gamma))
+        sa, sb, sg = (np.sin(np.radians(x)) for x in (alpha, beta, gamma))
+        tg = np.tan(np.radians(gamma))
+        G = np.array([[a * a, a * b * cg, a * c * cb], [a * b * cg, b * b, b * c * ca], [a * c * cb, b * c * ca, c * c]])
+        det = (a * b * c) ** 2 * (1.0 - ca ** 2 - cb ** 2 - cg ** 2 + 2.0 * ca * cb * cg)
+        V = np.sqrt(det)
+        Gs = np.array([[(b * c * sa) ** 2, a * b * c * c * (ca * cb - cg), a * b * b * c * (cg * ca - cb)],
+                       [a * b * c * c * (ca * cb - cg), (a * c * sb) ** 2, a * a * b * c * (cb * cg - ca)],
+                       [a * b * b * c * (cg * ca - cb), a * a * b * c * (cb * cg - ca), (a * b * sg) ** 2]]) / det
+        # direct structure matrix: columns are a, b, c in Cartesian coordinates
+        D = np.array([[a, b * cg, c * cb], [0.0, b * sg, -c * (cb * cg - ca) / sg], [0.0, 0.0, V / (a * b * sg)]])
+        # reciprocal structure matrix: columns are a*, b*, c* (times the cell volume convention)
+        Rs = np.array([[1.0 / a, 0.0, 0.0], [-1.0 / (a * tg), 1.0 / (b * sg), 0.0],
+                       [b * c * (cg * ca - cb) / (V * sg), a * c * (cb * cg - ca) / (V * sg), a * b * sg / V]])
+        for m in (G, Gs, D, Rs):
+            m[np.abs(m) < 1e-6] = 0.0
+        self.metricTensor = G
+        self.reciprocalMetricTensor = Gs
+        self.directStructureMatrix = D
+        self.reciprocalStructureMatrix = Rs
+        self.invReciprocalStructureMatrix = D.T.copy()
+        self.volume = V
+
+
+def hex4_to_hex3(poles) -> np.ndarray:
+    """Four-index (h k i l) plane normals -> three-index (h k l) (crystal_sym.py:469-473)."""
+    p = np.asarray(poles).reshape(-1, 4)
+    return p[:, [0, 1, 3]]
+
+
+def hex3_to_hex4(poles) -> np.ndarray:
+    """(h k l) -> (h k -(h+k) l) (crystal_sym.py:475-482)."""
+    p = np.asarray(poles, dtype=np.float64).reshape(-1, 3)
+    return np.stack([p[:, 0], p[:, 1], -(p[:, 0] + p[:, 1]), p[:, 2]], axis=1)
 
 
 def _unique_hkl(poles: np.ndarray) -> np.ndarray:
-    """Distinct integer poles, ordered by the key h + b k + b^2 l of the shifted indices
+    """Distinct poles, ordered by the key h + b k + b^2 l of the rounded, shifted indices
     (tripletvote.py:645-687 with reduceInversion=False), then reversed as build_trip_lib
     does (:350-351)."""
-    p = np.asarray(poles, dtype=np.int64).reshape(-1, 3)
-    q = p - p.min()
+    p = np.asarray(poles, dtype=np.float64).reshape(-1, 3)
+    q = np.rint(p).astype(np.int64)
+    q = q - q.min()
     b = q.max() + 1
     key = q @ np.array([1, b, b * b], dtype=np.int64)
     _, first = np.unique(key, return_index=True)
     return p[first][::-1]
 
 
-def _pole_cos(p1: np.ndarray, p2: np.ndarray) -> np.ndarray:
-    """Cosine of the angle between integer cubic poles, float64 (tripletvote.py:689-707
-    with the identity reciprocal metric of a cubic cell of unit edge)."""
+def _pole_cos(p1: np.ndarray, p2: np.ndarray, rmt=None) -> np.ndarray:
+    """Cosine of the angle between plane normals given by Miller indices, through the
+    reciprocal metric tensor (tripletvote.py:689-707), float64, clipped to [-1, 1]."""
+    G = np.eye(3) if rmt is None else np.asarray(rmt, dtype=np.float64)
     a = np.asarray(p1, dtype=np.float64).reshape(-1, 3)
     b = np.asarray(p2, dtype=np.float64).reshape(-1, 3)
-    dot = a @ b.T
-    na = np.sqrt(np.einsum("ij,ij->i", a, a)).reshape(-1, 1)
-    nb = np.sqrt(np.einsum("ij,ij->i", b, b)).reshape(1, -1)
+    dot = a @ G @ b.T
+    na = np.sqrt(np.einsum("ij,jk,ik->i", a, G, a)).reshape(-1, 1)
+    nb = np.sqrt(np.einsum("ij,jk,ik->i", b, G, b)).reshape(1, -1)
     return np.clip(dot / (na @ nb), -1.0, 1.0)
 
 
 def _normalise_families(polefamilies) -> np.ndarray:
-    """Drop zero poles and poles parallel or antiparallel to an earlier one
-    (tripletvote.py:245-266)."""
+    """Drop zero poles and poles parallel or antiparallel (in index space) to an earlier one
+    (tripletvote.py:245-266); any index count (3 or 4)."""
     poles = np.atleast_2d(np.asarray(polefamilies, dtype=np.float64))
     poles = poles[np.abs(poles).max(axis=1) > 1e-6]
     if poles.shape[0] == 0:
@@ -103,27 +212,50 @@ def _normalise_families(polefamilies) -> np.ndarray:
     return np.rint(poles[keep] * (1.0 + 1e-6)).astype(np.int64)
 
 
-def triplet_library(polefamilies) -> dict:
-    """Pair angles and angle triplets of a cubic (m-3m) phase (tripletvote.py:320-491).
+def _family(pole, rots, crystal) -> np.ndarray:
+    """The symmetric variants of one pole (Miller indices, float64): rotated in the
+    Cartesian frame and mapped back, snapped to integers where they are integers to 1e-6
+    (every consistent cell and Laue class), made distinct and ordered as _unique_hkl."""
+    cart = crystal.reciprocalStructureMatrix @ np.asarray(pole, dtype=np.float64)
+    var = np.einsum("rij,j->ri", rots, cart) @ crystal.invReciprocalStructureMatrix.T
+    near = np.abs(var - np.rint(var)) < 1e-6
+    var = np.where(near, np.rint(var), var)
+    return _unique_hkl(var)
 
-    Returns a dict with
-      ``polefamilies`` (nfam, 3) int, ``nfam``;
-      ``families`` list of (n_i, 3) int arrays, every symmetric variant of each family;
+
+def triplet_library(polefamilies, spacegroup: int = 225, latticeparameter=(1.0, 1.0, 1.0, 90.0, 90.0, 90.0)) -> dict:
+    """Pair angles, angle triplets and the complete pole list of a phase (tripletvote.py:
+    320-491), for any Laue class.
+
+    ``polefamilies``: (n, 3) Miller indices, or (n, 4) Miller-Bravais for the hexagonal Laue
+    classes.  Returns a dict with
+      ``polefamilies`` the distinct families as given (four-index for hexagonal input), ``nfam``;
+      ``families`` list of (n_i, 3) arrays, every symmetric variant of each family;
       ``angpairs``: ``angles`` (npair,) float64 degrees and ``familyid`` (npair, 2) int --
         for families i <= j, each distinct angle (to 0.01 degree) above 1 degree between
         family i's listed pole and the variants of family j;
       ``angtriplets``: ``angles`` (ntrip, 3) float64, ascending in each row, and
         ``familyid`` (ntrip, 3) int64, the family at each vertex, in the reference's row
-        order (duplicates removed, rows ordered by their family-id key)."""
-    poles = _normalise_families(polefamilies)
+        order (duplicates removed, rows ordered by their family-id key);
+      ``completelib``: ``poles`` (npoles, 3) every variant of every family, ``polesCart``
+        unit Cartesian normals, ``familyid`` per pole, ``angTable`` (npoles, npoles) degrees,
+        ``nFamily`` variants per family, ``famIndex`` first pole of each family."""
+    code = laue_class(spacegroup)
+    lp = np.asarray(latticeparameter, dtype=np.float64).reshape(6)
+    crystal = Crystal(*lp)
+    given = _normalise_families(polefamilies)
+    hexagonal = code in (62, 6)
+    poles = hex4_to_hex3(given) if (hexagonal and given.shape[-1] == 4) else given.reshape(-1, 3)
+    poles = np.asarray(poles, dtype=np.int64)
     nfam = poles.shape[0]
-    rots = _cubic_rotations()
-    families = [_unique_hkl(np.einsum("rij,j->ri", rots, p)) for p in poles]
+    rots = laue_rotations(code)
+    rmt = crystal.reciprocalMetricTensor
+    families = [_family(p, rots, crystal) for p in poles]
 
     angs, fid = [], []
     for i in range(nfam):
         for j in range(i, nfam):
-            c = np.abs(_pole_cos(poles[i], families[j]).ravel())
+            c = np.abs(_pole_cos(poles[i], families[j], rmt).ravel())
             cents = np.round(np.arccos(c) * RADEG * 100).astype(np.int32)
             u = np.unique(cents) / 100.0
             for a in u[u > 1.0]:
@@ -149,23 +281,38 @@ def triplet_library(polefamilies) -> dict:
     # ascending angles in each row, the family ids carried along (tripletvote.py:713-735)
     lut = triplet_lut()
     for r in range(lib_a.shape[0]):
-        s = np.argsort(lib_a[r])
-        lib_a[r] = lib_a[r][s]
-        lib_i[r] = lib_i[r][lut[:, s[0], s[1], s[2]]]
+        s_ = np.argsort(lib_a[r])
+        lib_a[r] = lib_a[r][s_]
+        lib_i[r] = lib_i[r][lut[:, s_[0], s_[1], s_[2]]]
     # duplicates (same rounded angle sum and ids) removed, then rows ordered by the id key
     # (tripletvote.py:737-755); the same numpy sorts, so the same row order
-    ang_key = np.sum(np.round(lib_a * 100), axis=1).astype(np.longlong)
-    base = np.longlong(lib_i.max() + 1)
-    id_key = lib_i.dot(np.array([1, base, base ** 2]))
-    digits = np.ceil(np.log10(id_key.max()))
-    digits = np.where(digits > 2, digits, 2)
-    uid = ang_key * 10 ** digits + id_key
-    _, first = np.unique(uid, return_index=True)
-    lib_a, lib_i, id_key = lib_a[first], lib_i[first], id_key[first]
-    order = np.argsort(id_key)
-    return {"polefamilies": poles, "nfam": nfam, "families": families,
+    if lib_a.shape[0]:
+        ang_key = np.sum(np.round(lib_a * 100), axis=1).astype(np.longlong)
+        base = np.longlong(lib_i.max() + 1)
+        id_key = lib_i.dot(np.array([1, base, base ** 2]))
+        digits = np.ceil(np.log10(max(int(id_key.max()), 1)))
+        digits = np.where(digits > 2, digits, 2)
+        uid = ang_key * 10 ** digits + id_key
+        _, first = np.unique(uid, return_index=True)
+        lib_a, lib_i, id_key = lib_a[first], lib_i[first], id_key[first]
+        order = np.argsort(id_key)
+        lib_a, lib_i = lib_a[order], lib_i[order]
+
+    # the complete pole list for band assignment (tripletvote.py:463-477)
+    allp = np.concatenate(families)
+    nF = np.array([f.shape[0] for f in families], dtype=np.int64)
+    famindex = np.concatenate(([0], np.cumsum(nF)))[:-1].astype(np.int64)
+    cart = allp @ crystal.reciprocalStructureMatrix.T
+    cart /= np.linalg.norm(cart, axis=1, keepdims=True)
+    complete = {"poles": allp, "polesCart": cart,
+                "familyid": np.repeat(np.arange(nfam, dtype=np.int32), nF),
+                "angTable": np.arccos(_pole_cos(allp, allp, rmt)) * RADEG,
+                "nFamily": nF, "famIndex": famindex}
+    shown = hex3_to_hex4(poles) if hexagonal else poles
+    return {"polefamilies": shown, "nfam": nfam, "families": families, "lauecode": code, "crystal": crystal,
             "angpairs": {"angles": angs, "familyid": fid},
-            "angtriplets": {"angles": lib_a[order], "familyid": lib_i[order]}}
+            "angtriplets": {"angles": lib_a, "familyid": lib_i},
+            "completelib": complete}
 
 
 def row_index(tripAngles, angTol: float, bin_width: float = 0.5, max_angle: float = 90.0):
@@ -187,35 +334,74 @@ def row_index(tripAngles, angTol: float, bin_width: float = 0.5, max_angle: floa
     return start, rows, float(bin_width), float(angTol)
 
 
-def addphase(libtype: str = "FCC", angTol: float = 2.0, polefamilies=None):
-    """A BandIndexer for an FCC (space group 225) or BCC (229) phase with the reference's
-    default pole families (tripletvote.py:70-140), or cubic families of one's own."""
-    t = str(libtype).upper()
-    if polefamilies is None:
+def addphase(libtype=None, phasename=None, spacegroup=None, latticeparameter=None, polefamilies=None,
+             nband_earlyexit=10, pointgroup=None, pointgroupid=None, angTol: float = 2.0):
+    """A BandIndexer for a phase (tripletvote.py:69-168): ``libtype`` 'FCC' (space group 225),
+    'BCC' (229) or 'HCP' (194, c/a 1.63) with the reference's default pole families and an
+    early exit at 8 matched bands for the cubic ones; or a space group, lattice parameters
+    (a, b, c, alpha, beta, gamma) and pole families of one's own (an FCC library when no
+    space group is given).  ``angTol`` (degrees) is the vote and assignment tolerance."""
+    if libtype is not None:
+        t = str(libtype).upper()
         if t == "FCC":
-            polefamilies = FCC_POLES
+            nband_earlyexit, spacegroup = 8, spacegroup or 225
+            phasename = phasename or "FCC"
+            default = FCC_POLES
         elif t == "BCC":
-            polefamilies = BCC_POLES
+            nband_earlyexit, spacegroup = 8, spacegroup or 229
+            phasename = phasename or "BCC"
+            default = BCC_POLES
+        elif t == "HCP":
+            spacegroup = spacegroup or 194
+            phasename = phasename or "HCP"
+            default = HEX_POLES
+            if latticeparameter is None:
+                latticeparameter = (1.0, 1.0, 1.63, 90.0, 90.0, 120.0)
         else:
-            raise NotImplementedError(f"libtype {libtype!r}: only cubic (m-3m) phases, FCC or BCC, are built")
-    return BandIndexer(phasename=t, polefamilies=polefamilies, angTol=angTol)
+            raise ValueError(f"libtype {libtype!r}: FCC, BCC or HCP")
+        if latticeparameter is None:
+            latticeparameter = (1.0, 1.0, 1.0, 90.0, 90.0, 90.0)
+        if polefamilies is None:
+            polefamilies = default
+    else:
+        if spacegroup is None:
+            return addphase(libtype="FCC", phasename=phasename, latticeparameter=latticeparameter,
+                            polefamilies=polefamilies, angTol=angTol)
+        if latticeparameter is None:
+            latticeparameter = (1.0, 1.0, 1.0, 90.0, 90.0, 90.0)
+        if polefamilies is None:
+            polefamilies = FCC_POLES
+    return BandIndexer(phasename=phasename, spacegroup=spacegroup, latticeparameter=latticeparameter,
+                       polefamilies=np.atleast_2d(polefamilies), angTol=angTol, nband_earlyexit=nband_earlyexit,
+                       pointgroup=pointgroup, pointgroupid=pointgroupid)
 
 
 class BandIndexer:
-    """A cubic phase's triplet library and its vote on the device (the library and voting
-    half of tripletvote.BandIndexer, tripletvote.py:171-491, :1019-1207)."""
+    """A phase's triplet library and its band indexing on the device (tripletvote.py:171-
+    1018): the library on the host, the vote, band assignment, QUEST orientation and
+    confidence metric on the device."""
 
-    def __init__(self, phasename: str = "", polefamilies=FCC_POLES, angTol: float = 2.0,
-                 nband_earlyexit: int = 8):
-        self.phasename = str(phasename)
+    def __init__(self, phasename=None, spacegroup=225, latticeparameter=(1.0, 1.0, 1.0, 90.0, 90.0, 90.0),
+                 polefamilies=FCC_POLES, angTol: float = 2.0, nband_earlyexit: int = 8, pointgroup=" ",
+                 pointgroupid=None):
+        self.phasename = "" if phasename is None else str(phasename)
+        self.spacegroup = int(spacegroup)
+        self.latticeparameter = np.array(latticeparameter, dtype=np.float64)
         self.angTol = float(angTol)
         self.nband_earlyexit = int(nband_earlyexit)
+        self.pointgroup = pointgroup
+        self.pointgroupid = pointgroupid
+        self.high_fidelity = True
+        self.simpleqweights = False
         self.lut = triplet_lut()
-        lib = triplet_library(polefamilies)
+        lib = triplet_library(polefamilies, self.spacegroup, self.latticeparameter)
+        self.lauecode = lib["lauecode"]
+        self.crystalmats = lib["crystal"]
         self.polefamilies = lib["polefamilies"]
         self.npolefamilies = lib["nfam"]
         self.angpairs = lib["angpairs"]
         self.angtriplets = lib["angtriplets"]
+        self.completelib = lib["completelib"]
         self.families = lib["families"]
         self._dev = {}
 

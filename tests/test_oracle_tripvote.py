@@ -49,6 +49,67 @@ def test_library_golden_counts(name, poles, nang, nuniq):
     assert a.size == nang and np.unique(a).size == nuniq
 
 
+def test_library_golden_counts_hcp_and_triclinic():
+    """The reference's own values (pyebsdindex/tests/test_tripletvote.py:50-96): HCP (space
+    group 194, c/a 1.63, the eight default four-index families) 82 pair angles, 74 distinct;
+    a triclinic cell (2, 3, 4, 70, 100, 120) with the 26 poles of {-1,0,1}^3 -> 13 families,
+    78 angles, 77 distinct."""
+    import itertools
+    ph = tv.addphase("HCP")
+    assert np.allclose(ph.polefamilies, [[1, 0, -1, 0], [0, 0, 0, 2], [1, 0, -1, 1], [1, 0, -1, 2],
+                                         [1, 1, -2, 0], [1, 0, -1, 3], [1, 1, -2, 2], [2, 0, -2, 1]])
+    a = ph.angpairs["angles"]
+    assert a.size == 82 and np.unique(a).size == 74
+    hkl = np.asarray(list(itertools.product(*[np.arange(-1, 2)] * 3)), dtype=int)
+    hkl = hkl[~np.all(hkl == 0, axis=1)]
+    tri = tv.addphase(spacegroup=1, latticeparameter=[2, 3, 4, 70, 100, 120], nband_earlyexit=5, polefamilies=hkl)
+    assert tri.spacegroup == 1 and tri.nband_earlyexit == 5
+    assert np.allclose(tri.latticeparameter, [2, 3, 4, 70, 100, 120])
+    assert tri.npolefamilies == hkl.shape[0] // 2
+    for h in tri.polefamilies:
+        assert any(np.array_equal(h, x) for x in hkl)
+    a = tri.angpairs["angles"]
+    assert a.size == 78 and np.unique(a).size == 77
+
+
+@pytest.mark.parametrize("phase", ["FCC", "BCC", "HCP", "triclinic"])
+def test_library_angles_clear_of_rounding_edges(phase):
+    """The families here are exact integer orbits; the reference rotates poles with float32
+    quaternions, so its pole indices carry ~1e-7 of noise.  Pair angles are rounded to 0.01
+    degree, so the two agree unless an exact angle lies within that noise of a rounding
+    edge: every exact pair angle (in units of 0.01 degree) is at least 1e-3 from an edge."""
+    import itertools
+    if phase == "triclinic":
+        hkl = np.asarray(list(itertools.product(*[np.arange(-1, 2)] * 3)), dtype=int)
+        ph = tv.addphase(spacegroup=1, latticeparameter=[2, 3, 4, 70, 100, 120], polefamilies=hkl[~np.all(hkl == 0, 1)])
+    else:
+        ph = tv.addphase(phase)
+    lib = tv.triplet_library(ph.polefamilies, ph.spacegroup, ph.latticeparameter)
+    rmt = ph.crystalmats.reciprocalMetricTensor
+    poles = tv.hex4_to_hex3(ph.polefamilies) if ph.lauecode in (6, 62) else ph.polefamilies
+    for i in range(lib["nfam"]):
+        for j in range(i, lib["nfam"]):
+            x = np.degrees(np.arccos(np.abs(tv._pole_cos(poles[i], lib["families"][j], rmt).ravel()))) * 100
+            frac = np.abs(x - np.floor(x) - 0.5)
+            assert frac.min() > 1e-3, (phase, i, j, x[np.argmin(frac)])
+
+
+def test_hcp_families_are_the_hexagonal_orbits():
+    """6/mmm orbits of the HCP families (both signs of a plane normal kept, as the
+    reference's complete list does): {10-10} and {11-20} 6 normals, {0002} 2, {10-11} 12;
+    every variant of a family makes the family's own angle with c, and all are unit."""
+    ph = tv.addphase("HCP")
+    c = ph.completelib
+    n = dict(zip(range(8), c["nFamily"]))
+    assert n[0] == 6 and n[1] == 2 and n[2] == 12 and n[4] == 6
+    z = np.array([0.0, 0.0, 1.0])
+    for f in range(8):
+        sel = c["familyid"] == f
+        cz = np.abs(c["polesCart"][sel] @ z)
+        assert np.allclose(cz, cz[0], atol=1e-12)
+    assert np.allclose(np.linalg.norm(c["polesCart"], axis=1), 1.0)
+
+
 def test_library_closed_form_angles():
     ind = tv.addphase("FCC")
     a, f = ind.angpairs["angles"], ind.angpairs["familyid"]
