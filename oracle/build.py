@@ -1,7 +1,9 @@
 """Build recipe for the oracle's C restatement (test infrastructure only).
 
 ``python oracle/build.py`` compiles ``oracle/csrc/ref_loops.c`` (band detection) and
-``oracle/csrc/ref_nlpar.c`` (NLPAR) and ``oracle/csrc/ref_tripvote.c`` (triplet vote) with gcc/OpenMP into ``oracle/libref_loops.so``.  The reference itself is Python + Numba (no C/C++ sources),
+``oracle/csrc/ref_nlpar.c`` (NLPAR), ``oracle/csrc/ref_tripvote.c`` (triplet vote) and
+``oracle/csrc/ref_bandindex.c`` (band assignment, QUEST, confidence metric) with gcc/OpenMP
+into ``oracle/libref_loops.so``.  The reference itself is Python + Numba (no C/C++ sources),
 so there is no ``oracle/_ref`` build: the reference is read, not run.
 """
 import os
@@ -10,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", "ref_loops.c"), os.path.join(HERE, "csrc", "ref_nlpar.c"),
-        os.path.join(HERE, "csrc", "ref_tripvote.c")]
+        os.path.join(HERE, "csrc", "ref_tripvote.c"), os.path.join(HERE, "csrc", "ref_bandindex.c")]
 LIB = os.path.join(HERE, "libref_loops.so")
 
 

@@ -125,3 +125,54 @@ def cubic_bands(indexer, npats, nbands, seed=0, noise_deg=0.0):
         out[p, :len(chosen)] = n
         fam[p, :len(chosen)] = famid[chosen]
     return out, fam
+
+
+def quat_to_rot(q):
+    """Active rotation matrix of a unit quaternion (w, x, y, z): R v = q v q* -- the rotation
+    the reference's QUEST applies to band normals (quat_vector with p = +1)."""
+    w, x, y, z = (float(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def phase_bands(indexer, npats, nbands, seed=0, noise_deg=0.0, min_sep=3.0):
+    """Band normals of npats random orientations of any phase: nbands poles of its complete
+    pole list (Cartesian, with its lattice), no two within min_sep degrees (antiparallel
+    counted as the same), rotated into the detector frame by R_true, optionally tilted by
+    Gaussian noise of noise_deg degrees per component.  Returns float32 normals (npats,
+    nbands, 3), the true pole index and family (npats, nbands) and R_true (npats, 3, 3)."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(seed)
+    lib = indexer.completelib
+    allp = np.asarray(lib["polesCart"], np.float64)
+    famid = np.asarray(lib["familyid"])
+    out = np.zeros((npats, nbands, 3), np.float32)
+    pole = np.full((npats, nbands), -1, np.int64)
+    fam = np.full((npats, nbands), -1, np.int32)
+    Rs = np.zeros((npats, 3, 3))
+    cs = np.cos(np.radians(min_sep))
+    for p in range(npats):
+        chosen = []
+        for idx in rng.permutation(allp.shape[0]):
+            if all(abs(allp[idx] @ allp[c]) < cs for c in chosen):
+                chosen.append(idx)
+            if len(chosen) == nbands:
+                break
+        R = Rotation.random(random_state=rng).as_matrix()
+        n = allp[chosen] @ R.T
+        if noise_deg:
+            n = n + rng.normal(0.0, np.radians(noise_deg), n.shape)
+            n /= np.linalg.norm(n, axis=1, keepdims=True)
+        out[p, :len(chosen)] = n
+        pole[p, :len(chosen)] = chosen
+        fam[p, :len(chosen)] = famid[chosen]
+        Rs[p] = R
+    return out, pole, fam, Rs
+
+
+def symmetric_misorientation_deg(Ra, Rb, rots):
+    """Smallest rotation angle (degrees) between Ra and Rb over the crystal symmetry rotations
+    ``rots`` (Cartesian, (n, 3, 3)) applied on the crystal side: min_S angle(Ra^T Rb S)."""
+    from oracle.orientation import misorientation_deg
+    return min(misorientation_deg(Ra, Rb @ S) for S in rots)
