@@ -29,7 +29,7 @@ NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_workspace_bytes", "ebsd_nlpar_sig
                  "ebsd_nlpar_last_error")
 
 #: exported symbols of the triplet-vote library, in declaration order of include/ebsd_tripvote.h
-TRIPVOTE_SYMBOLS = ("ebsd_tripvote", "ebsd_tripvote_last_error")
+TRIPVOTE_SYMBOLS = ("ebsd_tripvote", "ebsd_bandindex", "ebsd_tripvote_last_error")
 
 
 class HipLibraryError(RuntimeError):
@@ -163,6 +163,9 @@ def load_tripvote(path: str = TRIPVOTE_LIB_PATH):
         L.ebsd_tripvote.argtypes = [P, P, ctypes.c_int64, I, P, P, I, I, ctypes.c_double, P, P, I,
                                     ctypes.c_double, P, P, P, P, P, P]
         L.ebsd_tripvote.restype = ctypes.c_int
+        L.ebsd_bandindex.argtypes = [P, P, ctypes.c_int64, I, P, P, P, P, I, P, P, P, I, P, P, P, ctypes.c_double,
+                                     I, I, I, P, P, P, P, P, P, P, P, P, P, P, P]
+        L.ebsd_bandindex.restype = ctypes.c_int
         L.ebsd_tripvote_last_error.argtypes = []
         L.ebsd_tripvote_last_error.restype = ctypes.c_char_p
         _tripvote_lib = L

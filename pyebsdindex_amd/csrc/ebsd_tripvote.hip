@@ -619,6 +619,376 @@ __global__ __launch_bounds__(256) void k_tripvote_group(const float *__restrict_
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// k_bandindex: the steps of BandIndexer.bandindex after the vote (tripletvote.py:526-633) --
+// band assignment (_assign_bands_nb2, :1483-1688), the QUEST weights (:855-897), the QUEST
+// fit (:920-1018) and the confidence metric (:2056-2075).  One wavefront per pattern, lane
+// j holding band j (nbands <= 64); the (ii, jj) pair loop and each pair's candidate loop are
+// uniform over the wavefront, and every sum the reference takes over bands runs in band
+// order through v_readlane, so the float results follow the reference's operation order.
+// Arithmetic types follow Numba's typing: band normals, library poles, TRIAD matrices and
+// the pole-fit products float32 (products not contracted); p2 / (|p2| + 1e-35) and the cross
+// products with it float64; arccos of a float32 as the float64 arccos rounded to float32;
+// the weights and QUEST float64.
+// ---------------------------------------------------------------------------------------
+struct BandIndexArgs {
+    const float *bn, *inten, *accw, *rank, *acc;
+    const int32_t *fam;
+    int64_t npats;
+    int nb, nfam, npl;
+    const float *pflt;     // (npl, 3) unit Cartesian library poles, float32
+    const double *pc64;    // the same, float64 (QUEST)
+    const int32_t *famid;  // (npl) family of each pole
+    const double *angrows; // (nfam, npl) degrees between family f's first pole and every pole
+    const int32_t *famidx, *famcnt; // (nfam) first pole and number of poles of each family
+    double angtol;
+    int n_band_early, nfit, simple;
+    double *quat, *fit, *bandfit;
+    float *cm, *afit, *fitb;
+    int64_t *pm, *nmatch, *ij;
+    int32_t *acc_correct;
+    uint8_t *pv;
+};
+
+__device__ __forceinline__ float rl_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rl_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float acosf_cr(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+__global__ __launch_bounds__(256) void k_bandindex(BandIndexArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int npl = a.npl, nfam = a.nfam, nb = a.nb;
+    double *l_pc64 = reinterpret_cast<double *>(lds_raw);
+    double *l_ang = l_pc64 + 3 * npl;
+    float *l_pf = reinterpret_cast<float *>(l_ang + (size_t)nfam * npl);
+    int32_t *l_famid = reinterpret_cast<int32_t *>(l_pf + 3 * npl);
+    int32_t *l_fidx = l_famid + npl;
+    int32_t *l_fcnt = l_fidx + nfam;
+    for (int t = threadIdx.x; t < 3 * npl; t += blockDim.x) {
+        l_pc64[t] = a.pc64[t];
+        l_pf[t] = a.pflt[t];
+    }
+    for (int t = threadIdx.x; t < nfam * npl; t += blockDim.x) l_ang[t] = a.angrows[t];
+    for (int t = threadIdx.x; t < npl; t += blockDim.x) l_famid[t] = a.famid[t];
+    for (int t = threadIdx.x; t < nfam; t += blockDim.x) {
+        l_fidx[t] = a.famidx[t];
+        l_fcnt[t] = a.famcnt[t];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (p >= a.npats) return;
+    const bool act = lane < nb;
+    const int j = act ? lane : 0;
+    const float *bnp = a.bn + (p * nb + j) * 3;
+    const float n0 = bnp[0], n1 = bnp[1], n2 = bnp[2];
+    const int fam_j = a.fam[p * nb + j];
+    const float rank_j = act ? a.rank[p * nb + j] : 0.0f;
+    const float inten_j = a.inten[p * nb + j];
+    // stable ascending argsort of the ranks: band j sits at place pos
+    int pos = 0;
+    for (int i = 0; i < nb; ++i) {
+        const float ri = rl_f(rank_j, i);
+        pos += (ri < rank_j) || (ri == rank_j && i < lane);
+    }
+    const float eps = 1.0e-12f;
+    // the pattern's best assignment so far
+    float best_fit = 360.0f, best_fitb = 360.0f;
+    int best_n = 0, best_pm = -1000, best_ij0 = -1, best_ij1 = -1, best_ij2 = -1, best_ij3 = -1;
+    bool best_pv = false;
+    for (int ii = 0; ii < nb - 1; ++ii) {
+        for (int jj = ii + 1; jj < nb; ++jj) {
+            const int bnd1 = first_lane(__ballot(act && pos == nb - 1 - ii));
+            const int bnd2 = first_lane(__ballot(act && pos == nb - 1 - jj));
+            const float v1[3] = {rl_f(n0, bnd1), rl_f(n1, bnd1), rl_f(n2, bnd1)};
+            const float v2[3] = {rl_f(n0, bnd2), rl_f(n1, bnd2), rl_f(n2, bnd2)};
+            const int f1 = rl_i(fam_j, bnd1), f2 = rl_i(fam_j, bnd2);
+            float ang01 = (v1[0] * v2[0] + v1[1] * v2[1]) + v1[2] * v2[2];
+            if (ang01 > 1.0f) ang01 = (float)(1.0 - (double)eps);
+            if (ang01 < -1.0f) ang01 = (float)(-1.0 + (double)eps);
+            // this pair's best candidate (the reference's zero-initialised arrays)
+            float pf_fit = 0.0f;
+            int pf_pm = -1;
+            bool pf_pv = false;
+            int nGood = 0;
+            const double ptest = (double)acosf_cr(fabsf(ang01)) * kRadeg;
+            if (ptest > a.angtol) {
+                const double a01 = (double)acosf_cr(ang01) * kRadeg;
+                const int r1 = l_fidx[f1], base = l_fidx[f2], cnt = l_fcnt[f2];
+                const float *p1 = l_pf + 3 * r1;
+                float c12[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2],
+                                v1[0] * v2[1] - v1[1] * v2[0]};
+                const float nn = sqrtf((c12[0] * c12[0] + c12[1] * c12[1]) + c12[2] * c12[2]);
+                for (int m = 0; m < 3; ++m) c12[m] /= nn;
+                const float b2[3] = {v1[1] * c12[2] - v1[2] * c12[1], v1[2] * c12[0] - v1[0] * c12[2],
+                                     v1[0] * c12[1] - v1[1] * c12[0]};
+                const float B[9] = {v1[0], v1[1], v1[2], c12[0], c12[1], c12[2], b2[0], b2[1], b2[2]};
+                const double p1d[3] = {p1[0], p1[1], p1[2]};
+                double score = -1.0;
+                bool any = false;
+                for (int c0 = 0; c0 < cnt; c0 += kWave) {
+                    const int kk = base + c0 + lane;
+                    uint64_t hits = __ballot(c0 + lane < cnt && fabs(l_ang[(size_t)f1 * npl + kk] - a01) < a.angtol);
+                    while (hits) {
+                        const int t = first_lane(hits);
+                        hits &= hits - 1;
+                        any = true;
+                        const float *p2f = l_pf + 3 * (base + c0 + t);
+                        double nt = (double)sqrtf((p2f[0] * p2f[0] + p2f[1] * p2f[1]) + p2f[2] * p2f[2]) + 1.0e-35;
+                        double p2[3], pc[3], pc2[3];
+                        for (int m = 0; m < 3; ++m) p2[m] = (double)p2f[m] / nt;
+                        cross3(p1d, p2, pc);
+                        nt = sqrt((pc[0] * pc[0] + pc[1] * pc[1]) + pc[2] * pc[2]) + 1.0e-35;
+                        for (int m = 0; m < 3; ++m) pc[m] /= nt;
+                        cross3(p1d, pc, pc2);
+                        float A[9];
+                        for (int m = 0; m < 3; ++m) {
+                            A[m * 3 + 0] = p1[m];
+                            A[m * 3 + 1] = (float)pc[m];
+                            A[m * 3 + 2] = (float)pc2[m];
+                        }
+                        float AB[9];
+                        for (int r = 0; r < 3; ++r)
+                            for (int q = 0; q < 3; ++q)
+                                AB[r * 3 + q] = (A[r * 3 + 0] * B[0 * 3 + q] + A[r * 3 + 1] * B[1 * 3 + q]) +
+                                                A[r * 3 + 2] * B[2 * 3 + q];
+                        float tp[3];
+                        for (int r = 0; r < 3; ++r) tp[r] = (AB[r * 3 + 0] * n0 + AB[r * 3 + 1] * n1) + AB[r * 3 + 2] * n2;
+                        float mx = (l_pf[0] * tp[0] + l_pf[1] * tp[1]) + l_pf[2] * tp[2];
+                        int am = 0;
+                        for (int k = 1; k < npl; ++k) {
+                            const float *pk = l_pf + 3 * k;
+                            const float tv = (pk[0] * tp[0] + pk[1] * tp[1]) + pk[2] * tp[2];
+                            if (tv > mx) { mx = tv; am = k; }
+                        }
+                        if (mx < -1.0f) mx = -1.0f;
+                        if (mx > 1.0f) mx = 1.0f;
+                        const double angTry = (double)acosf_cr(mx) * kRadeg;
+                        const bool match = act && angTry < a.angtol;
+                        const uint64_t mm = __ballot(match);
+                        double sum = 0.0;
+                        for (int i = 0; i < nb; ++i) {
+                            const double ai = rl_d(angTry, i);
+                            if ((mm >> i) & 1ull) sum += ai;
+                        }
+                        const int nmatch = __popcll((unsigned long long)mm);
+                        const double scoreTry = (double)(float)nmatch / (sum / (double)nmatch + 1e-6);
+                        if (scoreTry > score) {
+                            score = scoreTry;
+                            pf_fit = (float)angTry;
+                            pf_pm = am;
+                            pf_pv = angTry < a.angtol;
+                        }
+                    }
+                }
+                if (any) nGood = __popcll((unsigned long long)__ballot(act && (double)pf_fit < a.angtol));
+            }
+            if (nGood < 3) continue;
+            const uint64_t gm = __ballot(act && (double)pf_fit < a.angtol);
+            float fit = 0.0f;
+            for (int i = 0; i < nb; ++i) {
+                const float fi = rl_f(pf_fit, i);
+                if ((gm >> i) & 1ull) fit += fi;
+            }
+            fit /= (float)nGood;
+            if (best_n < nGood || (best_n == nGood && best_fit > fit)) {
+                best_fit = fit;
+                best_fitb = pf_fit;
+                best_n = nGood;
+                best_pm = pf_pm;
+                best_pv = pf_pv;
+                best_ij0 = ii; best_ij1 = jj; best_ij2 = bnd1; best_ij3 = bnd2;
+            }
+            if (best_n >= a.n_band_early) break;
+        }
+        if (best_n >= a.n_band_early) break;
+    }
+    const int pmc = best_pm < 0 ? 0 : best_pm;
+    const bool valid = act && best_pv;
+    const uint64_t vmask = __ballot(valid);
+    const int nvalid = __popcll((unsigned long long)vmask);
+    const int corr = __popcll((unsigned long long)__ballot(valid && l_famid[pmc] == fam_j));
+    // QUEST weights: the nfit best-scoring matched bands keep their intensity, normalised
+    float w32 = 0.0f;
+    const float *accw = a.accw + p * (int64_t)nfam * nb;
+    const float *accn = a.acc + p * (int64_t)nfam * nb;
+    if (nvalid >= 3) {
+        float score = -1.0f;
+        if (valid) {
+            if (!a.simple) {
+                const int f = l_famid[pmc];
+                score = (float)((double)accw[f * nb + j] / fmax((double)accn[f * nb + j], 1.0e-12));
+            } else {
+                // the reference scores matched band whg with the intensity of band q, its
+                // place among the matched bands
+                const int q = __popcll((unsigned long long)(vmask & ((1ull << lane) - 1ull)));
+                score = a.inten[p * nb + q];
+            }
+        }
+        int spos = 0;
+        for (int i = 0; i < nb; ++i) {
+            const float si = rl_f(score, i);
+            spos += (si < score) || (si == score && i < lane);
+        }
+        const int kk = nvalid < a.nfit ? nvalid : a.nfit;
+        const double w = (act && spos >= nb - kk) ? (double)inten_j : 0.0;
+        double ws = 0.0;
+        for (int i = 0; i < nb; ++i) ws += rl_d(w, i);
+        w32 = act ? (float)(w / ws) : 0.0f;
+    }
+    // QUEST on the normalised weights (float64).  The reference normalises them with numpy's
+    // np.sum over the bands, whose contiguous reduction sums 8 or more terms pairwise: eight
+    // running sums over blocks of eight, combined as a tree, then the remainder in order
+    double wsum = 0.0;
+    if (nb < 8) {
+        for (int i = 0; i < nb; ++i) wsum += (double)rl_f(w32, i);
+    } else {
+        double r8[8];
+        for (int k = 0; k < 8; ++k) r8[k] = (double)rl_f(w32, k);
+        int i = 8;
+        for (; i < nb - (nb % 8); i += 8)
+            for (int k = 0; k < 8; ++k) r8[k] += (double)rl_f(w32, i + k);
+        wsum = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+        for (; i < nb; ++i) wsum += (double)rl_f(w32, i);
+    }
+    const double wnorm = (double)w32 / fmax(wsum, 1.0e-12);
+    const uint64_t qm = __ballot(act && wnorm > 1.0e-7);
+    const int nq = __popcll((unsigned long long)qm);
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    double lam = 3.14159265358979323846, mad = -1.0, bdot = -1.0;
+    if (nq >= 3) {
+        double ws = 0.0;
+        for (int i = 0; i < nb; ++i)
+            if ((qm >> i) & 1ull) ws += rl_d(wnorm, i);
+        const double wn = wnorm / ws;
+        const double bd[3] = {(double)n0, (double)n1, (double)n2};
+        const double pf[3] = {l_pc64[3 * pmc], l_pc64[3 * pmc + 1], l_pc64[3 * pmc + 2]};
+        double B[9], z[3] = {0.0, 0.0, 0.0}, S[9], S2[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const double term = (wn * bd[r]) * pf[c];
+                double s = 0.0;
+                for (int i = 0; i < nb; ++i)
+                    if ((qm >> i) & 1ull) s += rl_d(term, i);
+                B[r * 3 + c] = s;
+            }
+        double cr[3];
+        cross3(bd, pf, cr);
+        for (int m = 0; m < 3; ++m) {
+            const double term = wn * cr[m];
+            for (int i = 0; i < nb; ++i)
+                if ((qm >> i) & 1ull) z[m] += rl_d(term, i);
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) S[r * 3 + c] = B[r * 3 + c] + B[c * 3 + r];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                S2[r * 3 + c] = (S[r * 3 + 0] * S[0 * 3 + c] + S[r * 3 + 1] * S[1 * 3 + c]) + S[r * 3 + 2] * S[2 * 3 + c];
+        const double det = S[0] * (S[4] * S[8] - S[5] * S[7]) - S[1] * (S[3] * S[8] - S[5] * S[6]) +
+                           S[2] * (S[3] * S[7] - S[4] * S[6]);
+        const double kq = (S[4] * S[8] - S[5] * S[7]) + (S[0] * S[8] - S[2] * S[6]) + (S[0] * S[4] - S[3] * S[1]);
+        const double sig = 0.5 * ((S[0] + S[4]) + S[8]);
+        const double sig2 = sig * sig;
+        double t[3];
+        for (int c = 0; c < 3; ++c) t[c] = (z[0] * S2[0 * 3 + c] + z[1] * S2[1 * 3 + c]) + z[2] * S2[2 * 3 + c];
+        const double d = (t[0] * z[0] + t[1] * z[1]) + t[2] * z[2];
+        for (int c = 0; c < 3; ++c) t[c] = (z[0] * S[0 * 3 + c] + z[1] * S[1 * 3 + c]) + z[2] * S[2 * 3 + c];
+        const double cc = det + ((t[0] * z[0] + t[1] * z[1]) + t[2] * z[2]);
+        const double bq = sig2 + ((z[0] * z[0] + z[1] * z[1]) + z[2] * z[2]);
+        const double aq = sig2 - kq;
+        double dl = 1e6;
+        int iter = 0;
+        lam = 1.0;
+        while (dl > 1.0e-12 && iter < 10) {
+            const double l0 = lam, l2 = lam * lam;
+            lam = lam - (l2 * l2 - (aq + bq) * l2 - cc * lam + (aq * bq + cc * sig - d)) /
+                            (4.0 * (l2 * lam) - 2.0 * (aq + bq) * lam - cc);
+            dl = fabs(l0 - lam);
+            ++iter;
+        }
+        const double beta = lam - sig, alpha = lam * lam - sig2 + kq, gamma = (lam + sig) * alpha - det;
+        double X[3];
+        for (int r = 0; r < 3; ++r) {
+            double M0 = ((r == 0 ? alpha : 0.0) + beta * S[r * 3 + 0]) + S2[r * 3 + 0];
+            double M1 = ((r == 1 ? alpha : 0.0) + beta * S[r * 3 + 1]) + S2[r * 3 + 1];
+            double M2 = ((r == 2 ? alpha : 0.0) + beta * S[r * 3 + 2]) + S2[r * 3 + 2];
+            X[r] = (M0 * z[0] + M1 * z[1]) + M2 * z[2];
+        }
+        double qn = 0.0;
+        qn += ((gamma * gamma + X[0] * X[0]) + X[1] * X[1]) + X[2] * X[2];
+        qn = sqrt(qn);
+        q[0] = gamma / qn; q[1] = X[0] / qn; q[2] = X[1] / qn; q[3] = X[2] / qn;
+        if (gamma < 0.0)
+            for (int m = 0; m < 4; ++m) q[m] = -q[m];
+        // band normals rotated by q (convention p = +1), dotted with their poles
+        double vt[3], vt2[3], o[3];
+        vt[0] = q[2] * bd[2] - q[3] * bd[1];
+        vt[1] = q[3] * bd[0] - q[1] * bd[2];
+        vt[2] = q[1] * bd[1] - q[2] * bd[0];
+        for (int m = 0; m < 3; ++m) vt[m] += q[0] * bd[m];
+        vt2[0] = q[2] * vt[2] - q[3] * vt[1];
+        vt2[1] = q[3] * vt[0] - q[1] * vt[2];
+        vt2[2] = q[1] * vt[1] - q[2] * vt[0];
+        for (int m = 0; m < 3; ++m) o[m] = 2.0 * vt2[m] + bd[m];
+        const double bdj = (o[0] * pf[0] + o[1] * pf[1]) + o[2] * pf[2];
+        const bool inq = act && ((qm >> lane) & 1ull);
+        if (inq) bdot = bdj;
+        double msum = 0.0;
+        for (int i = 0; i < nb; ++i)
+            if ((qm >> i) & 1ull) msum += rl_d(bdj, i);
+        mad = msum / (double)nq;
+    }
+    // the confidence metric: matched votes over all votes, each cell at least 1 (float32)
+    float cm2 = -1.0f;
+    if (nvalid >= 3) {
+        const float mv = valid ? accw[l_famid[pmc] * nb + j] : 0.0f;
+        float cs = 0.0f;
+        for (int i = 0; i < nb; ++i) {
+            const float vi = rl_f(mv, i);
+            if ((vmask >> i) & 1ull) cs += vi;
+        }
+        float den = 0.0f;
+        for (int t = 0; t < nfam * nb; ++t) {
+            const float v = accw[t];
+            den += v < 1.0f ? 1.0f : v;
+        }
+        cm2 = cs / den;
+    }
+    if (act) {
+        a.pm[p * nb + j] = best_pm;
+        a.bandfit[p * nb + j] = acos(fmin(fmax(bdot, -1.0), 1.0)) * kRadeg;
+        if (a.pv) a.pv[p * nb + j] = (uint8_t)best_pv;
+        if (a.fitb) a.fitb[p * nb + j] = best_fitb;
+    }
+    if (lane < 4) a.quat[p * 4 + lane] = q[lane == 0 ? 0 : lane == 1 ? 1 : lane == 2 ? 2 : 3];
+    if (lane == 0) {
+        a.fit[p] = acos(fmin(fmax(mad, -1.0), 1.0)) * kRadeg;
+        a.cm[p] = cm2;
+        a.nmatch[p] = best_n;
+        a.ij[p * 4 + 0] = best_ij0;
+        a.ij[p * 4 + 1] = best_ij1;
+        a.ij[p * 4 + 2] = best_ij2;
+        a.ij[p * 4 + 3] = best_ij3;
+        a.acc_correct[p] = corr;
+        if (a.afit) a.afit[p] = best_fit;
+    }
+}
+
 }  // namespace
 
 extern "C" const char *ebsd_tripvote_last_error(void) { return g_err.c_str(); }
@@ -688,5 +1058,43 @@ extern "C" int ebsd_tripvote(const float *d_bandnorms, const float *d_intensity,
                            d_bin_start, d_bin_rows, nbins, bin_width, d_accw, d_fam, d_rank, d_cm, d_acc);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(-2, std::string("k_tripvote launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ebsd_bandindex(const float *d_bandnorms, const float *d_intensity, int64_t npats, int nbands,
+                              const float *d_accw, const int32_t *d_fam, const float *d_rank, const float *d_acc,
+                              int nfam, const float *d_polescart, const double *d_polescart64,
+                              const int32_t *d_famid, int npoles, const double *d_angrows,
+                              const int32_t *d_famindex, const int32_t *d_famcount, double angtol,
+                              int n_band_early, int nfit, int simple_weights, double *d_quat, double *d_fit,
+                              float *d_cm, int64_t *d_polematch, int64_t *d_nmatch, int64_t *d_ij,
+                              int32_t *d_acc_correct, double *d_bandfit, uint8_t *d_polevalid,
+                              float *d_assign_fit, float *d_fitb, void *stream)
+{
+    if (npats < 0) return fail(-1, "npats must be >= 0");
+    if (nbands < 1 || nbands > kMaxBands) return fail(-1, "nbands must be in [1, 64]");
+    if (nfam < 1 || npoles < 1) return fail(-1, "the library needs at least one family and one pole");
+    const size_t lds = (size_t)npoles * (3 * sizeof(double) + 3 * sizeof(float) + sizeof(int32_t)) +
+                       (size_t)nfam * npoles * sizeof(double) + (size_t)2 * nfam * sizeof(int32_t);
+    if (lds > kLdsBudget) return fail(-1, "library too large for the band assignment's LDS (64 KiB)");
+    if (npats == 0) return 0;
+    if (!d_bandnorms || !d_intensity || !d_accw || !d_fam || !d_rank || !d_acc || !d_polescart || !d_polescart64 ||
+        !d_famid || !d_angrows || !d_famindex || !d_famcount || !d_quat || !d_fit || !d_cm || !d_polematch ||
+        !d_nmatch || !d_ij || !d_acc_correct || !d_bandfit)
+        return fail(-1, "null device pointer");
+    BandIndexArgs a;
+    a.bn = d_bandnorms; a.inten = d_intensity; a.accw = d_accw; a.rank = d_rank; a.acc = d_acc; a.fam = d_fam;
+    a.npats = npats; a.nb = nbands; a.nfam = nfam; a.npl = npoles;
+    a.pflt = d_polescart; a.pc64 = d_polescart64; a.famid = d_famid; a.angrows = d_angrows;
+    a.famidx = d_famindex; a.famcnt = d_famcount;
+    a.angtol = angtol; a.n_band_early = n_band_early; a.nfit = nfit; a.simple = simple_weights;
+    a.quat = d_quat; a.fit = d_fit; a.bandfit = d_bandfit; a.cm = d_cm; a.afit = d_assign_fit; a.fitb = d_fitb;
+    a.pm = d_polematch; a.nmatch = d_nmatch; a.ij = d_ij; a.acc_correct = d_acc_correct; a.pv = d_polevalid;
+    const int per_block = 4;
+    const int64_t grid = (npats + per_block - 1) / per_block;
+    if (grid > 0x7fffffff) return fail(-1, "too many patterns for one launch");
+    hipLaunchKernelGGL(k_bandindex, dim3((unsigned)grid), dim3(per_block * kWave), lds, (hipStream_t)stream, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(-2, std::string("k_bandindex launch: ") + hipGetErrorString(e));
     return 0;
 }

@@ -11,12 +11,13 @@ their most likely family (``BandIndexer._tripvote_numba``, pyebsdindex/tripletvo
   rows are tested 64 at a time and the matches applied in library order, so the float
   sums follow the reference's order);
 * ``triplet_library`` / ``addphase``: the phase library (pole families, pair angles,
-  angle triplets) of a cubic phase of Laue class m-3m, the input the vote needs
+  angle triplets, the complete pole list) of a phase of any Laue class, the input the vote
+  and the band assignment need
   (tripletvote.py:320-491, :70-140 for the FCC/BCC defaults);
 * ``BandIndexer``: holds a phase's library on the host and the device and votes.
 
-Band assignment, the QUEST orientation fit and the confidence metrics that follow the
-vote in ``bandindex`` (tripletvote.py:548-633) are not part of this module.
+``BandIndexer.bandindex`` runs the rest on the device too (``ebsd_bandindex``): band
+assignment, the QUEST orientation fit and the confidence metric (tripletvote.py:526-633).
 """
 from __future__ import annotations
 
@@ -425,9 +426,74 @@ class BandIndexer:
         return tripvote_device(bandnorms, band_intensity, self.lut, self.angTol, a, f,
                                self.npolefamilies, stream=stream, index=idx if use_index else None)
 
-    def bandindex(self, *args, **kwargs):
-        raise NotImplementedError("band assignment and the QUEST orientation fit that follow the vote "
-                                  "(tripletvote.py:548-633) are not built on the device")
+    def _device_complete(self, device):
+        """The complete pole list on the device, for ebsd_bandindex (made once per device)."""
+        import torch
+        key = ("complete", str(device))
+        if key not in self._dev:
+            c = self.completelib
+            t = lambda x, dt: torch.as_tensor(np.ascontiguousarray(x, dtype=dt), device=device)
+            self._dev[key] = (t(c["polesCart"], np.float32), t(c["polesCart"], np.float64),
+                              t(c["familyid"], np.int32), t(c["angTable"][c["famIndex"]], np.float64),
+                              t(c["famIndex"], np.int32), t(c["nFamily"], np.int32))
+        return self._dev[key]
+
+    def bandindex(self, band_norms, band_intensity=None, band_widths=None, verbose=0, stream=None,
+                  return_assignment=False):
+        """BandIndexer.bandindex (tripletvote.py:492-633) on the device: the vote, band
+        assignment, QUEST orientation and confidence metric.  ``band_norms`` (npats, nbands,
+        3) or (nbands, 3), numpy or a device tensor; ``band_intensity`` likewise (default
+        ones).  Returns (avequat, fit, cm2, polematch, nMatch, ij, acc_correct, bandfit) as the
+        reference does -- numpy arrays for host input, device tensors for device input --
+        and, with ``return_assignment``, the assignment's own (polevalid, fit, per-band
+        fit) after them."""
+        import torch
+        if not self.high_fidelity:
+            raise NotImplementedError("high_fidelity=False (the TRIAD rotation as the orientation) is not built")
+        host = not (isinstance(band_norms, torch.Tensor) and band_norms.is_cuda)
+        dev = torch.device("cuda", torch.cuda.current_device()) if host else band_norms.device
+        bn = torch.as_tensor(np.asarray(band_norms, dtype=np.float32) if host else band_norms, device=dev)
+        bn = (bn.unsqueeze(0) if bn.dim() == 2 else bn).to(torch.float32).contiguous()
+        npats, nb = int(bn.shape[0]), int(bn.shape[1])
+        if band_intensity is None:
+            inten = torch.ones((npats, nb), dtype=torch.float32, device=dev)
+        else:
+            inten = torch.as_tensor(band_intensity, device=dev).to(torch.float32).reshape(npats, nb).contiguous()
+        accW, fam, rank, cm, acc = self.tripvote(bn, inten, stream=stream)
+        pf, pc64, famid, angrows, famidx, famcnt = self._device_complete(dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        quat = torch.empty((npats, 4), **f64)
+        fit = torch.empty(npats, **f64)
+        bandfit = torch.empty((npats, nb), **f64)
+        cm2 = torch.empty(npats, dtype=torch.float32, device=dev)
+        pm = torch.empty((npats, nb), dtype=torch.int64, device=dev)
+        nmatch = torch.empty(npats, dtype=torch.int64, device=dev)
+        ij = torch.empty((npats, 4), dtype=torch.int64, device=dev)
+        accc = torch.empty(npats, dtype=torch.int32, device=dev)
+        pv = torch.empty((npats, nb), dtype=torch.uint8, device=dev)
+        afit = torch.empty(npats, dtype=torch.float32, device=dev)
+        fitb = torch.empty((npats, nb), dtype=torch.float32, device=dev)
+        from ._hip import keep_for_stream, launch_stream
+        s = launch_stream(stream, dev)
+        lib = load_tripvote()
+        P = ctypes.c_void_p
+        ptr = lambda t: P(t.data_ptr())
+        rc = lib.ebsd_bandindex(ptr(bn), ptr(inten), ctypes.c_int64(npats), nb, ptr(accW), ptr(fam), ptr(rank),
+                                ptr(acc), int(self.npolefamilies), ptr(pf), ptr(pc64), ptr(famid), int(pf.shape[0]),
+                                ptr(angrows), ptr(famidx), ptr(famcnt), ctypes.c_double(self.angTol),
+                                int(self.nband_earlyexit), 6, int(bool(self.simpleqweights)), ptr(quat), ptr(fit),
+                                ptr(cm2), ptr(pm), ptr(nmatch), ptr(ij), ptr(accc), ptr(bandfit), ptr(pv),
+                                ptr(afit), ptr(fitb), P(s.cuda_stream))
+        check_tripvote(lib, rc)
+        outs = [quat, fit, cm2, pm, nmatch, ij, accc, bandfit]
+        if return_assignment:
+            outs += [pv, afit, fitb]
+        keep_for_stream(s, bn, inten, accW, fam, rank, cm, acc, *outs)
+        if host:
+            if stream is not None:
+                s.synchronize()
+            return tuple(o.cpu().numpy() for o in outs)
+        return tuple(outs)
 
 
 def tripvote_device(bandnorms, band_intensity, LUT, angTol, tripAngles, tripID, nfam, stream=None,
@@ -521,4 +587,5 @@ def bandangles(bandnorms: np.ndarray) -> np.ndarray:
 
 
 __all__ = ["triplet_lut", "triplet_library", "addphase", "BandIndexer", "tripvote_device", "bandangles",
+           "laue_class", "laue_rotations", "Crystal", "hex4_to_hex3", "hex3_to_hex4", "HEX_POLES",
            "FCC_POLES", "BCC_POLES"]

@@ -176,3 +176,17 @@ def symmetric_misorientation_deg(Ra, Rb, rots):
     ``rots`` (Cartesian, (n, 3, 3)) applied on the crystal side: min_S angle(Ra^T Rb S)."""
     from oracle.orientation import misorientation_deg
     return min(misorientation_deg(Ra, Rb @ S) for S in rots)
+
+
+def adj_intensity(bands, rhomax):
+    """The band weights EBSDIndexer passes to bandindex (_ebsd_index_single.py:757-763):
+    max scaled down away from rho = 0 and near theta = 0 or 180 degrees, zero for invalid
+    bands (float32)."""
+    rho = np.asarray(bands["rho"], np.float32)
+    theta = np.asarray(bands["theta"], np.float32)
+    rm = np.float32(rhomax)
+    a = (np.float32(-1) * np.abs(rho) * np.float32(0.5) / rm + np.float32(1)) * np.asarray(bands["max"], np.float32)
+    a *= ((theta > (2 * np.pi / 180)).astype(np.float32) + np.float32(0.5)) / np.float32(2)
+    a *= ((theta < (178.0 * np.pi / 180)).astype(np.float32) + np.float32(0.5)) / np.float32(2)
+    a *= np.asarray(bands["valid"]).astype(np.float32)
+    return a.astype(np.float32)

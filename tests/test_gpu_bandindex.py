@@ -1,0 +1,116 @@
+"""The device band indexer after the vote (ebsd_bandindex through BandIndexer.bandindex)
+against the oracle's restatement (oracle/tripvote_ref.py): band assignment, QUEST weights and
+orientation, confidence metric -- and the whole chain from patterns to orientations."""
+import numpy as np
+import pytest
+
+from helpers import adj_intensity, make_pair, oracle64, phase_bands, quat_to_rot, symmetric_misorientation_deg
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(ind, bn, inten, min_equal=0.995):
+    from oracle import tripvote_ref as tr
+    got = ind.bandindex(bn, inten, return_assignment=True)
+    q, fit, cm2, pm, nm, ij, accc, bandfit, pv, afit, fitb = got
+    wq, wfit, wcm, wpm, wnm, wij, waccc, wbandfit = tr.bandindex(ind, bn, inten)
+    same = (np.all(pm == wpm, axis=1) & (nm == wnm) & np.all(ij == wij, axis=1) & (accc == waccc))
+    exact = same & np.all(q == wq, axis=1) & (fit == wfit) & np.all(bandfit == wbandfit, axis=1) & (cm2 == wcm)
+    print(f"  assignment equal on {same.mean():.4f} of patterns, all outputs bit-identical on "
+          f"{exact.mean():.4f}", flush=True)
+    assert same.mean() >= min_equal, same.mean()
+    s = same
+    assert np.allclose(q[s], wq[s], rtol=0, atol=1e-12)
+    assert np.allclose(fit[s], wfit[s], rtol=0, atol=1e-9)
+    assert np.allclose(bandfit[s], wbandfit[s], rtol=0, atol=1e-9)
+    assert np.array_equal(cm2[s], wcm[s])
+    return got, same
+
+
+@pytest.mark.parametrize("phase,nb,noise,invalid", [("FCC", 9, 0.0, 0.0), ("FCC", 9, 0.3, 0.05),
+                                                    ("BCC", 9, 0.3, 0.05), ("HCP", 12, 0.3, 0.05),
+                                                    ("FCC", 20, 0.4, 0.1), ("HCP", 5, 0.2, 0.0)])
+def test_bandindex_matches_oracle(phase, nb, noise, invalid):
+    from pyebsdindex_amd import tripletvote as tv
+    print(f"bandindex {phase} nb={nb} noise={noise} invalid={invalid}", flush=True)
+    ind = tv.addphase(phase)
+    bn, pole, fam, Rt = phase_bands(ind, 300, nb, seed=17 + nb, noise_deg=noise)
+    rng = np.random.default_rng(5)
+    inten = rng.uniform(0.2, 1.0, bn.shape[:2]).astype(np.float32)
+    inten[rng.random(inten.shape) < invalid] = 0.0
+    (q, fit, cm2, pm, nm, ij, accc, bandfit, pv, afit, fitb), same = _compare(ind, bn, inten)
+    # the orientations themselves: the generator's rotation up to the Laue symmetry
+    rots = tv.laue_rotations(ind.lauecode)
+    ok = nm >= 3
+    assert ok.mean() > 0.9
+    err = [symmetric_misorientation_deg(quat_to_rot(q[p]).T, Rt[p], rots) for p in np.nonzero(ok)[0]]
+    assert np.median(err) < 0.5
+
+
+def test_bandindex_device_tensors_and_edges():
+    """Device input gives device output equal to the host call; no patterns; a pattern whose
+    bands are all parallel (no pair is tried: identity quaternion, fit 180, cm -1)."""
+    import torch
+    from pyebsdindex_amd import tripletvote as tv
+    ind = tv.addphase("FCC")
+    bn, pole, fam, Rt = phase_bands(ind, 8, 9, seed=2, noise_deg=0.2)
+    host = ind.bandindex(bn)
+    dev = ind.bandindex(torch.as_tensor(bn, device="cuda"))
+    for h, d in zip(host, dev):
+        assert isinstance(d, torch.Tensor) and d.is_cuda
+        assert np.array_equal(h, d.cpu().numpy())
+    empty = ind.bandindex(np.zeros((0, 9, 3), np.float32))
+    assert [e.shape for e in empty] == [(0, 4), (0,), (0,), (0, 9), (0,), (0, 4), (0,), (0, 9)]
+    bad = bn[:1].copy()
+    bad[0, 1:] = bad[0, 0]          # every band the same: every pair is parallel, nothing is assigned
+    q, fit, cm2, pm, nm, ij, accc, bandfit = ind.bandindex(bad)
+    assert nm[0] < 3 and cm2[0] == -1.0 and fit[0] == 180.0
+    assert np.array_equal(q[0], [1.0, 0.0, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("shape,dtype,family,n", [((120, 120), "uint16", "FCC", 64),
+                                                  ((60, 60), "uint8", "FCC", 64),
+                                                  ((160, 160), "uint8", "BCC", 24),
+                                                  ((240, 240), "uint16", "HCP", 12)])
+def test_chain_orientations_within_0p05(shape, dtype, family, n):
+    """The north-star bar: patterns -> device find_bands -> device radon2pole -> device
+    bandindex, against the FP64 oracle's bands -> the restated radon2pole -> the oracle's
+    bandindex, with the band weights EBSDIndexer uses; orientations within 0.05 degree where
+    both index a pattern, and within 0.5 degree (median) of the generator's own."""
+    import torch
+    from oracle import gnomonic
+    from oracle import tripvote_ref as tr
+    from pyebsdindex_amd import tripletvote as tv
+    from pyebsdindex_amd.differentiable import radon2pole
+    from pyebsdindex_amd.synthetic import kikuchi_patterns, quat_to_matrix
+    print(f"chain {shape} {dtype} {family} n={n}", flush=True)
+    pc = (0.5, 0.5, 0.6)
+    kw = dict(nTheta=360, nRho=180, nBands=12) if shape[0] == 240 else {}
+    pats, meta = kikuchi_patterns(n, shape, dtype, seed=71, family=family, pc=pc, return_meta=True)
+    pats = pats.numpy()
+    bd, ref = make_pair(shape, **kw)
+    got = bd.find_bands(pats)
+    res, _, _, _ = oracle64(ref, pats)
+    if family == "HCP":
+        ind = tv.addphase("HCP", latticeparameter=(1.0, 1.0, 1.587, 90.0, 90.0, 120.0))
+    else:
+        ind = tv.addphase(family)
+    dev = torch.device("cuda", 0)
+    nd = radon2pole(torch.as_tensor(np.ascontiguousarray(got["theta"]), device=dev),
+                    torch.as_tensor(np.ascontiguousarray(got["rho"]), device=dev), pc, shape[0], shape[1])
+    wd = torch.as_tensor(adj_intensity(got, bd.rhoMax), device=dev)
+    qd, fd, cmd, pmd, nmd, ijd, accd, bfd = (t.cpu().numpy() for t in ind.bandindex(nd, wd))
+    no = gnomonic.radon2pole(res["theta"], res["rho"], pc, shape).astype(np.float32)
+    qo, fo, cmo, pmo, nmo, ijo, acco, bfo = tr.bandindex(ind, no, adj_intensity(
+        {"rho": res["rho"], "theta": res["theta"], "max": res["max"], "valid": res["valid"]}, bd.rhoMax))
+    rots = tv.laue_rotations(ind.lauecode)
+    R_true = quat_to_matrix(meta["quat"])
+    both = (nmd >= 3) & (nmo >= 3)
+    assert both.mean() >= 0.9, (nmd, nmo)
+    worst = max(symmetric_misorientation_deg(quat_to_rot(qd[p]).T, quat_to_rot(qo[p]).T, rots)
+                for p in np.nonzero(both)[0])
+    vs_true = [symmetric_misorientation_deg(quat_to_rot(qd[p]).T, R_true[p], rots) for p in np.nonzero(nmd >= 3)[0]]
+    print(f"  {both.sum()}/{n} indexed by both, device vs oracle worst {worst:.2e} deg, "
+          f"device vs true median {np.median(vs_true):.3f} deg", flush=True)
+    assert worst < 0.05
+    assert np.median(vs_true) < 0.5

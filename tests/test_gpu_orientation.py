@@ -1,10 +1,8 @@
-"""Downstream orientations: the device's bands against the FP64 oracle's bands, each turned
-into an orientation by the same fit (oracle/orientation.py), must agree within 0.05 deg.
-
-The reference's consumer is triplet voting plus a QUEST fit (tripletvote.py:492-633), which
-needs numba and is not run here; its voting step is replaced by assigning each band to the
-generator's known plane normal, and its QUEST fit by the same weighted Wahba problem.  So
-this pins how band-position differences move the orientation, not the voting itself."""
+"""Downstream orientations through an independent fit: the device's bands against the FP64
+oracle's bands, each band assigned to the generator's known plane normal and the orientation
+solved as a weighted Wahba problem by SVD (oracle/orientation.py), must agree within 0.05 deg.
+This isolates how band-position differences move an orientation; the reference's own
+consumer (vote, band assignment, QUEST) is the chain of tests/test_gpu_bandindex.py."""
 import pytest
 
 from helpers import make_pair, oracle64, orientation_agreement
