@@ -373,13 +373,41 @@ def tripvote_measure(dev, npats=65536, nbands=9, reps=20, cpu_seconds=6.0, seed=
     if done:
         cpu = {"value": round(done / (time.perf_counter() - t0), 1), "unit": "patterns/s", "cores": os.cpu_count(),
                "kind": "port", "sample": f"{done} patterns in blocks of {sample}, OpenMP over patterns as numba.prange"}
+    # the whole BandIndexer.bandindex on the device: vote, band assignment, QUEST, cm
+    bi = ind.bandindex(bnd, intd)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        bi = ind.bandindex(bnd, intd)
+    e1.record(st)
+    torch.cuda.synchronize()
+    bi_ms = e0.elapsed_time(e1) / reps
+    nmatch = bi[4].cpu().numpy()
+    cpu_bi = None
+    if cpu_seconds > 0:
+        sample_bi = 1024
+        tripvote_ref.bandindex(ind, bn[:64], inten[:64])
+        done_bi, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            s0 = done_bi % (npats - sample_bi)
+            tripvote_ref.bandindex(ind, bn[s0:s0 + sample_bi], inten[s0:s0 + sample_bi])
+            done_bi += sample_bi
+        cpu_bi = {"value": round(done_bi / (time.perf_counter() - t0), 1), "unit": "patterns/s",
+                  "cores": os.cpu_count(), "kind": "port",
+                  "sample": f"{done_bi} patterns in blocks of {sample_bi}: the oracle's vote, assignment, QUEST "
+                            f"weights and fit, cm (C, OpenMP over patterns)"}
+    bandindex = {"workload": "BandIndexer.bandindex on the same bands: vote, band assignment, QUEST orientation, "
+                             "confidence metric (nband_earlyexit 8)",
+                 "patterns_per_s": round(npats / (bi_ms * 1e-3), 1), "ms": round(bi_ms, 4),
+                 "indexed_fraction": round(float((nmatch >= 3).mean()), 4),
+                 "mean_matched_bands": round(float(nmatch.mean()), 3), "cpu_baseline": cpu_bi}
     return {"workload": f"triplet vote of {npats} patterns x {nbands} bands, FCC library ({ntrip} triplets), "
                         f"angTol 2, 0.3 degree noise, 3 % invalid bands",
             "patterns_per_s": round(npats / (ms * 1e-3), 1), "kernel_ms": round(ms, 4),
             "library_row_tests_per_pattern": tests,
             "row_tests_per_s": round(npats * tests / (ms * 1e-3), 1),
             "band_family_agreement": round(agree, 4),
-            "cpu_baseline": cpu}
+            "cpu_baseline": cpu, "bandindex": bandindex}
 
 
 def nlpar_pmc_driver(dev, nrows=256, ncols=256, shape=(60, 60), sr=3, seed=17):
@@ -727,6 +755,10 @@ def main():
         print(f"[bench] triplet vote {tripvote['patterns_per_s']:.1f} patterns/s ({tripvote['kernel_ms']} ms per "
               f"65,536), CPU {cpu_tv:.1f} patterns/s, family agreement "
               f"{tripvote['band_family_agreement']}", file=sys.stderr, flush=True)
+        bi = tripvote["bandindex"]
+        print(f"[bench] bandindex {bi['patterns_per_s']:.1f} patterns/s ({bi['ms']} ms per 65,536), CPU "
+              f"{(bi['cpu_baseline'] or {}).get('value')} patterns/s, indexed {bi['indexed_fraction']}",
+              file=sys.stderr, flush=True)
 
     nlpar = None
     if not args.no_nlpar:
