@@ -371,7 +371,7 @@ class BandDetect:
                                             rhoMax=self.rhoMax, mask=patternmask,
                                             maskindex=patternmaskindex)
             self._setup_rdnmask()
-            self._device_plans = {}
+            self._drop_device_state()
 
         if tSigma is not None:
             self.tSigma = tSigma
@@ -390,14 +390,31 @@ class BandDetect:
             self.rSigma = 0.5 / np.float32(self.dRho)
             new_kernel = True
         if new_kernel and self.rSigma is not None and self.tSigma is not None:
+            old = self.kernel
             self._setup_kernel()
-            self._device_plans = {}
+            if old is None or old.shape != self.kernel.shape or not np.array_equal(old, self.kernel):
+                self._drop_device_state()   # the reference recomputes the kernel on every call
         self.padding = np.array([np.max([self.peakPad[0], self.padding[0]]),
                                  np.max([self.peakPad[1], self.padding[1]])])
         if nBands is not None:
             if nBands != self.nBands:
-                self._device_plans = {}
+                self._drop_device_state()
             self.nBands = nBands
+
+    def _drop_device_state(self):
+        """Forget the device plans and host pipelines built for the previous set-up."""
+        self._device_plans = {}
+        self.__dict__.pop("_pipelines", None)
+
+    def __getstate__(self):
+        """Pickled (Ray's object store pickles the indexer and its band-detection plan,
+        _ebsd_index_parallel.py:396) without device plans or pipelines: they hold library
+        handles, device memory and pinned host memory of this process, and are rebuilt on
+        the first call in the process that unpickles."""
+        state = self.__dict__.copy()
+        state["_device_plans"] = {}
+        state.pop("_pipelines", None)
+        return state
 
     def _line_sums(self, img):
         """Unnormalised line sums of one image over the index plan (host set-up only)."""

@@ -107,6 +107,14 @@ class Radon:
         pix = np.where(mi[pix] >= 0, mi[pix], sentinel)
         pix.sort(axis=-1)
         self.indexPlan = pix
+        self.__dict__.pop("_device_plans", None)   # device plans of the previous geometry
+
+    def __getstate__(self):
+        """Pickled (as Ray's object store does with the indexer) without device plans: they
+        hold library handles and device memory of this process."""
+        state = self.__dict__.copy()
+        state.pop("_device_plans", None)
+        return state
 
     # -- Radon on the GPU --------------------------------------------------------------
     def radon_faster(self, imageIn, padding=np.array([0, 0]), fixArtifacts=False, background=None,

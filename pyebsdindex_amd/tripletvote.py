@@ -406,6 +406,14 @@ class BandIndexer:
         self.families = lib["families"]
         self._dev = {}
 
+    def __getstate__(self):
+        """Pickled (Ray's object store pickles the indexer with its phases,
+        _ebsd_index_parallel.py:396) without the device copies of the library, which are
+        made again on first use in the process that unpickles."""
+        state = self.__dict__.copy()
+        state["_dev"] = {}
+        return state
+
     def _device_library(self, device):
         import torch
         key = (str(device), self.angTol)
