@@ -231,6 +231,12 @@ class NLPAR:
             p = Path(patternfile.filepath)
             appnd = "_NLPAR_l{:1.2f}".format(float(_scalar(self.lam))) + "sr{:d}".format(int(_scalar(self.searchradius)))
             out = p.parent / (p.stem + appnd + p.suffix)
+        if getattr(patternfile, "filetype", "UP") != "UP":
+            # .ebsp / .pat: a copy of the input whose patterns are then overwritten
+            # (nlpar_cpu.py:590,603-613; TF copies keep only the header and background)
+            patternfile.copy_file(str(out), empty_data=True)
+            self.filepathout = out
+            return
         w = ebsd_pattern.UPFile(out)
         for k in ("version", "patternW", "patternH", "filePos", "extraPatterns", "nCols", "nRows", "hexflag",
                   "xStep", "yStep", "filedatatype"):
@@ -324,7 +330,9 @@ class NLPAR:
     def _read_rows(self, f, a, b, dev):
         torch = _torch()
         ncols = int(self.ncols)
-        tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.uint16}[np.dtype(f.filedatatype)]
+        tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.uint16}.get(np.dtype(f.filedatatype))
+        if tdt is None:
+            raise ValueError(f"NLPAR reads uint8 and uint16 pattern files, not {np.dtype(f.filedatatype)}")
         host = torch.empty(((b - a) * ncols, int(f.patternH), int(f.patternW)), dtype=tdt, pin_memory=True)
         f.read_into(host, a * ncols, (b - a) * ncols)
         return host.to(dev, non_blocking=False)

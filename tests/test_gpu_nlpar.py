@@ -155,6 +155,42 @@ def test_nlpar_file_round_trip(tmp_path):
     assert np.allclose(lams, ref, atol=2e-3)
 
 
+@pytest.mark.parametrize("ext,dtype", [(".ebsp", "uint8"), (".pat", "uint16")])
+def test_nlpar_raw_files_equal_up(tmp_path, ext, dtype):
+    """NLPAR.calcnlpar on an .ebsp / .pat file writes the same patterns as on the same scan
+    in an UP file (the output is a copy of the input with its patterns replaced,
+    nlpar_cpu.py:590,603-613), and the copy keeps the input's scan layout."""
+    from pyebsdindex_amd.ebsd_pattern import EBSPFile, TFPatFile, UPFile, get_pattern_file_obj
+    from pyebsdindex_amd.nlpar import NLPAR
+    nrows, ncols, shape = 9, 8, (40, 40)
+    print(f"nlpar raw file {ext} {dtype}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=27)
+    n = nrows * ncols
+    up = tmp_path / ("scan.up1" if dtype == "uint8" else "scan.up2")
+    w = UPFile(up)
+    w.patternW, w.patternH, w.nCols, w.nRows, w.xStep, w.yStep = shape[1], shape[0], ncols, nrows, 1.0, 1.0
+    w.write_header()
+    w.pat_writer(pats, 0, n, pats.dtype)
+    raw = tmp_path / f"scan{ext}"
+    r = EBSPFile(raw) if ext == ".ebsp" else TFPatFile(raw)
+    r.patternW, r.patternH, r.nCols, r.nRows, r.xStep, r.yStep = shape[1], shape[0], ncols, nrows, 1.0, 1.0
+    r.nPatterns = n
+    if ext == ".ebsp":
+        r.version, r.hasxypos = 2, True
+        r.write_header(writeBlank=True, bitdepth=8)
+    else:
+        r.write_header(writeBlank=True, bitdepth=16)
+    r.pat_writer(pats, 0, n)
+    a = get_pattern_file_obj(NLPAR(str(up), lam=0.9, searchradius=2).calcnlpar(
+        fileout=str(tmp_path / ("o.up1" if dtype == "uint8" else "o.up2")))).pat_reader(0, n)[0]
+    out = NLPAR(str(raw), lam=0.9, searchradius=2).calcnlpar(fileout=str(tmp_path / f"o{ext}"))
+    fo = get_pattern_file_obj(out)
+    b = fo.pat_reader(0, n)[0]
+    assert type(fo) is type(r) and (int(fo.nCols), int(fo.nRows)) == (ncols, nrows)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(b, pats)
+
+
 @pytest.mark.parametrize("dtype", ["uint8", "uint16"])
 def test_saturated_patterns_and_both_paths(dtype):
     """Integer scans with saturated pixels in some patterns (1 to 60): the packed-word path

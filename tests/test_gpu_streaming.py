@@ -70,6 +70,50 @@ def test_file_streaming_equals_resident(tmp_path, shape, dtype, ext, start, coun
         assert np.array_equal(got[k].numpy(), want[k]), k
 
 
+def _write_raw(fn, pats, ncols, nrows):
+    """The patterns as an .ebsp (version 3, positions) or .pat file of the package's writers."""
+    from pyebsdindex_amd.ebsd_pattern import EBSPFile, TFPatFile
+    n, H, W = pats.shape
+    w = EBSPFile(fn) if str(fn).endswith(".ebsp") else TFPatFile(fn)
+    w.patternW, w.patternH, w.nCols, w.nRows, w.xStep, w.yStep = W, H, ncols, nrows, 1.0, 1.0
+    w.nPatterns = n
+    if isinstance(w, EBSPFile):
+        w.version, w.hasxypos = 3, True
+        w.write_header(writeBlank=True, bitdepth=8 * pats.dtype.itemsize)
+    else:
+        w.write_header(writeBlank=True, bitdepth=16)
+    w.pat_writer(pats, 0, n)
+
+
+@pytest.mark.parametrize("shape,dtype,ext,start,count,batch", [
+    ((60, 60), "uint8", ".ebsp", 3, 50, 16),
+    ((120, 120), "uint16", ".ebsp", 0, -1, 8),
+    ((120, 120), "uint16", ".pat", 7, -1, 16),
+])
+def test_raw_file_streaming_equals_resident(tmp_path, shape, dtype, ext, start, count, batch):
+    """Oxford .ebsp and ThermoFisher .pat files (records with headers and positions between
+    the patterns) read into the pinned ring by vector reads: the bands equal find_bands on the
+    same patterns resident on the device, bit for bit."""
+    import torch
+    from pyebsdindex_amd.ebsd_pattern import get_pattern_file_obj
+    from pyebsdindex_amd.streaming import StreamingBandDetect
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    print(f"raw file streaming {ext} {shape} {dtype} start {start} count {count} batch {batch}", flush=True)
+    n = 61
+    pats = kikuchi_patterns(n, shape, dtype, seed=58).numpy()
+    fn = tmp_path / f"scan{ext}"
+    _write_raw(fn, pats, 61, 1)
+    f = get_pattern_file_obj(str(fn))
+    assert np.array_equal(f.pat_reader(0, n)[0], pats)
+    bd, _ = make_pair(shape)
+    stop = n if count < 0 else start + count
+    want = {k: v.cpu().numpy() for k, v in bd.find_bands_device(torch.from_numpy(pats[start:stop]).cuda()).items()}
+    sb = StreamingBandDetect(bd, batch=batch, dtype=dtype, depth=2, chunk=batch)
+    got = sb.concat(sb.run_file(f, start, count, threads=3))
+    for k in want:
+        assert np.array_equal(got[k].numpy(), want[k]), k
+
+
 def test_file_streaming_rejects_mismatches(tmp_path):
     """A file of another pattern type or size, a range past its end and a ring of one
     buffer are refused before anything is copied."""
