@@ -95,6 +95,14 @@ int ebsd_plan_set_row_segments(ebsd_plan *plan, int segments);
  *     with peakPad = 2*ksz+1 and at most 16 wavefronts a pattern). */
 int ebsd_plan_set_peak_kernel(ebsd_plan *plan, int kind);
 
+/* Theta columns per wavefront of the streamed peak kernel (its theta block; the pattern's
+ * wavefronts = ceil(nTheta / theta_cols), at most 16), a launch-shape knob for the
+ * autotuner with the same results for every value.  0 = the default (the fewest
+ * wavefronts whose blocks fit 64 lanes less the halo).  Errors where the streamed kernel
+ * does not apply or the value is out of range.  No counterpart in the reference (whose
+ * OpenCL kernels take a fixed work-group shape). */
+int ebsd_plan_set_peak_block(ebsd_plan *plan, int theta_cols);
+
 /* Workspace (device bytes) that ebsd_find_bands needs to process `chunk` patterns per pass. */
 size_t ebsd_workspace_bytes(const ebsd_plan *plan, int64_t chunk);
 

@@ -1,9 +1,11 @@
 """Autotuner for the launch shape of one plan (pattern size × dtype × background mode).
 
 Sweeps the gather's patterns per LDS word, work-group size, row segments (integer sums) and
-LDS row pitch, and the tiled peak kernel's theta tile width and work-group size, timing each on a sample of the caller's patterns
-with HIP events; the fastest shape is applied to the plan (``ebsd_plan_set_tuning``) and
-remembered per (device, H, W, nTheta, nRho, dtype, background mode).
+LDS row pitch, then the peak kernel that runs: the streamed kernel's theta block (columns per
+wavefront, hence wavefronts per pattern), or -- where only the tiled kernel applies -- its
+theta tile width and work-group size.  Each shape is timed on a sample of the caller's
+patterns with HIP events; the fastest is applied to the plan and remembered per (device, H,
+W, nTheta, nRho, dtype, background mode).  Every shape gives the same results.
 """
 import itertools
 
@@ -34,9 +36,10 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
              gather_threads=(512, 1024), row_segments=(1, 2, 3, 4), peak_tiles=(24, 30, 36, 45, 60, 90),
              peak_threads=(256, 512), verbose=False):
     """Tune ``bd``'s device plan on (a sample of) the device tensor ``pats``; returns
-    ``{"gather_pack", "gather_threads", "row_segments", "stage_pitch", "peak_tile",
-    "peak_threads", "gather_ms", "peak_ms"}`` (``row_segments`` 0: the plan's automatic
-    choice)."""
+    ``{"gather_pack", "gather_threads", "row_segments", "stage_pitch", "peak_block",
+    "peak_tile", "peak_threads", "gather_ms", "peak_ms"}`` (``row_segments`` 0: the plan's
+    automatic choice; ``peak_block`` 0 where the tiled peak kernel runs, whose
+    ``peak_tile``/``peak_threads`` are 0 where the streamed kernel runs)."""
     import torch
     from . import _hip
     plan = bd.device_plan(gpu_id)
@@ -47,6 +50,8 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
         plan.set_tuning(best["gather_pack"], best["gather_threads"], best["peak_tile"], best["peak_threads"])
         plan.set_row_segments(best["row_segments"])
         plan.set_stage_pitch(best["stage_pitch"])
+        if best["peak_block"]:
+            plan.set_peak_block(best["peak_block"])
         return dict(best)
     xx, flags = bd._prepare(plan, x)
     rad = torch.empty((xx.shape[0], plan.nR, plan.nT), dtype=torch.float32, device=plan.device)
@@ -95,6 +100,28 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
     best_g = (min(best_g[0], best_s[0]),) + best_g[1:]
     plan.radon(xx, flags=flags, out=rad)
     best_p = None
+    block = plan.stream_block() if getattr(plan, "peak_kind", 0) != 1 else None
+    if block is not None:
+        # the streamed kernel: its theta block sets the wavefronts per pattern
+        default, widest = block
+        n0 = -(-plan.nT // widest)
+        cols = sorted({-(-plan.nT // nb) for nb in range(n0, min(16, n0 + 8) + 1)} | {default}, reverse=True)
+        for c in cols:
+            try:
+                plan.set_peak_block(c)
+                ms = _time(lambda: plan.peaks(rad, out=out), reps)
+            except _hip.HipLibraryError:
+                continue
+            if verbose:
+                print(f"  peaks (streamed) block={c} waves={-(-plan.nT // c)}: {ms:.4f} ms")
+            if best_p is None or ms < best_p[0]:
+                best_p = (ms, c)
+        plan.set_peak_block(best_p[1])
+        best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "row_segments": best_g[3],
+                "stage_pitch": best_s[1], "peak_block": best_p[1], "peak_tile": 0, "peak_threads": 0,
+                "gather_ms": best_g[0], "peak_ms": best_p[0]}
+        _CACHE[key] = best
+        return dict(best)
     for pt, pth in itertools.product(peak_tiles, peak_threads):
         try:
             plan.set_tuning(0, 0, pt, pth)
@@ -107,7 +134,7 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
             best_p = (ms, pt, pth)
     plan.set_tuning(0, 0, best_p[1], best_p[2])
     best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "row_segments": best_g[3],
-            "stage_pitch": best_s[1], "peak_tile": best_p[1],
+            "stage_pitch": best_s[1], "peak_block": 0, "peak_tile": best_p[1],
             "peak_threads": best_p[2], "gather_ms": best_g[0], "peak_ms": best_p[0]}
     _CACHE[key] = best
     return dict(best)

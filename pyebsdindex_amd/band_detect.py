@@ -62,6 +62,9 @@ class DevicePlan:
                 int(bd.padding[0]), int(bd.padding[1]), int(bd.peakPad[0]), int(bd.peakPad[1]),
                 self.nB, theta.ctypes.data, rho.ctypes.data, _hip.ctypes.byref(handle)))
         self.handle = handle
+        self.kernel_shape = (int(kern.shape[0]), int(kern.shape[1]))
+        self.peak_half = (int(bd.peakPad[0]) // 2, int(bd.peakPad[1]) // 2)
+        self.peak_kind = 0
         self.bg_key = None
 
     def __del__(self):
@@ -107,6 +110,25 @@ class DevicePlan:
     def set_peak_kernel(self, kind):
         """0 automatic, 1 theta-tiled work-group kernel, 2 streamed wavefront kernel."""
         _hip.check(self.lib.ebsd_plan_set_peak_kernel(self.handle, int(kind)))
+        self.peak_kind = int(kind)
+
+    def set_peak_block(self, theta_cols):
+        """Theta columns per wavefront of the streamed peak kernel (0 = default); the
+        same results for every value."""
+        _hip.check(self.lib.ebsd_plan_set_peak_block(self.handle, int(theta_cols)))
+
+    def stream_block(self):
+        """(default theta block, largest theta block) of the streamed peak kernel, or None
+        where it does not apply (host restatement of the plan's rule)."""
+        kr, kt = int(self.kernel_shape[0]), int(self.kernel_shape[1])
+        hr, ht = int(self.peak_half[0]), int(self.peak_half[1])
+        if not (hr == kr and ht == kt and (kr, kt) in ((9, 5), (5, 5))):
+            return None
+        usable = 64 - 2 * (kt // 2 + ht)
+        nblk = -(-self.nT // usable)
+        if nblk > 16:
+            return None
+        return -(-self.nT // nblk), usable
 
     # -- device entry points --------------------------------------------------------------
     def alloc_out(self, N):

@@ -1855,13 +1855,16 @@ static void split_kernel(const float *k, int kr, int kt, float *u, float *v)
 // Streamed peak kernel: one wavefront per block of sbw theta columns, each lane one padded
 // column, with KT/2 + HT halo lanes on either side; sbw = 0 where it does not apply
 // (kernel sizes without an instantiation, or more than 16 wavefronts a pattern).
+// theta columns a wavefront of the streamed kernel can own besides its halo lanes
+static int stream_usable(const PeakGeo &g) { return 64 - 2 * (g.kt / 2 + g.ht); }
+
 static void geo_stream(PeakGeo &g)
 {
     g.mcw = ((g.nR + 31) >> 5) + 2;
     g.sbw = 0;
     const bool fixed = g.hr == g.kr && g.ht == g.kt && ((g.kr == 9 && g.kt == 5) || (g.kr == 5 && g.kt == 5));
     if (!fixed) return;
-    const int usable = 64 - 2 * (g.kt / 2 + g.ht);
+    const int usable = stream_usable(g);
     const int nblk = (g.nT + usable - 1) / usable;
     if (nblk > 16) return;
     g.sbw = (g.nT + nblk - 1) / nblk;
@@ -2422,6 +2425,25 @@ extern "C" int ebsd_plan_set_peak_kernel(ebsd_plan *p, int kind)
     if (kind < 0 || kind > 2) return fail(-1, "ebsd_plan_set_peak_kernel: kind is 0, 1 or 2");
     if (kind == 2 && p->g.sbw <= 0) return fail(-1, "ebsd_plan_set_peak_kernel: the streamed kernel does not apply to this plan");
     p->pkind = kind;
+    return 0;
+}
+
+extern "C" int ebsd_plan_set_peak_block(ebsd_plan *p, int theta_cols)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_peak_block: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_peak_block")) return -1;
+    PeakGeo g = p->g;
+    geo_stream(g);                           // the default block (and whether it applies)
+    if (theta_cols == 0) {
+        p->g.sbw = g.sbw;
+        return 0;
+    }
+    if (g.sbw <= 0) return fail(-1, "ebsd_plan_set_peak_block: the streamed kernel does not apply to this plan");
+    if (theta_cols < 1 || theta_cols > stream_usable(g))
+        return fail(-1, "ebsd_plan_set_peak_block: theta_cols must be in 1.." + std::to_string(stream_usable(g)));
+    if ((g.nT + theta_cols - 1) / theta_cols > 16)
+        return fail(-1, "ebsd_plan_set_peak_block: more than 16 wavefronts a pattern");
+    p->g.sbw = theta_cols;
     return 0;
 }
 

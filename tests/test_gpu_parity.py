@@ -328,12 +328,61 @@ def test_autotune_keeps_results():
     assert np.array_equal(plan.radon(x).cpu().numpy(), rad0)
     b1 = {k: v.cpu().numpy() for k, v in bd.find_bands_device(x).items()}
     for k in b0:
-        if k == "stats":   # the tiled peak kernel's mean may round differently per tile width
+        # the mean's float64 sum runs in another grouping under another peak tile or theta
+        # block: the mean (and normmax, which divides by it) may differ in float32 rounding
+        if k in ("stats", "normmax"):
             np.testing.assert_allclose(b0[k], b1[k], rtol=1e-6)
         else:
             assert np.array_equal(b0[k], b1[k]), k
     plan.set_row_segments(0)
     plan.set_stage_pitch(0)
+    plan.set_peak_block(0)
+
+
+@pytest.mark.parametrize("case", ["120-u16", "240-360x180", "uneven-theta", "kernel-5x5"])
+def test_peak_blocks_agree(case):
+    """Every theta block of the streamed peak kernel (wavefronts per pattern from the fewest
+    to 16) gives the bands of the default block: identical positions, values, widths and
+    min/max; the mean within float32 rounding of its float64 sum.  Out-of-range blocks and
+    plans without the streamed kernel are refused."""
+    from pyebsdindex_amd._hip import HipLibraryError
+    print(f"peak blocks {case}", flush=True)
+    kw, shape, dtype, n = {}, (120, 120), "uint16", 64
+    if case == "240-360x180":
+        shape, kw, n = (240, 240), dict(nTheta=360, nRho=180, nBands=12), 8
+    elif case == "uneven-theta":
+        kw = dict(nTheta=190, nRho=100)
+    elif case == "kernel-5x5":
+        kw = dict(tSigma=1.0, rSigma=1.0)
+    bd, ref = make_pair(shape, **kw)
+    plan = bd.device_plan()
+    default, widest = plan.stream_block()
+    rad = plan.radon(_dev(_pats(n, shape, dtype, seed=43)))
+    want = {k: v.cpu().numpy() for k, v in plan.peaks(rad).items()}
+    nT = plan.nT
+    blocks = sorted({widest, default, -(-nT // 16), -(-nT // 7), 17, widest - 3} & set(range(-(-nT // 16), widest + 1)))
+    for c in blocks:
+        plan.set_peak_block(c)
+        got = {k: v.cpu().numpy() for k, v in plan.peaks(rad).items()}
+        for k in want:
+            if k == "stats":
+                assert np.array_equal(want[k][:, 0], got[k][:, 0]) and np.array_equal(want[k][:, 2], got[k][:, 2])
+                np.testing.assert_allclose(want[k][:, 1], got[k][:, 1], rtol=2e-7, atol=1e-30)
+            elif k == "normmax":
+                np.testing.assert_allclose(want[k], got[k], rtol=1e-6, atol=1e-6)
+            else:
+                assert np.array_equal(want[k], got[k]), (c, k)
+    for bad in (widest + 1, -(-nT // 16) - 1, -1):
+        with pytest.raises(HipLibraryError):
+            plan.set_peak_block(bad)
+    plan.set_peak_block(0)
+    assert {k: v.cpu().numpy() for k, v in plan.peaks(rad).items()}["max"].tolist() == want["max"].tolist()
+    if case == "120-u16":            # a 13x9 kernel: only the tiled peak kernel applies
+        bd13, _ = make_pair((60, 60), tSigma=2.0, rSigma=3.0)
+        p13 = bd13.device_plan()
+        assert p13.stream_block() is None
+        with pytest.raises(HipLibraryError):
+            p13.set_peak_block(20)
 
 
 @pytest.mark.parametrize("case", ["120-u16", "60-u8-plateau", "240-360x180", "uneven-theta", "kernel-5x5"])
