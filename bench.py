@@ -56,6 +56,8 @@ def parse():
                     help="short fixed NLPAR sequence for rocprofv3 --pmc: the 256x256 scan of 60x60 uint8 "
                          "patterns of the NLPAR measurement, sigma once, NLPAR twice (uint8 output)")
     ap.add_argument("--tune", default="", help="gather_pack,gather_threads,peak_tile,peak_threads (0 = keep)")
+    ap.add_argument("--line-unit", type=int, default=0,
+                    help="gather line order in units of this many neighbouring cells (0 = the plan's default)")
     ap.add_argument("--chunk", type=int, default=2048,
                     help="patterns per Radon+peak pass (the Radon of a chunk stays in the MALL)")
     return ap.parse_args()
@@ -81,18 +83,6 @@ def lds_ceiling(reps=2):
             best = max(best, v.value)
         out[f"ds_read_b{8 * w}"] = round(best / 1e9, 1)
     return out
-
-
-def gather_slots(bd):
-    """LDS reads per pattern of the whole-pattern integer gather: every 64-line batch (lines
-    sorted by their 8-entry group count, descending) is padded to its longest line's groups,
-    the filler entries reading a zero word -- the entry slots the kernel actually reads."""
-    plan = np.asarray(bd.radonPlan.indexPlan)
-    nPx = int(np.prod(bd.patDim))
-    cnt = (plan < nPx).sum(axis=2).ravel()
-    g = np.sort((cnt + 7) // 8, kind="stable")[::-1]
-    mx = np.maximum(g[::64], 1)  # the first line of each batch is its longest
-    return int(mx.sum()) * 64 * 8
 
 
 def valu_counts(H, W, dtype, chunk):
@@ -588,6 +578,8 @@ def main():
     plan = bd.device_plan()
     if args.tune:
         plan.set_tuning(*[int(v) for v in args.tune.split(",")])
+    if args.line_unit:
+        plan.set_line_unit(args.line_unit)
     info = plan.info()
     N = pats.shape[0]
     out = plan.alloc_out(N)
@@ -653,7 +645,7 @@ def main():
     # the bytes the gather's ds_read_b64 actually move: every padded entry slot, one 8-byte
     # word per 4 uint16 patterns (8 uint8 patterns)
     pack = {"uint16": 4, "uint8": 8}.get(args.dtype)
-    slots = gather_slots(bd)
+    slots = int(info["slots"])      # the plan's own padded entry slots per pattern
     actual = {"bytes_per_launch": slots * 8.0 * chunk / pack if pack else None}
     if pack:
         actual["achieved"] = round(actual["bytes_per_launch"] / (gather_ms * 1e-3) / 1e9, 1)

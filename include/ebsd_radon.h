@@ -89,6 +89,14 @@ int ebsd_plan_set_stage_pitch(ebsd_plan *plan, int pitch);
  * which the configured pack size fits), 1..8 = that many.  Results are identical. */
 int ebsd_plan_set_row_segments(ebsd_plan *plan, int segments);
 
+/* Line order of the Radon gather: lines in units of `unit` neighbouring Radon cells (a power
+ * of two <= 64; 0 = the default 8, one 32-byte write sector of float32 values), the units
+ * sorted by their longest line.  Larger units make a batch's output stores cover whole
+ * sectors (less HBM write traffic) at the price of padding shorter lines to the unit's
+ * longest; 1 sorts single lines.  Results are identical.  No counterpart in the reference
+ * (a launch-shape knob of this implementation). */
+int ebsd_plan_set_line_unit(ebsd_plan *plan, int unit);
+
 /* Which convolution/peak kernel runs the stages of band_detect.py:562-809 (same results):
  *   0 automatic (the streamed kernel where it applies), 1 theta-tiled work-group kernel,
  *   2 streamed kernel (one wavefront per block of theta columns; needs a 9x5 or 5x5 kernel
@@ -163,8 +171,11 @@ int ebsd_batch_run(ebsd_batch *batch, void *stream);
 int ebsd_batch_graph_launch(ebsd_batch *batch, void *stream);
 int ebsd_batch_destroy(ebsd_batch *batch);
 
-/* Plan geometry queries. */
-int ebsd_plan_info(const ebsd_plan *plan, int64_t *out8);
+/* Plan geometry queries: out10 = cells, line-pixel pairs per pattern, gather batches, tiled
+ * peak kernel's theta tile, its tiles, its LDS bytes, background mode, compute units, entry
+ * slots per pattern of the whole-pattern gather (its lines padded to their batch's group
+ * count), line unit. */
+int ebsd_plan_info(const ebsd_plan *plan, int64_t *out10);
 
 /* ---- adjoints: gradients of the band positions --------------------------------------
  * The reference has no derivatives; these are the adjoints of its forward stages, so that

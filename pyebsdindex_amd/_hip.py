@@ -18,7 +18,7 @@ EBSD_U8, EBSD_U16, EBSD_F32 = 0, 1, 2
 #: exported symbols, in declaration order of include/ebsd_radon.h
 SYMBOLS = (
     "ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background", "ebsd_plan_set_tuning",
-    "ebsd_plan_set_stage_pitch", "ebsd_plan_set_row_segments", "ebsd_plan_set_peak_kernel", "ebsd_plan_set_peak_block", "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_find_bands_noalloc", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
+    "ebsd_plan_set_stage_pitch", "ebsd_plan_set_row_segments", "ebsd_plan_set_peak_kernel", "ebsd_plan_set_peak_block", "ebsd_plan_set_line_unit", "ebsd_workspace_bytes", "ebsd_find_bands", "ebsd_find_bands_noalloc", "ebsd_radon", "ebsd_peaks", "ebsd_conv",
     "ebsd_plan_info", "ebsd_last_error", "ebsd_batch_workspace_bytes", "ebsd_batch_create",
     "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy", "ebsd_band_grad",
     "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands", "ebsd_radon2pole",
@@ -69,6 +69,7 @@ def load(path: str = LIB_PATH):
         L.ebsd_plan_set_tuning.argtypes = [P, I, I, I, I]
         L.ebsd_plan_set_peak_kernel.argtypes = [P, I]
         L.ebsd_plan_set_peak_block.argtypes = [P, I]
+        L.ebsd_plan_set_line_unit.argtypes = [P, I]
         L.ebsd_plan_set_stage_pitch.argtypes = [P, I]
         L.ebsd_plan_set_row_segments.argtypes = [P, I]
         L.ebsd_workspace_bytes.argtypes = [P, I64]
@@ -93,7 +94,7 @@ def load(path: str = LIB_PATH):
         L.ebsd_last_error.argtypes = []
         L.ebsd_last_error.restype = ctypes.c_char_p
         for name in ("ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background",
-                     "ebsd_plan_set_tuning", "ebsd_plan_set_peak_kernel", "ebsd_plan_set_peak_block", "ebsd_plan_set_stage_pitch",
+                     "ebsd_plan_set_tuning", "ebsd_plan_set_peak_kernel", "ebsd_plan_set_peak_block", "ebsd_plan_set_line_unit", "ebsd_plan_set_stage_pitch",
                      "ebsd_plan_set_row_segments", "ebsd_find_bands", "ebsd_find_bands_noalloc",
                      "ebsd_radon", "ebsd_peaks",
                      "ebsd_conv", "ebsd_plan_info", "ebsd_batch_create", "ebsd_batch_run",

@@ -1,7 +1,7 @@
 """Autotuner for the launch shape of one plan (pattern size × dtype × background mode).
 
-Sweeps the gather's patterns per LDS word, work-group size, row segments (integer sums) and
-LDS row pitch, then the peak kernel that runs: the streamed kernel's theta block (columns per
+Sweeps the gather's patterns per LDS word, work-group size, row segments (integer sums), LDS
+row pitch and line unit, then the peak kernel that runs: the streamed kernel's theta block (columns per
 wavefront, hence wavefronts per pattern), or -- where only the tiled kernel applies -- its
 theta tile width and work-group size.  Each shape is timed on a sample of the caller's
 patterns with HIP events; the fastest is applied to the plan and remembered per (device, H,
@@ -34,9 +34,9 @@ def _time(fn, reps):
 
 def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 8),
              gather_threads=(512, 1024), row_segments=(1, 2, 3, 4), peak_tiles=(24, 30, 36, 45, 60, 90),
-             peak_threads=(256, 512), verbose=False):
+             peak_threads=(256, 512), line_units=(1, 8, 16), verbose=False):
     """Tune ``bd``'s device plan on (a sample of) the device tensor ``pats``; returns
-    ``{"gather_pack", "gather_threads", "row_segments", "stage_pitch", "peak_block",
+    ``{"gather_pack", "gather_threads", "row_segments", "stage_pitch", "line_unit", "peak_block",
     "peak_tile", "peak_threads", "gather_ms", "peak_ms"}`` (``row_segments`` 0: the plan's
     automatic choice; ``peak_block`` 0 where the tiled peak kernel runs, whose
     ``peak_tile``/``peak_threads`` are 0 where the streamed kernel runs)."""
@@ -50,6 +50,7 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
         plan.set_tuning(best["gather_pack"], best["gather_threads"], best["peak_tile"], best["peak_threads"])
         plan.set_row_segments(best["row_segments"])
         plan.set_stage_pitch(best["stage_pitch"])
+        plan.set_line_unit(best["line_unit"])
         if best["peak_block"]:
             plan.set_peak_block(best["peak_block"])
         return dict(best)
@@ -98,6 +99,20 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
             best_s = (ms, pitch)
     plan.set_stage_pitch(best_s[1])
     best_g = (min(best_g[0], best_s[0]),) + best_g[1:]
+    # line order in units of neighbouring cells (whole output sectors against padding)
+    best_u = None
+    for unit in line_units:
+        try:
+            plan.set_line_unit(unit)
+            ms = _time(lambda: plan.radon(xx, flags=flags, out=rad), reps)
+        except _hip.HipLibraryError:
+            continue
+        if verbose:
+            print(f"  gather line unit={unit}: {ms:.4f} ms")
+        if best_u is None or ms < best_u[0]:
+            best_u = (ms, unit)
+    plan.set_line_unit(best_u[1])
+    best_g = (min(best_g[0], best_u[0]),) + best_g[1:]
     plan.radon(xx, flags=flags, out=rad)
     best_p = None
     block = plan.stream_block() if getattr(plan, "peak_kind", 0) != 1 else None
@@ -118,7 +133,8 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
                 best_p = (ms, c)
         plan.set_peak_block(best_p[1])
         best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "row_segments": best_g[3],
-                "stage_pitch": best_s[1], "peak_block": best_p[1], "peak_tile": 0, "peak_threads": 0,
+                "stage_pitch": best_s[1], "line_unit": best_u[1], "peak_block": best_p[1], "peak_tile": 0,
+                "peak_threads": 0,
                 "gather_ms": best_g[0], "peak_ms": best_p[0]}
         _CACHE[key] = best
         return dict(best)
@@ -134,7 +150,7 @@ def autotune(bd, pats, gpu_id=None, sample=2048, reps=3, gather_packs=(1, 2, 4, 
             best_p = (ms, pt, pth)
     plan.set_tuning(0, 0, best_p[1], best_p[2])
     best = {"gather_pack": best_g[1], "gather_threads": best_g[2], "row_segments": best_g[3],
-            "stage_pitch": best_s[1], "peak_block": 0, "peak_tile": best_p[1],
+            "stage_pitch": best_s[1], "line_unit": best_u[1], "peak_block": 0, "peak_tile": best_p[1],
             "peak_threads": best_p[2], "gather_ms": best_g[0], "peak_ms": best_p[0]}
     _CACHE[key] = best
     return dict(best)

@@ -77,9 +77,10 @@ class DevicePlan:
             self.handle = None
 
     def info(self):
-        out = (_hip.ctypes.c_int64 * 8)()
+        out = (_hip.ctypes.c_int64 * 10)()
         _hip.check(self.lib.ebsd_plan_info(self.handle, out))
-        keys = ("cells", "pairs", "batches", "peak_tile", "peak_tiles", "peak_lds", "bg_mode", "cus")
+        keys = ("cells", "pairs", "batches", "peak_tile", "peak_tiles", "peak_lds", "bg_mode", "cus", "slots",
+                "line_unit")
         return dict(zip(keys, list(out)))
 
     def set_background(self, bg, exact):
@@ -106,6 +107,11 @@ class DevicePlan:
     def set_row_segments(self, segments):
         """Row segments of the integer Radon gather (0 = automatic, 1..8); identical results."""
         _hip.check(self.lib.ebsd_plan_set_row_segments(self.handle, int(segments)))
+
+    def set_line_unit(self, unit):
+        """Gather line order in units of ``unit`` neighbouring cells (0 = default 8);
+        identical results, fewer partial output sectors for larger units."""
+        _hip.check(self.lib.ebsd_plan_set_line_unit(self.handle, int(unit)))
 
     def set_peak_kernel(self, kind):
         """0 automatic, 1 theta-tiled work-group kernel, 2 streamed wavefront kernel."""

@@ -72,6 +72,7 @@ struct ebsd_plan {
     int H = 0, W = 0, nPx = 0, nT = 0, nR = 0, nCells = 0;
     // compiled gather plan
     int spitch = 0;              // LDS words per staged pattern row (>= W); plan entries use it
+    int lunit = 8;               // lines ordered in units of this many neighbouring cells
     std::vector<int> h_cnt;      // per cell entry count
     std::vector<int64_t> h_off;  // per cell first entry in h_ent
     std::vector<uint16_t> h_ent; // per cell valid entries, plan order
@@ -2089,8 +2090,40 @@ static std::vector<int> row_counts(const ebsd_plan *p, int r0, int r1)
     return cs;
 }
 
-// The lines' entries in staged rows [r0, r1), lines in the given order (or sorted by their
-// group count there) and cut into 64-line batches of 8-entry groups (one lane per line); entries
+// The gather's line order: lines in units of U neighbouring cells (U x 4 bytes of a
+// pattern's Radon; U = 8 is one 32-byte write sector), the units by their largest group count
+// in each row range (descending, the first range first) and then in cell order.  A batch's
+// store of one pattern then writes whole sectors: with single lines sorted by group count a
+// batch's 64 values fall into about twice as many 32-byte sectors as they fill, and the
+// sectors are written piecewise (1.7x the Radon's bytes reach HBM).  The price is padding:
+// a unit's lines run to its longest line's group count.  A partial last unit is filled with
+// empty lanes (-1) so that no unit straddles a 64-line batch.
+static std::vector<int> unit_order(int nCells, int U, const std::vector<std::vector<int>> &grp)
+{
+    const int nu = (nCells + U - 1) / U;
+    const size_t S = grp.size();
+    std::vector<int> key((size_t)nu * S, 0);
+    for (int c = 0; c < nCells; ++c)
+        for (size_t s = 0; s < S; ++s) {
+            int &k = key[(size_t)(c / U) * S + s];
+            k = std::max(k, grp[s][c]);
+        }
+    std::vector<int> units(nu);
+    for (int u = 0; u < nu; ++u) units[u] = u;
+    std::stable_sort(units.begin(), units.end(), [&](int x, int y) {
+        for (size_t s = 0; s < S; ++s)
+            if (key[(size_t)x * S + s] != key[(size_t)y * S + s]) return key[(size_t)x * S + s] > key[(size_t)y * S + s];
+        return false;
+    });
+    std::vector<int> order;
+    order.reserve((size_t)nu * U);
+    for (int u : units)
+        for (int j = 0; j < U; ++j) order.push_back(u * U + j < nCells ? u * U + j : -1);
+    return order;
+}
+
+// The lines' entries in staged rows [r0, r1), lines in the given order (or in units ordered by
+// their group count there) and cut into 64-line batches of 8-entry groups (one lane per line; -1 an empty lane); entries
 // are LDS word indices of the staged pixels, (y - r0) * spitch + x.  Every line has a lane in
 // every row range (a line with no pixel there adds zero), and the lane keeps the count of the
 // whole line.  The device streams of a work-group size are laid out on first use.
@@ -2104,7 +2137,7 @@ static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const s
     if (g.nwords > 65535) return fail(-2, "ebsd_plan: staged pattern exceeds 65,535 LDS words");
     const int e0 = r0 * W, e1 = r1 * W;
     const std::vector<int> cs = row_counts(p, r0, r1);
-    std::vector<int> order(nCells);
+    std::vector<int> order;
     if (given) {
         order = *given;
     } else {
@@ -2112,10 +2145,12 @@ static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const s
         // counts pad alike, and a batch of neighbouring cells stores its line values into
         // few cache lines (0.28 -> 0.25 ms per 2,048 120x120 u16 patterns against a sort by
         // pixel count)
-        for (int i = 0; i < nCells; ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return (cs[x] + 7) / 8 > (cs[y] + 7) / 8; });
+        std::vector<std::vector<int>> grp(1, cs);
+        for (int &c : grp[0]) c = (c + 7) / 8;
+        order = unit_order(nCells, p->lunit, grp);
     }
-    g.nbatch = (nCells + 63) / 64;
+    const int nl = (int)order.size();
+    g.nbatch = (nl + 63) / 64;
     std::vector<uint2> bhdr(g.nbatch);
     std::vector<int2> lanes((size_t)g.nbatch * 64);
     std::vector<uint16_t> gidx;
@@ -2124,14 +2159,14 @@ static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const s
         int mx = 0;
         for (int l = 0; l < 64; ++l) {
             const int i = b * 64 + l;
-            if (i < nCells) mx = std::max(mx, cs[order[i]]);
+            if (i < nl && order[i] >= 0) mx = std::max(mx, cs[order[i]]);
         }
         const int ngrp = std::max(1, (mx + 7) / 8); // an empty batch still writes its cells
         bhdr[b] = make_uint2(blocks, (uint32_t)ngrp);
         gidx.resize(gidx.size() + (size_t)ngrp * 64 * 8, (uint16_t)g.nwords);
         for (int l = 0; l < 64; ++l) {
             const int i = b * 64 + l;
-            if (i >= nCells) { lanes[(size_t)b * 64 + l] = make_int2(-1, 0); continue; }
+            if (i >= nl || order[i] < 0) { lanes[(size_t)b * 64 + l] = make_int2(-1, 0); continue; }
             const int cell = order[i];
             lanes[(size_t)b * 64 + l] = make_int2(cell, p->h_cnt[cell]);
             int k = 0;
@@ -2160,9 +2195,10 @@ static int compile_seg(ebsd_plan *p, ebsd_plan::GSeg &g, int r0, int r1, const s
 
 // S row segments of (nearly) equal height, segment s holding rows [s H / S, (s + 1) H / S).
 // All segments share one line order, so that a line keeps its batch and lane -- its partial
-// sum's position -- from segment to segment: lines sorted by their group counts per segment,
-// segment 0 first (descending), which keeps the lines of a batch of similar length in every
-// segment (about 96% of the entry slots used at 240x240 in two segments, 87% in four).
+// sum's position -- from segment to segment: units of lines sorted by their group counts per
+// segment, segment 0 first (descending), which keeps the lines of a batch of similar length in
+// every segment (about 96% of the entry slots used at 240x240 in two segments, 87% in four,
+// with single-line units).
 static int compile_rows(ebsd_plan *p, int S)
 {
     if (S < 2 || S > 8) return fail(-1, "row segments must be in 2..8");
@@ -2176,13 +2212,7 @@ static int compile_rows(ebsd_plan *p, int S)
         q[s] = row_counts(p, s * p->H / S, (s + 1) * p->H / S);
         for (int &c : q[s]) c = (c + 7) / 8;
     }
-    std::vector<int> order(nCells);
-    for (int i = 0; i < nCells; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        for (int s = 0; s < S; ++s)
-            if (q[s][x] != q[s][y]) return q[s][x] > q[s][y];
-        return false;
-    });
+    const std::vector<int> order = unit_order(nCells, p->lunit, q);
     rows.resize(S);
     for (int s = 0; s < S; ++s) {
         const int rc = compile_seg(p, rows[s], s * p->H / S, (s + 1) * p->H / S, &order);
@@ -2409,6 +2439,23 @@ extern "C" int ebsd_plan_set_stage_pitch(ebsd_plan *p, int pitch)
     return rc;
 }
 
+extern "C" int ebsd_plan_set_line_unit(ebsd_plan *p, int unit)
+{
+    if (!p) return fail(-1, "ebsd_plan_set_line_unit: null plan");
+    if (check_unpinned(p, "ebsd_plan_set_line_unit")) return -1;
+    if (unit <= 0) unit = 8;
+    if (unit > 64 || (unit & (unit - 1))) return fail(-1, "ebsd_plan_set_line_unit: unit must be a power of two <= 64");
+    if (unit == p->lunit) return 0;
+    const int old = p->lunit;
+    p->lunit = unit;
+    const int rc = compile_groups(p);
+    if (rc) {
+        p->lunit = old;
+        (void)compile_groups(p);
+    }
+    return rc;
+}
+
 extern "C" int ebsd_plan_set_row_segments(ebsd_plan *p, int segments)
 {
     if (!p) return fail(-1, "ebsd_plan_set_row_segments: null plan");
@@ -2491,6 +2538,10 @@ extern "C" int ebsd_plan_info(const ebsd_plan *p, int64_t *o)
     o[5] = (int64_t)peak_lds_layout(p->g).total * 4;
     o[6] = p->bg_mode;
     o[7] = p->ncu;
+    int64_t slots = 0;
+    for (const uint2 &h : p->full.h_bhdr) slots += (int64_t)h.y * 64 * 8;
+    o[8] = slots;
+    o[9] = p->lunit;
     return 0;
 }
 
@@ -2841,7 +2892,7 @@ extern "C" int ebsd_find_bands_noalloc(ebsd_plan *p, const void *d_pats, int dty
     const bool seg = S > 1 && (int)p->rowsets[S].size() == S;
     // patterns per chunk: their Radons, then (row segments) their partial sums, in the scratch
     const size_t per = (size_t)p->nCells * sizeof(float);
-    const size_t npos = (size_t)((p->nCells + 63) / 64) * 64;
+    const size_t npos = seg ? (size_t)p->rowsets[S][0].nbatch * 64 : 0;
     const size_t pbytes = seg ? npos * (flt ? 8 : 4) : 0;
     int64_t chunk = (int64_t)(kScratchBytes / (per + pbytes));
     if (seg) chunk = chunk / 8 * 8; // whole packs of up to 8 patterns share the partial sums

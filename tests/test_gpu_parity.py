@@ -311,6 +311,51 @@ def test_float_row_segments_bit_identical(shape, dtype, mode):
     assert np.array_equal(plan.radon(x, flags=flags).cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("shape,dtype,kw,mode,segs", [
+    ((120, 120), "uint16", {}, "fast", (0,)),
+    ((120, 120), "uint8", dict(nTheta=190, nRho=95), None, (0, 2)),        # 18,050 cells: a partial unit
+    ((240, 240), "uint16", dict(nTheta=360, nRho=180, nBands=12), "fast", (0, 2, 3)),
+    ((240, 240), "uint16", dict(nTheta=360, nRho=180, nBands=12), "exact", (0,)),   # float row segments
+    ((61, 67), "float32", {}, None, (0,)),
+])
+def test_line_units_identical(shape, dtype, kw, mode, segs):
+    """The gather's line order in units of 1..64 neighbouring cells (the default 8) changes
+    which lanes and batches carry which lines, never a sum: the Radon (whole patterns and row
+    segments, integer and float sums, a cell count that leaves a partial unit) and the bands
+    equal those of single-line units bit for bit; the plan reports each unit's padded slots."""
+    print(f"line units {shape} {dtype} {kw} {mode} segments {segs}", flush=True)
+    bd, ref = make_pair(shape, **kw)
+    if mode:
+        bd.background_mode = mode
+        bd.backgroundsub = np.linspace(-3.0, 4.5, shape[0] * shape[1]).reshape(shape) + 0.25
+    n = 6 if shape[0] == 240 else 21
+    pats = _dev(_pats(n, shape, dtype, seed=61))
+    plan = bd.device_plan()
+    x, flags = bd._prepare(plan, pats)
+    plan.set_line_unit(1)
+    want = {S: (plan.set_row_segments(S), plan.radon(x, flags=flags).cpu().numpy())[1] for S in segs}
+    plan.set_row_segments(0)
+    bands = {k: v.cpu().numpy() for k, v in bd.find_bands_device(pats).items()}
+    slots = {1: plan.info()["slots"]}
+    for unit in (2, 8, 16, 64, 0):
+        plan.set_line_unit(unit)
+        info = plan.info()
+        assert info["line_unit"] == (unit or 8) and info["slots"] >= slots[1]
+        slots[unit] = info["slots"]
+        for S in segs:
+            plan.set_row_segments(S)
+            assert np.array_equal(plan.radon(x, flags=flags).cpu().numpy(), want[S]), (unit, S)
+        plan.set_row_segments(0)
+        got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(pats).items()}
+        for k in bands:
+            assert np.array_equal(got[k], bands[k]), (unit, k)
+    print(f"  entry slots per pattern by unit: {slots}", flush=True)
+    from pyebsdindex_amd._hip import HipLibraryError
+    for bad in (3, 128):
+        with pytest.raises(HipLibraryError):
+            plan.set_line_unit(bad)
+
+
 def test_autotune_keeps_results():
     """The autotuner (pack size, work-group size, row segments, row pitch, peak tile) only
     changes launch shapes: the Radon and the bands after tuning equal those before."""
