@@ -72,6 +72,30 @@ int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t P, const i
                int r0, int c0, int nrc, int ncc, void *d_out, int out_dtype, float *d_weights,
                void *d_work, size_t work_bytes, void *stream);
 
+/* ---- the options of NLPAR.calcnlpar (nlpar_cpu.py:283-540) ---------------------------- */
+
+/* Smallest pixel value of n patterns, as float32, into *d_min (device): the file minimum
+ * that stem_scale subtracts (NLPAR._getdatamaxmin, nlpar_cpu.py:948-969). */
+int ebsd_nlpar_min(const void *d_pats, int dtype, int64_t n, int64_t P, float *d_min, void *stream);
+
+/* The float32 patterns calcnlpar hands its kernels (nlpar_cpu.py:470-484): the values as
+ * float32; stem != 0: sqrt(v - dmin) in float32 (:472-476); d_back (P float64, or NULL):
+ * minus the fitted background, float32(v - back) (:481-482, NLPAR.backsub :679-735).
+ * d_out: n*P float32 (may not alias d_in unless dtype is float32). */
+int ebsd_nlpar_prepare(const void *d_in, int dtype, int64_t n, int64_t P, int stem, float dmin,
+                       const double *d_back, float *d_out, void *stream);
+
+/* d_acc[p] += d_x[i*P + p] for i = 0..n-1 in order, float32: the pixel sums of numpy's mean
+ * over the patterns (np.mean(data, axis=0), nlpar_cpu.py:697), accumulated across calls. */
+int ebsd_nlpar_colsum(const float *d_x, int64_t n, int64_t P, float *d_acc, void *stream);
+
+/* Averaged float32 patterns to the output file's values (nlpar_cpu.py:511-526): stem != 0:
+ * v*v + dmin (:511-513); rescale_top > 0: per pattern v -= min, v *= top / max (float32,
+ * :516-521); then out_dtype float32, or uint8/uint16 clipped to the type's range and
+ * truncated (pat_flt2int 'clip'; NaN from a flat rescaled pattern becomes 0). */
+int ebsd_nlpar_finish(const float *d_x, int64_t n, int64_t P, int stem, float dmin, float rescale_top,
+                      void *d_out, int out_dtype, void *stream);
+
 const char *ebsd_nlpar_last_error(void);
 
 #ifdef __cplusplus

@@ -8,11 +8,17 @@
   small scans: the normalised distance (d^2 - n (s_a^2 + s_b^2)) / ((s_a^2 + s_b^2)
   sqrt(2 n)), weights exp(-max(d - dthresh, 0) / lambda^2) with weight 1 on the centre.
 * ``makeautomask``: nlpar_cpu.py:737-751.
+* ``calcnlpar_options``: calcnlpar's data path with its options (nlpar_cpu.py:470-526):
+  stem_scale (file minimum, sqrt, square back), backsub (``background`` of the scan's mean,
+  ``backsub_fit`` = NLPAR.backsub :679-735), rescale, then the clip to the output type.
 
 How they are pinned: tests/test_oracle_nlpar.py (closed forms: constant-offset
 neighbours give sigma = |c| / sqrt(2); lambda -> infinity gives the shifted-window mean,
 lambda -> 0 the pattern itself; saturated pixels drop out of the counts; C against
-float64 within float32 rounding).
+float64 within float32 rounding).  The options (tests/test_oracle_nlpar_options.py):
+lambda -> 0 with stem_scale returns the patterns within one count; rescale maps every
+pattern onto [0, top]; numpy's mean over the patterns is the sequential float32 sum over
+them; the background fit recovers an exact Gaussian-on-a-plane surface.
 """
 import ctypes
 
@@ -141,3 +147,63 @@ def nlpar64(data, nrows, ncols, indices, sig, lam=0.7, sr=3, dthresh=0.0, satura
             ws = np.array(ws) / np.sum(ws)
             out[p] = ws @ full[qs]
     return out
+
+
+def backsub_fit(mean, mask):
+    """NLPAR.backsub's fit (nlpar_cpu.py:679-735) to a mean pattern: a 2-D Gaussian on a
+    plane by scipy's curve_fit at the mask's pixels, from the reference's starting point,
+    minus its mean; the zero-mean mean pattern when the fit does not converge."""
+    import scipy.optimize as opt
+    back = np.asarray(mean)
+    ny, nx = back.shape
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    x, y = xx.ravel().astype(float), yy.ravel().astype(float)
+
+    def g(M, a, x0, y0, sx, sy, c, d, e):
+        return a * np.exp(-((M[0] - x0) ** 2) / (2.0 * sx ** 2) - ((M[1] - y0) ** 2) / (2.0 * sy ** 2)) \
+            + c + d * M[0] + e * M[1]
+
+    sel = np.flatnonzero(np.asarray(mask).ravel())
+    zz = back.ravel()[sel]
+    iy, ix = np.unravel_index(np.argmax(back), back.shape)
+    start = [zz.max() - zz.min(), ix, iy, nx / 2.355, ny / 2.355, zz.min(), 0, 0]
+    try:
+        par, _ = opt.curve_fit(g, np.vstack((x[sel], y[sel])), zz, start)
+        surface = g(np.vstack((x, y)), *par).reshape(ny, nx)
+    except RuntimeError:
+        surface = back.copy()
+    return surface - np.mean(surface)
+
+
+def calcnlpar_options(pats, nrows, ncols, indices, lam=0.7, sr=3, stem_scale=False, backsub=False,
+                      rescale_top=None, out_dtype=np.uint8, saturation_protect=True, mask=None):
+    """The averaged patterns calcnlpar writes for a scan that fits one chunk
+    (nlpar_cpu.py:470-526) with its options, from the C restatement of the kernels: the
+    file's values as float32; stem_scale: sqrt(v - min); backsub: minus ``backsub_fit`` of
+    the mean pattern (numpy's float32 mean over the patterns); sigma and the averages on
+    that data; stem_scale: squared plus the minimum; rescale: each pattern shifted to 0 and
+    scaled by float32(top) / max; clipped to the output type and truncated."""
+    d = np.asarray(pats).astype(np.float32).reshape(nrows * ncols, -1)
+    H, W = np.asarray(pats).shape[-2:]
+    dmin = None
+    if stem_scale:
+        dmin = d.min()
+        d = np.sqrt(d - dmin).astype(np.float32)
+    if backsub:
+        mean = np.mean(d.reshape(-1, H, W), axis=0)
+        fit = backsub_fit(mean, mask)
+        d = d.reshape(-1, H, W)
+        d -= fit
+        d = d.reshape(nrows * ncols, -1)
+    sg = sigma(d, nrows, ncols, indices, nn=1, saturation_protect=saturation_protect)[0]
+    out = nlpar(d, nrows, ncols, indices, sg, lam=lam, sr=sr, saturation_protect=saturation_protect)
+    out = np.asarray(out, dtype=np.float32).reshape(nrows * ncols, H, W)
+    if stem_scale:
+        out = out ** 2 + dmin
+    if rescale_top is not None:
+        for i in range(out.shape[0]):
+            t = out[i]
+            t -= t.min()
+            t *= np.float32(rescale_top) / t.max()
+    top = np.iinfo(out_dtype).max
+    return out.clip(0, top).astype(out_dtype)

@@ -674,6 +674,114 @@ __global__ void k_key_to_float(unsigned *key, float *out)
 
 __global__ void k_zero_key(unsigned *__restrict__ key) { *key = 0u; }
 
+// ---- the options of calcnlpar (nlpar_cpu.py:283-540): stem_scale, backsub, rescale ----
+
+// the smallest value as an order-preserving key (atomicMin from all ones)
+template <typename T>
+__global__ void k_min(const T *__restrict__ pats, int64_t n, unsigned *__restrict__ key)
+{
+    unsigned best = 0xffffffffu;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride)
+        best = min(best, fkey(ldf(pats, t)));
+    for (int off = 32; off >= 1; off >>= 1) best = min(best, (unsigned)__shfl_xor((int)best, off, 64));
+    __shared__ unsigned s_best[kWaves];
+    if ((threadIdx.x & 63) == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned b = s_best[0];
+        for (int w = 1; w < kWaves; ++w) b = min(b, s_best[w]);
+        atomicMin(key, b);
+    }
+}
+
+__global__ void k_ones_key(unsigned *__restrict__ key) { *key = 0xffffffffu; }
+
+// float32 patterns as calcnlpar feeds them to its kernels: the file's values as float32;
+// stem_scale: sqrt(v - dmin) (a float32 difference, the correctly rounded float32 root --
+// the float64 root rounded to float32 is that, a 53-bit result rounded to 24 bits being
+// innocuous for sqrt); backsub: float32(float64(v) - back[pixel]), as the in-place
+// subtraction of a float64 background from a float32 array rounds.
+template <typename T>
+__global__ void k_prepare(const T *__restrict__ in, int64_t total, int64_t P, int stem, float dmin,
+                          const double *__restrict__ back, float *__restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += stride) {
+        float v = ldf(in, t);
+        if (stem) {
+            const float d = v - dmin;
+            v = (float)sqrt((double)d);
+        }
+        if (back) v = (float)((double)v - back[t % P]);
+        out[t] = v;
+    }
+}
+
+// column sums over patterns in order, float32, one thread per pixel: acc[p] += x[i][p] for
+// i = 0..n-1 -- numpy's mean over axis 0 of a float32 (n, H, W) array adds row after row
+__global__ void k_colsum(const float *__restrict__ x, int64_t n, int64_t P, float *__restrict__ acc)
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float a = acc[p];
+    for (int64_t i = 0; i < n; ++i) a = a + x[i * P + p];
+    acc[p] = a;
+}
+
+// One work-group per pattern: stem_scale's inverse v*v + dmin (float32, not fused), then
+// rescale's v -= min(v); v *= f32(top / max(v)) over the pattern (float32, as
+// nlpar_cpu.py:511-521), then the output type: float32, or clipped to [0, type max] and
+// truncated (pat_flt2int 'clip', ebsd_pattern.py:159-162); NaN (a flat pattern rescaled)
+// becomes 0.
+template <typename O>
+__global__ __launch_bounds__(kThreads) void k_finish(const float *__restrict__ x, int64_t P, int stem, float dmin,
+                                                     float top, O *__restrict__ out, float omax)
+{
+    const float *src = x + blockIdx.x * P;
+    O *dst = out + blockIdx.x * P;
+    __shared__ unsigned s_key[kWaves];
+    float mn = 0.0f, scale = 1.0f;
+    if (top > 0.0f) {
+        unsigned kmin = 0xffffffffu;
+        for (int64_t i = threadIdx.x; i < P; i += kThreads) {
+            float v = src[i];
+            if (stem) v = __fadd_rn(__fmul_rn(v, v), dmin);
+            kmin = min(kmin, fkey(v));
+        }
+        for (int off = 32; off >= 1; off >>= 1) kmin = min(kmin, (unsigned)__shfl_xor((int)kmin, off, 64));
+        if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = kmin;
+        __syncthreads();
+        kmin = s_key[0];
+        for (int w = 1; w < kWaves; ++w) kmin = min(kmin, s_key[w]);
+        mn = fkey_inv(kmin);
+        __syncthreads();
+        unsigned kmax = 0u;
+        for (int64_t i = threadIdx.x; i < P; i += kThreads) {
+            float v = src[i];
+            if (stem) v = __fadd_rn(__fmul_rn(v, v), dmin);
+            kmax = max(kmax, fkey(__fsub_rn(v, mn)));
+        }
+        for (int off = 32; off >= 1; off >>= 1) kmax = max(kmax, (unsigned)__shfl_xor((int)kmax, off, 64));
+        if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = kmax;
+        __syncthreads();
+        kmax = s_key[0];
+        for (int w = 1; w < kWaves; ++w) kmax = max(kmax, s_key[w]);
+        scale = (float)((double)top / (double)fkey_inv(kmax));
+    }
+    for (int64_t i = threadIdx.x; i < P; i += kThreads) {
+        float v = src[i];
+        if (stem) v = __fadd_rn(__fmul_rn(v, v), dmin);
+        if (top > 0.0f) v = __fmul_rn(__fsub_rn(v, mn), scale);
+        if (omax > 0.0f) {
+            v = v != v ? 0.0f : fminf(fmaxf(v, 0.0f), omax);
+            dst[i] = (O)v;
+        } else {
+            dst[i] = (O)v;
+        }
+    }
+}
+
 } // namespace
 
 static int grid_for(int64_t total)
@@ -702,6 +810,86 @@ extern "C" int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t 
             hipLaunchKernelGGL(k_max<float>, dim3(g), dim3(kThreads), 0, s, (const float *)d_pats, total, key);
     }
     hipLaunchKernelGGL(k_key_to_float, dim3(1), dim3(1), 0, s, key, d_max);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_nlpar_min(const void *d_pats, int dtype, int64_t n, int64_t P, float *d_min, void *stream)
+{
+    if (n < 0 || P <= 0) return fail(-1, "ebsd_nlpar_min: bad size");
+    if (!d_pats || !d_min) return fail(-1, "ebsd_nlpar_min: null pointer");
+    if (dtype < 0 || dtype > 2) return fail(-1, "ebsd_nlpar_min: dtype is 0 (u8), 1 (u16) or 2 (f32)");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned *key = reinterpret_cast<unsigned *>(d_min);
+    hipLaunchKernelGGL(k_ones_key, dim3(1), dim3(1), 0, s, key);
+    const int64_t total = n * P;
+    if (total > 0) {
+        const unsigned g = (unsigned)std::min<int64_t>(1024, (total + 16 * kThreads - 1) / (16 * kThreads));
+        if (dtype == EBSD_NL_U8)
+            hipLaunchKernelGGL(k_min<uint8_t>, dim3(g), dim3(kThreads), 0, s, (const uint8_t *)d_pats, total, key);
+        else if (dtype == EBSD_NL_U16)
+            hipLaunchKernelGGL(k_min<uint16_t>, dim3(g), dim3(kThreads), 0, s, (const uint16_t *)d_pats, total, key);
+        else
+            hipLaunchKernelGGL(k_min<float>, dim3(g), dim3(kThreads), 0, s, (const float *)d_pats, total, key);
+    }
+    hipLaunchKernelGGL(k_key_to_float, dim3(1), dim3(1), 0, s, key, d_min);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_nlpar_prepare(const void *d_in, int dtype, int64_t n, int64_t P, int stem, float dmin,
+                                  const double *d_back, float *d_out, void *stream)
+{
+    if (n < 0 || P <= 0) return fail(-1, "ebsd_nlpar_prepare: bad size");
+    if (!d_in || !d_out) return fail(-1, "ebsd_nlpar_prepare: null pointer");
+    if (dtype < 0 || dtype > 2) return fail(-1, "ebsd_nlpar_prepare: dtype is 0 (u8), 1 (u16) or 2 (f32)");
+    const int64_t total = n * P;
+    if (total == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned g = (unsigned)std::min<int64_t>(8192, (total + kThreads - 1) / kThreads);
+    if (dtype == EBSD_NL_U8)
+        hipLaunchKernelGGL(k_prepare<uint8_t>, dim3(g), dim3(kThreads), 0, s, (const uint8_t *)d_in, total, P, stem,
+                           dmin, d_back, d_out);
+    else if (dtype == EBSD_NL_U16)
+        hipLaunchKernelGGL(k_prepare<uint16_t>, dim3(g), dim3(kThreads), 0, s, (const uint16_t *)d_in, total, P,
+                           stem, dmin, d_back, d_out);
+    else
+        hipLaunchKernelGGL(k_prepare<float>, dim3(g), dim3(kThreads), 0, s, (const float *)d_in, total, P, stem,
+                           dmin, d_back, d_out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_nlpar_colsum(const float *d_x, int64_t n, int64_t P, float *d_acc, void *stream)
+{
+    if (n < 0 || P <= 0) return fail(-1, "ebsd_nlpar_colsum: bad size");
+    if (!d_x || !d_acc) return fail(-1, "ebsd_nlpar_colsum: null pointer");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       (hipStream_t)stream, d_x, n, P, d_acc);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ebsd_nlpar_finish(const float *d_x, int64_t n, int64_t P, int stem, float dmin, float rescale_top,
+                                 void *d_out, int out_dtype, void *stream)
+{
+    if (n < 0 || P <= 0) return fail(-1, "ebsd_nlpar_finish: bad size");
+    if (!d_x || !d_out) return fail(-1, "ebsd_nlpar_finish: null pointer");
+    if (out_dtype < 0 || out_dtype > 2) return fail(-1, "ebsd_nlpar_finish: out_dtype is 0 (u8), 1 (u16) or 2 (f32)");
+    if (n == 0) return 0;
+    if (n > 0x7fffffff) return fail(-1, "ebsd_nlpar_finish: too many patterns for one launch");
+    hipStream_t s = (hipStream_t)stream;
+    const float top = rescale_top > 0.0f ? rescale_top : 0.0f;
+    if (out_dtype == EBSD_NL_U8)
+        hipLaunchKernelGGL(k_finish<uint8_t>, dim3((unsigned)n), dim3(kThreads), 0, s, d_x, P, stem, dmin, top,
+                           (uint8_t *)d_out, 255.0f);
+    else if (out_dtype == EBSD_NL_U16)
+        hipLaunchKernelGGL(k_finish<uint16_t>, dim3((unsigned)n), dim3(kThreads), 0, s, d_x, P, stem, dmin, top,
+                           (uint16_t *)d_out, 65535.0f);
+    else
+        hipLaunchKernelGGL(k_finish<float>, dim3((unsigned)n), dim3(kThreads), 0, s, d_x, P, stem, dmin, top,
+                           (float *)d_out, 0.0f);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -17,11 +17,15 @@ scan into several chunks, the points next to a chunk boundary use the noise esti
 the chunk processed so far (nlpar_cpu.py:509-516); that order-dependent value is not
 reproduced.  The saturation threshold is taken over the whole scan.
 
-Options of the reference not built here raise NotImplementedError: ``stem_scale``,
-``backsub`` and ``rescale`` (they are off by default).  ``diff_offset`` is added once to
-each pattern's own weight; the reference keeps it in an array that all of its parallel
-workers share and that grows along a column (nlpar_cpu.py:890-892), which is not
-reproduced (the default, 0, is unaffected).
+The options of calcnlpar run on the device too (nlpar_cpu.py:470-526):
+``stem_scale`` takes sqrt(v - file minimum) before the averaging and squares back after;
+``backsub`` subtracts a Gaussian-on-a-plane fitted (on the host, with SciPy as the
+reference) to the mean pattern; ``rescale`` stretches each averaged pattern to the output
+type's range.  The background is fitted once, to the mean of the whole scan: the reference
+fits each of its chunks' means, which is the same for a scan that fits one chunk.
+``diff_offset`` is added once to each pattern's own weight; the reference keeps it in an
+array that all of its parallel workers share and that grows along a column
+(nlpar_cpu.py:890-892), which is not reproduced (the default, 0, is unaffected).
 """
 import ctypes
 from pathlib import Path
@@ -115,6 +119,90 @@ def pattern_max(pats, stream=None):
     if stream is not None:
         stream.synchronize()   # the reduction ran on ``stream``: done before the read
     return float(out.item())
+
+
+def pattern_min(pats, stream=None):
+    """Smallest pixel value of device patterns, as float32 (a device reduction)."""
+    torch = _torch()
+    L = _hip.load_nlpar()
+    pats = pats.contiguous()
+    out = torch.empty(1, dtype=torch.float32, device=pats.device)
+    n = pats.shape[0] if pats.dim() > 2 else 1
+    P = pats.numel() // max(n, 1)
+    _hip.check_nlpar(L.ebsd_nlpar_min(_ptr(pats), _code(pats), n, P, _ptr(out), _stream(stream, pats.device)))
+    if stream is not None:
+        stream.synchronize()
+    return float(out.item())
+
+
+def prepare_device(pats, stem_min=None, background=None, stream=None):
+    """The float32 patterns calcnlpar averages (nlpar_cpu.py:470-482): ``stem_min`` (a
+    float32 value, or None): sqrt(v - stem_min); ``background`` ((H, W) float64 device
+    tensor, or None): minus it.  Returns a new float32 device tensor of the same shape."""
+    torch = _torch()
+    L = _hip.load_nlpar()
+    x = pats.contiguous()
+    n = x.shape[0] if x.dim() > 2 else 1
+    P = x.numel() // max(n, 1)
+    out = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+    bg = None
+    if background is not None:
+        bg = background.to(torch.float64).contiguous().reshape(-1)
+        if bg.numel() != P:
+            raise ValueError("the background must have one value per pattern pixel")
+    _hip.check_nlpar(L.ebsd_nlpar_prepare(_ptr(x), _code(x), n, P, int(stem_min is not None),
+                                          float(np.float32(0.0 if stem_min is None else stem_min)),
+                                          None if bg is None else _ptr(bg), _ptr(out), _stream(stream, x.device)))
+    if stream is not None:
+        _hip.keep_for_stream(stream, x, bg, out)
+    return out
+
+
+def finish_device(x, stem_min=None, rescale_top=None, out_dtype=None, stream=None):
+    """Averaged float32 device patterns (n, H, W) to the output values (nlpar_cpu.py:511-526):
+    squared plus ``stem_min`` when given, stretched to [0, ``rescale_top``] per pattern when
+    given, then ``out_dtype`` (torch.float32, or uint8/uint16 clipped and truncated)."""
+    torch = _torch()
+    L = _hip.load_nlpar()
+    x = x.contiguous()
+    n = x.shape[0] if x.dim() > 2 else 1
+    P = x.numel() // max(n, 1)
+    odt = torch.float32 if out_dtype is None else out_dtype
+    out = torch.empty(x.shape, dtype=odt, device=x.device)
+    _hip.check_nlpar(L.ebsd_nlpar_finish(_ptr(x), n, P, int(stem_min is not None),
+                                         float(np.float32(0.0 if stem_min is None else stem_min)),
+                                         float(rescale_top or 0.0), _ptr(out), _code(out), _stream(stream, x.device)))
+    if stream is not None:
+        _hip.keep_for_stream(stream, x, out)
+    return out
+
+
+def fit_background(mean, mask):
+    """NLPAR.backsub's background (nlpar_cpu.py:679-735): a 2-D Gaussian on a plane fitted by
+    least squares to the mean pattern at the mask's pixels, started from (max - min, the
+    maximum's position, size / 2.355, min, flat), then shifted to zero mean; the mean pattern
+    itself (zero-mean) if the fit does not converge.  Returns float64 (H, W)."""
+    import scipy.optimize as opt
+    back = np.asarray(mean)
+    ny, nx = back.shape
+    x = np.broadcast_to(np.arange(nx, dtype=float).reshape(1, nx), (ny, nx)).ravel()
+    y = np.broadcast_to(np.arange(ny, dtype=float).reshape(ny, 1), (ny, nx)).ravel()
+
+    def surf(x, y, a, x0, y0, sx, sy, c, d, e):
+        return a * np.exp(-((x - x0) ** 2) / (2.0 * sx ** 2) - ((y - y0) ** 2) / (2.0 * sy ** 2)) + c + d * x + e * y
+
+    wh = np.nonzero(np.asarray(mask).ravel())[0]
+    z = back.ravel()[wh]
+    top = np.unravel_index(back.argmax(), back.shape)
+    p0 = [(z.max() - z.min()), top[1], top[0], nx / 2.355, ny / 2.355, z.min(), 0, 0]
+    try:
+        popt, _ = opt.curve_fit(lambda M, *a: surf(M[0], M[1], *a), np.vstack((x[wh], y[wh])), z, p0)
+        fit = surf(x, y, *popt).reshape(ny, nx)
+    except RuntimeError:
+        print('Warning: no convergence on back subtract ... using mean of the patterns.')
+        fit = back.copy()
+    fit -= np.mean(fit)
+    return fit
 
 
 def sigma_device(pats, nrows, ncols, indices, nn=1, saturation_protect=True, return_d2=False, mxval=None,
@@ -298,15 +386,42 @@ class NLPAR:
             self.mask = np.ones((H, W), dtype=np.uint8)
         return np.asarray(self.mask.flatten().nonzero()[0], np.int32)
 
-    def _check_options(self, stem_scale=False, backsub=False, rescale=False):
-        if stem_scale or backsub or rescale:
-            raise NotImplementedError("stem_scale, backsub and rescale are not built in this NLPAR")
+    def _options(self, f, dev, stem, backsub, bands):
+        """The device transform of the patterns for stem_scale and backsub: the file's
+        smallest value (stem_scale, nlpar_cpu.py:948-969) and the background fitted to the
+        mean of the (stem-scaled) scan (backsub, :679-735).  ``bands``: row ranges that
+        cover the scan once."""
+        torch = _torch()
+        self._rows_cache = None
+        self._opt = None
+        if not (stem or backsub):
+            return
+        dmin = None
+        if stem:
+            m = np.inf
+            for r0, r1 in bands:
+                m = min(m, pattern_min(self._read_rows(f, r0, r1, dev)))
+            dmin = np.float32(m)
+        back = None
+        if backsub:
+            H, W = int(f.patternH), int(f.patternW)
+            L = _hip.load_nlpar()
+            acc = torch.zeros(H * W, dtype=torch.float32, device=dev)
+            for r0, r1 in bands:
+                x = prepare_device(self._read_rows(f, r0, r1, dev), stem_min=dmin)
+                _hip.check_nlpar(L.ebsd_nlpar_colsum(_ptr(x), x.shape[0], H * W, _ptr(acc), _stream(None, dev)))
+            n = int(self.nrows) * int(self.ncols)
+            # np.mean divides the float32 sums by the count in float64 and rounds to float32
+            mean = (acc.cpu().numpy().astype(np.float64) / n).astype(np.float32).reshape(H, W)
+            back = torch.as_tensor(fit_background(mean, self.mask), dtype=torch.float64, device=dev)
+        self._opt = (dmin, back)
 
-    def _bands(self, f, halo):
-        """Row bands (r0, r1) covering the scan, sized to the device budget."""
+    def _bands(self, f, halo, float_rows=False):
+        """Row bands (r0, r1) covering the scan, sized to the device budget (``float_rows``:
+        the patterns are held as float32)."""
         torch = _torch()
         nrows, ncols = int(self.nrows), int(self.ncols)
-        row_bytes = ncols * f.pattern_bytes
+        row_bytes = ncols * (4 * int(f.patternW) * int(f.patternH) if float_rows else f.pattern_bytes)
         budget = self.band_bytes
         if budget is None:
             free, _ = torch.cuda.mem_get_info(self._device())
@@ -318,12 +433,15 @@ class NLPAR:
         """Rows [a, b) of the scan as device patterns (file type), through pinned memory.
         The last rows loaded are kept, so a scan that fits one band is read from the file
         once for the maximum, the sigma and the averaging passes."""
-        key = (str(f.filepath), a, b, str(dev))
+        opt = getattr(self, "_opt", None)
+        key = (str(f.filepath), a, b, str(dev), None if opt is None else id(opt))
         cached = getattr(self, "_rows_cache", None)
         if cached is not None and cached[0] == key:
             return cached[1]
         self._rows_cache = None
         x = self._read_rows(f, a, b, dev)
+        if opt is not None:
+            x = prepare_device(x, stem_min=opt[0], background=opt[1])
         self._rows_cache = (key, x)
         return x
 
@@ -345,7 +463,7 @@ class NLPAR:
 
     # -- sigma (nlpar_cpu.py:186-281) ------------------------------------------------------
     def calcsigma(self, chunksize=0, nn=1, saturation_protect=None, automask=None, stem_scale=None, verbose=0,
-                  **kwargs):
+                  _keep_options=False, **kwargs):
         """Noise estimate of every scan point; returns (sigma, d2, n2) as numpy arrays
         ((nrows, ncols), and (nrows, ncols, (2nn+1)^2) normalised distances and counts).
         ``chunksize`` > 0 bounds the device bytes of patterns per band, as the reference's
@@ -359,14 +477,17 @@ class NLPAR:
             self.automask = automask
         if stem_scale is not None:
             self.stem_scale = stem_scale
-        self._check_options(stem_scale=_scalar(self.stem_scale))
+        stem = _scalar(self.stem_scale) is True
         f = self.getinfileobj()
         dev = self._device()
         nrows, ncols = int(self.nrows), int(self.ncols)
         idx = self._indices(int(f.patternH), int(f.patternW), _scalar(self.automask))
         # bands overlap by 2 nn rows: the neighbours of a kept point then have their whole
         # neighbourhood in the band too, so the sigmas that normalise its distances are exact
-        bands = self._bands(f, 2 * nn)
+        keep = _keep_options and getattr(self, "_opt", None) is not None
+        bands = self._bands(f, 2 * nn, float_rows=stem or keep)
+        if not keep:
+            self._options(f, dev, stem, False, bands)
         mx = self._scan_max(f, bands, dev)
         win = (2 * nn + 1) ** 2
         sigma = np.zeros((nrows, ncols), np.float32)
@@ -382,6 +503,9 @@ class NLPAR:
             n2[r0:r1] = cnt[r0 - a:r1 - a].cpu().numpy()
         if verbose >= 1:
             print(sigma.min(), sigma.max())
+        if not keep:
+            self._opt = None
+            self._rows_cache = None
         return sigma, d2, n2
 
     # -- lambda (nlpar_cpu.py:94-184) ------------------------------------------------------
@@ -447,7 +571,8 @@ class NLPAR:
             self.automask = automask
         if stem_scale is not None:
             self.stem_scale = stem_scale
-        self._check_options(_scalar(self.stem_scale), backsub, rescale)
+        stem = _scalar(self.stem_scale) is True
+        backsub = backsub is True
         if filename is not None:
             self.setfile(filepath=filename)
         f = self.getinfileobj()
@@ -461,12 +586,18 @@ class NLPAR:
             self.sigma = None
         if self.sigma is not None and np.asarray(self.sigma).size == 1:
             self.sigma = np.zeros((nrows, ncols), np.float32) + np.asarray(self.sigma).reshape(-1)[0]
+        bands = self._bands(f, sr, float_rows=stem or backsub)
+        self._options(f, dev, stem, backsub, bands)
+        dmin = self._opt[0] if self._opt is not None else None
         if self.sigma is None or np.asarray(self.sigma).shape != (nrows, ncols):
-            self.sigma = self.calcsigma(nn=1, saturation_protect=satp)[0]
+            self.sigma = self.calcsigma(nn=1, saturation_protect=satp, _keep_options=True)[0]
         sigma = np.asarray(self.sigma, np.float32)
-        bands = self._bands(f, sr)
         mx = self._scan_max(f, bands, dev)
         odt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.uint16}[np.dtype(fo.filedatatype)]
+        # rescale: each averaged pattern stretched to the output type's range (:392-397)
+        top = float(np.iinfo(fo.filedatatype).max) if (rescale == True and  # noqa: E712 (as the reference)
+                                                       np.issubdtype(fo.filedatatype, np.integer)) else None
+        post = stem or top is not None
         newmax, newmin = -np.inf, np.inf
         ws = 2 * sr + 1
         if nrows < ws or ncols < ws:
@@ -479,7 +610,10 @@ class NLPAR:
             x = self._load_rows(f, a, b, dev)
             sg = torch.as_tensor(sigma[a:b], device=dev)
             out = nlpar_device(x, b - a, ncols, idx, sg, lam=lamv, sr=sr, dthresh=dth, saturation_protect=satp,
-                               diff_offset=doff, calclim=(0, r0 - a, ncols, r1 - r0), out_dtype=odt, mxval=mx)
+                               diff_offset=doff, calclim=(0, r0 - a, ncols, r1 - r0),
+                               out_dtype=torch.float32 if post else odt, mxval=mx)
+            if post:
+                out = finish_device(out, stem_min=dmin if stem else None, rescale_top=top, out_dtype=odt)
             host = out.cpu().numpy()
             newmax, newmin = max(newmax, float(host.max())), min(newmin, float(host.min()))
             fo.pat_writer(host, r0 * ncols, (r1 - r0) * ncols, fo.filedatatype)
@@ -487,6 +621,7 @@ class NLPAR:
                 print("rows complete: ", r1, "/", nrows, sep='', end='\r')
         fo.datamin, fo.datamax = newmin, newmax
         self._rows_cache = None
+        self._opt = None
         return str(fo.filepath)
 
     def auto_nlpar(self, filename=None, fileout=None, lindex=1, **kwargs):

@@ -155,6 +155,42 @@ def test_nlpar_file_round_trip(tmp_path):
     assert np.allclose(lams, ref, atol=2e-3)
 
 
+@pytest.mark.parametrize("dtype,stem,backsub,rescale", [("uint8", True, False, False), ("uint16", False, True, False),
+                                                       ("uint8", False, False, True), ("uint16", True, True, True)])
+def test_nlpar_options_match_oracle(tmp_path, dtype, stem, backsub, rescale):
+    """calcnlpar with stem_scale, backsub and rescale (nlpar_cpu.py:470-526) on the device
+    against the oracle's restatement of the same data path around the C kernels: within one
+    count on at most 0.1% of the pixels; with rescale, which multiplies the averages by
+    top / range, within 1 + 2e-4 top (the averages' 1e-4 relative agreement, scaled)."""
+    from pyebsdindex_amd.ebsd_pattern import UPFile, get_pattern_file_obj
+    from pyebsdindex_amd.nlpar import NLPAR
+    nrows, ncols, shape, sr, lam = 9, 10, (40, 40), 2, 0.8
+    print(f"nlpar options {dtype} stem={stem} backsub={backsub} rescale={rescale}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=29)
+    n = nrows * ncols
+    fn = tmp_path / ("scan.up1" if dtype == "uint8" else "scan.up2")
+    w = UPFile(fn)
+    w.patternW, w.patternH, w.nCols, w.nRows, w.xStep, w.yStep = shape[1], shape[0], ncols, nrows, 1.0, 1.0
+    w.write_header()
+    w.pat_writer(pats, 0, n, pats.dtype)
+    nl = NLPAR(str(fn), lam=lam, searchradius=sr, stem_scale=stem)
+    out = nl.calcnlpar(fileout=str(tmp_path / ("o.up1" if dtype == "uint8" else "o.up2")), backsub=backsub,
+                       rescale=rescale)
+    got = get_pattern_file_obj(out).pat_reader(0, n)[0].astype(np.int64)
+    top = np.iinfo(dtype).max
+    mask = nr.makeautomask(*shape)
+    want = nr.calcnlpar_options(pats, nrows, ncols, np.flatnonzero(mask.ravel()), lam=lam, sr=sr, stem_scale=stem,
+                                backsub=backsub, rescale_top=top if rescale else None, out_dtype=np.dtype(dtype),
+                                mask=mask).reshape(got.shape).astype(np.int64)
+    diff = np.abs(got - want)
+    print(f"  max diff {diff.max()}, pixels differing {(diff > 0).mean():.5f}", flush=True)
+    if rescale:
+        assert diff.max() <= 1 + 2e-4 * top
+    else:
+        assert diff.max() <= 1 and (diff > 0).mean() <= 1e-3
+    assert np.array_equal(nl.mask, mask)
+
+
 @pytest.mark.parametrize("ext,dtype", [(".ebsp", "uint8"), (".pat", "uint16")])
 def test_nlpar_raw_files_equal_up(tmp_path, ext, dtype):
     """NLPAR.calcnlpar on an .ebsp / .pat file writes the same patterns as on the same scan

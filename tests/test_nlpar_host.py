@@ -74,7 +74,7 @@ def test_sigma_bands_overlap_two_rows():
                 assert max(0, jj - nn) >= a and min(nrows, jj + nn + 1) <= b
 
 
-def test_output_name_and_unbuilt_options(tmp_path):
+def test_output_name(tmp_path):
     from pyebsdindex_amd.ebsd_pattern import UPFile, get_pattern_file_obj
     fn = tmp_path / "scan.up1"
     w = UPFile(fn)
@@ -90,5 +90,3 @@ def test_output_name_and_unbuilt_options(tmp_path):
     assert out.nPatterns == 6
     with pytest.raises(ValueError):
         nl.setoutfile(f, filepath=str(fn))
-    with pytest.raises(NotImplementedError):
-        nl._check_options(stem_scale=True)
