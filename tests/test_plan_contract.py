@@ -78,8 +78,11 @@ def test_plan_methods_take_the_indexers_keywords():
     p = inspect.signature(bd.radonPlan.radon2pole).parameters
     for k in ("PC", "vendor"):
         assert k in p, k
-    # GnomonicCorrection reads radonPlan.imDim (gnomonic_correction.py:71)
+    # GnomonicCorrection reads radonPlan.imDim (gnomonic_correction.py:71); the indexer's
+    # save/restore reads the pattern mask and mask index (_ebsd_index_single.py:964-965)
     assert tuple(bd.radonPlan.imDim) == (120, 120)
+    assert np.asarray(bd.radonPlan.mask).shape == (120, 120)
+    assert np.array_equal(np.asarray(bd.radonPlan.maskindex).ravel(), np.arange(120 * 120))
     # a second set-up with new dimensions, as the indexer does for a new file (:697-700).  The
     # plan keeps the previous pattern mask and mask index, as the reference's does: smaller
     # patterns without new ones are refused by the mask-index check (radon_fast.py:189-190)
