@@ -695,6 +695,7 @@ __global__ __launch_bounds__(256) void k_bandindex(BandIndexArgs a)
     if (p >= a.npats) return;
     const bool act = lane < nb;
     const int j = act ? lane : 0;
+    const int G = nb <= 16 ? 4 : (nb <= 32 ? 2 : 1); // lanes per band in the nearest-pole search
     const float *bnp = a.bn + (p * nb + j) * 3;
     const float n0 = bnp[0], n1 = bnp[1], n2 = bnp[2];
     const int fam_j = a.fam[p * nb + j];
@@ -769,13 +770,30 @@ __global__ __launch_bounds__(256) void k_bandindex(BandIndexArgs a)
                                                 A[r * 3 + 2] * B[2 * 3 + q];
                         float tp[3];
                         for (int r = 0; r < 3; ++r) tp[r] = (AB[r * 3 + 0] * n0 + AB[r * 3 + 1] * n1) + AB[r * 3 + 2] * n2;
-                        float mx = (l_pf[0] * tp[0] + l_pf[1] * tp[1]) + l_pf[2] * tp[2];
-                        int am = 0;
-                        for (int k = 1; k < npl; ++k) {
+                        // the nearest library pole of every band, G lanes per band: lane L scans
+                        // poles L % G, L % G + G, ... for band L / G, and the G lanes keep the
+                        // larger value, or on equal (or unordered) values the smaller pole --
+                        // the first index of the maximum, as the reference's scan with strict >
+                        const int bsrc = lane / G;
+                        const float s0 = __shfl(tp[0], bsrc), s1 = __shfl(tp[1], bsrc), s2 = __shfl(tp[2], bsrc);
+                        float gmx = 0.0f;
+                        int gam = -1;
+                        for (int k = lane % G; k < npl; k += G) {
                             const float *pk = l_pf + 3 * k;
-                            const float tv = (pk[0] * tp[0] + pk[1] * tp[1]) + pk[2] * tp[2];
-                            if (tv > mx) { mx = tv; am = k; }
+                            const float tv = (pk[0] * s0 + pk[1] * s1) + pk[2] * s2;
+                            if (gam < 0 || tv > gmx) { gmx = tv; gam = k; }
                         }
+                        for (int off = 1; off < G; off <<= 1) {
+                            const float omx = __shfl_xor(gmx, off);
+                            const int oam = __shfl_xor(gam, off);
+                            if (oam >= 0 && (gam < 0 || omx > gmx || (!(gmx > omx) && oam < gam))) {
+                                gmx = omx;
+                                gam = oam;
+                            }
+                        }
+                        const int gsrc = (lane < kWave / G ? lane : 0) * G;
+                        float mx = __shfl(gmx, gsrc);
+                        const int am = __shfl(gam, gsrc);
                         if (mx < -1.0f) mx = -1.0f;
                         if (mx > 1.0f) mx = 1.0f;
                         const double angTry = (double)acosf_cr(mx) * kRadeg;

@@ -47,6 +47,25 @@ def test_bandindex_matches_oracle(phase, nb, noise, invalid):
     assert np.median(err) < 0.5
 
 
+@pytest.mark.parametrize("nb_lib,nb_rand", [(25, 15), (20, 28)])
+def test_bandindex_many_bands(nb_lib, nb_rand):
+    """Above 32 bands (one lane per band in the nearest-pole search) and between 17 and 32
+    (two lanes per band): library bands mixed with random normals, in shuffled band order,
+    against the oracle -- assignment identical, outputs as in the other cases."""
+    from pyebsdindex_amd import tripletvote as tv
+    print(f"bandindex many bands {nb_lib}+{nb_rand}", flush=True)
+    ind = tv.addphase("FCC")
+    bn, pole, fam, Rt = phase_bands(ind, 120, nb_lib, seed=31, noise_deg=0.2)
+    rng = np.random.default_rng(8)
+    extra = rng.standard_normal((bn.shape[0], nb_rand, 3))
+    extra /= np.linalg.norm(extra, axis=2, keepdims=True)
+    allb = np.concatenate([bn, extra.astype(np.float32)], axis=1)
+    order = rng.permutation(allb.shape[1])
+    allb = np.ascontiguousarray(allb[:, order])
+    inten = rng.uniform(0.2, 1.0, allb.shape[:2]).astype(np.float32)
+    _compare(ind, allb, inten, min_equal=0.99)
+
+
 def test_bandindex_device_tensors_and_edges():
     """Device input gives device output equal to the host call; no patterns; a pattern whose
     bands are all parallel (no pair is tried: identity quaternion, fit 180, cm -1)."""
