@@ -37,7 +37,7 @@ int ebsd_nlpar_max(const void *d_pats, int dtype, int64_t n, int64_t P, float *d
  * patterns of P pixels.  With uint8 or uint16 patterns (P a multiple of 4) and this
  * workspace the kernels compute the distances as exact integer sums over packed words
  * (uint8: byte dot products), taking out the pixels saturated in either pattern from a
- * per-pattern list; patterns with more than 32 saturated pixels, float32 patterns, or no
+ * per-pattern list; patterns with more than 64 saturated pixels, float32 patterns, or no
  * workspace (NULL): pixel by pixel.  No counterpart in the reference (its Numba kernels
  * allocate nothing on a device). */
 size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P);

@@ -16,7 +16,7 @@
 // the centre value once per pixel and each neighbour's, and sum the squared float32
 // differences in float64 (the reference sums sequentially in float32).  uint8 path (with a
 // workspace): a prepass stores each pattern's masked sum of squares and the list of its
-// masked pixels at or above the saturation threshold (up to 32); then
+// masked pixels at or above the saturation threshold (up to 64, one per lane); then
 // d2 = sum a^2 + sum b^2 - 2 sum ab exactly, with sum ab from v_dot4_u32_u8 over 4 pixels of
 // the byte-masked centre and the neighbour per instruction, minus the exact terms of the
 // pixels saturated in either pattern (which the reference leaves out).  Pairs that touch a
@@ -104,7 +104,10 @@ template <typename V> __device__ __forceinline__ V wave_sum(V x)
 // uint8 workspace: byte mask (0xff on mask pixels), per-pattern masked sum of squares, the
 // number of masked pixels at or above the saturation threshold (kNoList when more than
 // kSatList) and the list of those pixels.  Null mask4 = no workspace (per-pixel path only).
-constexpr int kSatList = 32;
+// 64: one list entry per lane in the corrections.  (With 32, the patterns of the bench's
+// clipped-Poisson scan that hold 33-48 saturated pixels sent every group of pairs they were
+// in down the per-pixel path: 2.74 against 2.50 ms per 65,536 points with 64.)
+constexpr int kSatList = 64;
 constexpr uint8_t kNoList = 255;
 struct U8Work {
     const uint32_t *mask4; // P/4 words of the byte mask

@@ -212,7 +212,7 @@ def sigma_device(pats, nrows, ncols, indices, nn=1, saturation_protect=True, ret
     ``return_d2``, the normalised distances and counts (nrows, ncols, (2nn+1)^2), all device
     float32.  ``mxval``: the saturation reference value (default: the patterns' maximum).
     ``fast``: uint8/uint16 distances as exact integer sums over packed words (a workspace
-    of about one pattern plus 137 bytes per pattern); False: pixel by pixel."""
+    of about one pattern plus 265 bytes per pattern); False: pixel by pixel."""
     torch = _torch()
     L = _hip.load_nlpar()
     x = pats.contiguous().reshape(nrows * ncols, -1)
