@@ -78,7 +78,7 @@ def test_nlpar_error_paths_without_device_compute():
                          None, None, 0, None) < 0
     assert b"2*searchradius+1" in nl.ebsd_nlpar_last_error()
     assert nl.ebsd_nlpar_max(None, 0, 1, 16, None, None) < 0
-    assert nl.ebsd_nlpar_workspace_bytes(10, 3600) == 3840 + 256 + 256 + 1280
+    assert nl.ebsd_nlpar_workspace_bytes(10, 3600) == 3840 + 256 + 256 + 2560   # lists of 64 int32
 
 
 def test_error_paths_without_device_compute():
