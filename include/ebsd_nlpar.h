@@ -72,6 +72,20 @@ int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t P, const i
                int r0, int c0, int nrc, int ncc, void *d_out, int out_dtype, float *d_weights,
                void *d_work, size_t work_bytes, void *stream);
 
+/* Which kernel ebsd_nlpar runs (process-wide; the results are identical): 1 (the default)
+ * a 2 x 2 tile of scan points per work-group, whose weighted average loads each pattern
+ * of the tile's windows once for all four points (where the patterns start on 16-byte
+ * boundaries and P is a multiple of 16), 0 one point per work-group.  No counterpart in the
+ * reference (a launch-shape switch of this implementation). */
+int ebsd_nlpar_set_tiled(int on);
+
+/* Bytes of the pair table the tiled kernel uses when ebsd_nlpar's workspace holds it after
+ * the integer part (work_bytes >= ebsd_nlpar_workspace_bytes(nr*nc, P) rounded up to 256,
+ * plus this): the squared distance and pixel count of every (point, window entry) of the
+ * npoints = nrc*ncc calculation points, each unordered pair computed once -- half the
+ * distance work, the same results.  No counterpart in the reference. */
+size_t ebsd_nlpar_pair_bytes(int64_t npoints, int sr);
+
 /* ---- the options of NLPAR.calcnlpar (nlpar_cpu.py:283-540) ---------------------------- */
 
 /* Smallest pixel value of n patterns, as float32, into *d_min (device): the file minimum

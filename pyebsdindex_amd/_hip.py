@@ -26,8 +26,8 @@ SYMBOLS = (
 
 #: exported symbols of the NLPAR library, in declaration order of include/ebsd_nlpar.h
 NLPAR_SYMBOLS = ("ebsd_nlpar_max", "ebsd_nlpar_workspace_bytes", "ebsd_nlpar_sigma", "ebsd_nlpar",
-                 "ebsd_nlpar_min", "ebsd_nlpar_prepare", "ebsd_nlpar_colsum", "ebsd_nlpar_finish",
-                 "ebsd_nlpar_last_error")
+                 "ebsd_nlpar_set_tiled", "ebsd_nlpar_pair_bytes", "ebsd_nlpar_min", "ebsd_nlpar_prepare", "ebsd_nlpar_colsum",
+                 "ebsd_nlpar_finish", "ebsd_nlpar_last_error")
 
 #: exported symbols of the triplet-vote library, in declaration order of include/ebsd_tripvote.h
 TRIPVOTE_SYMBOLS = ("ebsd_tripvote", "ebsd_bandindex", "ebsd_tripvote_last_error")
@@ -132,6 +132,10 @@ def load_nlpar(path: str = NLPAR_LIB_PATH):
         L.ebsd_nlpar_workspace_bytes.restype = ctypes.c_size_t
         L.ebsd_nlpar_sigma.argtypes = [P, I, I, I, I64, P, I, I, F, P, P, P, P, ctypes.c_size_t, P]
         L.ebsd_nlpar.argtypes = [P, I, I, I, I64, P, I, P, I, F, F, F, F, I, I, I, I, P, I, P, P, ctypes.c_size_t, P]
+        L.ebsd_nlpar_pair_bytes.argtypes = [I64, I]
+        L.ebsd_nlpar_pair_bytes.restype = ctypes.c_size_t
+        L.ebsd_nlpar_set_tiled.argtypes = [I]
+        L.ebsd_nlpar_set_tiled.restype = ctypes.c_int
         L.ebsd_nlpar_min.argtypes = [P, I, I64, I64, P, P]
         L.ebsd_nlpar_prepare.argtypes = [P, I, I64, I64, I, F, P, P, P]
         L.ebsd_nlpar_colsum.argtypes = [P, I64, I64, P, P]

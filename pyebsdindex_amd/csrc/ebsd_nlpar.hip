@@ -626,6 +626,270 @@ __global__ __launch_bounds__(kThreads) void k_nlpar(const T *__restrict__ pats, 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// k_nlpar_pairs: the squared distances and pixel counts of every pair (point, window entry)
+// the averaging needs, each unordered pair once.  A point of the calculation region computes
+// the entries of its (shifted) window except those whose partner q is in the region, holds
+// the point in its own window and has the smaller scan index: q computes that pair and
+// writes it to both points' slots.  The distance of a pair does not depend on which of the
+// two computes it (exact integer sums, or float64 sums of exact squares of float32
+// differences in the same pixel order), so the table holds what k_nlpar computes itself.
+// tab[o * win + e] = (float(d2), n2) for region point o and window entry e.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void shifted_window(int j, int i, int nr, int nc, int sr, int &wy0, int &wx0)
+{
+    wx0 = max(i - sr, 0) - max(i + sr - (nc - 1), 0);
+    wy0 = max(j - sr, 0) - max(j + sr - (nr - 1), 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_nlpar_pairs(const T *__restrict__ pats, int nr, int nc, int64_t P,
+                                                          const int32_t *__restrict__ idx, int M, int sr, float sat,
+                                                          int r0, int c0, int nrc, int ncc, U8Work wk,
+                                                          float2 *__restrict__ tab)
+{
+    __shared__ int s_e[kMaxWin];
+    __shared__ int s_n;
+    int jr, ic;
+    if (!blocked_point(blockIdx.x, nrc, ncc, jr, ic)) return;
+    const int j = r0 + jr, i = c0 + ic;
+    const int64_t p = (int64_t)j * nc + i;
+    const int64_t o = (int64_t)jr * ncc + ic;
+    const int ws = 2 * sr + 1, win = ws * ws;
+    int wy0, wx0;
+    shifted_window(j, i, nr, nc, sr, wy0, wx0);
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < win; e += kThreads) {
+        const int qy = wy0 + e / ws, qx = wx0 + e % ws;
+        if (qy == j && qx == i) continue;
+        const int qr = qy - r0, qc = qx - c0;
+        bool skip = false;
+        if (qr >= 0 && qr < nrc && qc >= 0 && qc < ncc) {
+            int qy0, qx0;
+            shifted_window(qy, qx, nr, nc, sr, qy0, qx0);
+            const bool holds = j >= qy0 && j < qy0 + ws && i >= qx0 && i < qx0 + ws;
+            skip = holds && (int64_t)qy * nc + qx < p;
+        }
+        if (!skip) s_e[atomicAdd(&s_n, 1)] = e;
+    }
+    __syncthreads();
+    const int n = s_n;
+    constexpr int NGW = 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int g = wave; g * NGW < n; g += kWaves) {
+        int64_t q[NGW];
+        int es[NGW];
+#pragma unroll
+        for (int k = 0; k < NGW; ++k) {
+            const int t = g * NGW + k;
+            const int e = s_e[min(t, n - 1)];
+            es[k] = t < n ? e : -1;
+            q[k] = (int64_t)(wy0 + e / ws) * nc + (wx0 + e % ws);
+        }
+        double d2[NGW];
+        int n2[NGW];
+        wave_pair_sums<NGW>(pats, P, idx, M, p, q, sat, wk, d2, n2);
+        // lane k writes pair k: this point's slot, and the partner's when it needs the pair
+        double dk = d2[0];
+        int nk = n2[0], e = es[0];
+#pragma unroll
+        for (int k = 1; k < NGW; ++k)
+            if (lane == k) {
+                dk = d2[k];
+                nk = n2[k];
+                e = es[k];
+            }
+        if (lane < NGW && e >= 0) {
+            const float2 v = make_float2((float)dk, (float)nk);
+            tab[o * win + e] = v;
+            const int qy = wy0 + e / ws, qx = wx0 + e % ws, qr = qy - r0, qc = qx - c0;
+            if (qr >= 0 && qr < nrc && qc >= 0 && qc < ncc) {
+                int qy0, qx0;
+                shifted_window(qy, qx, nr, nc, sr, qy0, qx0);
+                if (j >= qy0 && j < qy0 + ws && i >= qx0 && i < qx0 + ws)
+                    tab[((int64_t)qr * ncc + qc) * win + (j - qy0) * ws + (i - qx0)] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_nlpar_tile: k_nlpar for a tile of TR x TC scan points per work-group.  The distances and
+// weights of each point are those of k_nlpar (the same code per point); the weighted
+// average walks the union of the tile's windows once in row-major order, loading and
+// converting each pattern's 16-pixel vector once for all the points whose window holds it.
+// A point's window entries then still come in row-major (window) order, so every point sees
+// the same fused multiply-adds in the same order as in k_nlpar -- bit-identical results --
+// while the patterns read per point drop from (2sr+1)^2 to about (2sr+TR)(2sr+TC) / (TR TC).
+// ---------------------------------------------------------------------------------------
+constexpr int kTBR = 4, kTBC = 16; // tiles per XCD block (8 x 32 points for 2 x 2 tiles)
+
+__device__ __forceinline__ bool blocked_tile(int64_t b, int ntr, int ntc, int &ty, int &tx)
+{
+    const int64_t items = (int64_t)((ntr + kTBR - 1) / kTBR) * ((ntc + kTBC - 1) / kTBC) * (kTBR * kTBC);
+    const int64_t l = xcd_point(b, items);
+    const int nbc = (ntc + kTBC - 1) / kTBC;
+    const int64_t blk = l / (kTBR * kTBC);
+    const int w = (int)(l % (kTBR * kTBC));
+    ty = (int)(blk / nbc) * kTBR + w / kTBC;
+    tx = (int)(blk % nbc) * kTBC + w % kTBC;
+    return ty < ntr && tx < ntc;
+}
+
+template <typename T, typename O, int TR, int TC>
+__global__ __launch_bounds__(kThreads) void k_nlpar_tile(const T *__restrict__ pats, int nr, int nc, int64_t P,
+                                                         const int32_t *__restrict__ idx, int M,
+                                                         const float *__restrict__ sigma, int sr, double lam2,
+                                                         float dthresh, float diff_offset, float sat, int r0, int c0,
+                                                         int nrc, int ncc, U8Work wk, const float2 *__restrict__ tab,
+                                                         O *__restrict__ out, float *__restrict__ wout)
+{
+    constexpr int NT = TR * TC;
+    __shared__ float s_w[NT][kMaxWin];
+    __shared__ float s_sum[NT];
+    int ty, tx;
+    if (!blocked_tile(blockIdx.x, (nrc + TR - 1) / TR, (ncc + TC - 1) / TC, ty, tx)) return;
+    const int ws = 2 * sr + 1, win = ws * ws;
+    // point t of the tile: scan position, window origin (shifted inside the chunk,
+    // nlpar_cpu.py:872-876) and whether it lies in the calculation region
+    auto geom = [&](int t, int &j, int &i, int &wy0, int &wx0) -> bool {
+        const int jr = ty * TR + t / TC, ic = tx * TC + t % TC;
+        j = r0 + min(jr, nrc - 1);
+        i = c0 + min(ic, ncc - 1);
+        wx0 = max(i - sr, 0) - max(i + sr - (nc - 1), 0);
+        wy0 = max(j - sr, 0) - max(j + sr - (nr - 1), 0);
+        return jr < nrc && ic < ncc;
+    };
+    // ---- distances of the non-centre entries: from the pair table, or point by point, NGW
+    //      per wavefront pass (the same normalisation either way) ----
+    constexpr int NGW = 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ne = win - 1, ngroups = (ne + NGW - 1) / NGW;
+    if (tab) {
+        for (int f = threadIdx.x; f < NT * win; f += kThreads) {
+            const int t = f / win, e = f - t * win;
+            int j, i, wy0, wx0;
+            if (!geom(t, j, i, wy0, wx0)) continue;
+            if (e == (j - wy0) * ws + (i - wx0)) continue;
+            const float2 v = tab[((int64_t)(j - r0) * ncc + (i - c0)) * win + e];
+            const float sig0 = sigma[(int64_t)j * nc + i];
+            const float s0 = sig0 * sig0;
+            const float sg1 = sigma[(int64_t)(wy0 + e / ws) * nc + (wx0 + e % ws)];
+            const float s1 = sg1 * sg1;
+            float dd = v.x;
+            const float nf = v.y;
+            dd -= nf * (s0 + s1);
+            const float dnorm = (s1 + s0) * sqrtf(2.0f * nf);
+            dd = dnorm > 1.e-8f ? dd / dnorm : 1.0e6f * nf;
+            s_w[t][e] = dd;
+        }
+    }
+    for (int it = wave; it < (tab ? 0 : NT * ngroups); it += kWaves) {
+        const int t = it / ngroups, g = it - t * ngroups;
+        int j, i, wy0, wx0;
+        if (!geom(t, j, i, wy0, wx0)) continue;
+        const int64_t p = (int64_t)j * nc + i;
+        const int eself = (j - wy0) * ws + (i - wx0);
+        int64_t q[NGW];
+        int es[NGW];
+#pragma unroll
+        for (int k = 0; k < NGW; ++k) {
+            const int e0 = g * NGW + k, tt = min(e0, ne - 1);
+            const int e = tt < eself ? tt : tt + 1;
+            es[k] = e0 < ne ? e : -1;
+            q[k] = (int64_t)(wy0 + e / ws) * nc + (wx0 + e % ws);
+        }
+        double d2[NGW];
+        int n2[NGW];
+        wave_pair_sums<NGW>(pats, P, idx, M, p, q, sat, wk, d2, n2);
+        if (lane == 0) {
+            const float sig0 = sigma[p];
+            const float s0 = sig0 * sig0;
+#pragma unroll
+            for (int k = 0; k < NGW; ++k) {
+                if (es[k] < 0) continue;
+                const float sg1 = sigma[q[k]];
+                const float s1 = sg1 * sg1;
+                float dd = (float)d2[k];
+                const float nf = (float)n2[k];
+                dd -= nf * (s0 + s1);
+                const float dnorm = (s1 + s0) * sqrtf(2.0f * nf);
+                dd = dnorm > 1.e-8f ? dd / dnorm : 1.0e6f * nf;
+                s_w[t][es[k]] = dd;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- weights, one (point, entry) per thread; each point's sum in window order ----
+    for (int f = threadIdx.x; f < NT * win; f += kThreads) {
+        const int t = f / win, e = f - t * win;
+        int j, i, wy0, wx0;
+        if (!geom(t, j, i, wy0, wx0)) continue;
+        const bool self = e == (j - wy0) * ws + (i - wx0);
+        float w = self ? -1.0e6f : s_w[t][e];
+        w = fmaxf(w - dthresh, 0.f);
+        w = (float)exp(-1.0 * (double)w * lam2);
+        if (self) w += diff_offset;
+        s_w[t][e] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < NT) {
+        float sum = 0.f;
+        for (int e = 0; e < win; ++e) sum += s_w[threadIdx.x][e];
+        s_sum[threadIdx.x] = sum;
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < NT * win; f += kThreads) {
+        const int t = f / win, e = f - t * win;
+        int j, i, wy0, wx0;
+        if (!geom(t, j, i, wy0, wx0)) continue;
+        const float w = s_w[t][e] / s_sum[t];
+        s_w[t][e] = w;
+        if (wout) wout[((int64_t)(j - r0) * ncc + (i - c0)) * win + e] = w;
+    }
+    __syncthreads();
+    // ---- weighted average over the union of the tile's windows ----
+    int pj[NT], pi[NT], py[NT], px[NT];
+    bool pv[NT];
+    int uy0 = nr, uy1 = 0, ux0 = nc, ux1 = 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        pv[t] = geom(t, pj[t], pi[t], py[t], px[t]);
+        if (pv[t]) {
+            uy0 = min(uy0, py[t]);
+            uy1 = max(uy1, py[t] + ws);
+            ux0 = min(ux0, px[t]);
+            ux1 = max(ux1, px[t] + ws);
+        }
+    }
+    const int64_t P16 = P / 16;
+    for (int64_t v = threadIdx.x; v < P16; v += kThreads) {
+        float acc[NT][16];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
+        for (int uy = uy0; uy < uy1; ++uy) {
+            for (int ux = ux0; ux < ux1; ++ux) {
+                float x[16];
+                load16(pats + ((int64_t)uy * nc + ux) * P + 16 * v, x);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if (pv[t] && uy >= py[t] && uy < py[t] + ws && ux >= px[t] && ux < px[t] + ws) {
+                        const float w = s_w[t][(uy - py[t]) * ws + (ux - px[t])];
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) acc[t][k] = fmaf(x[k], w, acc[t][k]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            if (pv[t]) store16(out + ((int64_t)(pj[t] - r0) * ncc + (pi[t] - c0)) * P + 16 * v, acc[t]);
+    }
+}
+
 // max of the pixel values as order-preserving integer keys
 __device__ __forceinline__ unsigned fkey(float x)
 {
@@ -899,6 +1163,21 @@ extern "C" int ebsd_nlpar_finish(const float *d_x, int64_t n, int64_t P, int ste
 
 static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
+extern "C" size_t ebsd_nlpar_pair_bytes(int64_t npoints, int sr)
+{
+    if (npoints <= 0 || sr < 1 || sr > 10) return 0;
+    return al256((size_t)npoints * (size_t)((2 * sr + 1) * (2 * sr + 1)) * sizeof(float2));
+}
+
+// the tiled NLPAR kernel (k_nlpar_tile) where it applies; ebsd_nlpar_set_tiled switches it
+static bool g_nlpar_tiled = true;
+
+extern "C" int ebsd_nlpar_set_tiled(int on)
+{
+    g_nlpar_tiled = on != 0;
+    return 0;
+}
+
 extern "C" size_t ebsd_nlpar_workspace_bytes(int64_t n, int64_t P)
 {
     if (n <= 0 || P <= 0) return 0;
@@ -999,6 +1278,7 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
     const double lam2 = 1.0 / (double)(lam * lam);
     // 16-pixel vector loads and stores when every pattern starts on a 16-byte boundary
     const int vec = (P % 16 == 0) && ((uintptr_t)d_pats % 16 == 0) && ((uintptr_t)d_out % 16 == 0);
+    const bool tiled = g_nlpar_tiled;
     U8Work wk;
     if (int rc = u8_setup(d_pats, dtype, (int64_t)nr * nc, P, d_idx, M, sat, d_work, work_bytes, s, &wk)) return rc;
 #define EBSD_NL_RUN(T, O)                                                                                   \
@@ -1012,7 +1292,50 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
         EBSD_NL_RUN(T, uint16_t);      \
     else                               \
         EBSD_NL_RUN(T, float);
-    if (dtype == EBSD_NL_U8) {
+    if (vec && tiled) {
+        // 2 x 2 points per work-group (k_nlpar_tile); the same results.  With room in the
+        // workspace after the integer part, each pair's distance is computed once first.
+        float2 *tab = nullptr;
+        const size_t base_bytes = al256(ebsd_nlpar_workspace_bytes((int64_t)nr * nc, P));
+        if (d_work && ((uintptr_t)d_work % 256) == 0 && work_bytes >= base_bytes + ebsd_nlpar_pair_bytes(npts, sr))
+            tab = (float2 *)((uint8_t *)d_work + base_bytes);
+        if (tab) {
+#define EBSD_NLP_RUN(T)                                                                                        \
+    hipLaunchKernelGGL((k_nlpar_pairs<T>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, nc, \
+                       P, d_idx, M, sr, sat, r0, c0, nrc, ncc, wk, tab)
+            if (dtype == EBSD_NL_U8)
+                EBSD_NLP_RUN(uint8_t);
+            else if (dtype == EBSD_NL_U16)
+                EBSD_NLP_RUN(uint16_t);
+            else
+                EBSD_NLP_RUN(float);
+#undef EBSD_NLP_RUN
+        }
+        const int ntr = (nrc + 1) / 2, ntc = (ncc + 1) / 2;
+        const int64_t titems = (int64_t)((ntr + kTBR - 1) / kTBR) * ((ntc + kTBC - 1) / kTBC) * (kTBR * kTBC);
+        const int64_t tgrid = ((titems + 7) / 8) * 8;
+        if (tgrid > INT32_MAX) return fail(-1, "ebsd_nlpar: region too large");
+#define EBSD_NLT_RUN(T, O)                                                                                           \
+    hipLaunchKernelGGL((k_nlpar_tile<T, O, 2, 2>), dim3((unsigned)tgrid), dim3(kThreads), 0, s, (const T *)d_pats, nr, \
+                       nc, P, d_idx, M, d_sigma, sr, lam2, dthresh, diff_offset, sat, r0, c0, nrc, ncc, wk, tab,     \
+                       (O *)d_out, d_weights)
+#define EBSD_NLT_OUT(T)                \
+    if (out_dtype == EBSD_NL_U8)       \
+        EBSD_NLT_RUN(T, uint8_t);      \
+    else if (out_dtype == EBSD_NL_U16) \
+        EBSD_NLT_RUN(T, uint16_t);     \
+    else                               \
+        EBSD_NLT_RUN(T, float);
+        if (dtype == EBSD_NL_U8) {
+            EBSD_NLT_OUT(uint8_t)
+        } else if (dtype == EBSD_NL_U16) {
+            EBSD_NLT_OUT(uint16_t)
+        } else {
+            EBSD_NLT_OUT(float)
+        }
+#undef EBSD_NLT_OUT
+#undef EBSD_NLT_RUN
+    } else if (dtype == EBSD_NL_U8) {
         EBSD_NL_OUT(uint8_t)
     } else if (dtype == EBSD_NL_U16) {
         EBSD_NL_OUT(uint16_t)

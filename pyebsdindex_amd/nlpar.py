@@ -61,13 +61,19 @@ def _indices_dev(indices, P, device):
     return torch.as_tensor(idx.astype(np.int32), device=device)
 
 
-def _workspace(x):
-    """(buffer, bytes) of the integer-pattern workspace for the chunk ``x`` (n, P), or
-    (None, 0) for float32 patterns."""
+def _workspace(x, pairs=None):
+    """(buffer, bytes) of the workspace for the chunk ``x`` (n, P): the integer-pattern part
+    (none for float32 patterns), and with ``pairs`` = (calculation points, search radius)
+    room for ebsd_nlpar's pair table after it."""
     torch = _torch()
-    if x.dtype not in (torch.uint8, torch.uint16):
+    L = _hip.load_nlpar()
+    base = int(L.ebsd_nlpar_workspace_bytes(x.shape[0], x.shape[1]))
+    if pairs is not None:
+        nbytes = -(-base // 256) * 256 + int(L.ebsd_nlpar_pair_bytes(int(pairs[0]), int(pairs[1])))
+    elif x.dtype not in (torch.uint8, torch.uint16):
         return None, 0
-    nbytes = int(_hip.load_nlpar().ebsd_nlpar_workspace_bytes(x.shape[0], x.shape[1]))
+    else:
+        nbytes = base
     buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
     return buf, nbytes
 
@@ -258,7 +264,7 @@ def nlpar_device(pats, nrows, ncols, indices, sigma, lam=0.7, sr=3, dthresh=0.0,
     wts = torch.empty((nrc * ncc, (2 * sr + 1) ** 2), dtype=torch.float32, device=x.device) \
         if return_weights else None
     mx = pattern_max(x, stream) if mxval is None else float(mxval)
-    work, wbytes = _workspace(x) if fast else (None, 0)
+    work, wbytes = _workspace(x, pairs=(nrc * ncc, sr)) if fast else (None, 0)
     _hip.check_nlpar(L.ebsd_nlpar(_ptr(x), _code(x), nrows, ncols, P, _ptr(idx), idx.numel(), _ptr(sig), int(sr),
                                   float(lam), float(dthresh), float(diff_offset), _sat_nlpar(mx, saturation_protect),
                                   r0, c0, nrc, ncc, _ptr(out), _code(out), None if wts is None else _ptr(wts),

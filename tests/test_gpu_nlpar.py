@@ -112,6 +112,38 @@ def test_calclim_region_is_bit_identical():
     assert np.array_equal(part, full[2:8, 3:8])
 
 
+@pytest.mark.parametrize("shape,dtype,nrows,ncols,sr,calclim,odt", [
+    ((40, 40), "uint8", 11, 13, 3, None, "uint8"),
+    ((48, 32), "uint16", 9, 10, 2, (1, 2, 7, 5), "float32"),     # odd calculation region
+    ((32, 32), "float32", 8, 9, 1, (0, 1, 9, 7), "uint16"),
+    ((40, 40), "uint8", 7, 7, 3, (2, 3, 3, 1), "float32"),       # one row of three points
+])
+def test_tiled_kernel_is_bit_identical(shape, dtype, nrows, ncols, sr, calclim, odt):
+    """The 2 x 2 tiled NLPAR kernel (each window pattern loaded once per tile) gives the
+    one-point-per-work-group kernel's averages and weights bit for bit: partial tiles at the
+    region's edges, shifted windows at the scan's edges, every output type."""
+    import torch
+    from pyebsdindex_amd import _hip
+    from pyebsdindex_amd.nlpar import makeautomask, nlpar_device, sigma_device
+    print(f"nlpar tiled {shape} {dtype} {nrows}x{ncols} sr={sr} calclim={calclim} out={odt}", flush=True)
+    pats = _grain_scan(nrows, ncols, shape, dtype, seed=33)
+    idx = np.nonzero(makeautomask(*shape).ravel())[0]
+    x = _dev(pats)
+    s = sigma_device(x, nrows, ncols, idx)
+    L = _hip.load_nlpar()
+    res = []
+    try:
+        for tiled in (0, 1):
+            L.ebsd_nlpar_set_tiled(tiled)
+            o, w = nlpar_device(x, nrows, ncols, idx, s, sr=sr, calclim=calclim, out_dtype=getattr(torch, odt),
+                                return_weights=True)
+            res.append((o.cpu().numpy(), w.cpu().numpy()))
+    finally:
+        L.ebsd_nlpar_set_tiled(1)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
 def test_nlpar_file_round_trip(tmp_path):
     """NLPAR.calcnlpar on a UP1 file equals the oracle on the whole scan (clipped to uint8),
     and a run in row bands equals the one-band run bit for bit; opt_lambda matches the same
