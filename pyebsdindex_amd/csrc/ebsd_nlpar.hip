@@ -642,7 +642,7 @@ __device__ __forceinline__ void shifted_window(int j, int i, int nr, int nc, int
     wy0 = max(j - sr, 0) - max(j + sr - (nr - 1), 0);
 }
 
-template <typename T>
+template <typename T, int NGW>
 __global__ __launch_bounds__(kThreads) void k_nlpar_pairs(const T *__restrict__ pats, int nr, int nc, int64_t P,
                                                           const int32_t *__restrict__ idx, int M, int sr, float sat,
                                                           int r0, int c0, int nrc, int ncc, U8Work wk,
@@ -674,8 +674,7 @@ __global__ __launch_bounds__(kThreads) void k_nlpar_pairs(const T *__restrict__ 
         if (!skip) s_e[atomicAdd(&s_n, 1)] = e;
     }
     __syncthreads();
-    const int n = s_n;
-    constexpr int NGW = 4;
+    const int n = s_n; // sr (sr + 1) / 2 * 4 for an interior point: one group of NGW per wave
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int g = wave; g * NGW < n; g += kWaves) {
         int64_t q[NGW];
@@ -1300,15 +1299,28 @@ extern "C" int ebsd_nlpar(const void *d_pats, int dtype, int nr, int nc, int64_t
         if (d_work && ((uintptr_t)d_work % 256) == 0 && work_bytes >= base_bytes + ebsd_nlpar_pair_bytes(npts, sr))
             tab = (float2 *)((uint8_t *)d_work + base_bytes);
         if (tab) {
-#define EBSD_NLP_RUN(T)                                                                                        \
-    hipLaunchKernelGGL((k_nlpar_pairs<T>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, nc, \
-                       P, d_idx, M, sr, sat, r0, c0, nrc, ncc, wk, tab)
-            if (dtype == EBSD_NL_U8)
-                EBSD_NLP_RUN(uint8_t);
-            else if (dtype == EBSD_NL_U16)
-                EBSD_NLP_RUN(uint16_t);
-            else
-                EBSD_NLP_RUN(float);
+            // an interior point owns sr (sr + 1) / 2 * 4 pairs: groups of NGW pairs so that
+            // each of the four wavefronts takes about one
+#define EBSD_NLP_RUN(T, G)                                                                                      \
+    hipLaunchKernelGGL((k_nlpar_pairs<T, G>), dim3((unsigned)grid), dim3(kThreads), 0, s, (const T *)d_pats, nr, \
+                       nc, P, d_idx, M, sr, sat, r0, c0, nrc, ncc, wk, tab)
+#define EBSD_NLP_G(T)            \
+    if (sr == 1)                 \
+        EBSD_NLP_RUN(T, 1);      \
+    else if (sr == 2)            \
+        EBSD_NLP_RUN(T, 3);      \
+    else if (sr == 3)            \
+        EBSD_NLP_RUN(T, 6);      \
+    else                         \
+        EBSD_NLP_RUN(T, 8);
+            if (dtype == EBSD_NL_U8) {
+                EBSD_NLP_G(uint8_t)
+            } else if (dtype == EBSD_NL_U16) {
+                EBSD_NLP_G(uint16_t)
+            } else {
+                EBSD_NLP_G(float)
+            }
+#undef EBSD_NLP_G
 #undef EBSD_NLP_RUN
         }
         const int ntr = (nrc + 1) / 2, ntc = (ncc + 1) / 2;
