@@ -58,8 +58,9 @@ def parse():
     ap.add_argument("--tune", default="", help="gather_pack,gather_threads,peak_tile,peak_threads (0 = keep)")
     ap.add_argument("--line-unit", type=int, default=0,
                     help="gather line order in units of this many neighbouring cells (0 = the plan's default)")
-    ap.add_argument("--chunk", type=int, default=2048,
-                    help="patterns per Radon+peak pass (the Radon of a chunk stays in the MALL)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="patterns per Radon+peak pass (0: the library's automatic size, all "
+                         "patterns while their Radons fit 1 GiB)")
     return ap.parse_args()
 
 
@@ -165,7 +166,7 @@ def streaming_measure(dev, total=1048576, batch=8192, ring=4, seed=11):
     from pyebsdindex_amd.synthetic import kikuchi_patterns
     shape = (240, 240)
     bd = BandDetect(patDim=shape, nTheta=360, nRho=180, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=12)
-    sb = StreamingBandDetect(bd, batch=batch, dtype="uint16", depth=2, chunk=2048)
+    sb = StreamingBandDetect(bd, batch=batch, dtype="uint16", depth=2)
     host = []
     for k in range(ring):
         x = kikuchi_patterns(batch, shape, "uint16", seed=seed + k, family="HCP", device=dev, chunk=1024)
@@ -240,7 +241,7 @@ def file_streaming_measure(bd, host, npats=16384, batch=2048, threads=16):
             w.pat_writer(host[k % len(host)].numpy(), off, n, np.uint16)
             off, k = off + n, k + 1
         f = get_pattern_file_obj(fn)
-        sb = StreamingBandDetect(bd, batch=batch, dtype="uint16", depth=3, chunk=2048)
+        sb = StreamingBandDetect(bd, batch=batch, dtype="uint16", depth=3)
         ring = [sb.pinned_batch() for _ in range(sb.depth + 1)]
         blocks = sb.pinned_blocks(-(-npats // batch))
         sb.run_file(f, 0, 2 * batch, blocks=blocks, ring=ring, threads=threads)      # warm-up
@@ -495,9 +496,9 @@ def batched_measure(dev, steps, warmup, npats=32768, seed=7, autotune_sets=True)
             pats.append(x)
     torch.cuda.synchronize()
     outs = [bd.device_plan().alloc_out(x.shape[0]) for bd, x in zip(dets, pats)]
-    works = [bd.device_plan().workspace(x.shape[0], 2048) for bd, x in zip(dets, pats)]
+    works = [bd.device_plan().workspace(x.shape[0]) for bd, x in zip(dets, pats)]
 
-    def plain(chunk=2048):
+    def plain(chunk=None):
         for bd, x, o, w in zip(dets, pats, outs, works):
             bd.find_bands_device(x, out=o, work=w, chunk=chunk)
 
@@ -511,7 +512,7 @@ def batched_measure(dev, steps, warmup, npats=32768, seed=7, autotune_sets=True)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    plain_s = time_plain(2048)
+    plain_s = time_plain(None)      # the library's pass size (all of a set's patterns)
     # the same per-size launches at the reference's default chunk (band_detect.py:427-434)
     plain512_s = time_plain(512)
     batch = BatchBandDetect(dets, pats)
@@ -583,7 +584,7 @@ def main():
     info = plan.info()
     N = pats.shape[0]
     out = plan.alloc_out(N)
-    chunk = max(1, min(args.chunk, N))
+    chunk = plan.auto_chunk(N) if args.chunk <= 0 else max(1, min(args.chunk, N))
     work = plan.workspace(N, chunk)
     print(f"[bench] plan {info}", file=sys.stderr, flush=True)
 

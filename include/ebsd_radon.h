@@ -115,8 +115,9 @@ int ebsd_plan_set_peak_block(ebsd_plan *plan, int theta_cols);
 size_t ebsd_workspace_bytes(const ebsd_plan *plan, int64_t chunk);
 
 /* BandDetect.find_bands on N patterns resident on the device (band_detect.py:411-526).
- * d_pats: (N, H, W) of `dtype`; d_work: workspace of ws_bytes (>= ebsd_workspace_bytes(plan,1)),
- * patterns are processed in chunks that fit it.  stream: a hipStream_t (NULL = default).
+ * d_pats: (N, H, W) of `dtype`; d_work: workspace of ws_bytes (>= ebsd_workspace_bytes(plan,1));
+ * the patterns run in passes of equal size whose Radons fit it (a larger workspace, fewer
+ * and fuller passes).  stream: a hipStream_t (NULL = default).
  * Replaces: band_detect_cl.BandDetect.find_bands (opencl/band_detect_cl.py:51-199). */
 int ebsd_find_bands(ebsd_plan *plan, const void *d_pats, int dtype, int64_t N,
                     const ebsd_bands *out, void *d_work, size_t ws_bytes, void *stream);
@@ -152,8 +153,8 @@ int ebsd_conv(ebsd_plan *plan, const float *d_radon, int64_t N, float *d_conv, v
 
 /* Batches of mixed pattern sizes (each set with its own plan): persistent Radon launches
  * (one per LDS-occupancy class) and peak launches (one per Radon geometry) over
- * heterogeneous work lists; a set of 2,048 patterns or more whose gather needs no row
- * segments is gathered by its own per-size launch in the same sequence (run once by
+ * heterogeneous work lists; a set of 1,024 patterns or more runs its own per-size gather
+ * and peak launches, one after the other, in the same sequence (run once by
  * ebsd_batch_create, so that later launches allocate nothing).  The batch binds the given
  * device pointers; ebsd_batch_graph_launch captures the launch sequence into a hipGraph on
  * its first call (on a non-default stream) and replays it afterwards.  The batch's launches

@@ -4,9 +4,9 @@ Radon and peak launches whose work lists mix the sets (optionally one hipGraph r
 The reference handles one pattern size per plan and re-plans when the size changes
 (pyebsdindex/_ebsd_index_single.py:696-700); here the per-size plans stay resident and one
 launch per LDS-occupancy class (Radon) or per Radon geometry (peaks) walks a work list that
-mixes them (``ebsd_batch_*`` in include/ebsd_radon.h).  A set of 2,048 patterns or more,
-which fills the GPU by itself, is gathered by its own per-size launch inside the same
-launch sequence and graph.
+mixes them (``ebsd_batch_*`` in include/ebsd_radon.h).  A set of 1,024 patterns or more,
+which fills the GPU by itself, runs its own per-size gather and peak launches inside the
+same launch sequence and graph.
 """
 import ctypes
 

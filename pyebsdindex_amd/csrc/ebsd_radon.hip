@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
 #include <type_traits>
 #include <mutex>
@@ -63,7 +64,7 @@ struct PeakGeo {
     int tile, ntiles;
     int rh, apitch, bpitch, maskw; // LDS tile rows, pitches of the two tile buffers, mask words per row
     int nB;
-    int sbw, mcw;         // streamed peak kernel: theta columns per wavefront (0: not used), mask words
+    int sbw;              // streamed peak kernel: theta columns per wavefront (0: not used)
     float u[KMAX], v[KMAX]; // C = sum_a u[a] sum_b v[b] Rpad[r + kr/2 - a][c + kt/2 - b]
 };
 
@@ -115,7 +116,12 @@ struct ebsd_plan {
     // peaks
     PeakGeo g{};
     uint8_t *d_mask = nullptr;   // nR*nT
-    uint32_t *d_mcol = nullptr;  // rho-mask bits per theta column (PeakGeo::mcw words each)
+    std::vector<uint8_t> h_rdnmask; // nR*nT, the peak mask (host copy for the lane-mask tables)
+    // streamed peak kernel: per theta block width sbw, the rho-mask as 64-bit lane masks,
+    // [wavefront][padded row d + 8] for d in [-8, nRp + 8); built when sbw is set, kept until
+    // the plan is destroyed (launches queued on any stream may hold an older one)
+    std::map<int, uint64_t *> mlane_tabs;
+    uint64_t *d_mlane = nullptr; // the table of the current sbw
     uint32_t *d_mrow = nullptr;  // rho-mask bits per rho row (PeakGeo::maskw words each)
     // adjoint structures, built on first use
     int *d_px_off = nullptr, *d_px_cell = nullptr; // pixel -> lines through it (CSR)
@@ -728,7 +734,7 @@ struct PeakPtrs {
     int64_t N;
     const uint8_t *mask; // (nR, nT)
     const double *grid;  // theta_deg[nT], rho[nR]
-    const uint32_t *mcol; // rho-mask bits per theta column, rows -32 .. (mcw words a column)
+    const uint64_t *mlane; // streamed kernel: rho-mask lane masks [wavefront][padded row + 8]
     const uint32_t *mrow; // rho-mask bits per rho row (maskw words a row)
     ebsd_bands out;
 };
@@ -1423,13 +1429,13 @@ constexpr int KEY_MASKED = INT_MIN; // below the key of every float
 
 // LDS layout of k_peaks_stream (4-byte words), per wavefront w of nw:
 //   list: v[WCAP], p[WCAP], nn[9*WCAP]; run: v[nB], p[nB], nn[9*nB]; sel: same as run;
-//   red: min, max, sum (double); cnt: running-list count; mask: [mcw][64] rho-mask words
+//   red: min, max, sum (double); cnt: running-list count
 // then the final list (v, p, nn) of nB and the pattern statistics.
 struct StreamLds {
-    int per_wave, list_v, list_p, list_nn, run_v, run_p, run_nn, sel_v, sel_p, sel_nn, red, cnt, mask, fin_v,
+    int per_wave, list_v, list_p, list_nn, run_v, run_p, run_nn, sel_v, sel_p, sel_nn, red, cnt, fin_v,
         fin_p, fin_nn, stats, total;
 };
-__host__ __device__ inline StreamLds stream_lds_layout(int nw, int nB, int mcw)
+__host__ __device__ inline StreamLds stream_lds_layout(int nw, int nB)
 {
     StreamLds L;
     L.list_v = 0;
@@ -1444,8 +1450,7 @@ __host__ __device__ inline StreamLds stream_lds_layout(int nw, int nB, int mcw)
     L.red = L.sel_nn + 9 * nB;
     L.red += L.red & 1;        // 8-byte alignment of the double sum
     L.cnt = L.red + 4;         // running-list count
-    L.mask = L.cnt + 2;        // rho-mask words of the wavefront's columns, [mcw][64]
-    L.per_wave = L.mask + 64 * mcw; // even: every wave's red stays 8-byte aligned
+    L.per_wave = L.cnt + 2;    // even: every wave's red stays 8-byte aligned
     L.fin_v = nw * L.per_wave;
     L.fin_p = L.fin_v + nB;
     L.fin_nn = L.fin_p + nB;
@@ -1505,7 +1510,7 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     const PeakPtrs &S = MULTI ? a.sets[sidx] : a.s;
     const int64_t q = MULTI ? (int64_t)a.items[it].y : it;
     const int nB = g.nB;
-    const StreamLds L = stream_lds_layout(nw, nB, g.mcw);
+    const StreamLds L = stream_lds_layout(nw, nB);
     float *wl = sm + wave * L.per_wave;
     float *lst_v = wl + L.list_v, *lst_nn = wl + L.list_nn;
     int *lst_p = reinterpret_cast<int *>(wl + L.list_p);
@@ -1529,26 +1534,30 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     const int jc = !cval ? 0 : (left ? g.nT - g.p1 + c : (right ? c - g.p1 - g.nT : jint));
     const bool flip = cval && (left || right);
     const int jc4 = jc * 4;
-    const uint32_t *mcol = S.mcol + (size_t)(useful ? jint : 0) * g.mcw;
+    // this wavefront's rho-mask rows as 64-bit lane masks (useful lanes only), read with
+    // scalar loads: padded row d at mlane[d + 8]
+    const uint64_t *mlane = S.mlane + (size_t)__builtin_amdgcn_readfirstlane(wave) * (size_t)(g.nRp + 16) + 8;
 
-    // staged value of padded row r: issued unconditionally (the row is clamped into the
-    // array, so the address is always valid) and masked where it is consumed, which keeps
-    // the loads of a whole strip in flight together
+    // staged value of padded row r (zero outside the padded array and in columns outside
+    // it), issued unconditionally so that the loads of a whole strip stay in flight
+    // together.  The zeros come from the buffer's range check: a read at or beyond
+    // nR*nT*4 bytes returns 0, so a masked lane or row gets an offset far out of range
+    // instead of a select.  Lane offset = (S ^ fm) + fb with S the row's byte offset
+    // i*nT*4 (uniform): fm = 0, fb = jc4 reads row i; fm = -1, fb = (nR-1)*nT*4 + jc4 + 1
+    // reads the flipped row nR-1-i (two's complement: (S ^ -1) + 1 = -S).  Every lane
+    // offset is a single 32-bit VALU add of the uniform S (the range check ignores the
+    // scalar offset, so the whole offset is in the lane register).
+    const uint32_t fm = flip ? 0xffffffffu : 0u;
+    const uint32_t fb = !cval ? 0x80000000u                      // column outside: >= 2^31
+                              : (flip ? (uint32_t)((g.nR - 1) * g.nT * 4 + jc4 + 1) : (uint32_t)jc4);
     auto stage_load = [&](int r) -> float {
         int i = r - g.p0;
         i = i < 0 ? 0 : (i > g.nR - 1 ? g.nR - 1 : i);
-        // the whole offset in the lane register (the range check ignores the scalar offset)
-        const int row = flip ? g.nR - 1 - i : i;
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Rrs, row * g.nT * 4 + jc4, 0, 0));
+        // a row outside the padded array: S = 2^30 puts every lane's offset at or above
+        // 2^30 - (nR*nT*4) (flipped lanes) or 2^30 (others), far beyond the buffer
+        const uint32_t S = (r >= 0 && r < g.nRp) ? (uint32_t)(i * g.nT * 4) : 0x40000000u;
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Rrs, (int)((S ^ fm) + fb), 0, 0));
     };
-
-    // this wavefront's mask columns, lane-major in LDS (read without touching the vector
-    // memory counter that the prefetched Radon rows are waited on with)
-    uint32_t *wmask = reinterpret_cast<uint32_t *>(wl + L.mask);
-    for (int w = 0; w < g.mcw; ++w) {
-        const uint32_t mv = mcol[w];
-        wmask[w * 64 + lane] = useful ? mv : 0u;
-    }
 
     float Tr[TRING], pf[SROWS];
     int Mr[MRING]; // keys of the convolved rows (KEY_MASKED outside the array and the kept rows)
@@ -1565,18 +1574,12 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
     for (int k = 0; k < SROWS; ++k) pf[k] = stage_load(rbeg + CKR + k);
 
     for (int r0 = rbeg; r0 < rend; r0 += SROWS) {
-        // the rdnmask bits of this strip's dilation rows d0 .. d0+7 (words cover rows -32 ..)
-        const int d0 = r0 - HR;
-        const int mi0 = min(max(d0 - g.p0 + 32, 0), 32 * (g.mcw - 1) - 1); // word mi0>>5 <= mcw-2
-        const uint32_t mlo = wmask[(mi0 >> 5) * 64 + lane], mhi = wmask[((mi0 >> 5) + 1) * 64 + lane];
-        // ---- stage rows r0+CKR .. r0+CKR+SROWS-1 (zero outside the padded array); fetch
-        //      the next strip's ----
+        const int d0 = r0 - HR; // this strip's dilation rows d0 .. d0+7
+        // ---- stage rows r0+CKR .. r0+CKR+SROWS-1 (loaded as zero outside the padded
+        //      array); fetch the next strip's ----
         float x[SROWS];
 #pragma unroll
-        for (int k = 0; k < SROWS; ++k) {
-            const int r = r0 + CKR + k;
-            x[k] = (cval && r >= 0 && r < g.nRp) ? pf[k] : 0.0f;
-        }
+        for (int k = 0; k < SROWS; ++k) x[k] = pf[k];
 #pragma unroll
         for (int k = 0; k < SROWS; ++k) pf[k] = stage_load(r0 + SROWS + CKR + k);
         // ---- theta convolution: T[c] = sum_b v[b] S[c + CKT - b] (fma order of k_peaks) ----
@@ -1603,19 +1606,21 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
             for (int aa = 0; aa < KR; ++aa) acc = fmaf(g.u[aa], Tr[k + 2 * CKR - aa], acc);
             const int r = r0 + k;
             const int key = fkey(acc);
-            if (r >= g.sr0 && r < g.sr1) { // every lane; lanes outside the block are dropped below
-                kmin = min(kmin, key);
-                kmax = max(kmax, key);
-                ssum += (double)acc;
-            }
+            // statistics rows [sr0, sr1), every lane (lanes outside the block are dropped
+            // below).  The row test is uniform but taken as scalar factors, not a branch
+            // (which the compiler turns into four selects per row): a row outside adds
+            // acc * 0.0 (a zero: the float64 sum, which starts at +0, is unchanged; the
+            // convolved values are finite) and clamps its key to the identity of min/max.
+            const bool srow = r >= g.sr0 && r < g.sr1;
+            kmin = min(kmin, max(key, srow ? INT_MIN : INT_MAX));
+            kmax = max(kmax, min(key, srow ? INT_MAX : KEY_MASKED));
+            ssum = fma((double)acc, srow ? 1.0 : 0.0, ssum);
             Mr[2 * HR + k] = (cval && r >= g.cr0 && r < g.cr1) ? key : KEY_MASKED;
         }
         // ---- rows d = d0 + k: rho dilation (windows Mr[k .. k + 2HR]), theta dilation,
         //      D == C under the mask -> candidates ----
         if (d0 + SROWS > g.mr0 && d0 < g.mr1) {
-            // (in this branch d0 - p0 + 32 >= 25: mi0 is not clamped)
-            const uint64_t v64 = (uint64_t)mlo | ((uint64_t)mhi << 32);
-            const uint32_t bits = useful ? (uint32_t)(v64 >> (mi0 & 31)) & ((1u << SROWS) - 1u) : 0u;
+            // (in this branch -8 < d0 < nRp: rows d0 .. d0+7 lie in the lane-mask table)
             // all eight windows share Mr[SROWS-1 .. 2HR]
             int mid = Mr[SROWS - 1];
 #pragma unroll
@@ -1627,7 +1632,7 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
             rg[0] = KEY_MASKED;
 #pragma unroll
             for (int k = 1; k < SROWS; ++k) rg[k] = max(rg[k - 1], Mr[2 * HR + k]);
-            uint32_t cand = 0u;
+            uint64_t cand[SROWS], any = 0; // per row: the lanes whose cell is a candidate
 #pragma unroll
             for (int k = 0; k < SROWS; ++k) {
                 const int vk = max(mid, max(lf[k], rg[k]));
@@ -1638,14 +1643,15 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
                     rgt = max(vk, lane_next_i(rgt));
                 }
                 const int dk = max(lft, rgt);
-                cand |= (((bits >> k) & 1u) && dk == Mr[k + HR]) ? (1u << k) : 0u;
+                cand[k] = mlane[d0 + k] & __ballot(dk == Mr[k + HR]);
+                any |= cand[k];
             }
-            if (__ballot(cand != 0u)) {
+            if (any) {
 #pragma unroll
                 for (int k = 0; k < SROWS; ++k) {
-                    const bool ck = (cand >> k) & 1u;
-                    const uint64_t bal = __ballot(ck);
+                    const uint64_t bal = cand[k];
                     if (bal == 0) continue;
+                    const bool ck = (bal >> lane) & 1u;
                     const int n = __popcll(bal);
                     if (ncand + n > WCAP) {
                         nrun = wave_merge(lane, nB, nrun, ncand, run_v, run_p, run_nn, lst_v, lst_p, lst_nn, sel_v,
@@ -1830,7 +1836,7 @@ static void plan_free(ebsd_plan *p)
     (void)hipFree(p->d_linebg);
     (void)hipFree(p->d_bg);
     (void)hipFree(p->d_mask);
-    (void)hipFree(p->d_mcol);
+    for (auto &t : p->mlane_tabs) (void)hipFree(t.second);
     (void)hipFree(p->d_mrow);
     (void)hipFree(p->d_px_off);
     (void)hipFree(p->d_px_cell);
@@ -1861,7 +1867,6 @@ static int stream_usable(const PeakGeo &g) { return 64 - 2 * (g.kt / 2 + g.ht); 
 
 static void geo_stream(PeakGeo &g)
 {
-    g.mcw = ((g.nR + 31) >> 5) + 2;
     g.sbw = 0;
     const bool fixed = g.hr == g.kr && g.ht == g.kt && ((g.kr == 9 && g.kt == 5) || (g.kr == 5 && g.kt == 5));
     if (!fixed) return;
@@ -1874,6 +1879,38 @@ static void geo_stream(PeakGeo &g)
 static int stream_waves(const PeakGeo &g) { return g.sbw > 0 ? (g.nT + g.sbw - 1) / g.sbw : 0; }
 
 static bool use_stream(const ebsd_plan *p) { return p->g.sbw > 0 && p->pkind != 1; }
+
+// The streamed kernel's rho-mask for the plan's current theta block width: per wavefront w
+// and padded row d in [-8, nRp + 8), bit `lane` is set where the lane owns an interior
+// column (lanes [halo, halo + sbw), column w*sbw + lane - halo < nT) whose cell
+// (d - p0, column) is in the mask.  One table per width, kept until the plan is destroyed.
+static int build_mlane(ebsd_plan *p)
+{
+    const PeakGeo &g = p->g;
+    if (g.sbw <= 0) { p->d_mlane = nullptr; return 0; }
+    const auto it = p->mlane_tabs.find(g.sbw);
+    if (it != p->mlane_tabs.end()) { p->d_mlane = it->second; return 0; }
+    if (g.mr0 < 0) return fail(-3, "lane-mask table: mask rows start above the padded array");
+    const int nw = stream_waves(g), rows = g.nRp + 16, halo = g.kt / 2 + g.ht;
+    std::vector<uint64_t> t((size_t)nw * rows, 0ull);
+    for (int w = 0; w < nw; ++w)
+        for (int d = -8; d < g.nRp + 8; ++d) {
+            const int i = d - g.p0;
+            if (i < 0 || i >= g.nR) continue;
+            uint64_t m = 0ull;
+            for (int lane = halo; lane < halo + g.sbw && lane < 64; ++lane) {
+                const int j = w * g.sbw + lane - halo;
+                if (j < g.nT && p->h_rdnmask[(size_t)i * g.nT + j]) m |= 1ull << lane;
+            }
+            t[(size_t)w * rows + (d + 8)] = m;
+        }
+    uint64_t *d = nullptr;
+    int rc;
+    if ((rc = dev_upload(&d, t.data(), t.size()))) return rc;
+    p->mlane_tabs[g.sbw] = d;
+    p->d_mlane = d;
+    return 0;
+}
 
 static int geo_tiles(ebsd_plan *p)
 {
@@ -2357,12 +2394,8 @@ extern "C" int ebsd_plan_create(int H, int W, int nTheta, int nRho, const int64_
     geo_stream(g);
     if ((rc = dev_upload(&p->d_mask, rdnmask, (size_t)nCells))) { plan_free(p); return rc; }
     {
-        // column-major mask bits; bit (i + 32) of column j is rdnmask[i][j]
-        std::vector<uint32_t> mcol((size_t)nTheta * g.mcw, 0u);
-        for (int j = 0; j < nTheta; ++j)
-            for (int i = 0; i < nRho; ++i)
-                if (rdnmask[i * nTheta + j]) mcol[(size_t)j * g.mcw + ((i + 32) >> 5)] |= 1u << ((i + 32) & 31);
-        if ((rc = dev_upload(&p->d_mcol, mcol.data(), mcol.size()))) { plan_free(p); return rc; }
+        p->h_rdnmask.assign(rdnmask, rdnmask + nCells);
+        if ((rc = build_mlane(p))) { plan_free(p); return rc; }
         // row-major mask bits; bit b of word w of row i is rdnmask[i][32 w + b]
         std::vector<uint32_t> mrow((size_t)nRho * g.maskw, 0u);
         for (int i = 0; i < nRho; ++i)
@@ -2483,15 +2516,21 @@ extern "C" int ebsd_plan_set_peak_block(ebsd_plan *p, int theta_cols)
     geo_stream(g);                           // the default block (and whether it applies)
     if (theta_cols == 0) {
         p->g.sbw = g.sbw;
-        return 0;
+        return build_mlane(p);
     }
     if (g.sbw <= 0) return fail(-1, "ebsd_plan_set_peak_block: the streamed kernel does not apply to this plan");
     if (theta_cols < 1 || theta_cols > stream_usable(g))
         return fail(-1, "ebsd_plan_set_peak_block: theta_cols must be in 1.." + std::to_string(stream_usable(g)));
     if ((g.nT + theta_cols - 1) / theta_cols > 16)
         return fail(-1, "ebsd_plan_set_peak_block: more than 16 wavefronts a pattern");
+    const int prev = p->g.sbw;
     p->g.sbw = theta_cols;
-    return 0;
+    const int rc = build_mlane(p);
+    if (rc) {
+        p->g.sbw = prev;
+        (void)build_mlane(p); // the previous width's table exists: no allocation
+    }
+    return rc;
 }
 
 extern "C" int ebsd_plan_set_tuning(ebsd_plan *p, int gather_pack, int gather_threads, int peak_tile,
@@ -2773,7 +2812,7 @@ static int launch_peaks_t(ebsd_plan *p, const float *d_radon, int64_t N, const e
     a.s.N = N;
     a.s.mask = p->d_mask;
     a.s.grid = p->d_grid;
-    a.s.mcol = p->d_mcol;
+    a.s.mlane = p->d_mlane;
     a.s.mrow = p->d_mrow;
     a.s.out = *out;
     a.g = p->g;
@@ -2799,12 +2838,12 @@ static int launch_peaks_stream_t(ebsd_plan *p, const float *d_radon, int64_t N, 
     a.s.N = N;
     a.s.mask = p->d_mask;
     a.s.grid = p->d_grid;
-    a.s.mcol = p->d_mcol;
+    a.s.mlane = p->d_mlane;
     a.s.mrow = p->d_mrow;
     a.s.out = *out;
     a.g = p->g;
     const int nw = stream_waves(p->g);
-    const size_t lds = (size_t)stream_lds_layout(nw, p->g.nB, p->g.mcw).total * 4;
+    const size_t lds = (size_t)stream_lds_layout(nw, p->g.nB).total * 4;
     auto kern = k_peaks_stream<KR, KT, false>;
     HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3((unsigned)N), dim3(64 * nw), lds, s, a);
@@ -2858,8 +2897,13 @@ extern "C" int ebsd_find_bands(ebsd_plan *p, const void *d_pats, int dtype, int6
     if (chunk < 1) return fail(-1, "ebsd_find_bands: workspace smaller than one Radon");
     hipStream_t s = (hipStream_t)stream;
     const size_t esz = (dtype & 0xff) == EBSD_U8 ? 1 : ((dtype & 0xff) == EBSD_U16 ? 2 : 4);
-    for (int64_t q0 = 0; q0 < N; q0 += chunk) {
-        const int64_t n = std::min(chunk, N - q0);
+    // passes of equal size (whole packs of 8 where the workspace allows), not full passes
+    // and a short last one: a short pass leaves most of the GPU idle
+    const int64_t npass = (N + chunk - 1) / chunk;
+    int64_t piece = (N + npass - 1) / npass;
+    piece = std::min(chunk, (piece + 7) / 8 * 8);
+    for (int64_t q0 = 0; q0 < N; q0 += piece) {
+        const int64_t n = std::min(piece, N - q0);
         const void *src = (const char *)d_pats + (size_t)q0 * p->nPx * esz;
         int rc = launch_gather(p, src, dtype, n, (float *)d_work, s);
         if (rc) return rc;
@@ -3588,9 +3632,10 @@ struct PeakClass {
     int grid = 0;
 };
 
-// patterns from which a set of a batch is gathered by the per-size launcher (as many as one
-// per-size chunk: the set then fills the GPU by itself)
-constexpr int64_t kSingleSetMin = 2048;
+// patterns from which a set of a batch runs the per-size launchers (a gather round of the
+// whole GPU at 4 patterns per CU: the set then fills the GPU by itself; measured on mixed
+// batches of 1,024 patterns a set, the per-size launches beat the merged ones by 7%)
+constexpr int64_t kSingleSetMin = 1024;
 
 struct ebsd_batch {
     int nsets = 0, device = 0, ncu = 0;
@@ -3603,13 +3648,15 @@ struct ebsd_batch {
     int2 *d_pitems = nullptr;
     int64_t npitems = 0;
     std::vector<PeakClass> pclasses;
-    // sets gathered by the per-size launcher (large sets whose gather needs no row segments)
+    // sets that fill the GPU by themselves: the per-size gather and peak launchers, one
+    // set after the other, inside the batch
     struct Single {
         ebsd_plan *plan;
         const void *pats;
         int dtype;
         int64_t N;
         float *radon;
+        ebsd_bands out;
     };
     std::vector<Single> singles;
     uint32_t *part = nullptr;        // the singles' row-segment partial sums (in the workspace;
@@ -3717,22 +3764,26 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         S.N = N[i];
         S.mask = p->d_mask;
         S.grid = p->d_grid;
-        S.mcol = p->d_mcol;
+        S.mlane = p->d_mlane;
         S.mrow = p->d_mrow;
         S.out = outs[i];
     }
     // a set that fills the GPU by itself (kSingleSetMin patterns or more) runs the per-size
-    // launcher, with the plan's tuned shape and row segments, inside the batch; the partial
-    // sums of its row segments go to the batch's own buffer (after the Radons in the
-    // workspace; the singles run one after another and share it).  The warm-up below
-    // compiles every row segmentation they use, and a plan keeps its segmentations (one per
-    // segment count) until it is destroyed; the merged launches are for the others
+    // launchers, gather (with the plan's tuned shape and row segments) then peaks, inside
+    // the batch, as a plain call would: the peak kernel reads the Radon its gather has just
+    // written, while the cache still holds much of it.  The partial sums of its row
+    // segments go to the batch's own buffer (after the Radons in the workspace; the singles
+    // run one after another and share it).  The warm-up below compiles every row
+    // segmentation they use, and a plan keeps its segmentations (one per segment count)
+    // until it is destroyed; the merged launches are for the others
     b->part = (uint32_t *)work;
     b->part_bytes = (size_t)((char *)d_work + ws_bytes - work);
+    std::vector<char> single(nsets, 0);
     for (int i = 0; i < nsets; ++i) {
         if (N[i] < kSingleSetMin) continue;
-        b->singles.push_back(ebsd_batch::Single{plans[i], d_pats[i], dtypes[i], N[i], gs[i].out});
+        b->singles.push_back(ebsd_batch::Single{plans[i], d_pats[i], dtypes[i], N[i], gs[i].out, outs[i]});
         cls[i] = 0;
+        single[i] = 1;
     }
     // gather items, class by class (4 work-groups per CU first); inside a class the largest
     // patterns first, so the static round-robin over persistent work-groups ends cheap
@@ -3792,6 +3843,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     };
     std::vector<int2> pitems;
     std::vector<int> done(nsets, 0);
+    for (int i = 0; i < nsets; ++i) done[i] = single[i];
     for (int i0 = 0; i0 < nsets; ++i0) {
         if (done[i0]) continue;
         const PeakGeo g = geo_key(plans[i0]);
@@ -3806,7 +3858,7 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
         pc.stream = use_stream(plans[i0]);
         if (pc.stream) {
             pc.threads = 64 * stream_waves(g);
-            pc.lds = (size_t)stream_lds_layout(stream_waves(g), g.nB, g.mcw).total * 4;
+            pc.lds = (size_t)stream_lds_layout(stream_waves(g), g.nB).total * 4;
         }
         pc.item0 = (int64_t)pitems.size();
         for (int i : order) {
@@ -3877,7 +3929,8 @@ extern "C" int ebsd_batch_create(int nsets, ebsd_plan *const *plans, const void 
     go.part = b->part;
     go.part_bytes = b->part_bytes;
     for (const ebsd_batch::Single &g : b->singles) {
-        if ((rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, nullptr, go))) {
+        if ((rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, nullptr, go)) ||
+            (rc = launch_peaks(g.plan, g.radon, g.N, &g.out, nullptr))) {
             batch_free(b);
             return rc;
         }
@@ -3901,7 +3954,8 @@ static int batch_launch(ebsd_batch *b, hipStream_t s)
     go.part = b->part;
     go.part_bytes = b->part_bytes;
     for (const ebsd_batch::Single &g : b->singles) {
-        const int rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, s, go);
+        int rc = launch_gather(g.plan, g.pats, g.dtype, g.N, g.radon, s, go);
+        if (!rc) rc = launch_peaks(g.plan, g.radon, g.N, &g.out, s);
         if (rc) return rc;
     }
     for (const GatherClass &gc : b->classes) {

@@ -52,7 +52,7 @@ class StreamingBandDetect:
 
     FIELDS = ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid", "stats")
 
-    def __init__(self, bd, batch=8192, dtype="uint16", gpu_id=None, depth=3, chunk=2048):
+    def __init__(self, bd, batch=8192, dtype="uint16", gpu_id=None, depth=3, chunk=None):
         import torch
         if depth < 2:
             raise ValueError("depth must be >= 2 (one slot copying while another computes)")
@@ -61,7 +61,8 @@ class StreamingBandDetect:
         self.device = self.plan.device
         self.batch = int(batch)
         self.depth = int(depth)
-        self.chunk = int(chunk)
+        # patterns per Radon+peak pass inside a batch (None: the plan's automatic size)
+        self.chunk = self.plan.auto_chunk(self.batch) if chunk is None else int(chunk)
         self.tdtype = {"uint8": torch.uint8, "uint16": torch.uint16, "float32": torch.float32}[str(np.dtype(dtype))]
         H, W = self.plan.H, self.plan.W
         nB = self.plan.nB

@@ -68,9 +68,9 @@ def test_batch_against_fp64_oracle():
 
 
 def test_batch_with_large_sets():
-    """Sets of 2,048 patterns or more whose gather needs no row segments run the per-size
-    gather inside the batch (and its graph); a large set that needs row segments (160x160
-    uint16) and a small set stay in the merged launch.  Results equal each set's own
+    """Sets of 1,024 patterns or more run the per-size gather and peak launches inside the
+    batch (and its graph), row segments included (160x160 uint16); a small set stays in the
+    merged launches.  Results equal each set's own
     find_bands bit for bit, eagerly and on graph replays with eager work in between."""
     import torch
     from pyebsdindex_amd import BandDetect
@@ -78,7 +78,7 @@ def test_batch_with_large_sets():
     from pyebsdindex_amd.synthetic import kikuchi_patterns
     print("batch with large sets", flush=True)
     spec = [((60, 60), "uint8", 2300), ((120, 120), "uint16", 2100), ((160, 160), "uint16", 2048),
-            ((80, 80), "uint8", 40)]
+            ((80, 80), "uint16", 1100), ((80, 80), "uint8", 40)]
     dets, pats = [], []
     for k, (shape, dtype, n) in enumerate(spec):
         pats.append(kikuchi_patterns(n, shape, dtype, seed=120 + k, family="BCC", device="cuda"))
