@@ -38,6 +38,19 @@ def _dtype_code(dt):
     return None
 
 
+def auto_pass_size(N, cells, cus, budget_bytes):
+    """Patterns per Radon+peak pass when the caller names none: all N while their float32
+    Radons (``cells`` each) fit ``budget_bytes``; else the largest whole number of gather
+    rounds (4 patterns per CU, ``cus`` CUs) that fits, or what fits where not even one
+    round does."""
+    N = max(1, int(N))
+    cap = max(1, int(budget_bytes) // (int(cells) * 4))
+    if N <= cap:
+        return N
+    unit = 4 * max(1, int(cus))
+    return cap // unit * unit if cap >= unit else cap
+
+
 class DevicePlan:
     """The compiled device plan of one BandDetect on one GPU (``ebsd_plan*``)."""
 
@@ -179,12 +192,7 @@ class DevicePlan:
         4-wavefront work-groups take 1.6 rounds of the 1,280 resident slots at 2,048
         patterns, and one 16,384-pattern pass at 120x120 runs 5% faster per pattern than
         eight passes of 2,048."""
-        N = max(1, int(N))
-        cap = max(1, self.AUTO_WORKSPACE_BYTES // (self.nR * self.nT * 4))
-        if N <= cap:
-            return N
-        unit = 4 * max(1, int(self.info()["cus"]))
-        return max(unit, cap // unit * unit) if cap >= unit else cap
+        return auto_pass_size(N, self.nR * self.nT, int(self.info()["cus"]), self.AUTO_WORKSPACE_BYTES)
 
     def workspace(self, N, chunk=None):
         import torch

@@ -495,3 +495,34 @@ def test_masked_plans(kind):
     res, _, C64, _ = oracle64(ref, pats)
     total, ties = compare_bands(got, res, C64, ref.padding, ref.peakPad)
     assert ties <= max(1, total // 100)
+
+
+def test_pass_sizes_identical():
+    """find_bands gives the same bits whatever the pass size: the default (one pass while
+    the Radons fit the workspace budget), equal passes from a workspace that holds 700 or
+    1,000 Radons (2,500 patterns -> passes of 632/632/632/604 and 840/840/820), one pass
+    per pattern, and the background modes."""
+    import torch
+    print("pass sizes 2500 x 120x120 uint16", flush=True)
+    shape = (120, 120)
+    bd, ref = make_pair(shape)
+    pats = _pats(2500, shape, "uint16", seed=71)
+    bg = pats[:256].astype(np.float64).mean(axis=0)
+    x = _dev(pats)
+    for mode in ("none", "fast", "exact"):
+        bd.backgroundsub = None if mode == "none" else bg
+        bd.background_mode = "exact" if mode == "exact" else "fast"
+        plan = bd.device_plan()
+        want = {k: v.cpu().numpy() for k, v in bd.find_bands_device(x).items()}
+        for chunk in (700, 1000, 1):
+            n = 7 if chunk == 1 else x.shape[0]
+            got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(x[:n], chunk=chunk).items()}
+            for k in want:
+                assert np.array_equal(got[k], want[k][:n]), (mode, chunk, k)
+        # a workspace that is not a whole number of Radons
+        work = plan.workspace(x.shape[0], 900)
+        work = work[: work.numel() - 7]
+        got = {k: v.cpu().numpy() for k, v in bd.find_bands_device(x, work=work).items()}
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (mode, "ragged workspace", k)
+    torch.cuda.synchronize()

@@ -216,3 +216,25 @@ def test_phase_exposes_what_the_indexer_reads(libtype):
     ph2 = pickle.loads(pickle.dumps(ph))
     assert ph2._dev == {} and ph2.lauecode == ph.lauecode
     assert np.array_equal(ph2.angtriplets["angles"], ph.angtriplets["angles"])
+
+
+def test_auto_pass_size_rule():
+    """The pass size taken when a caller names no chunk: all patterns while their Radons fit
+    the workspace budget, else whole gather rounds (4 patterns per CU) that fit it."""
+    cells = 180 * 90                 # 64,800-byte float32 Radon
+    gib = 1 << 30
+    # the headline workload: 16,384 Radons are 1.06e9 bytes < 1 GiB -> one pass
+    assert bdmod.auto_pass_size(16384, cells, 256, gib) == 16384
+    assert bdmod.auto_pass_size(1, cells, 256, gib) == 1
+    assert bdmod.auto_pass_size(0, cells, 256, gib) == 1
+    # 20,000 do not fit: 1 GiB holds 16,570 -> 16 rounds of 1,024
+    assert bdmod.auto_pass_size(20000, cells, 256, gib) == 16384
+    # 240x240 with Radon 360x180 (259,200 bytes): 4,142 fit -> 4 rounds
+    assert bdmod.auto_pass_size(8192, 360 * 180, 256, gib) == 4096
+    # a budget below one round keeps what fits
+    assert bdmod.auto_pass_size(5000, cells, 256, 100 * cells * 4) == 100
+    # every answer fits the budget and never exceeds N
+    for n in (1, 7, 1023, 1024, 4097, 50000):
+        for b in (cells * 4, 10 ** 6, 10 ** 8, gib):
+            k = bdmod.auto_pass_size(n, cells, 256, b)
+            assert 1 <= k <= max(1, n) and (k * cells * 4 <= b or k == 1)
