@@ -514,9 +514,14 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
         //      wave walks its stream of plan groups with three groups in flight ----
         const int nslot = nwaves * a.nslice, slot = slice * nwaves + wave;
         // one walk, instantiated for the wide and the narrow sums (chosen per pack)
-        // mode 0: wide sums; 1: narrow, flushed every group; 2: narrow, flushed every F groups
-        auto walk = [&](auto mode_tag) {
+        // mode 0: wide sums; 1: narrow, flushed every group; 2: narrow, flushed every F groups.
+        // SEG: row segments after the first (segmode 2, 3), whose lines start from the earlier
+        // segments' partial sums -- a compile-time choice, so that the other walks hold no
+        // load of them: a register loaded on one path and read after the paths merge makes
+        // the wave wait there for every load in flight, at each line end (5-6% of the gather)
+        auto walk = [&](auto mode_tag, auto seg_tag) {
           constexpr int MODE = decltype(mode_tag)::value;
+          constexpr bool SEG = decltype(seg_tag)::value;
           constexpr bool NARROW = MODE > 0;
           int b = slot;
           if (b < nb) {
@@ -549,7 +554,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             auto load_prev = [&](int bb, int cell) {
 #pragma unroll
                 for (int p = 0; p < P; ++p)
-                    prev[p] = (!FLT && a.segmode >= 2 && cell >= 0 && q0 + p < a.N)
+                    prev[p] = (SEG && !FLT && cell >= 0 && q0 + p < a.N)
                                   ? a.part[(q0 + p) * (int64_t)a.npos + bb * 64 + lane] : 0u;
             };
             load_prev(b, li.x);
@@ -561,7 +566,7 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             auto load_pd = [&](double *d, int bb, int cell) {
 #pragma unroll
                 for (int p = 0; p < P; ++p)
-                    d[p] = (FLT && a.segmode >= 2 && bb < nb && cell >= 0 && q0 + p < a.N)
+                    d[p] = (SEG && FLT && bb < nb && cell >= 0 && q0 + p < a.N)
                                ? partd[(q0 + p) * (int64_t)a.npos + bb * 64 + lane] : 0.0;
             };
             if constexpr (FLT) {
@@ -665,12 +670,16 @@ __device__ __forceinline__ void gather_pack(const GatherArgs &a, int64_t pack, i
             }
           }
         };
+        auto walk_seg = [&](auto mode_tag) {
+            if (a.segmode >= 2) walk(mode_tag, std::true_type{});
+            else walk(mode_tag, std::false_type{});
+        };
         if constexpr (Narrow<IN, FLT, P>::ok) {
-            if (F == 1) walk(std::integral_constant<int, 1>{});
-            else if (F > 1) walk(std::integral_constant<int, 2>{});
-            else walk(std::integral_constant<int, 0>{});
+            if (F == 1) walk_seg(std::integral_constant<int, 1>{});
+            else if (F > 1) walk_seg(std::integral_constant<int, 2>{});
+            else walk_seg(std::integral_constant<int, 0>{});
         } else {
-            walk(std::integral_constant<int, 0>{});
+            walk_seg(std::integral_constant<int, 0>{});
         }
         __syncthreads();
     }
