@@ -1405,7 +1405,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // (one padded column per lane, with a halo of KT/2 + HT lanes each side).  Per strip of
 // SROWS rows a lane stages its padded-Radon column, convolves along theta with DPP lane
 // shifts and along rho from a register ring, then -- on rows that can hold a candidate --
-// dilates along rho from a second ring and along theta with DPP (the separable max in the
+// dilates along rho from a second ring and along theta with lane reads (the separable max in the
 // other order: the same exact result) and tests D == C against the rho mask.  Only the
 // candidate lists, the per-wave statistics and the final top-nB pass through LDS; the
 // work-group synchronises twice per pattern.
@@ -1422,7 +1422,6 @@ __device__ __forceinline__ float lane_prev(float x) // lane i receives lane i-1 
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ int lane_next_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true); }
-__device__ __forceinline__ int lane_prev_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true); }
 
 // Order-preserving integer key of a float (an involution): the dilation maxima and the
 // D == C test run on keys, exact and without the canonicalising moves float max needs
@@ -1503,11 +1502,31 @@ __device__ __forceinline__ int wave_merge(int lane, int nB, int nrun, int ncand,
     return nsel;
 }
 
+// Lane offsets of the windows of three lanes that, with the window at c, cover the theta
+// dilation window c-HT .. c+HT: left of c they start at c-HT and step by 3 (the last may
+// overlap the window at c); right of c+2 they step by 3 from c+3, the last moved back so
+// that it ends at c+HT.
+template <int N> struct StaticOffsets { int v[N > 0 ? N : 1]; __host__ __device__ constexpr int operator[](int i) const { return v[i]; } };
+template <int N> __host__ __device__ constexpr StaticOffsets<N> theta_windows(int ht)
+{
+    StaticOffsets<N> o{};
+    int n = 0;
+    for (int s = -ht; s < 0; s += 3) o.v[n++] = s;
+    for (int s = 3; s + 2 < ht + 3 && n < N; s += 3) o.v[n++] = s + 2 > ht ? ht - 2 : s;
+    return o;
+}
+
 template <int KR, int KT, bool MULTI>
-__global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MULTI ? 1 : 5))) void k_peaks_stream(PeakArgs a)
 {
     constexpr int CKR = KR / 2, CKT = KT / 2, HR = KR, HT = KT; // dilation half-windows = kernel sizes
     constexpr int HALO = CKT + HT;
+    // theta dilation window 2HT+1 from windows of three lanes: the one at c and NWIN more
+    // at lanes c + WOFS[j]
+    static_assert(HT >= 2 && HT <= 8, "theta dilation half-window outside 2 .. 8");
+    constexpr int NLEFT = (HT + 2) / 3, NRIGHT = HT / 3; // windows left of c, right of c+2
+    constexpr int NWIN = NLEFT + NRIGHT;
+    constexpr StaticOffsets<NWIN> WOFS = theta_windows<NWIN>(HT);
     constexpr int TRING = SROWS + 2 * CKR;            // theta-convolved rows r0-CKR .. r0+SROWS-1+CKR
     constexpr int MRING = SROWS + 2 * HR;             // convolved rows r0-2HR .. r0+SROWS-1
     static_assert(2 * CKR <= SROWS && SROWS - 1 <= 2 * HR, "strip shorter than the rho windows");
@@ -1567,6 +1586,12 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
         const uint32_t S = (r >= 0 && r < g.nRp) ? (uint32_t)(i * g.nT * 4) : 0x40000000u;
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Rrs, (int)((S ^ fm) + fb), 0, 0));
     };
+
+    // ds_bpermute byte addresses of the theta dilation's lane reads, lanes c + WOFS[j]
+    // taken modulo the wave
+    int badr[NWIN];
+#pragma unroll
+    for (int j = 0; j < NWIN; ++j) badr[j] = ((lane + WOFS[j]) & 63) << 2;
 
     float Tr[TRING], pf[SROWS];
     int Mr[MRING]; // keys of the convolved rows (KEY_MASKED outside the array and the kept rows)
@@ -1641,19 +1666,34 @@ __global__ __launch_bounds__(1024) void k_peaks_stream(PeakArgs a)
             rg[0] = KEY_MASKED;
 #pragma unroll
             for (int k = 1; k < SROWS; ++k) rg[k] = max(rg[k - 1], Mr[2 * HR + k]);
+            // theta dilation: w = the maximum over lanes c .. c+2 (two DPP lane shifts fused
+            // into maxima), then D = the maximum of w at c and at lanes c + WOFS[j], windows of
+            // three that together cover c-HT .. c+HT.  Those other lanes' w are ds_bpermute
+            // lane reads, which issue to the LDS unit rather than the VALU.  A useful lane's
+            // result depends on lanes c-HT .. c+HT only.
+            // DGRP rows at a time: their lane reads all in flight, then their maxima (four rows
+            // at a time need more than the 96 VGPRs of five waves per SIMD)
             uint64_t cand[SROWS], any = 0; // per row: the lanes whose cell is a candidate
+            constexpr int DGRP = 2;
 #pragma unroll
-            for (int k = 0; k < SROWS; ++k) {
-                const int vk = max(mid, max(lf[k], rg[k]));
-                int lft = vk, rgt = vk;
+            for (int h = 0; h < SROWS; h += DGRP) {
+                int w[DGRP], wr[DGRP][NWIN];
 #pragma unroll
-                for (int d = 0; d < HT; ++d) {
-                    lft = max(vk, lane_prev_i(lft));
-                    rgt = max(vk, lane_next_i(rgt));
+                for (int kk = 0; kk < DGRP; ++kk) {
+                    const int vk = max(mid, max(lf[h + kk], rg[h + kk]));
+                    w[kk] = max(vk, lane_next_i(max(vk, lane_next_i(vk))));
+#pragma unroll
+                    for (int j = 0; j < NWIN; ++j) wr[kk][j] = __builtin_amdgcn_ds_bpermute(badr[j], w[kk]);
                 }
-                const int dk = max(lft, rgt);
-                cand[k] = mlane[d0 + k] & __ballot(dk == Mr[k + HR]);
-                any |= cand[k];
+#pragma unroll
+                for (int kk = 0; kk < DGRP; ++kk) {
+                    const int k = h + kk;
+                    int dk = w[kk];
+#pragma unroll
+                    for (int j = 0; j < NWIN; ++j) dk = max(dk, wr[kk][j]);
+                    cand[k] = mlane[d0 + k] & __ballot(dk == Mr[k + HR]);
+                    any |= cand[k];
+                }
             }
             if (any) {
 #pragma unroll
