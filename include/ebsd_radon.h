@@ -218,6 +218,17 @@ int ebsd_radon2pole(int64_t N, int nB, const float *d_theta, const float *d_rho,
                     const double *d_pc, int pc_per_pattern, int H, int W, int vendor,
                     float *d_normals, double *d_dn_dpc, void *stream);
 
+/* The band records of a find_bands result in the byte layout of BandDetect.dataType
+ * (band_detect.py:110-115, numpy's packed layout): N*nB records of 49 + 16*nPhases bytes
+ * (id, max, normmax, maxloc[2], avemax, aveloc[2], pqmax, width, theta, rho, valid,
+ * band_match_index[nPhases], bandfit[nPhases]) into the device buffer d_records.  The
+ * fields find_bands does not compute are zero, as in the reference's result (its records
+ * come from a zeroed chunk array, band_detect.py:425, :466).  Replaces the host-side
+ * assembly of the structured array (band_detect.py:466), so that a host caller receives the
+ * array with one copy. */
+int ebsd_pack_band_records(const ebsd_bands *bands, int64_t N, int nB, int nPhases, void *d_records,
+                           void *stream);
+
 const char *ebsd_last_error(void);
 
 #ifdef __cplusplus

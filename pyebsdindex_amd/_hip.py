@@ -22,6 +22,7 @@ SYMBOLS = (
     "ebsd_plan_info", "ebsd_last_error", "ebsd_batch_workspace_bytes", "ebsd_batch_create",
     "ebsd_batch_run", "ebsd_batch_graph_launch", "ebsd_batch_destroy", "ebsd_band_grad",
     "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands", "ebsd_radon2pole",
+    "ebsd_pack_band_records",
 )
 
 #: exported symbols of the NLPAR library, in declaration order of include/ebsd_nlpar.h
@@ -92,6 +93,7 @@ def load(path: str = LIB_PATH):
         L.ebsd_backproject.argtypes = [P, P, I64, P, P]
         L.ebsd_gnomonic_bands.argtypes = [I64, I, P, P, I, I, I, I, P, P, P, P]
         L.ebsd_radon2pole.argtypes = [I64, I, P, P, P, I, I, I, I, P, P, P]
+        L.ebsd_pack_band_records.argtypes = [ctypes.POINTER(EbsdBands), I64, I, I, P, P]
         L.ebsd_last_error.argtypes = []
         L.ebsd_last_error.restype = ctypes.c_char_p
         for name in ("ebsd_plan_create", "ebsd_plan_destroy", "ebsd_plan_set_background",
@@ -101,7 +103,7 @@ def load(path: str = LIB_PATH):
                      "ebsd_conv", "ebsd_plan_info", "ebsd_batch_create", "ebsd_batch_run",
                      "ebsd_batch_graph_launch", "ebsd_batch_destroy",
                      "ebsd_band_grad", "ebsd_conv_transpose", "ebsd_backproject", "ebsd_gnomonic_bands",
-                     "ebsd_radon2pole"):
+                     "ebsd_radon2pole", "ebsd_pack_band_records"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
         return L

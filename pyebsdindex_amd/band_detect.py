@@ -38,6 +38,27 @@ def _dtype_code(dt):
     return None
 
 
+def record_phases(dtype):
+    """The phase count if ``dtype`` (a ``BandDetect.dataType``) has the packed layout that
+    ``ebsd_pack_band_records`` writes (band_detect.py:110-115), else None."""
+    dtype = np.dtype(dtype)
+    if dtype.names is None or "band_match_index" not in dtype.names:
+        return None
+    nph = int(np.prod(dtype["band_match_index"].shape))
+    f4, i4, i1, i8, f8 = (np.dtype(t) for t in ("<f4", "<i4", "i1", "<i8", "<f8"))
+    want = [("id", i4, (), 0), ("max", f4, (), 4), ("normmax", f4, (), 8), ("maxloc", f4, (2,), 12),
+            ("avemax", f4, (), 20), ("aveloc", f4, (2,), 24), ("pqmax", f4, (), 32), ("width", f4, (), 36),
+            ("theta", f4, (), 40), ("rho", f4, (), 44), ("valid", i1, (), 48),
+            ("band_match_index", i8, (nph,), 49), ("bandfit", f8, (nph,), 49 + 8 * nph)]
+    if list(dtype.names) != [w[0] for w in want] or dtype.itemsize != 49 + 16 * nph:
+        return None
+    for name, base, shape, off in want:
+        ft, fo = dtype.fields[name][:2]
+        if fo != off or ft.base != base or tuple(ft.shape) != tuple(shape):
+            return None
+    return nph
+
+
 def auto_pass_size(N, cells, cus, budget_bytes):
     """Patterns per Radon+peak pass when the caller names none: all N while their float32
     Radons (``cells`` each) fit ``budget_bytes``; else the largest whole number of gather
@@ -168,6 +189,25 @@ class DevicePlan:
         for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid", "stats"):
             setattr(b, "d_" + k, out[k].data_ptr())
         return b
+
+    def pack_records(self, out, n_phases, dst=None, stream=None):
+        """``ebsd_pack_band_records``: the bands of ``out`` (the dict ``find_bands`` returns,
+        N patterns) as N*nB records in the byte layout of ``BandDetect.dataType`` with
+        ``n_phases`` phases, into ``dst`` (a device uint8 tensor of N*nB*(49+16*n_phases)
+        bytes; allocated here unless given).  Returns ``dst``."""
+        import torch
+        N = int(out["max"].shape[0])
+        nbytes = N * self.nB * (49 + 16 * int(n_phases))
+        if dst is None:
+            dst = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        if dst.dtype != torch.uint8 or dst.device != self.device or dst.numel() < nbytes or not dst.is_contiguous():
+            raise ValueError(f"dst must be a contiguous uint8 tensor of at least {nbytes} bytes on {self.device}")
+        s = _hip.launch_stream(stream, self.device)
+        b = self.bands_struct(out)
+        _hip.check(self.lib.ebsd_pack_band_records(_hip.ctypes.byref(b), N, self.nB, int(n_phases),
+                                                   dst.data_ptr(), s.cuda_stream))
+        _hip.keep_for_stream(s, dst, *out.values())
+        return dst
 
     def _check_pats(self, pats):
         import torch
@@ -624,13 +664,20 @@ class BandDetect:
 
     #: host inputs of at least this many patterns take the pipelined path of find_bands
     STREAM_MIN = 512
+    #: that path's patterns per batch (when the caller names no chunksize) and device slots
+    #: (at 120x120 uint16: 9.6 ms per 16,384 patterns with 2,048 and 3 slots, against 9.8 with
+    #: 2 slots, 10.2 with 4,096 and 12.4 with 1,024)
+    STREAM_BATCH = 2048
+    STREAM_DEPTH = 3
 
     def _find_bands_streamed(self, patternsIn, plan, chunksize):
         """find_bands of host patterns (numpy or CPU tensor, uint8/uint16/float32, at least
         STREAM_MIN of them) through a pinned-host pipeline: batches are copied into a pinned
         ring by a thread pool, uploaded on a copy stream and detected on a compute stream
-        while the next batch is copied, and the results come back in pinned blocks.  Returns
-        the structured array, or None when the input does not take this path."""
+        while the next batch is copied, and the results come back into pinned memory (the
+        structured array's own, packed on the device, when dataType has the reference's
+        layout).  Returns the structured array, or None when the input does not take this
+        path."""
         import torch
         from .streaming import StreamingBandDetect
         if isinstance(patternsIn, torch.Tensor):
@@ -645,28 +692,40 @@ class BandDetect:
             raise ValueError(f"patterns must be (N, {plan.H}, {plan.W})")
         arr = np.ascontiguousarray(arr)
         N = int(arr.shape[0])
-        batch = int(chunksize) if chunksize and chunksize > 0 else 2048
+        batch = int(chunksize) if chunksize and chunksize > 0 else self.STREAM_BATCH
         batch = max(1, min(batch, N))
-        key = (id(plan), str(arr.dtype), batch)
+        # the bands come back as records in the byte layout of dataType when it has the
+        # reference's layout: the device writes the returned array itself
+        nph = record_phases(self.dataType)
+        key = (id(plan), str(arr.dtype), batch, nph, self.STREAM_DEPTH)
         pipes = self.__dict__.setdefault("_pipelines", {})
         sb = pipes.get(key)
         if sb is None:
             pipes.clear()   # one pipeline at a time: its device slots and pinned ring are large
             sb = StreamingBandDetect(self, batch=batch, dtype=str(arr.dtype), gpu_id=plan.device.index,
-                                     depth=2, chunk=batch)
+                                     depth=self.STREAM_DEPTH, chunk=batch, records=nph)
             pipes[key] = sb
         blocks = sb.__dict__.get("_blocks")
         nb = -(-N // batch)
         if blocks is None or len(blocks) < nb:
             blocks = sb._blocks = sb.pinned_blocks(nb)
-        res = sb.run_array(arr, blocks=blocks[:nb])
-        bandData = np.zeros((N, self.nBands), dtype=self.dataType)
+        if nph is not None:
+            # the structured array lives in pinned host memory that the device writes its
+            # records into (torch's caching host allocator recycles it once the array is
+            # dropped): no host-side assembly and no host copy of the result
+            rec = torch.empty(N * sb.record_row_bytes, dtype=torch.uint8, pin_memory=True)
+            res = sb.run_array(arr, blocks=blocks[:nb], rec_out=rec)
+            bandData = rec.numpy().view(self.dataType).reshape(N, self.nBands)
+        else:
+            res = sb.run_array(arr, blocks=blocks[:nb])
+            bandData = np.zeros((N, self.nBands), dtype=self.dataType)
         stats = np.empty((N, 3), dtype=np.float32)
         off = 0
         for r in res:
-            n = int(r["max"].shape[0])
-            for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid"):
-                bandData[k][off:off + n] = r[k].numpy()
+            n = int(r["stats"].shape[0])
+            if nph is None:
+                for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid"):
+                    bandData[k][off:off + n] = r[k].numpy()
             stats[off:off + n] = r["stats"].numpy()
             off += n
         self.last_stats = stats

@@ -3628,6 +3628,57 @@ extern "C" int ebsd_radon2pole(int64_t N, int nB, const float *d_theta, const fl
     return 0;
 }
 
+// Band records in the byte layout of BandDetect.dataType (band_detect.py:110-115, numpy's
+// packed layout: fields in declaration order, no padding): id @0, max @4, normmax @8,
+// maxloc[2] @12, avemax @20, aveloc[2] @24, pqmax @32, width @36, theta @40, rho @44,
+// valid @48, band_match_index[nPhases] @49, bandfit[nPhases] @49+8*nPhases; record size
+// 49 + 16*nPhases.  One thread per output byte, grid-stride (the stores of a wavefront
+// cover 64 consecutive bytes).  Fields that find_bands does not compute are zero, as in the
+// reference's result.
+__global__ void k_pack_band_records(ebsd_bands b, int64_t nbytes, int R, uint8_t *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / R; // record = pattern * nB + band
+        const int o = (int)(i - r * R);
+        uint8_t v = 0;
+        if (o == 48) {
+            v = (uint8_t)b.d_valid[r];
+        } else if (o >= 4 && o < 48 && (o < 32 || o >= 36)) {
+            // the float field word: fi = 0 max, 1 normmax, 2-3 maxloc, 4 avemax, 5-6 aveloc,
+            // 8 width, 9 theta, 10 rho -- chosen by selects, without branches (a switch in
+            // their place gave rho's bytes theta's values on the GPU)
+            const int fi = (o - 4) >> 2;
+            const float *f = fi == 0 ? b.d_max : b.d_normmax;
+            f = (fi == 2 || fi == 3) ? b.d_maxloc : f;
+            f = fi == 4 ? b.d_avemax : f;
+            f = (fi == 5 || fi == 6) ? b.d_aveloc : f;
+            f = fi == 8 ? b.d_width : f;
+            f = fi == 9 ? b.d_theta : f;
+            f = fi == 10 ? b.d_rho : f;
+            const int64_t k = (fi == 2 || fi == 3) ? 2 * r + (fi - 2) : ((fi == 5 || fi == 6) ? 2 * r + (fi - 5) : r);
+            v = (uint8_t)(__float_as_uint(f[k]) >> (8 * (o & 3)));
+        }
+        dst[i] = v;
+    }
+}
+
+extern "C" int ebsd_pack_band_records(const ebsd_bands *bands, int64_t N, int nB, int nPhases, void *d_records,
+                                      void *stream)
+{
+    if (N < 0 || nB < 0 || nPhases < 0) return fail(-1, "ebsd_pack_band_records: negative size");
+    if (N == 0 || nB == 0) return 0;
+    if (!bands || !d_records || !bands->d_max || !bands->d_normmax || !bands->d_avemax || !bands->d_maxloc ||
+        !bands->d_aveloc || !bands->d_width || !bands->d_theta || !bands->d_rho || !bands->d_valid)
+        return fail(-1, "ebsd_pack_band_records: null pointer");
+    const int R = 49 + 16 * nPhases;
+    const int64_t nbytes = N * (int64_t)nB * R;
+    hipLaunchKernelGGL(k_pack_band_records, dim3(grid_for(nbytes)), dim3(256), 0, (hipStream_t)stream, *bands,
+                       nbytes, R, (uint8_t *)d_records);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // ---------------------------------------------------------------------------------------
 // batches of mixed pattern sizes: one persistent gather launch and one persistent peak
 // launch over heterogeneous work lists, optionally captured once into a hipGraph

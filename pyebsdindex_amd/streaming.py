@@ -52,10 +52,14 @@ class StreamingBandDetect:
 
     FIELDS = ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid", "stats")
 
-    def __init__(self, bd, batch=8192, dtype="uint16", gpu_id=None, depth=3, chunk=None):
+    def __init__(self, bd, batch=8192, dtype="uint16", gpu_id=None, depth=3, chunk=None, records=None):
         import torch
         if depth < 2:
             raise ValueError("depth must be >= 2 (one slot copying while another computes)")
+        # records: a phase count -> each batch's bands come back as packed records in the byte
+        # layout of BandDetect.dataType with that many phases ("records", uint8 (n, nB*R)),
+        # beside "stats", instead of one tensor per field
+        self.records = None if records is None else int(records)
         self.bd = bd
         self.plan = bd.device_plan(gpu_id)
         self.device = self.plan.device
@@ -69,10 +73,14 @@ class StreamingBandDetect:
         # field layout of one result block: (name, torch dtype, per-pattern shape), 256-byte aligned
         self.layout = []
         off = 0
-        for k in self.FIELDS:
-            shp = {"maxloc": (nB, 2), "aveloc": (nB, 2), "stats": (3,)}.get(k, (nB,))
-            dt = torch.int8 if k == "valid" else torch.float32
-            nbytes = self.batch * int(np.prod(shp)) * (1 if k == "valid" else 4)
+        fields = self.FIELDS if self.records is None else ("records", "stats")
+        for k in fields:
+            if k == "records":
+                shp, dt, esz = (nB * (49 + 16 * self.records),), torch.uint8, 1
+            else:
+                shp = {"maxloc": (nB, 2), "aveloc": (nB, 2), "stats": (3,)}.get(k, (nB,))
+                dt, esz = (torch.int8, 1) if k == "valid" else (torch.float32, 4)
+            nbytes = self.batch * int(np.prod(shp)) * esz
             self.layout.append((k, dt, shp, off, nbytes))
             off += (nbytes + 255) // 256 * 256
         self.block_bytes = off
@@ -82,6 +90,13 @@ class StreamingBandDetect:
             self.blocks = [torch.empty(self.block_bytes, dtype=torch.uint8, device=self.device)
                            for _ in range(self.depth)]
             self.outs = [self._views(b, self.batch) for b in self.blocks]
+            if self.records is not None:
+                # the fields stay on the device (one set per slot); the block holds the records
+                # packed from them and the statistics
+                for o in self.outs:
+                    rec = o.pop("records")
+                    o.update({k: v for k, v in self.plan.alloc_out(self.batch).items() if k != "stats"})
+                    o["records"] = rec
             self.work = self.plan.workspace(self.batch, self.chunk)
             self.copy_stream = torch.cuda.Stream(self.device)      # host -> device batches
             self.compute_stream = torch.cuda.Stream(self.device)
@@ -162,13 +177,16 @@ class StreamingBandDetect:
 
         return self._run(len(counts), get, counts, blocks, uploaded)
 
-    def run_array(self, arr, blocks=None, ring=None, threads=8):
+    def run_array(self, arr, blocks=None, ring=None, threads=8, rec_out=None):
         """Stream a host array (N, H, W) of the pipeline's dtype (numpy, or a CPU tensor)
         through band detection: each batch is copied by ``threads`` threads into one buffer
         of a pinned host ring (``ring``, default ``depth + 1`` buffers from
         ``pinned_batch``) just before its upload, and a ring buffer is written again only
         after the upload that read it has finished, so the host copy of one batch overlaps
-        the upload and band detection of the ones before.  Returns the results of ``run``."""
+        the upload and band detection of the ones before.  Returns the results of ``run``.
+        ``rec_out`` (pipelines made with ``records``): a pinned uint8 tensor of N records
+        rows, ``N * record_row_bytes`` bytes, that receives every batch's records straight
+        from the device (the results' own "records" views are then not filled)."""
         import torch
         if isinstance(arr, torch.Tensor):
             arr = arr.numpy()
@@ -204,7 +222,7 @@ class StreamingBandDetect:
             ev.record(self.copy_stream)
             freed[j % R] = ev
 
-        return self._run(len(counts), get, counts, blocks, uploaded)
+        return self._run(len(counts), get, counts, blocks, uploaded, rec_out)
 
     def host_ring(self):
         """The pipeline's own pinned host ring (``depth + 1`` batches), made on first use."""
@@ -219,7 +237,12 @@ class StreamingBandDetect:
         if hb.dtype != self.tdtype:
             raise TypeError(f"batch dtype {hb.dtype}, pipeline dtype {self.tdtype}")
 
-    def _run(self, nb, get_batch, counts, blocks, on_upload=None):
+    @property
+    def record_row_bytes(self):
+        """Bytes of one pattern's records (nB records of 49 + 16 * phases bytes)."""
+        return None if self.records is None else self.plan.nB * (49 + 16 * self.records)
+
+    def _run(self, nb, get_batch, counts, blocks, on_upload=None, rec_out=None):
         """The pipeline over ``nb`` batches: ``get_batch(j)`` returns batch j as a pinned host
         tensor of ``counts[j]`` patterns, called in order just before its upload is
         enqueued; ``on_upload(j)`` runs right after."""
@@ -228,6 +251,15 @@ class StreamingBandDetect:
             blocks = self.pinned_blocks(nb)
         if len(blocks) < nb:
             raise ValueError("fewer result blocks than batches")
+        if rec_out is not None:
+            rb = self.record_row_bytes
+            if rb is None:
+                raise ValueError("rec_out needs a pipeline made with records")
+            if (rec_out.dtype != torch.uint8 or rec_out.is_cuda or not rec_out.is_pinned()
+                    or not rec_out.is_contiguous() or rec_out.numel() < sum(counts) * rb):
+                raise ValueError(f"rec_out must be a pinned contiguous uint8 tensor of {sum(counts) * rb} bytes")
+            # the statistics' span of a result block (the records lead it, at offset 0)
+            st = next((off, nbytes) for k, dt, shp, off, nbytes in self.layout if k == "stats")
         cs, ks, rs = self.copy_stream, self.compute_stream, self.return_stream
         D = self.depth
 
@@ -255,14 +287,21 @@ class StreamingBandDetect:
                 ks.wait_event(self.copied[s])
                 if self._used[s]:
                     ks.wait_event(self.returned[s])          # slot s's last results are back on the host
-                sub = {k: v[:n] for k, v in self.outs[s].items()}
+                sub = {k: v[:n] for k, v in self.outs[s].items() if k != "records"}
                 self.bd.find_bands_device(self.slots[s][:n], gpu_id=self.device.index, out=sub, work=self.work,
                                           chunk=self.chunk, stream=ks)
+                if self.records is not None:
+                    self.plan.pack_records(sub, self.records, dst=self.outs[s]["records"], stream=ks)
                 self.computed[s].record(ks)
                 self._used[s] = True
                 rs.wait_event(self.computed[s])
                 with torch.cuda.stream(rs):
-                    blocks[j].copy_(self.blocks[s], non_blocking=True)
+                    if rec_out is None:
+                        blocks[j].copy_(self.blocks[s], non_blocking=True)
+                    else:
+                        o0 = j * self.batch * rb
+                        rec_out[o0:o0 + n * rb].copy_(self.blocks[s][:n * rb], non_blocking=True)
+                        blocks[j][st[0]:st[0] + st[1]].copy_(self.blocks[s][st[0]:st[0] + st[1]], non_blocking=True)
                 self.returned[s].record(rs)
                 results.append(self._views(blocks[j], n))
             rs.synchronize()

@@ -175,3 +175,59 @@ def test_find_bands_api_pipeline_equals_device(shape, dtype, n, chunk, mode):
     bd.STREAM_MIN = 512
     again = bd.find_bands(pats, chunksize=chunk)
     assert np.array_equal(again, got)
+
+
+@pytest.mark.parametrize("n_phases", [1, 2, 10])
+def test_pack_band_records_bytes(n_phases):
+    """ebsd_pack_band_records writes the bands in the byte layout of BandDetect.dataType
+    (band_detect.py:110-115): every byte equals numpy's own assembly of the structured array
+    from the fields, for arbitrary float bit patterns (NaN payloads, -0, infinities) and
+    int8 flags, with the fields find_bands does not compute left zero."""
+    import torch
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.band_detect import record_phases
+    print(f"pack records n_phases={n_phases}", flush=True)
+    bd = BandDetect(patDim=(60, 60), nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=7,
+                    nPhases=n_phases)
+    assert record_phases(bd.dataType) == n_phases
+    plan = bd.device_plan()
+    N = 37
+    rng = np.random.default_rng(5 + n_phases)
+    out = plan.alloc_out(N)
+    host = {}
+    for k, v in out.items():
+        if k == "valid":
+            a = rng.integers(-128, 128, size=tuple(v.shape), dtype=np.int8)
+        else:
+            a = rng.integers(0, 2**32, size=tuple(v.shape), dtype=np.uint32).view(np.float32)
+        host[k] = a
+        v.copy_(torch.from_numpy(a.copy()))
+    rec = plan.pack_records(out, n_phases).cpu().numpy()
+    want = np.zeros((N, bd.nBands), dtype=bd.dataType)
+    for k in ("max", "normmax", "avemax", "maxloc", "aveloc", "width", "theta", "rho", "valid"):
+        want[k] = host[k]
+    assert rec.shape == (N * bd.nBands * bd.dataType.itemsize,)
+    assert np.array_equal(rec, want.view(np.uint8).reshape(-1))
+
+
+def test_find_bands_api_record_layouts():
+    """The drop-in call with two phases (records of 81 bytes) and with a dataType of another
+    layout (the per-field path): both equal the direct path's structured array."""
+    from pyebsdindex_amd import BandDetect
+    from pyebsdindex_amd.band_detect import record_phases
+    from pyebsdindex_amd.synthetic import kikuchi_patterns
+    print("api record layouts", flush=True)
+    pats = kikuchi_patterns(700, (60, 60), "uint8", seed=83).numpy()
+    for variant in ("two_phases", "other_layout"):
+        bd = BandDetect(patDim=(60, 60), nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9,
+                        nPhases=2)
+        if variant == "other_layout":
+            bd.dataType = np.dtype([(n, bd.dataType.fields[n][0]) for n in bd.dataType.names], align=True)
+            assert record_phases(bd.dataType) is None
+        got = bd.find_bands(pats, chunksize=256)
+        assert getattr(bd, "_pipelines", None), "the host input did not take the pipelined path"
+        assert got.dtype == bd.dataType and got.shape == (700, 9)
+        bd.STREAM_MIN = 701
+        direct = bd.find_bands(pats, chunksize=256)
+        assert np.array_equal(direct, got), variant
+        assert np.array_equal(got.view(np.uint8), direct.view(np.uint8)) or variant == "other_layout"

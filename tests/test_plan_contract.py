@@ -238,3 +238,21 @@ def test_auto_pass_size_rule():
         for b in (cells * 4, 10 ** 6, 10 ** 8, gib):
             k = bdmod.auto_pass_size(n, cells, 256, b)
             assert 1 <= k <= max(1, n) and (k * cells * 4 <= b or k == 1)
+
+
+def test_record_layout_of_the_structured_array():
+    """The drop-in call has the device write its records into the returned array only for a
+    dataType with the reference's packed layout (band_detect.py:110-115): record_phases
+    gives its phase count, and None for a layout that differs in order, offsets or types."""
+    for nph in (1, 2, 10):
+        bd = BandDetect(patDim=(60, 60), nTheta=180, nRho=90, nPhases=nph)
+        assert bd.dataType.itemsize == 49 + 16 * nph
+        assert bdmod.record_phases(bd.dataType) == nph
+    dt = BandDetect(patDim=(60, 60), nTheta=180, nRho=90).dataType
+    aligned = np.dtype([(n, dt.fields[n][0]) for n in dt.names], align=True)
+    assert bdmod.record_phases(aligned) is None
+    reordered = np.dtype([(n, dt.fields[n][0]) for n in reversed(dt.names)])
+    assert bdmod.record_phases(reordered) is None
+    wider = np.dtype([(n, np.float64 if n == "theta" else dt.fields[n][0]) for n in dt.names])
+    assert bdmod.record_phases(wider) is None
+    assert bdmod.record_phases(np.dtype(np.float32)) is None
