@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--pmc-driver", action="store_true",
                     help="short fixed launch sequence for rocprofv3 --pmc: gather x3, peaks x3 on one "
                          "chunk, then a 256 MiB device copy x3 (the counter calibration)")
+    ap.add_argument("--pmc-driver-probe", action="store_true",
+                    help="the VALU issue probe alone for rocprofv3 --pmc: each of its four instruction "
+                         "mixes once (an untimed and a timed launch of 2,048 work-groups x 4,096 steps)")
     ap.add_argument("--pmc-driver-nlpar", action="store_true",
                     help="short fixed NLPAR sequence for rocprofv3 --pmc: the 256x256 scan of 60x60 uint8 "
                          "patterns of the NLPAR measurement, sigma once, NLPAR twice (uint8 output)")
@@ -83,6 +86,31 @@ def lds_ceiling(reps=2):
                 raise RuntimeError("LDS probe failed")
             best = max(best, v.value)
         out[f"ds_read_b{8 * w}"] = round(best / 1e9, 1)
+    return out
+
+
+def valu_ceiling(reps=2):
+    """Vector-instruction issue rate of this GPU (wave-instructions/s) for independent chains
+    of v_fma_f32, of v_xor_b32/v_add_u32, of v_max_i32 with a DPP source and v_sub_u32, and of
+    v_fma_f64, from the measurement-only probe library, the best of ``reps`` runs per kind."""
+    import ctypes
+    import torch
+    from pyebsdindex_amd.build import PROBE_LIB
+    L = ctypes.CDLL(PROBE_LIB)
+    L.ebsd_probe_valu_issue.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    s = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for kind, name in ((0, "v_fma_f32"), (1, "v_xor_b32+v_add_u32"), (2, "v_max_i32_dpp+v_sub_u32"),
+                       (3, "v_fma_f64")):
+        best = 0.0
+        for _ in range(reps):
+            v = ctypes.c_double()
+            # 2,048 work-groups of 256 threads: 8 waves on every SIMD of 256 CUs
+            if L.ebsd_probe_valu_issue(kind, 2048, 4096, ctypes.byref(v), s) != 0:
+                raise RuntimeError("VALU probe failed")
+            best = max(best, v.value)
+        out[name] = round(best / 1e9, 1)
     return out
 
 
@@ -565,6 +593,10 @@ def main():
     if args.pmc_driver_nlpar:
         nlpar_pmc_driver(dev)
         return
+    if args.pmc_driver_probe:
+        torch.zeros(1, device=dev)
+        print(json.dumps({"pmc_driver_probe": True, "valu_ceiling_GIPS": valu_ceiling(reps=1)}), flush=True)
+        return
     H = W = args.size
     params = dict(nTheta=180, nRho=90, tSigma=1, rSigma=2, rhoMaskFrac=0.1, nBands=9)
     bd = BandDetect(patDim=(H, W), **params)
@@ -654,6 +686,9 @@ def main():
         actual["frac"] = round(actual["achieved"] / actual["peak"], 4)
         actual["entry_slots_per_pattern"] = slots
     vc = valu_counts(H, W, args.dtype, chunk)
+    print("[bench] VALU issue ceiling (independent chains, measured now)", file=sys.stderr, flush=True)
+    valu_ceil = valu_ceiling()
+    valu_peak = max(valu_ceil.values()) * 1e9
 
     # throughput variants of the headline workload: the exact background mode (the
     # reference's per-pixel float32 rounding, float64 sums), and the drop-in API from host
@@ -799,13 +834,19 @@ def main():
                      "valu": None if vc is None else {
                          "instructions_per_launch": vc["k_radon_gather"],
                          "achieved": round(vc["k_radon_gather"] / (gather_ms * 1e-3) / 1e9, 1),
-                         "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instructions/s",
-                         "frac": round(vc["k_radon_gather"] / (gather_ms * 1e-3) / VALU_ISSUE_PEAK, 4)}},
+                         "peak": round(valu_peak / 1e9, 1), "unit": "G wave-instructions/s",
+                         "frac": round(vc["k_radon_gather"] / (gather_ms * 1e-3) / valu_peak, 4),
+                         "peak_kind": "VALU issue ceiling measured in this run (the best of the "
+                                      "probe's instruction mixes)",
+                         "peak_nominal": VALU_ISSUE_PEAK / 1e9}},
         "roofline_peaks": None if vc is None else {
             "bound": "valu", "kernel": "k_peaks_stream", "unit": "G wave-instructions/s",
             "achieved": round(vc["k_peaks_stream"] / (peaks_ms * 1e-3) / 1e9, 1),
-            "peak": VALU_ISSUE_PEAK / 1e9,
-            "frac": round(vc["k_peaks_stream"] / (peaks_ms * 1e-3) / VALU_ISSUE_PEAK, 4),
+            "peak": round(valu_peak / 1e9, 1),
+            "frac": round(vc["k_peaks_stream"] / (peaks_ms * 1e-3) / valu_peak, 4),
+            "peak_kind": "VALU issue ceiling measured in this run (the best of the probe's "
+                         "instruction mixes)",
+            "valu_ceiling_measured": valu_ceil, "peak_nominal": VALU_ISSUE_PEAK / 1e9,
             "instructions_per_launch": vc["k_peaks_stream"], "avg_launch_ms": round(peaks_ms, 4),
             "counts_from": f"profiles/valu_counts_{H}x{W}_{args.dtype}.json (rocprofv3 SQ_INSTS_VALU)"},
         "exact_mode": exact,

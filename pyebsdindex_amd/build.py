@@ -4,7 +4,7 @@
 ``csrc/ebsd_radon.hip`` (band detection -> ``libebsd_radon.so``) and
 ``csrc/ebsd_nlpar.hip`` (NLPAR -> ``libebsd_nlpar.so``) and ``csrc/ebsd_tripvote.hip``
 (triplet voting -> ``libebsd_tripvote.so``), plus the measurement-only
-``csrc/ebsd_ldsprobe.hip`` (the LDS read ceiling -> ``libebsd_ldsprobe.so``, used by bench.py).  All are plain C-ABI shared objects (no torch
+``csrc/ebsd_ldsprobe.hip`` (the LDS read and VALU issue ceilings -> ``libebsd_ldsprobe.so``, used by bench.py).  All are plain C-ABI shared objects (no torch
 symbols); ``pyebsdindex_amd._hip`` loads them with ctypes.
 """
 import os

@@ -1578,12 +1578,16 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MULTI ? 1 
     const uint32_t fm = flip ? 0xffffffffu : 0u;
     const uint32_t fb = !cval ? 0x80000000u                      // column outside: >= 2^31
                               : (flip ? (uint32_t)((g.nR - 1) * g.nT * 4 + jc4 + 1) : (uint32_t)jc4);
+    // row range tests as one unsigned compare each (r in [lo, lo+n) <=> (unsigned)(r-lo) < n),
+    // so that each is one scalar compare and select rather than two and a mask operation
+    const int nT4 = g.nT * 4;
+    const uint32_t sr_n = (uint32_t)max(g.sr1 - g.sr0, 0), cr_n = (uint32_t)max(g.cr1 - g.cr0, 0);
     auto stage_load = [&](int r) -> float {
-        int i = r - g.p0;
-        i = i < 0 ? 0 : (i > g.nR - 1 ? g.nR - 1 : i);
-        // a row outside the padded array: S = 2^30 puts every lane's offset at or above
-        // 2^30 - (nR*nT*4) (flipped lanes) or 2^30 (others), far beyond the buffer
-        const uint32_t S = (r >= 0 && r < g.nRp) ? (uint32_t)(i * g.nT * 4) : 0x40000000u;
+        const int i = min(max(r - g.p0, 0), g.nR - 1);
+        // a row outside the padded array: bit 30 of S set puts every lane's offset at or
+        // above 2^30 - (nR*nT*4) (flipped lanes) or 2^30 (others), far beyond the buffer
+        // (computed without a branch: i*nT*4 < nR*nT*4 < 2^30)
+        const uint32_t S = (uint32_t)(i * nT4) | ((uint32_t)((uint32_t)r >= (uint32_t)g.nRp) << 30);
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Rrs, (int)((S ^ fm) + fb), 0, 0));
     };
 
@@ -1645,11 +1649,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MULTI ? 1 
             // (which the compiler turns into four selects per row): a row outside adds
             // acc * 0.0 (a zero: the float64 sum, which starts at +0, is unchanged; the
             // convolved values are finite) and clamps its key to the identity of min/max.
-            const bool srow = r >= g.sr0 && r < g.sr1;
+            const bool srow = (uint32_t)(r - g.sr0) < sr_n;
             kmin = min(kmin, max(key, srow ? INT_MIN : INT_MAX));
             kmax = max(kmax, min(key, srow ? INT_MAX : KEY_MASKED));
             ssum = fma((double)acc, srow ? 1.0 : 0.0, ssum);
-            Mr[2 * HR + k] = (cval && r >= g.cr0 && r < g.cr1) ? key : KEY_MASKED;
+            Mr[2 * HR + k] = (cval && (uint32_t)(r - g.cr0) < cr_n) ? key : KEY_MASKED;
         }
         // ---- rows d = d0 + k: rho dilation (windows Mr[k .. k + 2HR]), theta dilation,
         //      D == C under the mask -> candidates ----

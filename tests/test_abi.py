@@ -40,14 +40,15 @@ def test_library_exports_every_declared_symbol():
     assert set(declared_tv) == set(_hip.TRIPVOTE_SYMBOLS)
     for name in declared_tv:
         assert hasattr(tv, name), name
-    # the measurement-only probe (LDS read ceiling for the bench's roofline)
+    # the measurement-only probe (LDS read and VALU issue ceilings for the bench's rooflines)
     from pyebsdindex_amd.build import PROBE_LIB
     pr = ctypes.CDLL(PROBE_LIB)
     declared_pr = _declared("ebsd_probe.h")
-    assert declared_pr == ["ebsd_probe_lds_read"]
+    assert declared_pr == ["ebsd_probe_lds_read", "ebsd_probe_valu_issue"]
     for name in declared_pr:
         assert hasattr(pr, name), name
     assert pr.ebsd_probe_lds_read(3, 1, 1, None, None) == -1   # bad arguments, no launch
+    assert pr.ebsd_probe_valu_issue(4, 1, 1, None, None) == -1
     # every header's declarations are covered by one of the libraries
     assert set(_declared()) == set(declared) | set(declared_nl) | set(declared_tv) | set(declared_pr)
 
