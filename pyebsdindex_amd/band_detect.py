@@ -669,6 +669,10 @@ class BandDetect:
     #: 2 slots, 10.2 with 4,096 and 12.4 with 1,024)
     STREAM_BATCH = 2048
     STREAM_DEPTH = 3
+    #: the pipelined path returns its structured array in pinned host memory (the records are
+    #: copied there straight from the device); False returns a pageable copy instead, for
+    #: callers that keep many results alive
+    PINNED_RESULT = True
 
     def _find_bands_streamed(self, patternsIn, plan, chunksize):
         """find_bands of host patterns (numpy or CPU tensor, uint8/uint16/float32, at least
@@ -716,6 +720,9 @@ class BandDetect:
             rec = torch.empty(N * sb.record_row_bytes, dtype=torch.uint8, pin_memory=True)
             res = sb.run_array(arr, blocks=blocks[:nb], rec_out=rec)
             bandData = rec.numpy().view(self.dataType).reshape(N, self.nBands)
+            if not self.PINNED_RESULT:
+                bandData = bandData.copy()
+                del rec
         else:
             res = sb.run_array(arr, blocks=blocks[:nb])
             bandData = np.zeros((N, self.nBands), dtype=self.dataType)

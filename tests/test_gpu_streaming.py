@@ -227,6 +227,9 @@ def test_find_bands_api_record_layouts():
         got = bd.find_bands(pats, chunksize=256)
         assert getattr(bd, "_pipelines", None), "the host input did not take the pipelined path"
         assert got.dtype == bd.dataType and got.shape == (700, 9)
+        bd.PINNED_RESULT = False
+        pageable = bd.find_bands(pats, chunksize=256)
+        assert np.array_equal(pageable, got), variant
         bd.STREAM_MIN = 701
         direct = bd.find_bands(pats, chunksize=256)
         assert np.array_equal(direct, got), variant
