@@ -1,7 +1,7 @@
 """Reference-faithful CPU restatement of PyEBSDIndex's CPU band detection.
 
-TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  This is the parity target
-named by the task and the timed CPU baseline.  It follows, step by step:
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  This is the parity target of the
+device path and the timed CPU baseline.  It follows, step by step:
 
 * ``pyebsdindex/radon_fast.py:64-193``   plan set-up (same numpy expressions, so the
   same float32/float64 promotions and half-even ``np.round``),

@@ -92,7 +92,7 @@ def test_bandindex_device_tensors_and_edges():
                                                   ((160, 160), "uint8", "BCC", 24),
                                                   ((240, 240), "uint16", "HCP", 12)])
 def test_chain_orientations_within_0p05(shape, dtype, family, n):
-    """The north-star bar: patterns -> device find_bands -> device radon2pole -> device
+    """The downstream orientation bar: patterns -> device find_bands -> device radon2pole -> device
     bandindex, against the FP64 oracle's bands -> the restated radon2pole -> the oracle's
     bandindex, with the band weights EBSDIndexer uses; orientations within 0.05 degree where
     both index a pattern, and within 0.5 degree (median) of the generator's own."""
